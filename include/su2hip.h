@@ -1,0 +1,312 @@
+/* su2hip.h -- C ABI of the MI355X (gfx950) lattice update sweep for 3d SU(2)
+ * (+U(1)) gauge theory with fundamental / adjoint / singlet scalars.
+ *
+ * One handle (su2hip_batch) owns one or more independent lattices that are
+ * swept together on one HIP stream: every kernel launch covers every active
+ * lattice of the batch.  A batch of one is the plain single-lattice run.
+ * Lattices of one batch share the field content, precision and pass schedule
+ * (algorithms and per-sweep update counts); they may differ in extents,
+ * couplings and RNG seed.
+ *
+ * Fields cross this boundary in the reference's host layout and site order
+ * (src/alloc.c:30-75, src/layout.c:826-851): su2link double[V][3][4],
+ * u1link double[V][3], doublet double[V][4], triplet double[V][3],
+ * singlet double[V], with the V sites ordered "even parity first, odd
+ * second, natural x-fastest order inside each parity".  All functions return
+ * 0 on success and a negative SU2HIP_E* code otherwise; su2hip_error_string()
+ * and su2hip_last_error() describe the failure.  No function calls exit().
+ *
+ * Threading: a handle is used by one host thread at a time.
+ */
+#ifndef SU2HIP_H
+#define SU2HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Identifiers with the reference's values (src/generic/stddefs.h:46-65). */
+#define SU2HIP_METROPOLIS 1
+#define SU2HIP_HEATBATH 2
+#define SU2HIP_OVERRELAX 3
+
+#define SU2HIP_EVEN 0
+#define SU2HIP_ODD 1
+
+#define SU2HIP_PHISQ 1
+#define SU2HIP_SIGMASQ 2
+#define SU2HIP_PHI2MINUSSIGMA2 3
+
+#define SU2HIP_READONLY 0
+#define SU2HIP_FAST 1
+#define SU2HIP_SLOW 2
+
+/* Arithmetic of the sweep kernels.  Reductions, multicanonical weights and
+ * the global accept test are FP64 in both modes. */
+#define SU2HIP_FP64 64
+#define SU2HIP_FP32 32
+
+/* Gradient-flow integrators. */
+#define SU2HIP_FLOW_EULER 1 /* the reference's scheme, src/gradflow.c:14-18 */
+#define SU2HIP_FLOW_RK3 3   /* Luscher's 3rd-order Runge-Kutta, arXiv:1006.4518 */
+
+/* Error codes. */
+#define SU2HIP_OK 0
+#define SU2HIP_EINVAL -1      /* bad argument */
+#define SU2HIP_ENOMEM -2      /* device or host allocation failed / budget exceeded */
+#define SU2HIP_EDEVICE -3     /* HIP runtime error */
+#define SU2HIP_ERANGE -4      /* index or size out of range */
+#define SU2HIP_EUNSUPPORTED -5/* model combination not supported */
+#define SU2HIP_ESTATE -6      /* call not valid in the current state */
+#define SU2HIP_EIO -7         /* file I/O or parse error (host I/O layer) */
+
+/* Run parameters of one lattice.  The scalar couplings keep the reference's
+ * names and meaning (src/su2.h:76-181, parsed at src/parameters.c:157-338).
+ * Field content is a runtime choice here: nhiggs (0/1), u1, triplet and
+ * singlet replace the reference's -DNHIGGS/-DU1/-DTRIPLET/-DSINGLET flags. */
+typedef struct su2hip_params {
+  int32_t L[3];              /* extents; all even */
+  int32_t nhiggs;            /* fundamental doublets: 0 or 1 */
+  int32_t u1;                /* compact U(1) hypercharge links */
+  int32_t triplet;           /* adjoint scalar */
+  int32_t singlet;           /* real singlet scalar */
+  int32_t precision;         /* SU2HIP_FP64 or SU2HIP_FP32 */
+  int32_t algorithm_su2link; /* HEATBATH or METROPOLIS */
+  int32_t algorithm_u1link;  /* METROPOLIS */
+  int32_t algorithm_su2doublet; /* OVERRELAX or METROPOLIS */
+  int32_t algorithm_su2triplet; /* OVERRELAX or METROPOLIS */
+  int32_t algorithm_singlet;    /* OVERRELAX or METROPOLIS */
+  int32_t update_links;      /* link rounds per sweep (update_links) */
+  int32_t link_overrelax;    /* SU(2) link overrelaxation rounds after each link round (no reference counterpart) */
+  int32_t scalar_sweeps;     /* scalar rounds per sweep (scalar_sweeps) */
+  int32_t update_su2doublet; /* doublet passes per scalar round (update_doublet) */
+  int32_t update_su2triplet; /* triplet passes per scalar round (update_triplet) */
+  int32_t update_singlet;    /* singlet passes per scalar round (update_singlet) */
+  int32_t reunitarize;       /* FP32 only: renormalise every link a kernel writes */
+  int32_t reserved;
+  double betasu2;
+  double betau1, r_u1;
+  double msq_phi, lambda_phi;
+  double msq_triplet, a2, b4;
+  double b1_s, msq_s, b3_s, b4_s, a1_s, a2_s;
+  uint64_t seed;             /* Philox key of this lattice */
+} su2hip_params;
+
+/* Accept/total counters, the reference's `counters` (src/su2.h:208-231) for
+ * one lattice and one doublet, plus the link overrelaxation counts. */
+typedef struct su2hip_counters {
+  int64_t total_su2link, accepted_su2link;
+  int64_t total_u1link, accepted_u1link;
+  int64_t total_doublet, accepted_doublet;
+  int64_t total_overrelax_doublet, acc_overrelax_doublet;
+  int64_t total_triplet, accepted_triplet;
+  int64_t total_overrelax_triplet, acc_overrelax_triplet;
+  int64_t total_singlet, accepted_singlet;
+  int64_t total_overrelax_singlet, acc_overrelax_singlet;
+  int64_t total_muca, accepted_muca;
+  int64_t total_link_or, accepted_link_or;
+} su2hip_counters;
+
+/* Multicanonical state of one lattice (the scalar part of the reference's
+ * `weight`, src/su2.h:235-274).  The bins+2 arrays pos[] and W[] travel
+ * separately; index 0 and bins+1 are the reference's virtual end bins. */
+typedef struct su2hip_muca {
+  int32_t enabled;           /* multicanonical != 0 */
+  int32_t orderparam;        /* SU2HIP_PHISQ / SIGMASQ / PHI2MINUSSIGMA2 */
+  int32_t mode;              /* SU2HIP_READONLY / FAST / SLOW */
+  int32_t checks_per_sweep;
+  int32_t bins;
+  int32_t last_max;
+  int32_t do_acceptance;
+  int32_t muca_count;
+  int32_t update_interval;
+  int32_t reserved;
+  double delta, min, max, wrk_min, wrk_max;
+  double param_value[2];     /* order parameter contribution of EVEN / ODD sites */
+} su2hip_muca;
+
+/* Layout of the FP64 volume sums returned by su2hip_measure().  Every model
+ * uses the same slots; absent fields leave theirs at 0.  The column values
+ * of the reference's `measure` file (src/measure.c:252-306) are these sums
+ * divided as su2hip_measure_columns() does. */
+enum {
+  SU2HIP_SUM_ACTION = 0,
+  SU2HIP_SUM_WILSON,        /* sum of local_su2wilson (includes beta) */
+  SU2HIP_SUM_U1WILSON,
+  SU2HIP_SUM_COV_PHI,
+  SU2HIP_SUM_PHI2,
+  SU2HIP_SUM_PHI4,
+  SU2HIP_SUM_HOP_SIGMA,
+  SU2HIP_SUM_SIGMA2,
+  SU2HIP_SUM_SIGMA4,
+  SU2HIP_SUM_PHI2SIGMA2,
+  SU2HIP_SUM_MAGCHARGE,
+  SU2HIP_SUM_MAGCHARGE_ABS,
+  SU2HIP_SUM_S_COV,
+  SU2HIP_SUM_S,
+  SU2HIP_SUM_S2,
+  SU2HIP_SUM_S3,
+  SU2HIP_SUM_S4,
+  SU2HIP_SUM_SPHISQ,
+  SU2HIP_SUM_S2PHISQ,
+  SU2HIP_NSUMS
+};
+
+/* Kinds of kernel launches, used by su2hip_pass() and the timing API. */
+enum {
+  SU2HIP_K_LINK_HEATBATH = 0,
+  SU2HIP_K_LINK_METROPOLIS,
+  SU2HIP_K_LINK_OVERRELAX,
+  SU2HIP_K_U1_METROPOLIS,
+  SU2HIP_K_DOUBLET_OVERRELAX,
+  SU2HIP_K_DOUBLET_METROPOLIS,
+  SU2HIP_K_TRIPLET_OVERRELAX,
+  SU2HIP_K_TRIPLET_METROPOLIS,
+  SU2HIP_K_SINGLET_OVERRELAX,
+  SU2HIP_K_SINGLET_METROPOLIS,
+  SU2HIP_K_MEASURE,
+  SU2HIP_K_MUCA,
+  SU2HIP_K_FLOW,
+  SU2HIP_K_AUX,
+  SU2HIP_NKINDS
+};
+
+/* Opaque handle: one or more lattices on one GPU stream. */
+typedef struct su2hip_batch su2hip_batch_t;
+
+/* ---- errors and device ---------------------------------------------- */
+
+/* Text for an error code. */
+const char *su2hip_error_string(int code);
+/* Detailed message of the last failure on this host thread. */
+const char *su2hip_last_error(void);
+
+/* ---- handle lifecycle --------------------------------------------------
+ * Replaces the reference's layout()+alloc_fields() (src/su2.h:310,335;
+ * src/layout.c:657-704, src/alloc.c:103-125) for n lattices at once.
+ * max_slots >= n reserves room for lattices added later (serving);
+ * hbm_budget (bytes, 0 = unlimited) caps the field memory of all slots. */
+int su2hip_create(int n, const su2hip_params *params, int max_slots,
+                  uint64_t hbm_budget, su2hip_batch_t **out);
+int su2hip_destroy(su2hip_batch_t *b);
+
+/* Device bytes one lattice with these params occupies (fields, multicanonical
+ * backup, and, with with_flow, the flow scratch). */
+int su2hip_footprint(const su2hip_params *p, int with_flow, uint64_t *bytes);
+
+/* Add a lattice in a free slot (fields cold-started); fails with
+ * SU2HIP_ENOMEM when the HBM budget or the slots are exhausted. */
+int su2hip_add(su2hip_batch_t *b, const su2hip_params *p, int *slot);
+/* Free a slot and its device memory. */
+int su2hip_remove(su2hip_batch_t *b, int slot);
+/* Number of slots / whether a slot is in use. */
+int su2hip_slots(su2hip_batch_t *b, int *max_slots, int *active);
+int su2hip_slot_active(su2hip_batch_t *b, int slot, int *active);
+
+/* ---- fields ------------------------------------------------------------
+ * Host <-> device in the reference layout (src/checkpoint.c:101-115 order of
+ * fields; a NULL pointer skips a field).  Replaces load_lattice/save_lattice's
+ * field transfer (src/su2.h:458-459). */
+int su2hip_set_fields(su2hip_batch_t *b, int slot, const double *su2link,
+                      const double *u1link, const double *doublet,
+                      const double *triplet, const double *singlet);
+int su2hip_get_fields(su2hip_batch_t *b, int slot, double *su2link,
+                      double *u1link, double *doublet, double *triplet,
+                      double *singlet);
+/* Cold start, the reference's setfields() (src/init.c:92-112). */
+int su2hip_cold_start(su2hip_batch_t *b, int slot, double phi0, double sigma0,
+                      double singlet0);
+/* Hot start from the lattice's Philox key: links normalised N(0,1)
+ * quaternions, scalar components N(0,1), U(1) angles uniform in [-pi,pi).
+ * No reference counterpart (the reference only cold-starts). */
+int su2hip_hot_start(su2hip_batch_t *b, int slot);
+
+/* ---- sweep ---------------------------------------------------------------
+ * n_sweeps compound sweeps of every active lattice whose sweep budget is not
+ * exhausted; one sweep is the reference's update_lattice() (src/su2.h:426,
+ * src/update.c:270-430) with link_overrelax extra link rounds.
+ * Asynchronous: returns once the work is queued. */
+int su2hip_update(su2hip_batch_t *b, int n_sweeps);
+/* Per-lattice sweep budget: a lattice with budget 0 is skipped by
+ * su2hip_update(); each completed sweep decrements it.  -1 = unlimited. */
+int su2hip_set_budget(su2hip_batch_t *b, int slot, int64_t sweeps);
+/* Budgets of all slots (max_slots entries, -2 for an empty slot). */
+int su2hip_get_budgets(su2hip_batch_t *b, int64_t *budgets);
+/* One checkerboard pass of one kind over every active lattice (the
+ * reference's checkerboard_sweep_* for one parity and direction,
+ * src/update.c:35-259).  dir is ignored for scalar kinds. */
+int su2hip_pass(su2hip_batch_t *b, int kind, int parity, int dir);
+/* Capture each compound sweep into a hipGraph that is replayed (1) or launch
+ * kernels one by one (0, default). */
+int su2hip_use_graph(su2hip_batch_t *b, int enable);
+/* Wait for all queued work. */
+int su2hip_sync(su2hip_batch_t *b);
+
+/* Counters of a lattice since the last reset (src/init.c:167-199). */
+int su2hip_get_counters(su2hip_batch_t *b, int slot, su2hip_counters *c);
+int su2hip_reset_counters(su2hip_batch_t *b, int slot);
+/* RNG position: Philox key and the 64-bit pass counter (for exact resume). */
+int su2hip_rng_get(su2hip_batch_t *b, int slot, uint64_t *seed, uint64_t *pass);
+int su2hip_rng_set(su2hip_batch_t *b, int slot, uint64_t seed, uint64_t pass);
+
+/* ---- measurement -----------------------------------------------------------
+ * FP64 volume sums of the reference's measure() site loop
+ * (src/su2.h:475, src/measure.c:110-174), deterministic tree order.
+ * sums has SU2HIP_NSUMS entries. */
+int su2hip_measure(su2hip_batch_t *b, int slot, double *sums);
+/* Sums of every slot at once: sums[max_slots][SU2HIP_NSUMS] (empty slots 0). */
+int su2hip_measure_all(su2hip_batch_t *b, double *sums);
+/* Column values as written to `measure` (src/measure.c:252-306) from sums,
+ * for the field content of p; weight/muca_param fill the first two columns.
+ * Returns the number of columns in *ncols. */
+int su2hip_measure_columns(const su2hip_params *p, const double *sums,
+                           double weight, double muca_param, double *cols,
+                           int *ncols);
+
+/* ---- multicanonical --------------------------------------------------------
+ * Install the weight of a lattice (load_weight's result, src/su2.h:483,
+ * src/multicanonical.c:118-288) and compute both parity contributions of
+ * the order parameter (calc_orderparam, src/multicanonical.c:709-751).
+ * pos/W have m->bins+2 entries.  The global accept/reject then runs on the
+ * device inside every scalar pass of the order-parameter field. */
+int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m,
+                    const double *pos, const double *W);
+/* Read back the weight state (pos/W/hits may be NULL). */
+int su2hip_muca_get(su2hip_batch_t *b, int slot, su2hip_muca *m, double *pos,
+                    double *W, int32_t *hits);
+/* W(R) of the installed weight (get_weight, src/multicanonical.c:350-379). */
+int su2hip_muca_weight_of(su2hip_batch_t *b, int slot, double value,
+                          double *weight);
+
+/* ---- gradient flow ---------------------------------------------------------
+ * Flow a copy of the lattice's links (+ triplet) from t=0 to t_max
+ * (grad_flow, src/su2.h:543, src/gradflow.c:46-173); the lattice itself is
+ * not changed.  Every meas_interval steps (and at the end) a row of
+ * 1 + SU2HIP_NSUMS doubles {t, sums...} is stored in rows (row 0 is t=0).
+ * *nrows returns the number of rows written (at most max_rows). */
+int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt,
+                double t_max, int meas_interval, double *rows, int max_rows,
+                int *nrows);
+/* Gradient force on every link, double[V][3][4] with component 0 = 0
+ * (calc_gradient's link part, src/gradflow.c:335-354). */
+int su2hip_flow_force(su2hip_batch_t *b, int slot, double *force);
+
+/* ---- timing ----------------------------------------------------------------
+ * With timing on, every launch is bracketed by HIP events on the handle's
+ * stream; su2hip_kernel_times() returns the summed milliseconds and the
+ * launch count per SU2HIP_K_* kind since the last reset. */
+int su2hip_set_timing(su2hip_batch_t *b, int enable);
+int su2hip_kernel_times(su2hip_batch_t *b, double *ms, int64_t *launches);
+int su2hip_reset_timing(su2hip_batch_t *b);
+/* Copy `bytes` of device memory with the library's streaming copy kernel and
+ * return its duration (ms): the calibration kernel for HBM counters. */
+int su2hip_copy_probe(uint64_t bytes, int repeats, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SU2HIP_H */

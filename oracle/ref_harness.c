@@ -1,0 +1,523 @@
+/* ref_harness.c -- drives the reference su2higgs C code compiled from its own
+ * sources (oracle/Makefile builds it into oracle/_ref/), for checking the
+ * oracle restatement.  TEST INFRASTRUCTURE: loaded only by tests/ and by
+ * bench.py's cpu_baseline leg.
+ *
+ * The reference's RNG entry points (src/generic/mersenne.h:9-11) are
+ * defined here instead of mersenne.o: dran() returns the oracle's Philox
+ * uniform for the current (pass, site, k).  The per-site update functions
+ * and multicanonical_acceptance are linked with --wrap, so the wrappers see
+ * every call the reference's own update_lattice (src/update.c:270-430)
+ * makes: a call at the first site of a parity opens a new pass, and each
+ * call sets the site context before running the reference function.
+ */
+#include "su2.h"
+
+#include <stdint.h>
+#include <string.h>
+
+#include "su2oracle.h"
+
+MPI_Comm MPI_COMM_WORLD;
+
+static lattice l;
+static params p;
+static fields f;
+static counters c;
+static weight w;
+
+static uint64_t h_seed, h_next_pass, h_cur_pass;
+static uint32_t h_site, h_k, h_check;
+
+double genrand64_real2(void) {
+  return orc_uniform(h_seed, h_cur_pass, h_site, h_k++);
+}
+unsigned long long genrand64_int64(void) {
+  double u = genrand64_real2();
+  return (unsigned long long)(u * 18446744073709551616.0);
+}
+void init_genrand64(unsigned long long s) { (void)s; }
+
+static void site_ctx(long i) {
+  if (i == 0 || i == l.evensites) {
+    h_cur_pass = h_next_pass++;
+    h_check = 0;
+  }
+  h_site = (uint32_t)i;
+  h_k = 0;
+}
+
+#define WRAP(ret, name, decl, args) \
+  ret __real_##name decl;          \
+  ret __wrap_##name decl {         \
+    site_ctx(i);                   \
+    return __real_##name args;     \
+  }
+
+WRAP(int, heatbath_su2link, (lattice const *ll, fields *ff, params const *pp, long i, int dir), (ll, ff, pp, i, dir))
+WRAP(int, metro_su2link, (lattice const *ll, fields *ff, params const *pp, long i, int dir), (ll, ff, pp, i, dir))
+#ifdef U1
+WRAP(int, metro_u1link, (lattice const *ll, fields *ff, params const *pp, long i, int dir), (ll, ff, pp, i, dir))
+#endif
+#if (NHIGGS > 0)
+WRAP(int, metro_doublet, (lattice const *ll, fields *ff, params const *pp, long i, int h), (ll, ff, pp, i, h))
+WRAP(int, overrelax_doublet, (lattice const *ll, fields *ff, params const *pp, long i), (ll, ff, pp, i))
+#endif
+#ifdef TRIPLET
+WRAP(int, metro_triplet, (lattice const *ll, fields *ff, params const *pp, long i), (ll, ff, pp, i))
+WRAP(int, overrelax_triplet, (lattice const *ll, fields *ff, params const *pp, long i), (ll, ff, pp, i))
+#endif
+#ifdef SINGLET
+WRAP(int, metro_singlet, (lattice const *ll, fields *ff, params const *pp, long i), (ll, ff, pp, i))
+WRAP(int, overrelax_singlet, (lattice const *ll, fields *ff, params const *pp, long i), (ll, ff, pp, i))
+#endif
+
+int __real_multicanonical_acceptance(lattice const *ll, weight *ww, double o, double n);
+int __wrap_multicanonical_acceptance(lattice const *ll, weight *ww, double o, double n) {
+  h_site = ORC_MUCA_SITE;
+  h_k = h_check++;
+  return __real_multicanonical_acceptance(ll, ww, o, n);
+}
+
+/* which model flags this build has: bit0 NHIGGS, bit1 U1, bit2 TRIPLET,
+ * bit3 SINGLET, bit4 GRADFLOW */
+int ref_model(void) {
+  int m = 0;
+#if (NHIGGS > 0)
+  m |= 1;
+#endif
+#ifdef U1
+  m |= 2;
+#endif
+#ifdef TRIPLET
+  m |= 4;
+#endif
+#ifdef SINGLET
+  m |= 8;
+#endif
+#ifdef GRADFLOW
+  m |= 16;
+#endif
+  return m;
+}
+
+int ref_setup(const su2hip_params *P) {
+  memset(&l, 0, sizeof(l));
+  memset(&p, 0, sizeof(p));
+  memset(&w, 0, sizeof(w));
+  myRank = 0;
+  MPISize = 1;
+  l.rank = 0;
+  l.size = 1;
+  l.dim = 3;
+  l.L = malloc(3 * sizeof(int));
+  l.vol = 1;
+  for (int d = 0; d < 3; d++) {
+    l.L[d] = P->L[d];
+    l.vol *= P->L[d];
+  }
+  layout(&l, 0, 0);
+  alloc_fields(&l, &f);
+  p.betasu2 = P->betasu2;
+  p.algorithm_su2link = P->algorithm_su2link;
+  p.algorithm_su2doublet = P->algorithm_su2doublet;
+  p.update_links = P->update_links;
+  p.scalar_sweeps = P->scalar_sweeps;
+  p.update_su2doublet = P->update_su2doublet;
+  p.random_sweeps = 0;
+  p.multicanonical = 0;
+  p.msq_phi = P->msq_phi;
+  p.lambda_phi = P->lambda_phi;
+  p.msq_triplet = P->msq_triplet;
+  p.a2 = P->a2;
+  p.b4 = P->b4;
+  p.update_su2triplet = P->update_su2triplet;
+  p.algorithm_su2triplet = P->algorithm_su2triplet;
+#ifdef U1
+  p.betau1 = P->betau1;
+  p.r_u1 = P->r_u1;
+  p.algorithm_u1link = P->algorithm_u1link;
+#endif
+#ifdef SINGLET
+  p.update_singlet = P->update_singlet;
+  p.algorithm_singlet = P->algorithm_singlet;
+  p.b1_s = P->b1_s;
+  p.msq_s = P->msq_s;
+  p.b3_s = P->b3_s;
+  p.b4_s = P->b4_s;
+#if (NHIGGS == 1)
+  p.a1_s = P->a1_s;
+  p.a2_s = P->a2_s;
+#endif
+#endif
+  w.do_acceptance = 0;
+  w.orderparam = -1;
+  init_counters(&c);
+  h_seed = P->seed;
+  h_next_pass = 0;
+  return 0;
+}
+
+long ref_evensites(void) { return l.evensites; }
+
+void ref_tables(long *next, long *prev) {
+  for (long i = 0; i < l.sites; i++)
+    for (int d = 0; d < 3; d++) {
+      next[i * 3 + d] = l.next[i][d];
+      prev[i * 3 + d] = l.prev[i][d];
+    }
+}
+
+void ref_put_fields(const double *link, const double *u1, const double *db,
+                    const double *tr, const double *sg) {
+  long V = l.sites;
+  if (link) memcpy(&f.su2link[0][0][0], link, sizeof(double) * V * 12);
+#ifdef U1
+  if (u1) memcpy(&f.u1link[0][0], u1, sizeof(double) * V * 3);
+#endif
+#if (NHIGGS > 0)
+  if (db) memcpy(&f.su2doublet[0][0][0], db, sizeof(double) * V * 4);
+#endif
+#ifdef TRIPLET
+  if (tr) memcpy(&f.su2triplet[0][0], tr, sizeof(double) * V * 3);
+#endif
+#ifdef SINGLET
+  if (sg) memcpy(&f.singlet[0][0], sg, sizeof(double) * V);
+#endif
+  (void)u1;
+  (void)db;
+  (void)tr;
+  (void)sg;
+}
+
+void ref_get_fields(double *link, double *u1, double *db, double *tr, double *sg) {
+  long V = l.sites;
+  if (link) memcpy(link, &f.su2link[0][0][0], sizeof(double) * V * 12);
+#ifdef U1
+  if (u1) memcpy(u1, &f.u1link[0][0], sizeof(double) * V * 3);
+#endif
+#if (NHIGGS > 0)
+  if (db) memcpy(db, &f.su2doublet[0][0][0], sizeof(double) * V * 4);
+#endif
+#ifdef TRIPLET
+  if (tr) memcpy(tr, &f.su2triplet[0][0], sizeof(double) * V * 3);
+#endif
+#ifdef SINGLET
+  if (sg) memcpy(sg, &f.singlet[0][0], sizeof(double) * V);
+#endif
+  (void)u1;
+  (void)db;
+  (void)tr;
+  (void)sg;
+}
+
+void ref_rng_set(uint64_t seed, uint64_t pass) {
+  h_seed = seed;
+  h_next_pass = pass;
+}
+uint64_t ref_rng_pass(void) { return h_next_pass; }
+
+void ref_counters(su2hip_counters *o) {
+  memset(o, 0, sizeof(*o));
+  o->total_su2link = c.total_su2link;
+  o->accepted_su2link = c.accepted_su2link;
+  o->total_u1link = c.total_u1link;
+  o->accepted_u1link = c.accepted_u1link;
+#if (NHIGGS > 0)
+  o->total_doublet = c.total_doublet[0];
+  o->accepted_doublet = c.accepted_doublet[0];
+  o->total_overrelax_doublet = c.total_overrelax_doublet[0];
+  o->acc_overrelax_doublet = c.acc_overrelax_doublet[0];
+#endif
+  o->total_triplet = c.total_triplet;
+  o->accepted_triplet = c.accepted_triplet;
+  o->total_overrelax_triplet = c.total_overrelax_triplet;
+  o->acc_overrelax_triplet = c.acc_overrelax_triplet;
+#ifdef SINGLET
+  o->total_singlet = c.total_singlet;
+  o->accepted_singlet = c.accepted_singlet;
+  o->total_overrelax_singlet = c.total_overrelax_singlet;
+  o->acc_overrelax_singlet = c.acc_overrelax_singlet;
+#endif
+  o->total_muca = c.total_muca;
+  o->accepted_muca = c.accepted_muca;
+}
+
+/* the reference's own update_lattice, n times */
+void ref_update(int n) {
+  for (int k = 0; k < n; k++) update_lattice(&l, &f, &p, &c, &w);
+}
+
+/* one per-site update with an explicit pass number */
+int ref_site(int kind, long i, int dir, uint64_t pass) {
+  h_cur_pass = pass;
+  h_site = (uint32_t)i;
+  h_k = 0;
+  switch (kind) {
+    case SU2HIP_K_LINK_HEATBATH: return __real_heatbath_su2link(&l, &f, &p, i, dir);
+    case SU2HIP_K_LINK_METROPOLIS: return __real_metro_su2link(&l, &f, &p, i, dir);
+#ifdef U1
+    case SU2HIP_K_U1_METROPOLIS: return __real_metro_u1link(&l, &f, &p, i, dir);
+#endif
+#if (NHIGGS > 0)
+    case SU2HIP_K_DOUBLET_OVERRELAX: return __real_overrelax_doublet(&l, &f, &p, i);
+    case SU2HIP_K_DOUBLET_METROPOLIS: return __real_metro_doublet(&l, &f, &p, i, 0);
+#endif
+#ifdef TRIPLET
+    case SU2HIP_K_TRIPLET_OVERRELAX: return __real_overrelax_triplet(&l, &f, &p, i);
+    case SU2HIP_K_TRIPLET_METROPOLIS: return __real_metro_triplet(&l, &f, &p, i);
+#endif
+#ifdef SINGLET
+    case SU2HIP_K_SINGLET_OVERRELAX: return __real_overrelax_singlet(&l, &f, &p, i);
+    case SU2HIP_K_SINGLET_METROPOLIS: return __real_metro_singlet(&l, &f, &p, i);
+#endif
+  }
+  return -1;
+}
+
+void ref_link_staple(long i, int dir, double *V) { su2link_staple(&l, &f, &p, i, dir, V); }
+void ref_wilson_staple(long i, int dir, double *V) { su2staple_wilson(&l, &f, i, dir, V); }
+double ref_localact_su2link(long i, int dir) { return localact_su2link(&l, &f, &p, i, dir); }
+double ref_action_local(long i) { return action_local(&l, &f, &p, i); }
+double ref_polysolve3(double a, double b, double cc, double d) { return polysolve3(a, b, cc, d); }
+#if (NHIGGS > 0)
+void ref_doublet_staple(long i, double *s) { staple_doublet(s, &l, &f, &p, i, 0); }
+double ref_localact_doublet(long i) { return localact_doublet(&l, &f, &p, i, 0); }
+#endif
+#ifdef U1
+double ref_localact_u1link(long i, int dir) { return localact_u1link(&l, &f, &p, i, dir); }
+#endif
+#ifdef TRIPLET
+double ref_localact_triplet(long i) { return localact_triplet(&l, &f, &p, i); }
+double ref_magcharge_cube(long i) { return magcharge_cube(&l, &f, &p, i); }
+#endif
+#ifdef SINGLET
+double ref_localact_singlet(long i) { return localact_singlet(&l, &f, &p, i); }
+#endif
+
+/* measure()'s site loop through the reference's own per-site functions,
+ * in the SU2HIP_SUM_* layout */
+void ref_measure_sums(double *s) {
+  memset(s, 0, sizeof(double) * SU2HIP_NSUMS);
+  for (long i = 0; i < l.sites; i++) {
+    s[SU2HIP_SUM_ACTION] += action_local(&l, &f, &p, i);
+    s[SU2HIP_SUM_WILSON] += local_su2wilson(&l, &f, &p, i);
+#ifdef U1
+    s[SU2HIP_SUM_U1WILSON] += local_u1wilson(&l, &f, &p, i);
+#endif
+#if (NHIGGS > 0)
+    double mod = doubletsq(f.su2doublet[0][i]);
+    s[SU2HIP_SUM_COV_PHI] += covariant_doublet(&l, &f, i, 0);
+    s[SU2HIP_SUM_PHI2] += mod;
+    s[SU2HIP_SUM_PHI4] += mod * mod;
+#ifdef TRIPLET
+    s[SU2HIP_SUM_PHI2SIGMA2] += mod * tripletsq(f.su2triplet[i]);
+#endif
+#endif
+#ifdef TRIPLET
+    double tm = tripletsq(f.su2triplet[i]);
+    s[SU2HIP_SUM_SIGMA2] += tm;
+    s[SU2HIP_SUM_SIGMA4] += tm * tm;
+    for (int d = 0; d < 3; d++) s[SU2HIP_SUM_HOP_SIGMA] += hopping_triplet_forward(&l, &f, &p, i, d);
+    double ch = magcharge_cube(&l, &f, &p, i);
+    s[SU2HIP_SUM_MAGCHARGE] += ch;
+    s[SU2HIP_SUM_MAGCHARGE_ABS] += fabs(ch);
+#endif
+#ifdef SINGLET
+    double S = f.singlet[i][0];
+    for (int d = 0; d < 3; d++) s[SU2HIP_SUM_S_COV] += S * S - S * f.singlet[l.next[i][d]][0];
+    s[SU2HIP_SUM_S] += S;
+    s[SU2HIP_SUM_S2] += S * S;
+    s[SU2HIP_SUM_S3] += S * S * S;
+    s[SU2HIP_SUM_S4] += S * S * S * S;
+#if (NHIGGS == 1)
+    s[SU2HIP_SUM_SPHISQ] += doubletsq(f.su2doublet[0][i]) * S;
+    s[SU2HIP_SUM_S2PHISQ] += doubletsq(f.su2doublet[0][i]) * S * S;
+#endif
+#endif
+  }
+}
+
+/* the reference's measure() line, as text */
+int ref_measure_text(char *buf, int n) {
+  char *mem = NULL;
+  size_t sz = 0;
+  FILE *fp = open_memstream(&mem, &sz);
+  measure(fp, &l, &f, &p, &w);
+  fclose(fp);
+  int len = (int)sz;
+  if (len >= n) len = n - 1;
+  memcpy(buf, mem, len);
+  buf[len] = 0;
+  free(mem);
+  return (int)sz;
+}
+
+/* labels file of this build into the current directory */
+void ref_print_labels(void) { print_labels(); }
+
+/* multicanonical: load a weight file the reference's way and compute the
+ * order parameter of the current fields */
+int ref_muca_load(const char *weightfile, const su2hip_muca *m) {
+  p.multicanonical = 1;
+  w.orderparam = m->orderparam;
+  w.mode = m->mode;
+  w.checks_per_sweep = m->checks_per_sweep;
+  w.bins = m->bins;
+  w.min = m->min;
+  w.max = m->max;
+  w.delta = m->delta;
+  strncpy(w.weightfile, weightfile, sizeof(w.weightfile) - 1);
+  load_weight(&w);
+  alloc_muca_backups(&l, &w);
+  calc_orderparam(&l, &f, &p, &w, EVEN);
+  calc_orderparam(&l, &f, &p, &w, ODD);
+  return 0;
+}
+
+void ref_muca_get(su2hip_muca *m, double *pos, double *W, int32_t *hits) {
+  m->enabled = p.multicanonical;
+  m->orderparam = w.orderparam;
+  m->mode = w.mode;
+  m->checks_per_sweep = w.checks_per_sweep;
+  m->bins = w.bins;
+  m->last_max = w.last_max;
+  m->do_acceptance = w.do_acceptance;
+  m->muca_count = w.muca_count;
+  m->update_interval = w.update_interval;
+  m->delta = w.delta;
+  m->min = w.min;
+  m->max = w.max;
+  m->wrk_min = w.wrk_min;
+  m->wrk_max = w.wrk_max;
+  m->param_value[0] = w.param_value[0];
+  m->param_value[1] = w.param_value[1];
+  for (int i = 0; i < w.bins + 2; i++) {
+    if (pos) pos[i] = w.pos[i];
+    if (W) W[i] = w.W[i];
+    if (hits) hits[i] = w.hits[i];
+  }
+}
+
+void ref_muca_set_mode(int mode) { w.mode = mode; }
+double ref_muca_weight(double v) { return get_weight(&w, v); }
+int ref_muca_whichbin(double v) { return whichbin(&w, v); }
+double ref_muca_orderparam(int par) { return calc_orderparam(&l, &f, &p, &w, (char)par); }
+
+#ifdef GRADFLOW
+void ref_flow_force(double *force, double *tforce) {
+  fields fo;
+  alloc_fields(&l, &fo);
+  calc_gradient(&l, &f, &p, &fo);
+  memcpy(force, &fo.su2link[0][0][0], sizeof(double) * l.sites * 12);
+#ifdef TRIPLET
+  if (tforce) memcpy(tforce, &fo.su2triplet[0][0], sizeof(double) * l.sites * 3);
+#endif
+  (void)tforce;
+  free_fields(&l, &fo);
+}
+
+void ref_flow_euler_step(double dt) {
+  fields fo;
+  alloc_fields(&l, &fo);
+  calc_gradient(&l, &f, &p, &fo);
+  flow_gauge(&l, &f, &fo, dt);
+#ifdef TRIPLET
+  flow_triplet(&l, &f, &fo, dt);
+#endif
+  free_fields(&l, &fo);
+}
+
+/* grad_flow() writing measure_flow into the current directory */
+void ref_grad_flow(double t_max, double dt, int meas_interval, int id) {
+  p.flow_meas_interval = meas_interval;
+  grad_flow(&l, &f, &p, &w, t_max, dt, id);
+}
+#endif
+
+/* checkpoint I/O through the reference (src/checkpoint.c:84-214) */
+void ref_save_lattice(const char *path, long iter) {
+  c.iter = iter;
+  Global_total_time = 0.0;
+  Global_comms_time = 0.0;
+  save_lattice(&l, f, c, (char *)path);
+}
+long ref_load_lattice(const char *path) {
+  load_lattice(&l, &f, &c, (char *)path);
+  return c.iter;
+}
+
+/* config parsing through the reference (src/parameters.c:157-402) */
+int ref_get_parameters(const char *path, su2hip_params *out, long *longs,
+                       su2hip_muca *m, char *latticefile, char *resultsfile) {
+  lattice l2;
+  params p2;
+  weight w2;
+  memset(&l2, 0, sizeof(l2));
+  memset(&p2, 0, sizeof(p2));
+  memset(&w2, 0, sizeof(w2));
+  get_parameters((char *)path, &l2, &p2);
+  get_weight_parameters((char *)path, &p2, &w2);
+  memset(out, 0, sizeof(*out));
+  for (int d = 0; d < 3 && d < l2.dim; d++) out->L[d] = l2.L[d];
+  out->algorithm_su2link = p2.algorithm_su2link;
+  out->algorithm_su2doublet = p2.algorithm_su2doublet;
+  out->algorithm_su2triplet = p2.algorithm_su2triplet;
+  out->update_links = p2.update_links;
+  out->scalar_sweeps = p2.scalar_sweeps;
+  out->update_su2doublet = p2.update_su2doublet;
+  out->update_su2triplet = p2.update_su2triplet;
+  out->betasu2 = p2.betasu2;
+  out->msq_phi = p2.msq_phi;
+  out->lambda_phi = p2.lambda_phi;
+  out->msq_triplet = p2.msq_triplet;
+  out->a2 = p2.a2;
+  out->b4 = p2.b4;
+#ifdef U1
+  out->betau1 = p2.betau1;
+  out->r_u1 = p2.r_u1;
+  out->algorithm_u1link = p2.algorithm_u1link;
+#endif
+#ifdef SINGLET
+  out->update_singlet = p2.update_singlet;
+  out->algorithm_singlet = p2.algorithm_singlet;
+  out->b1_s = p2.b1_s;
+  out->msq_s = p2.msq_s;
+  out->b3_s = p2.b3_s;
+  out->b4_s = p2.b4_s;
+#if (NHIGGS == 1)
+  out->a1_s = p2.a1_s;
+  out->a2_s = p2.a2_s;
+#endif
+#endif
+  longs[0] = p2.iterations;
+  longs[1] = p2.interval;
+  longs[2] = p2.checkpoint;
+  longs[3] = p2.n_thermalize;
+  longs[4] = p2.reset;
+  longs[5] = p2.multicanonical;
+  longs[6] = p2.run_checks;
+  longs[7] = p2.do_local_meas;
+#ifdef GRADFLOW
+  longs[8] = p2.do_flow;
+  longs[9] = p2.flow_interval;
+  longs[10] = p2.flow_meas_interval;
+#endif
+  m->enabled = p2.multicanonical;
+  m->orderparam = w2.orderparam;
+  m->mode = w2.mode;
+  m->checks_per_sweep = w2.checks_per_sweep;
+  m->bins = w2.bins;
+  m->min = w2.min;
+  m->max = w2.max;
+  m->delta = w2.delta;
+  strcpy(latticefile, p2.latticefile);
+  (void)resultsfile;
+  if (p2.resultsfile) fclose(p2.resultsfile);
+  return 0;
+}
+
+double ref_phi0(const char *path) {
+  (void)path;
+  return p.phi0;
+}

@@ -1,0 +1,1674 @@
+/* su2oracle.c -- CPU FP64 restatement of su2higgs's lattice update sweep.
+ *
+ * TEST INFRASTRUCTURE (see su2oracle.h): the checker and the CPU baseline,
+ * never part of the product path.
+ *
+ * Each function names the reference function it restates (file:line under
+ * the reference's src/).  The algebra is written with quaternion products
+ * (U = u0 + i u.sigma, product rule of su2rot, src/su2u1.c:60-70) instead of
+ * the reference's expanded polynomials; the results agree to rounding.
+ * Where the reference computes in long double (polysolve3, local_su2wilson)
+ * this file does too.
+ *
+ * Parity status: every per-site function, the sweep, measure sums, the
+ * multicanonical steps and the Euler flow are pinned against the reference
+ * compiled from source (oracle/_ref, driven by oracle/ref_harness.c) and the
+ * committed golden fixtures.  Functions with no reference counterpart are
+ * pinned by invariants in tests/: link overrelaxation (action preserved,
+ * involution, det 1), the RK3 flow (third-order convergence, agreement with
+ * small-step Euler) and the hot start (moments of N(0,1)).
+ */
+#include "su2oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define D 3
+
+struct orc {
+  su2hip_params p;
+  int L[D];
+  long vol, even;
+  long *next, *prev, *nat;
+  double *U, *A, *phi, *sig, *S;
+  double *bu_phi, *bu_sig;
+  uint64_t seed, pass;
+  su2hip_counters c;
+  su2hip_muca m;
+  double *pos, *W, *slope, *b, *hgram;
+  int32_t *hits;
+  long *gsum, *nsum;
+  int nb;
+  uint64_t r_pass;
+  uint32_t r_site, r_k;
+};
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10: Salmon et al., "Parallel random numbers: as easy as
+ * 1, 2, 3", SC'11.  Round: (hi,lo)=M0*c0, (hi',lo')=M1*c2,
+ * c' = (hi'^c1^k0, lo', hi^c3^k1, lo); key bumped by the Weyl constants
+ * between rounds. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
+                       uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0;
+    c1 = (uint32_t)p1;
+    c2 = n2;
+    c3 = (uint32_t)p0;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+double orc_uniform(uint64_t seed, uint64_t pass, uint32_t site, uint32_t k) {
+  uint32_t ctr[4] = {k >> 1, site, (uint32_t)pass, (uint32_t)(pass >> 32)};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t o[4];
+  orc_philox4x32_10(ctr, key, o);
+  uint64_t x = (k & 1) ? (((uint64_t)o[3] << 32) | o[2])
+                       : (((uint64_t)o[1] << 32) | o[0]);
+  return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static inline double dran(orc_t *o) {
+  return orc_uniform(o->seed, o->r_pass, o->r_site, o->r_k++);
+}
+
+static inline void ctx(orc_t *o, uint64_t pass, uint32_t site) {
+  o->r_pass = pass;
+  o->r_site = site;
+  o->r_k = 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* quaternion algebra, U = u0 + i u.sigma */
+static inline void qmul(const double *a, const double *b, double *r) {
+  double r0 = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  double r1 = a[0] * b[1] + a[1] * b[0] - (a[2] * b[3] - a[3] * b[2]);
+  double r2 = a[0] * b[2] + a[2] * b[0] - (a[3] * b[1] - a[1] * b[3]);
+  double r3 = a[0] * b[3] + a[3] * b[0] - (a[1] * b[2] - a[2] * b[1]);
+  r[0] = r0;
+  r[1] = r1;
+  r[2] = r2;
+  r[3] = r3;
+}
+static inline void qconj(const double *a, double *r) {
+  r[0] = a[0];
+  r[1] = -a[1];
+  r[2] = -a[2];
+  r[3] = -a[3];
+}
+/* r = a * b^dagger */
+static inline void qmul_nc(const double *a, const double *b, double *r) {
+  double bc[4];
+  qconj(b, bc);
+  qmul(a, bc, r);
+}
+/* r = a^dagger * b */
+static inline void qmul_cn(const double *a, const double *b, double *r) {
+  double ac[4];
+  qconj(a, ac);
+  qmul(ac, b, r);
+}
+static inline double qsq(const double *a) {
+  return a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
+}
+/* hypercharge factor exp(-i alpha sigma_3) as a quaternion (src/staples.c:161-177) */
+static inline void u1twist(double alpha, double *e) {
+  e[0] = cos(alpha);
+  e[1] = 0.0;
+  e[2] = 0.0;
+  e[3] = -sin(alpha);
+}
+/* R(U) v = vector part of U (0,v) U^dagger */
+static inline void rotate(const double *u, const double *v, double *r) {
+  double q[4] = {0.0, v[0], v[1], v[2]}, t[4];
+  qmul(u, q, t);
+  qmul_nc(t, u, q);
+  r[0] = q[1];
+  r[1] = q[2];
+  r[2] = q[3];
+}
+
+#define LNK(o, i, d) ((o)->U + ((size_t)(i) * D + (d)) * 4)
+#define PHI(o, i) ((o)->phi + (size_t)(i) * 4)
+#define SIG(o, i) ((o)->sig + (size_t)(i) * 3)
+#define ALP(o, i, d) ((o)->A[(size_t)(i) * D + (d)])
+#define NX(o, i, d) ((o)->next[(size_t)(i) * D + (d)])
+#define PV(o, i, d) ((o)->prev[(size_t)(i) * D + (d)])
+
+/* ------------------------------------------------------------------ */
+/* geometry: serial layout (src/layout.c:657-756, paritymap :826-851,
+ * indexToCoords/coordsToIndex :945-972) */
+static void build_layout(orc_t *o) {
+  long V = o->vol;
+  int *L = o->L;
+  long *newidx = (long *)malloc(sizeof(long) * V);
+  long ne = 0;
+  for (long i = 0; i < V; i++) {
+    long x = i % L[0], y = (i / L[0]) % L[1], z = i / ((long)L[0] * L[1]);
+    if ((x + y + z) % 2 == 0) ne++;
+  }
+  o->even = ne;
+  long e = 0, od = 0;
+  for (long i = 0; i < V; i++) {
+    long x = i % L[0], y = (i / L[0]) % L[1], z = i / ((long)L[0] * L[1]);
+    if ((x + y + z) % 2 == 0)
+      newidx[i] = e++;
+    else
+      newidx[i] = ne + od++;
+  }
+  for (long i = 0; i < V; i++) {
+    long c[D] = {i % L[0], (i / L[0]) % L[1], i / ((long)L[0] * L[1])};
+    long ni = newidx[i];
+    o->nat[ni] = i;
+    for (int d = 0; d < D; d++) {
+      long cp[D], cm[D];
+      for (int k = 0; k < D; k++) cp[k] = cm[k] = c[k];
+      cp[d] = (c[d] + 1) % L[d];
+      cm[d] = (c[d] - 1 + L[d]) % L[d];
+      long ip = cp[0] + L[0] * (cp[1] + (long)L[1] * cp[2]);
+      long im = cm[0] + L[0] * (cm[1] + (long)L[1] * cm[2]);
+      o->next[ni * D + d] = newidx[ip];
+      o->prev[ni * D + d] = newidx[im];
+    }
+  }
+  free(newidx);
+}
+
+orc_t *orc_create(const su2hip_params *p) {
+  orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
+  o->p = *p;
+  for (int d = 0; d < D; d++) o->L[d] = p->L[d];
+  o->vol = (long)p->L[0] * p->L[1] * p->L[2];
+  long V = o->vol;
+  o->next = (long *)malloc(sizeof(long) * V * D);
+  o->prev = (long *)malloc(sizeof(long) * V * D);
+  o->nat = (long *)malloc(sizeof(long) * V);
+  build_layout(o);
+  o->U = (double *)calloc((size_t)V * D * 4, sizeof(double));
+  o->A = (double *)calloc((size_t)V * D, sizeof(double));
+  o->phi = (double *)calloc((size_t)V * 4, sizeof(double));
+  o->sig = (double *)calloc((size_t)V * 3, sizeof(double));
+  o->S = (double *)calloc((size_t)V, sizeof(double));
+  o->bu_phi = (double *)calloc((size_t)V * 4, sizeof(double));
+  o->bu_sig = (double *)calloc((size_t)V * 3, sizeof(double));
+  o->seed = p->seed;
+  o->pass = 0;
+  return o;
+}
+
+static void free_muca(orc_t *o) {
+  free(o->pos);
+  free(o->W);
+  free(o->slope);
+  free(o->b);
+  free(o->hgram);
+  free(o->hits);
+  free(o->gsum);
+  free(o->nsum);
+  o->pos = o->W = o->slope = o->b = o->hgram = NULL;
+  o->hits = NULL;
+  o->gsum = o->nsum = NULL;
+}
+
+void orc_destroy(orc_t *o) {
+  if (!o) return;
+  free(o->next);
+  free(o->prev);
+  free(o->nat);
+  free(o->U);
+  free(o->A);
+  free(o->phi);
+  free(o->sig);
+  free(o->S);
+  free(o->bu_phi);
+  free(o->bu_sig);
+  free_muca(o);
+  free(o);
+}
+
+long orc_volume(const orc_t *o) { return o->vol; }
+long orc_evensites(const orc_t *o) { return o->even; }
+
+void orc_tables(const orc_t *o, long *next, long *prev, long *natural) {
+  if (next) memcpy(next, o->next, sizeof(long) * o->vol * D);
+  if (prev) memcpy(prev, o->prev, sizeof(long) * o->vol * D);
+  if (natural) memcpy(natural, o->nat, sizeof(long) * o->vol);
+}
+
+void orc_set_fields(orc_t *o, const double *su2link, const double *u1link,
+                    const double *doublet, const double *triplet,
+                    const double *singlet) {
+  size_t V = (size_t)o->vol;
+  if (su2link) memcpy(o->U, su2link, sizeof(double) * V * D * 4);
+  if (u1link) memcpy(o->A, u1link, sizeof(double) * V * D);
+  if (doublet) memcpy(o->phi, doublet, sizeof(double) * V * 4);
+  if (triplet) memcpy(o->sig, triplet, sizeof(double) * V * 3);
+  if (singlet) memcpy(o->S, singlet, sizeof(double) * V);
+}
+
+void orc_get_fields(const orc_t *o, double *su2link, double *u1link,
+                    double *doublet, double *triplet, double *singlet) {
+  size_t V = (size_t)o->vol;
+  if (su2link) memcpy(su2link, o->U, sizeof(double) * V * D * 4);
+  if (u1link) memcpy(u1link, o->A, sizeof(double) * V * D);
+  if (doublet) memcpy(doublet, o->phi, sizeof(double) * V * 4);
+  if (triplet) memcpy(triplet, o->sig, sizeof(double) * V * 3);
+  if (singlet) memcpy(singlet, o->S, sizeof(double) * V);
+}
+
+/* setfields (src/init.c:92-112) */
+void orc_cold_start(orc_t *o, double phi0, double sigma0, double singlet0) {
+  for (long i = 0; i < o->vol; i++) {
+    for (int d = 0; d < D; d++) {
+      double *u = LNK(o, i, d);
+      u[0] = 1.0;
+      u[1] = u[2] = u[3] = 0.0;
+      ALP(o, i, d) = 0.0;
+    }
+    double *f = PHI(o, i);
+    f[0] = phi0;
+    f[1] = f[2] = f[3] = 0.0;
+    double *s = SIG(o, i);
+    s[0] = sigma0;
+    s[1] = s[2] = 0.0;
+    o->S[i] = singlet0;
+  }
+}
+
+/* Box-Muller pair from two uniforms */
+static inline void gauss2(double u1, double u2, double *z0, double *z1) {
+  double r = sqrt(-2.0 * log(1.0 - u1));
+  double t = 2.0 * M_PI * u2;
+  *z0 = r * cos(t);
+  *z1 = r * sin(t);
+}
+
+/* Hot start (no reference counterpart).  Draw order at each site: links
+ * dir 0..2 (4 Gaussians each), U(1) angles, doublet (4 Gaussians), triplet
+ * (2 pairs, last Gaussian unused), singlet (1 pair, second unused). */
+void orc_hot_start(orc_t *o) {
+  for (long i = 0; i < o->vol; i++) {
+    ctx(o, ORC_HOTSTART_PASS, (uint32_t)i);
+    for (int d = 0; d < D; d++) {
+      double z[4], u0, u1;
+      u0 = dran(o);
+      u1 = dran(o);
+      gauss2(u0, u1, &z[0], &z[1]);
+      u0 = dran(o);
+      u1 = dran(o);
+      gauss2(u0, u1, &z[2], &z[3]);
+      double n = sqrt(qsq(z));
+      double *u = LNK(o, i, d);
+      for (int k = 0; k < 4; k++) u[k] = z[k] / n;
+    }
+    if (o->p.u1)
+      for (int d = 0; d < D; d++) ALP(o, i, d) = M_PI * (2.0 * dran(o) - 1.0);
+    if (o->p.nhiggs) {
+      double z[4], u0, u1;
+      u0 = dran(o);
+      u1 = dran(o);
+      gauss2(u0, u1, &z[0], &z[1]);
+      u0 = dran(o);
+      u1 = dran(o);
+      gauss2(u0, u1, &z[2], &z[3]);
+      for (int k = 0; k < 4; k++) PHI(o, i)[k] = z[k];
+    }
+    if (o->p.triplet) {
+      double z[4], u0, u1;
+      u0 = dran(o);
+      u1 = dran(o);
+      gauss2(u0, u1, &z[0], &z[1]);
+      u0 = dran(o);
+      u1 = dran(o);
+      gauss2(u0, u1, &z[2], &z[3]);
+      for (int k = 0; k < 3; k++) SIG(o, i)[k] = z[k];
+    }
+    if (o->p.singlet) {
+      double z0, z1, u0, u1;
+      u0 = dran(o);
+      u1 = dran(o);
+      gauss2(u0, u1, &z0, &z1);
+      o->S[i] = z0;
+    }
+  }
+}
+
+void orc_rng_set(orc_t *o, uint64_t seed, uint64_t pass) {
+  o->seed = seed;
+  o->pass = pass;
+}
+void orc_rng_get(const orc_t *o, uint64_t *seed, uint64_t *pass) {
+  *seed = o->seed;
+  *pass = o->pass;
+}
+void orc_get_counters(const orc_t *o, su2hip_counters *c) { *c = o->c; }
+void orc_reset_counters(orc_t *o) { memset(&o->c, 0, sizeof(o->c)); }
+
+/* ------------------------------------------------------------------ */
+/* local action pieces (src/su2u1.c) */
+
+/* su2ptrace (src/su2u1.c:108-116): Re Tr U_a(x) U_b(x+a) U_a(x+b)^+ U_b(x)^+ */
+static double ptrace(const orc_t *o, long i, int a, int b) {
+  double t[4], s[4];
+  qmul(LNK(o, i, a), LNK(o, NX(o, i, a), b), t);
+  qmul_nc(t, LNK(o, NX(o, i, b), a), s);
+  qmul_nc(s, LNK(o, i, b), t);
+  return 2.0 * t[0];
+}
+
+/* local_su2wilson (src/su2u1.c:143-157), long double accumulation */
+static long double local_su2wilson(const orc_t *o, long i) {
+  long double res = 0.0;
+  for (int d1 = 0; d1 < D; d1++)
+    for (int d2 = 0; d2 < d1; d2++) res += (1.0 - 0.5 * ptrace(o, i, d2, d1));
+  return o->p.betasu2 * res;
+}
+
+/* u1ptrace (src/su2u1.c:300-309) */
+static double u1ptrace(const orc_t *o, long i, int a, int b) {
+  return ALP(o, i, a) + ALP(o, NX(o, i, a), b) - ALP(o, NX(o, i, b), a) -
+         ALP(o, i, b);
+}
+
+/* local_u1wilson (src/su2u1.c:316-328) */
+static double local_u1wilson(const orc_t *o, long i) {
+  double res = 0.0;
+  for (int d1 = 0; d1 < D; d1++)
+    for (int d2 = 0; d2 < d1; d2++)
+      res += (1.0 - cos(o->p.r_u1 * u1ptrace(o, i, d2, d1)));
+  return o->p.betau1 * res;
+}
+
+static inline double doubletsq(const double *a) {
+  return 0.5 * (a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3]);
+}
+static inline double tripletsq(const double *a) {
+  return 0.5 * (a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+}
+
+/* hopping_doublet_forward (src/su2u1.c:428-445):
+ * -Re Tr Phi(x)^+ U_j(x) Phi(x+j) exp(-i alpha_j(x) sigma_3) */
+static double hop_doublet_fwd(const orc_t *o, long i, int d) {
+  double t[4], s[4];
+  qmul_cn(PHI(o, i), LNK(o, i, d), t);
+  qmul(t, PHI(o, NX(o, i, d)), s);
+  if (o->p.u1) {
+    double e[4];
+    u1twist(ALP(o, i, d), e);
+    qmul(s, e, t);
+    return -t[0];
+  }
+  return -s[0];
+}
+
+/* hopping_doublet_backward (src/su2u1.c:451-472) */
+static double hop_doublet_bwd(const orc_t *o, long i, int d) {
+  long pv = PV(o, i, d);
+  double t[4], s[4];
+  qmul_cn(PHI(o, pv), LNK(o, pv, d), t);
+  qmul(t, PHI(o, i), s);
+  if (o->p.u1) {
+    double e[4];
+    u1twist(ALP(o, pv, d), e);
+    qmul(s, e, t);
+    return -t[0];
+  }
+  return -s[0];
+}
+
+/* covariant_doublet (src/su2u1.c:477-488) */
+static double covariant_doublet(const orc_t *o, long i) {
+  double tot = 0.0, mod = doubletsq(PHI(o, i));
+  for (int d = 0; d < D; d++) tot += 2.0 * mod + hop_doublet_fwd(o, i, d);
+  return tot;
+}
+
+/* hopping_triplet_forward (src/su2u1.c:605-617): -2 Tr A(x) U A(x+j) U^+ */
+static double hop_triplet_fwd(const orc_t *o, long i, int d) {
+  double r[3];
+  rotate(LNK(o, i, d), SIG(o, NX(o, i, d)), r);
+  const double *a = SIG(o, i);
+  return -(a[0] * r[0] + a[1] * r[1] + a[2] * r[2]);
+}
+
+/* hopping_triplet_backward (src/su2u1.c:623-637) */
+static double hop_triplet_bwd(const orc_t *o, long i, int d) {
+  long pv = PV(o, i, d);
+  double r[3];
+  rotate(LNK(o, pv, d), SIG(o, i), r);
+  const double *a = SIG(o, pv);
+  return -(a[0] * r[0] + a[1] * r[1] + a[2] * r[2]);
+}
+
+/* covariant_triplet (src/su2u1.c:643-651) */
+static double covariant_triplet(const orc_t *o, long i) {
+  double tot = 0.0, mod = tripletsq(SIG(o, i));
+  for (int d = 0; d < D; d++) tot += 2.0 * mod + hop_triplet_fwd(o, i, d);
+  return tot;
+}
+
+/* potential_singlet (src/su2u1.c:708-723) */
+static double potential_singlet(const orc_t *o, long i) {
+  const su2hip_params *p = &o->p;
+  double S = o->S[i];
+  double pot = p->b1_s * S + 0.5 * p->msq_s * S * S +
+               1.0 / 3.0 * p->b3_s * S * S * S + 0.25 * p->b4_s * S * S * S * S;
+  if (p->nhiggs == 1) {
+    double mod = doubletsq(PHI(o, i));
+    pot += 0.5 * p->a1_s * S * mod + 0.5 * p->a2_s * S * S * mod;
+  }
+  return pot;
+}
+
+/* higgspotential (src/su2u1.c:528-572) */
+static double higgspotential(const orc_t *o, long i) {
+  const su2hip_params *p = &o->p;
+  double pot = 0.0, mod = 0.0;
+  if (p->nhiggs) {
+    mod = doubletsq(PHI(o, i));
+    pot += p->msq_phi * mod + p->lambda_phi * mod * mod;
+  }
+  if (p->triplet) {
+    double mt = tripletsq(SIG(o, i));
+    pot += p->msq_triplet * mt + p->b4 * mt * mt;
+    if (p->nhiggs) pot += p->a2 * mod * mt;
+  }
+  if (p->singlet) pot += potential_singlet(o, i);
+  return pot;
+}
+
+/* localact_su2link (src/su2u1.c:166-198) */
+double orc_localact_su2link(const orc_t *o, long i, int dir) {
+  double tot = 0.0;
+  for (int d2 = 0; d2 < D; d2++) {
+    if (d2 == dir) continue;
+    tot += (1.0 - 0.5 * ptrace(o, i, dir, d2));
+    tot += (1.0 - 0.5 * ptrace(o, PV(o, i, d2), dir, d2));
+  }
+  tot *= o->p.betasu2;
+  if (o->p.nhiggs) tot += hop_doublet_fwd(o, i, dir);
+  if (o->p.triplet) tot += hop_triplet_fwd(o, i, dir);
+  return tot;
+}
+
+/* localact_u1link (src/su2u1.c:336-354) */
+double orc_localact_u1link(const orc_t *o, long i, int dir) {
+  double tot = 0.0, r = o->p.r_u1;
+  for (int d2 = 0; d2 < D; d2++) {
+    if (d2 == dir) continue;
+    tot += (1.0 - cos(r * u1ptrace(o, i, dir, d2)));
+    tot += (1.0 - cos(r * u1ptrace(o, PV(o, i, d2), dir, d2)));
+  }
+  tot *= o->p.betau1;
+  if (o->p.nhiggs) tot += hop_doublet_fwd(o, i, dir);
+  return tot;
+}
+
+/* localact_doublet (src/su2u1.c:495-510) */
+double orc_localact_doublet(const orc_t *o, long i) {
+  double tot = covariant_doublet(o, i);
+  for (int d = 0; d < D; d++) tot += hop_doublet_bwd(o, i, d);
+  tot += higgspotential(o, i);
+  return tot;
+}
+
+/* localact_triplet (src/su2u1.c:657-679) */
+double orc_localact_triplet(const orc_t *o, long i) {
+  double tot = covariant_triplet(o, i);
+  for (int d = 0; d < D; d++) tot += hop_triplet_bwd(o, i, d);
+  double mod = tripletsq(SIG(o, i));
+  tot += o->p.msq_triplet * mod + o->p.b4 * mod * mod;
+  if (o->p.nhiggs) tot += o->p.a2 * doubletsq(PHI(o, i)) * mod;
+  return tot;
+}
+
+/* localact_singlet (src/su2u1.c:688-705) */
+double orc_localact_singlet(const orc_t *o, long i) {
+  double S = o->S[i];
+  double res = D * S * S;
+  for (int d = 0; d < D; d++)
+    res -= S * (o->S[NX(o, i, d)] + o->S[PV(o, i, d)]);
+  res += potential_singlet(o, i);
+  return res;
+}
+
+/* action_local (src/measure.c:314-343) */
+double orc_action_local(const orc_t *o, long i) {
+  double tot = 0.0;
+  tot += local_su2wilson(o, i);
+  if (o->p.u1) tot += local_u1wilson(o, i);
+  tot += higgspotential(o, i);
+  if (o->p.nhiggs) tot += covariant_doublet(o, i);
+  if (o->p.triplet) tot += covariant_triplet(o, i);
+  if (o->p.singlet) {
+    double S = o->S[i];
+    tot += D * S * S;
+    for (int d = 0; d < D; d++) tot -= S * o->S[NX(o, i, d)];
+  }
+  return tot;
+}
+
+/* ------------------------------------------------------------------ */
+/* staples (src/staples.c) */
+
+/* su2staple_wilson (src/staples.c:84-115):
+ * sum_{nu!=mu} U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+ *            + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu) */
+void orc_wilson_staple(const orc_t *o, long i, int mu, double V[4]) {
+  double tot[4] = {0.0, 0.0, 0.0, 0.0}, t[4], s[4];
+  long xp = NX(o, i, mu);
+  for (int nu = 0; nu < D; nu++) {
+    if (nu == mu) continue;
+    qmul_nc(LNK(o, xp, nu), LNK(o, NX(o, i, nu), mu), t);
+    qmul_nc(t, LNK(o, i, nu), s);
+    for (int k = 0; k < 4; k++) tot[k] += s[k];
+    long xm = PV(o, i, nu);
+    double ac[4], bc[4];
+    qconj(LNK(o, PV(o, xp, nu), nu), ac);
+    qconj(LNK(o, xm, mu), bc);
+    qmul(ac, bc, t);
+    qmul(t, LNK(o, xm, nu), s);
+    for (int k = 0; k < 4; k++) tot[k] += s[k];
+  }
+  for (int k = 0; k < 4; k++) V[k] = tot[k];
+}
+
+/* su2link_staple (src/staples.c:134-193): -beta/2 * Wilson staple
+ * - 1/2 Phi(x+mu) exp(-i alpha sigma_3) Phi(x)^+ */
+void orc_link_staple(const orc_t *o, long i, int dir, double V[4]) {
+  orc_wilson_staple(o, i, dir, V);
+  for (int k = 0; k < 4; k++) V[k] *= -0.5 * o->p.betasu2;
+  if (o->p.nhiggs) {
+    double b[4], t[4];
+    const double *nb = PHI(o, NX(o, i, dir));
+    if (o->p.u1) {
+      double e[4];
+      u1twist(ALP(o, i, dir), e);
+      qmul(nb, e, b);
+    } else {
+      memcpy(b, nb, sizeof(b));
+    }
+    qmul_nc(b, PHI(o, i), t);
+    for (int k = 0; k < 4; k++) V[k] -= 0.5 * t[k];
+  }
+}
+
+/* staple_doublet (src/staples.c:243-302): s_a with S = phi_a s_a + ... */
+void orc_doublet_staple(const orc_t *o, long i, double s[4]) {
+  for (int k = 0; k < 4; k++) s[k] = 0.0;
+  for (int d = 0; d < D; d++) {
+    double t[4], q[4];
+    /* forward: -(U_d(x) Phi(x+d) E_d(x)) */
+    qmul(LNK(o, i, d), PHI(o, NX(o, i, d)), t);
+    if (o->p.u1) {
+      double e[4];
+      u1twist(ALP(o, i, d), e);
+      qmul(t, e, q);
+    } else {
+      memcpy(q, t, sizeof(q));
+    }
+    for (int k = 0; k < 4; k++) s[k] -= q[k];
+    /* backward: -(U_d(x-d)^+ Phi(x-d) E_d(x-d)^+) */
+    long pv = PV(o, i, d);
+    qmul_cn(LNK(o, pv, d), PHI(o, pv), t);
+    if (o->p.u1) {
+      double e[4], ec[4];
+      u1twist(ALP(o, pv, d), e);
+      qconj(e, ec);
+      qmul(t, ec, q);
+    } else {
+      memcpy(q, t, sizeof(q));
+    }
+    for (int k = 0; k < 4; k++) s[k] -= q[k];
+  }
+}
+
+/* hopping staple of the triplet (src/overrelax.c:143-180):
+ * s = -sum_d [ R(U_d(x)) Sigma(x+d) + R(U_d(x-d))^T Sigma(x-d) ] */
+void orc_triplet_staple(const orc_t *o, long i, double s[3]) {
+  s[0] = s[1] = s[2] = 0.0;
+  for (int d = 0; d < D; d++) {
+    double r[3], uc[4];
+    rotate(LNK(o, i, d), SIG(o, NX(o, i, d)), r);
+    for (int k = 0; k < 3; k++) s[k] -= r[k];
+    long pv = PV(o, i, d);
+    qconj(LNK(o, pv, d), uc);
+    rotate(uc, SIG(o, pv), r);
+    for (int k = 0; k < 3; k++) s[k] -= r[k];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* polysolve3 (src/overrelax.c:14-34): the real root of a x^3+b x^2+c x+d
+ * by Cardano's formula when the discriminant is negative; otherwise the
+ * reference returns b/a (with a warning).  long double like the reference,
+ * with its double-precision sqrt and cbrt. */
+static double polysolve3_ld(long double a, long double b, long double c,
+                            long double d) {
+  long double disc = 18.0 * a * b * c * d - 4.0 * b * b * b * d + b * b * c * c -
+                     4.0 * a * c * c * c - 27.0 * a * a * d * d;
+  long double d1 = 2.0 * b * b * b - 9.0 * a * b * c + 27.0 * a * a * d;
+  double x;
+  if (disc < 0.0) {
+    long double A = 0.5 * (d1 + a * sqrt((double)(-27.0 * disc)));
+    A = cbrt((double)A);
+    x = -1.0 * (b + A + (b * b - 3.0 * a * c) / A) / (3.0 * a);
+  } else {
+    x = b / a;
+  }
+  return x;
+}
+
+double orc_polysolve3(double a, double b, double c, double d) {
+  return polysolve3_ld(a, b, c, d);
+}
+
+/* ------------------------------------------------------------------ */
+/* per-site updates */
+
+/* heatbath_su2link (src/heatbath.c:15-124), Kennedy-Pendleton */
+static int site_heatbath(orc_t *o, long i, int dir) {
+  double old[4], oldact = 0.0;
+  double *u = LNK(o, i, dir);
+  if (o->p.triplet) {
+    memcpy(old, u, sizeof(old));
+    oldact += hop_triplet_fwd(o, i, dir);
+  }
+  double V[4];
+  orc_link_staple(o, i, dir, V);
+  double a = sqrt(qsq(V));
+  for (int k = 0; k < 4; k++) V[k] *= -1.0 / a;
+  a *= 2.0;
+  double r1, r2, r3, d, a0;
+  int loop = 1;
+  do {
+    r1 = -1.0 * log(1.0 - dran(o)) / a;
+    r2 = -1.0 * log(1.0 - dran(o)) / a;
+    r3 = cos(2 * M_PI * dran(o));
+    r3 *= r3;
+    d = r1 * r3 + r2;
+    r3 = 1.0 - dran(o);
+    loop++;
+  } while (r3 * r3 > 1 - 0.5 * d && loop <= 200);
+  a0 = (loop > 200) ? 1e-9 : 1.0 - d;
+  double rad = 1.0 - a0 * a0;
+  if (rad < 0.0) return 0;
+  rad = sqrt(rad);
+  d = 2.0;
+  while (d > 1.0 || d == 0.0) {
+    r1 = 2.0 * dran(o) - 1.0;
+    r2 = 2.0 * dran(o) - 1.0;
+    r3 = 2.0 * dran(o) - 1.0;
+    d = r1 * r1 + r2 * r2 + r3 * r3;
+  }
+  d = sqrt(d);
+  r1 *= rad / d;
+  r2 *= rad / d;
+  r3 *= rad / d;
+  double A[4] = {a0, r1, r2, r3};
+  qmul_nc(A, V, u);
+  if (o->p.triplet) {
+    double diff = oldact - hop_triplet_fwd(o, i, dir);
+    if (dran(o) < exp(diff)) return 1;
+    memcpy(u, old, sizeof(old));
+    return 0;
+  }
+  return 1;
+}
+
+/* accept rule of every Metropolis update in src/metropolis.c */
+static inline int metro_accept(orc_t *o, double diff) {
+  if (diff < 0) return 1;
+  if (diff > 0 && exp(-diff) > dran(o)) return 1;
+  return 0;
+}
+
+/* metro_su2link (src/metropolis.c:15-41) + random_su2link (src/init.c:43-58) */
+static int site_metro_link(orc_t *o, long i, int dir) {
+  double *u = LNK(o, i, dir), old[4];
+  memcpy(old, u, sizeof(old));
+  double act_old = orc_localact_su2link(o, i, dir);
+  double r[4];
+  r[0] = 10.0 * (dran(o) - 0.5);
+  r[1] = 1.0 * (dran(o) - 0.5);
+  r[2] = 1.0 * (dran(o) - 0.5);
+  r[3] = 1.0 * (dran(o) - 0.5);
+  double n = sqrt(qsq(r));
+  for (int k = 0; k < 4; k++) r[k] = r[k] / n;
+  qmul(u, r, u);
+  double diff = orc_localact_su2link(o, i, dir) - act_old;
+  if (metro_accept(o, diff)) return 1;
+  memcpy(u, old, sizeof(old));
+  return 0;
+}
+
+/* SU(2) link overrelaxation (no reference counterpart):
+ * U' = W^+ U^+ W^+ with W = staple/|staple| keeps Re Tr(U staple); with an
+ * adjoint scalar the hopping term quadratic in U gets a Metropolis test
+ * like the heatbath's (src/heatbath.c:107-120). */
+static int site_link_or(orc_t *o, long i, int dir) {
+  double *u = LNK(o, i, dir), old[4], oldact = 0.0;
+  if (o->p.triplet) {
+    memcpy(old, u, sizeof(old));
+    oldact = hop_triplet_fwd(o, i, dir);
+  }
+  double V[4];
+  orc_link_staple(o, i, dir, V);
+  double a = sqrt(qsq(V));
+  if (a == 0.0) return 1;
+  double w[4], t[4];
+  for (int k = 0; k < 4; k++) w[k] = V[k] / a;
+  double wc[4], uc[4];
+  qconj(w, wc);
+  qconj(u, uc);
+  qmul(wc, uc, t);
+  qmul(t, wc, u);
+  if (o->p.triplet) {
+    double diff = oldact - hop_triplet_fwd(o, i, dir);
+    if (dran(o) < exp(diff)) return 1;
+    memcpy(u, old, sizeof(old));
+    return 0;
+  }
+  return 1;
+}
+
+/* metro_u1link (src/metropolis.c:48-70) */
+static int site_metro_u1(orc_t *o, long i, int dir) {
+  double old = ALP(o, i, dir);
+  double act_old = orc_localact_u1link(o, i, dir);
+  ALP(o, i, dir) += 0.95 * (dran(o) - 0.5);
+  double diff = orc_localact_u1link(o, i, dir) - act_old;
+  if (metro_accept(o, diff)) return 1;
+  ALP(o, i, dir) = old;
+  return 0;
+}
+
+/* overrelax_doublet (src/overrelax.c:54-128), Cartesian (X,Y) overrelax */
+static int site_or_doublet(orc_t *o, long i) {
+  const su2hip_params *p = &o->p;
+  double *f = PHI(o, i);
+  double s[4];
+  orc_doublet_staple(o, i, s);
+  double F = 0.0;
+  for (int k = 0; k < 4; k++) F += s[k] * s[k];
+  F = sqrt(F);
+  double X = 0.0;
+  for (int k = 0; k < 4; k++) X += f[k] * s[k];
+  X /= F;
+  double Ysq = 0.0;
+  for (int k = 0; k < 4; k++) {
+    double y = f[k] - X * s[k] / F;
+    Ysq += y * y;
+  }
+  double B = 0.5 * p->msq_phi + 1.0 * D;
+  if (p->triplet) B += 0.5 * p->a2 * tripletsq(SIG(o, i));
+  if (p->singlet) {
+    double S = o->S[i];
+    B += 0.25 * p->a1_s * S + 0.25 * p->a2_s * S * S;
+  }
+  double C = 0.25 * p->lambda_phi;
+  long double al = C;
+  long double be = C * X;
+  long double cc = B + 2.0 * Ysq * C;
+  long double ga = (cc + C * X * X);
+  long double de = (F + cc * X + C * X * X * X);
+  double nX = polysolve3_ld(al, be, ga, de);
+  double dV = F + 2.0 * B * X + 4.0 * C * (X * X * X + Ysq * X);
+  double dVn = F + 2.0 * B * nX + 4.0 * C * (nX * nX * nX + Ysq * nX);
+  double ratio = fabs(dV / dVn);
+  if (ratio >= dran(o)) {
+    for (int k = 0; k < 4; k++) f[k] = -1.0 * f[k] + (nX + X) * s[k] / F;
+    return 1;
+  }
+  return 0;
+}
+
+/* metro_doublet (src/metropolis.c:77-106) */
+static int site_metro_doublet(orc_t *o, long i) {
+  double *f = PHI(o, i), old[4];
+  memcpy(old, f, sizeof(old));
+  double act_old = orc_localact_doublet(o, i);
+  for (int k = 0; k < 4; k++) f[k] += 1.0 * (dran(o) - 0.5);
+  double diff = orc_localact_doublet(o, i) - act_old;
+  if (metro_accept(o, diff)) return 1;
+  memcpy(f, old, sizeof(old));
+  return 0;
+}
+
+/* overrelax_triplet (src/overrelax.c:137-236).  The reference's portal term
+ * sits under `#ifdef HIGGS` (src/overrelax.c:202), a flag no build defines,
+ * so B never contains it; kept that way. */
+static int site_or_triplet(orc_t *o, long i) {
+  const su2hip_params *p = &o->p;
+  double *f = SIG(o, i);
+  double s[3];
+  orc_triplet_staple(o, i, s);
+  double F = 0.0;
+  for (int k = 0; k < 3; k++) F += s[k] * s[k];
+  F = sqrt(F);
+  double X = 0.0;
+  for (int k = 0; k < 3; k++) X += f[k] * s[k];
+  X /= F;
+  double Ysq = 0.0;
+  for (int k = 0; k < 3; k++) {
+    double y = f[k] - X * s[k] / F;
+    Ysq += y * y;
+  }
+  double B = 0.5 * p->msq_triplet + 1.0 * D;
+  double C = 0.25 * p->b4;
+  long double al = C;
+  long double be = C * X;
+  long double cc = B + 2.0 * Ysq * C;
+  long double ga = (cc + C * X * X);
+  long double de = (F + cc * X + C * X * X * X);
+  double nX = polysolve3_ld(al, be, ga, de);
+  double dV = F + 2.0 * B * X + 4.0 * C * (X * X * X + Ysq * X);
+  double dVn = F + 2.0 * B * nX + 4.0 * C * (nX * nX * nX + Ysq * nX);
+  double ratio = fabs(dV / dVn);
+  if (ratio >= dran(o)) {
+    for (int k = 0; k < 3; k++) f[k] = -1.0 * f[k] + (nX + X) * s[k] / F;
+    return 1;
+  }
+  return 0;
+}
+
+/* metro_triplet (src/metropolis.c:113-143) */
+static int site_metro_triplet(orc_t *o, long i) {
+  double *f = SIG(o, i), old[3];
+  memcpy(old, f, sizeof(old));
+  double act_old = orc_localact_triplet(o, i);
+  for (int k = 0; k < 3; k++) f[k] += 1.0 * (dran(o) - 0.5);
+  double diff = orc_localact_triplet(o, i) - act_old;
+  if (metro_accept(o, diff)) return 1;
+  memcpy(f, old, sizeof(old));
+  return 0;
+}
+
+/* overrelax_singlet (src/overrelax.c:243-285) */
+static int site_or_singlet(orc_t *o, long i) {
+  const su2hip_params *p = &o->p;
+  double S = o->S[i];
+  double c1 = p->b1_s;
+  for (int d = 0; d < D; d++) c1 -= (o->S[NX(o, i, d)] + o->S[PV(o, i, d)]);
+  double c2 = D + 0.5 * p->msq_s;
+  double c3 = 1.0 / 3.0 * p->b3_s;
+  double c4 = 0.25 * p->b4_s;
+  if (p->nhiggs == 1) {
+    double ph = doubletsq(PHI(o, i));
+    c1 += 0.5 * p->a1_s * ph;
+    c2 += 0.5 * p->a2_s * ph;
+  }
+  double d0 = c1 + c2 * S + c3 * S * S + c4 * S * S * S;
+  double d1 = c2 + c3 * S + c4 * S * S;
+  double d2 = c3 + c4 * S;
+  double d3 = c4;
+  double Y = polysolve3_ld(d3, d2, d1, d0);
+  double dV = c1 + 2.0 * c2 * S + 3.0 * c3 * S * S + 4.0 * c4 * S * S * S;
+  double dVn = c1 + 2.0 * c2 * Y + 3.0 * c3 * Y * Y + 4.0 * c4 * Y * Y * Y;
+  if (fabs(dV / dVn) >= dran(o)) {
+    o->S[i] = Y;
+    return 1;
+  }
+  return 0;
+}
+
+/* metro_singlet (src/metropolis.c:148-167) */
+static int site_metro_singlet(orc_t *o, long i) {
+  double old = o->S[i];
+  double act_old = orc_localact_singlet(o, i);
+  o->S[i] += 1.0 * (dran(o) - 0.5);
+  double diff = orc_localact_singlet(o, i) - act_old;
+  if (metro_accept(o, diff)) return 1;
+  o->S[i] = old;
+  return 0;
+}
+
+static int site_dispatch(orc_t *o, int kind, long i, int dir) {
+  switch (kind) {
+    case SU2HIP_K_LINK_HEATBATH: return site_heatbath(o, i, dir);
+    case SU2HIP_K_LINK_METROPOLIS: return site_metro_link(o, i, dir);
+    case SU2HIP_K_LINK_OVERRELAX: return site_link_or(o, i, dir);
+    case SU2HIP_K_U1_METROPOLIS: return site_metro_u1(o, i, dir);
+    case SU2HIP_K_DOUBLET_OVERRELAX: return site_or_doublet(o, i);
+    case SU2HIP_K_DOUBLET_METROPOLIS: return site_metro_doublet(o, i);
+    case SU2HIP_K_TRIPLET_OVERRELAX: return site_or_triplet(o, i);
+    case SU2HIP_K_TRIPLET_METROPOLIS: return site_metro_triplet(o, i);
+    case SU2HIP_K_SINGLET_OVERRELAX: return site_or_singlet(o, i);
+    case SU2HIP_K_SINGLET_METROPOLIS: return site_metro_singlet(o, i);
+  }
+  return 0;
+}
+
+int orc_site_update(orc_t *o, int kind, long i, int dir, uint64_t pass) {
+  ctx(o, pass, (uint32_t)i);
+  return site_dispatch(o, kind, i, dir);
+}
+
+/* ------------------------------------------------------------------ */
+/* multicanonical (src/multicanonical.c) */
+
+/* linearize_weight (src/multicanonical.c:337-346) */
+static void linearize(orc_t *o) {
+  int n = o->m.bins;
+  o->slope[0] = 0;
+  o->slope[n + 1] = 0;
+  o->b[0] = o->W[0];
+  o->b[n + 1] = o->W[n + 1];
+  for (int i = 1; i < n + 1; i++) {
+    o->slope[i] = (o->W[i + 1] - o->W[i]) / (o->pos[i + 1] - o->pos[i]);
+    o->b[i] = o->W[i] - o->slope[i] * o->pos[i];
+  }
+}
+
+/* whichbin (src/multicanonical.c:671-704) */
+int orc_muca_whichbin(const orc_t *o, double val) {
+  if (val < o->m.min) return 0;
+  if (val >= o->m.max) return o->m.bins + 1;
+  int lo = 1, hi = o->m.bins;
+  while (lo <= hi) {
+    int bin = lo + (hi - lo) / 2;
+    double cur = o->pos[bin], nxt = o->pos[bin + 1];
+    if ((val < nxt && val > cur) || (fabs(val - cur) < 1e-12)) return bin;
+    if (val > cur)
+      lo = bin + 1;
+    else
+      hi = bin - 1;
+  }
+  return o->m.bins;
+}
+
+/* get_weight (src/multicanonical.c:350-379) */
+double orc_muca_weight(const orc_t *o, double val) {
+  int n = o->m.bins;
+  if (val >= o->m.max) return o->b[n + 1];
+  if (val < o->m.min) return o->b[0];
+  int bin = orc_muca_whichbin(o, val);
+  if (bin >= n + 1) return o->b[n + 1];
+  return o->b[bin] + o->slope[bin] * val;
+}
+
+/* muca_accumulate_hits (src/multicanonical.c:451-498) */
+static void accumulate_hits(orc_t *o, double val) {
+  int n = o->m.bins;
+  int bin = orc_muca_whichbin(o, val);
+  if (bin == 0) {
+    double dmin = o->pos[2] - o->pos[1];
+    if (val < o->pos[1] - 0.5 * dmin) return;
+    bin += 1;
+  } else if (bin < n + 1) {
+    if (val - o->pos[bin] > o->pos[bin + 1] - val) bin = bin + 1;
+  } else {
+    double dmax = o->pos[n + 1] - o->pos[n];
+    if (val > o->pos[n + 1] + 0.5 * dmax) return;
+  }
+  if (o->m.mode == SU2HIP_FAST) {
+    o->hits[bin] += 5;
+    if (bin > 1) o->hits[bin - 1] += 3;
+    if (bin > 2) o->hits[bin - 2] += 1;
+    if (bin < n) o->hits[bin + 1] += 3;
+    if (bin < n - 1) o->hits[bin + 2] += 1;
+  } else if (o->m.mode == SU2HIP_SLOW) {
+    o->hits[bin]++;
+    o->hgram[bin] += exp(orc_muca_weight(o, val));
+  }
+}
+
+/* update_weight (src/multicanonical.c:504-557), FAST recursion */
+static int update_weight_fast(orc_t *o) {
+  int n = o->m.bins, first = -1;
+  double delta = 0.0;
+  for (int i = 0; i < n + 2; i++) {
+    if (o->pos[i] >= o->m.wrk_min && o->m.wrk_max >= o->pos[i]) {
+      if (first < 0) first = i;
+      delta = (o->hits[i] - o->hits[first]) * o->m.delta / ((double)n);
+      o->W[i] += delta;
+    } else if (o->m.wrk_max < o->pos[i]) {
+      o->W[i] += delta;
+    }
+  }
+  o->W[0] = o->W[1];
+  o->W[n + 1] = o->W[n];
+  linearize(o);
+  int tunnel = 0;
+  if (o->m.last_max && o->hits[1] > 0) {
+    o->m.last_max = 0;
+    tunnel = 1;
+  } else if (!o->m.last_max && o->hits[n] > 0) {
+    o->m.last_max = 1;
+    tunnel = 1;
+  }
+  if (tunnel) o->m.delta /= 1.5;
+  for (int i = 0; i < n + 2; i++) o->hits[i] = 0;
+  return tunnel;
+}
+
+/* update_weight_slow (src/multicanonical.c:561-666), hep-lat/9804019 */
+static void update_weight_slow(orc_t *o) {
+  int n = o->m.bins;
+  double *wold = (double *)malloc(sizeof(double) * (n + 2));
+  memcpy(wold, o->W, sizeof(double) * (n + 2));
+  double dmin = o->pos[2] - o->pos[1];
+  o->hgram[1] /= dmin;
+  for (int i = 2; i < n + 1; i++) o->hgram[i] /= (o->pos[i + 1] - o->pos[i - 1]);
+  int first = -1, last = -1;
+  const int hits_min = 8;
+  for (int i = 1; i < n + 1; i++) {
+    if (o->pos[i] >= o->m.wrk_min && o->m.wrk_max >= o->pos[i]) {
+      if (first < 0) {
+        first = i;
+        continue;
+      }
+      o->nsum[i] += o->hits[i];
+      int g;
+      if (o->hits[i] >= hits_min && o->hits[i - 1] >= hits_min) {
+        g = o->hits[i] + o->hits[i - 1];
+      } else {
+        o->W[i] = wold[i] - wold[i - 1] + o->W[i - 1];
+        continue;
+      }
+      long gnew = o->gsum[i] + g;
+      double ln = o->hgram[i - 1] / o->hgram[i];
+      ln = -1.0 * g * log(ln);
+      o->W[i] = o->W[i - 1] + ((wold[i] - wold[i - 1]) * o->gsum[i] + ln) / ((double)gnew);
+      o->gsum[i] = gnew;
+    } else if (o->m.wrk_max < o->pos[i]) {
+      if (last < 0) last = i - 1;
+      o->W[i] += o->W[last] - wold[last];
+    }
+  }
+  for (int i = 0; i < n + 2; i++) {
+    o->hits[i] = 0;
+    o->hgram[i] = 0.0;
+  }
+  o->W[0] = o->W[1];
+  o->W[n + 1] = o->W[n];
+  memcpy(wold, o->W, sizeof(double) * (n + 2));
+  double corr = 0.0;
+  if (first == 1)
+    dmin = o->pos[first + 1] - o->pos[first];
+  else
+    dmin = o->pos[first + 1] - o->pos[first - 1];
+  for (int i = first + 1; i < n + 1; i++) {
+    if (i <= last) {
+      double d1 = o->pos[i + 1] - o->pos[i - 1];
+      corr = 2.0 * log((dmin / d1) * o->nsum[i] / o->nsum[first]);
+    }
+    o->W[i] += corr;
+  }
+  o->W[0] = o->W[1];
+  o->W[n + 1] = o->W[n];
+  linearize(o);
+  memcpy(o->W, wold, sizeof(double) * (n + 2));
+  free(wold);
+}
+
+/* calc_orderparam (src/multicanonical.c:709-751) */
+double orc_muca_orderparam(orc_t *o, int par) {
+  long off = par == SU2HIP_EVEN ? 0 : o->even;
+  long max = par == SU2HIP_EVEN ? o->even : o->vol;
+  double tot = 0.0;
+  switch (o->m.orderparam) {
+    case SU2HIP_SIGMASQ:
+      for (long i = off; i < max; i++) tot += tripletsq(SIG(o, i));
+      break;
+    case SU2HIP_PHI2MINUSSIGMA2:
+      for (long i = off; i < max; i++)
+        tot += doubletsq(PHI(o, i)) - tripletsq(SIG(o, i));
+      break;
+    case SU2HIP_PHISQ:
+      for (long i = off; i < max; i++) tot += doubletsq(PHI(o, i));
+      break;
+  }
+  tot = tot / o->vol;
+  o->m.param_value[par] = tot;
+  return tot + o->m.param_value[1 - par];
+}
+
+/* multicanonical_acceptance (src/multicanonical.c:385-445); one draw */
+static int muca_acceptance(orc_t *o, double oldval, double newval) {
+  double diff = orc_muca_weight(o, newval) - orc_muca_weight(o, oldval);
+  int accept = exp(-(diff)) > dran(o);
+  if (o->m.mode != SU2HIP_READONLY) {
+    if (!accept) newval = oldval;
+    accumulate_hits(o, newval);
+    o->m.muca_count++;
+    if (o->m.muca_count % o->m.update_interval == 0) {
+      if (o->m.mode == SU2HIP_FAST)
+        update_weight_fast(o);
+      else if (o->m.mode == SU2HIP_SLOW)
+        update_weight_slow(o);
+      o->m.muca_count = 0;
+    }
+  }
+  return accept;
+}
+
+/* muca_check (src/update.c:13-30) */
+static int muca_check(orc_t *o, int par, uint64_t pass, uint32_t check) {
+  double old = o->m.param_value[0] + o->m.param_value[1];
+  double nw = orc_muca_orderparam(o, par);
+  ctx(o, pass, ORC_MUCA_SITE);
+  o->r_k = check;
+  int acc = muca_acceptance(o, old, nw);
+  if (!acc) o->m.param_value[par] = old - o->m.param_value[1 - par];
+  o->c.accepted_muca += acc;
+  o->c.total_muca++;
+  return acc;
+}
+
+void orc_muca_set(orc_t *o, const su2hip_muca *m, const double *pos,
+                  const double *W) {
+  free_muca(o);
+  o->m = *m;
+  int n = m->bins;
+  o->nb = n;
+  o->pos = (double *)malloc(sizeof(double) * (n + 2));
+  o->W = (double *)malloc(sizeof(double) * (n + 2));
+  o->slope = (double *)malloc(sizeof(double) * (n + 2));
+  o->b = (double *)malloc(sizeof(double) * (n + 2));
+  o->hgram = (double *)calloc(n + 2, sizeof(double));
+  o->hits = (int32_t *)calloc(n + 2, sizeof(int32_t));
+  o->gsum = (long *)malloc(sizeof(long) * (n + 2));
+  o->nsum = (long *)malloc(sizeof(long) * (n + 2));
+  for (int i = 0; i < n + 2; i++) {
+    o->gsum[i] = 5;
+    o->nsum[i] = 1;
+  }
+  memcpy(o->pos, pos, sizeof(double) * (n + 2));
+  memcpy(o->W, W, sizeof(double) * (n + 2));
+  if (o->m.update_interval <= 0)
+    o->m.update_interval = (o->m.mode == SU2HIP_SLOW) ? 5000 : 100;
+  if (o->m.mode == SU2HIP_SLOW)
+    for (int i = 0; i < n + 2; i++) o->hgram[i] = exp(o->W[i]);
+  linearize(o);
+  if (o->m.enabled) {
+    orc_muca_orderparam(o, SU2HIP_EVEN);
+    orc_muca_orderparam(o, SU2HIP_ODD);
+  }
+}
+
+void orc_muca_get(const orc_t *o, su2hip_muca *m, double *pos, double *W,
+                  int32_t *hits) {
+  *m = o->m;
+  int n = o->m.bins;
+  if (pos && o->pos) memcpy(pos, o->pos, sizeof(double) * (n + 2));
+  if (W && o->W) memcpy(W, o->W, sizeof(double) * (n + 2));
+  if (hits && o->hits) memcpy(hits, o->hits, sizeof(int32_t) * (n + 2));
+}
+
+/* ------------------------------------------------------------------ */
+/* passes and the sweep (src/update.c) */
+
+static void count(orc_t *o, int kind, int acc) {
+  su2hip_counters *c = &o->c;
+  switch (kind) {
+    case SU2HIP_K_LINK_HEATBATH:
+    case SU2HIP_K_LINK_METROPOLIS:
+      c->accepted_su2link += acc;
+      c->total_su2link++;
+      break;
+    case SU2HIP_K_LINK_OVERRELAX:
+      c->accepted_link_or += acc;
+      c->total_link_or++;
+      break;
+    case SU2HIP_K_U1_METROPOLIS:
+      c->accepted_u1link += acc;
+      c->total_u1link++;
+      break;
+    case SU2HIP_K_DOUBLET_OVERRELAX:
+      c->acc_overrelax_doublet += acc;
+      c->total_overrelax_doublet++;
+      break;
+    case SU2HIP_K_DOUBLET_METROPOLIS:
+      c->accepted_doublet += acc;
+      c->total_doublet++;
+      break;
+    case SU2HIP_K_TRIPLET_OVERRELAX:
+      c->acc_overrelax_triplet += acc;
+      c->total_overrelax_triplet++;
+      break;
+    case SU2HIP_K_TRIPLET_METROPOLIS:
+      c->accepted_triplet += acc;
+      c->total_triplet++;
+      break;
+    case SU2HIP_K_SINGLET_OVERRELAX:
+      c->acc_overrelax_singlet += acc;
+      c->total_overrelax_singlet++;
+      break;
+    case SU2HIP_K_SINGLET_METROPOLIS:
+      c->accepted_singlet += acc;
+      c->total_singlet++;
+      break;
+  }
+}
+
+/* cp_field over one parity (src/init.c:152-163) */
+static void cp_parity(const orc_t *o, const double *src, double *dst, int dofs,
+                      int par) {
+  long off = par == SU2HIP_EVEN ? 0 : o->even;
+  long max = par == SU2HIP_EVEN ? o->even : o->vol;
+  memcpy(dst + off * dofs, src + off * dofs, sizeof(double) * dofs * (max - off));
+}
+
+/* checkerboard_sweep_* (src/update.c:35-259) for one kind/parity/dir,
+ * including the multicanonical checks of src/update.c:95-153 */
+void orc_pass(orc_t *o, int kind, int par, int dir) {
+  uint64_t pass = o->pass++;
+  long off = par == SU2HIP_EVEN ? 0 : o->even;
+  long max = par == SU2HIP_EVEN ? o->even : o->vol;
+  int field = 0; /* 1 doublet, 2 triplet */
+  if (kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS) field = 1;
+  if (kind == SU2HIP_K_TRIPLET_OVERRELAX || kind == SU2HIP_K_TRIPLET_METROPOLIS) field = 2;
+  int do_muca = 0;
+  if (field && o->m.enabled && o->m.do_acceptance) {
+    int op = o->m.orderparam;
+    if (field == 1 && (op == SU2HIP_PHISQ || op == SU2HIP_PHI2MINUSSIGMA2)) do_muca = 1;
+    if (field == 2 && (op == SU2HIP_SIGMASQ || op == SU2HIP_PHI2MINUSSIGMA2)) do_muca = 1;
+  }
+  double *fld = field == 1 ? o->phi : o->sig, *bu = field == 1 ? o->bu_phi : o->bu_sig;
+  int dofs = field == 1 ? 4 : 3;
+  long interval = o->vol, cnt = 0;
+  uint32_t check = 0;
+  if (do_muca) {
+    cp_parity(o, fld, bu, dofs, par);
+    interval = (max - off) / o->m.checks_per_sweep;
+    if (interval <= 0) interval = 1;
+  }
+  for (long i = off; i < max; i++) {
+    ctx(o, pass, (uint32_t)i);
+    int acc = site_dispatch(o, kind, i, dir);
+    count(o, kind, acc);
+    if (do_muca) {
+      cnt++;
+      if (cnt % interval == 0) {
+        int more = (i < max - 1);
+        int a = muca_check(o, par, pass, check++);
+        if (!a)
+          cp_parity(o, bu, fld, dofs, par);
+        else if (more)
+          cp_parity(o, fld, bu, dofs, par);
+      }
+    }
+  }
+  if (do_muca && (max - off) % o->m.checks_per_sweep != 0) {
+    int a = muca_check(o, par, pass, check++);
+    if (!a) cp_parity(o, bu, fld, dofs, par);
+  }
+}
+
+/* scalar kind of this round: OR unless forced Metropolis (src/update.c:333-360) */
+static int scalar_round(int alg, int last, int kor, int kme) {
+  if (last) return alg == SU2HIP_OVERRELAX ? kme : -1;
+  return alg == SU2HIP_OVERRELAX ? kor : kme;
+}
+
+/* update_lattice (src/update.c:270-430) with link_overrelax rounds after
+ * the heatbath/Metropolis link passes */
+void orc_update(orc_t *o, int n) {
+  const su2hip_params *p = &o->p;
+  for (int it = 0; it < n; it++) {
+    for (int k = 0; k < p->update_links; k++) {
+      int lk = p->algorithm_su2link == SU2HIP_HEATBATH ? SU2HIP_K_LINK_HEATBATH
+                                                        : SU2HIP_K_LINK_METROPOLIS;
+      for (int d = 0; d < D; d++)
+        for (int par = 0; par < 2; par++) orc_pass(o, lk, par, d);
+      for (int r = 0; r < p->link_overrelax; r++)
+        for (int d = 0; d < D; d++)
+          for (int par = 0; par < 2; par++) orc_pass(o, SU2HIP_K_LINK_OVERRELAX, par, d);
+      if (p->u1)
+        for (int d = 0; d < D; d++)
+          for (int par = 0; par < 2; par++) orc_pass(o, SU2HIP_K_U1_METROPOLIS, par, d);
+    }
+    int ns = p->scalar_sweeps + 1;
+    for (int s = 0; s < ns; s++) {
+      int last = (s >= ns - 1);
+      if (p->nhiggs) {
+        int kd = scalar_round(p->algorithm_su2doublet, last, SU2HIP_K_DOUBLET_OVERRELAX,
+                              SU2HIP_K_DOUBLET_METROPOLIS);
+        if (kd >= 0)
+          for (int k = 0; k < p->update_su2doublet; k++)
+            for (int par = 0; par < 2; par++) orc_pass(o, kd, par, 0);
+      }
+      if (p->triplet) {
+        int kt = scalar_round(p->algorithm_su2triplet, last, SU2HIP_K_TRIPLET_OVERRELAX,
+                              SU2HIP_K_TRIPLET_METROPOLIS);
+        if (kt >= 0)
+          for (int k = 0; k < p->update_su2triplet; k++)
+            for (int par = 0; par < 2; par++) orc_pass(o, kt, par, 0);
+      }
+      if (p->singlet) {
+        int ks = scalar_round(p->algorithm_singlet, last, SU2HIP_K_SINGLET_OVERRELAX,
+                              SU2HIP_K_SINGLET_METROPOLIS);
+        if (ks >= 0)
+          for (int k = 0; k < p->update_singlet; k++)
+            for (int par = 0; par < 2; par++) orc_pass(o, ks, par, 0);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* magnetic charge (src/magfield.c, TRIPLET measure columns) */
+
+typedef struct {
+  double re[4], im[4]; /* row-major 2x2 */
+} cmat;
+
+static void cmat_mul(const cmat *a, const cmat *b, int dag, cmat *r) {
+  cmat bb = *b;
+  if (dag) {
+    bb.re[0] = b->re[0];
+    bb.im[0] = -b->im[0];
+    bb.re[1] = b->re[2];
+    bb.im[1] = -b->im[2];
+    bb.re[2] = b->re[1];
+    bb.im[2] = -b->im[1];
+    bb.re[3] = b->re[3];
+    bb.im[3] = -b->im[3];
+  }
+  cmat t;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+      double sr = 0, si = 0;
+      for (int k = 0; k < 2; k++) {
+        double ar = a->re[2 * i + k], ai = a->im[2 * i + k];
+        double br = bb.re[2 * k + j], bi = bb.im[2 * k + j];
+        sr += ar * br - ai * bi;
+        si += ar * bi + ai * br;
+      }
+      t.re[2 * i + j] = sr;
+      t.im[2 * i + j] = si;
+    }
+  *r = t;
+}
+
+/* (1 + n.sigma)/2 with n = Sigma/|Sigma| (projector, src/magfield.c:90-104) */
+static void halfproj(const double *a, cmat *m) {
+  double md = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  double n0 = a[0] / md, n1 = a[1] / md, n2 = a[2] / md;
+  m->re[0] = 0.5 * (1 + n2);
+  m->im[0] = 0.0;
+  m->re[1] = 0.5 * n0;
+  m->im[1] = -0.5 * n1;
+  m->re[2] = 0.5 * n0;
+  m->im[2] = 0.5 * n1;
+  m->re[3] = 0.5 * (1 - n2);
+  m->im[3] = 0.0;
+}
+
+/* project_u1 (src/magfield.c:115-184): Pi(x) U_d(x) Pi(x+d) */
+static void project_u1(const orc_t *o, long i, int d, cmat *r) {
+  cmat pl, pr, u, t;
+  halfproj(SIG(o, i), &pl);
+  halfproj(SIG(o, NX(o, i, d)), &pr);
+  const double *q = LNK(o, i, d);
+  u.re[0] = q[0];
+  u.im[0] = q[3];
+  u.re[1] = q[2];
+  u.im[1] = q[1];
+  u.re[2] = -q[2];
+  u.im[2] = q[1];
+  u.re[3] = q[0];
+  u.im[3] = -q[3];
+  cmat_mul(&pl, &u, 0, &t);
+  cmat_mul(&t, &pr, 0, r);
+}
+
+/* alpha_proj (src/magfield.c:189-216) */
+static double alpha_proj(const orc_t *o, long i, int d1, int d2) {
+  cmat u1, u2, u3, u4;
+  project_u1(o, i, d1, &u1);
+  project_u1(o, NX(o, i, d1), d2, &u2);
+  project_u1(o, NX(o, i, d2), d1, &u3);
+  project_u1(o, i, d2, &u4);
+  cmat_mul(&u1, &u2, 0, &u1);
+  cmat_mul(&u1, &u3, 1, &u1);
+  cmat_mul(&u1, &u4, 1, &u1);
+  double al = atan2(u1.im[0] + u1.im[3], u1.re[0] + u1.re[3]);
+  return al * sqrt(o->p.betasu2);
+}
+
+/* magfield (src/magfield.c:221-257) */
+static double magfield(const orc_t *o, long i, int dir) {
+  double res = 0.0;
+  for (int d1 = 0; d1 < D; d1++) {
+    if (d1 == dir) continue;
+    for (int d2 = d1 + 1; d2 < D; d2++) {
+      if (d2 == dir) continue;
+      double al = alpha_proj(o, i, d1, d2);
+      if (d1 > dir)
+        res += al;
+      else if (d2 > dir)
+        res -= al;
+      else
+        res += al;
+    }
+  }
+  return res;
+}
+
+/* magcharge_cube (src/magfield.c:262-280) */
+static double magcharge_cube(const orc_t *o, long i) {
+  double res = 0.0;
+  for (int d = 0; d < D; d++)
+    res += magfield(o, NX(o, i, d), d) - magfield(o, i, d);
+  return res;
+}
+
+/* ------------------------------------------------------------------ */
+/* measure() site loop (src/measure.c:110-174), raw sums */
+void orc_measure(const orc_t *o, double *sums) {
+  const su2hip_params *p = &o->p;
+  double s[SU2HIP_NSUMS];
+  memset(s, 0, sizeof(s));
+  for (long i = 0; i < o->vol; i++) {
+    s[SU2HIP_SUM_ACTION] += orc_action_local(o, i);
+    s[SU2HIP_SUM_WILSON] += local_su2wilson(o, i);
+    if (p->u1) s[SU2HIP_SUM_U1WILSON] += local_u1wilson(o, i);
+    if (p->nhiggs) {
+      double mod = doubletsq(PHI(o, i));
+      s[SU2HIP_SUM_COV_PHI] += covariant_doublet(o, i);
+      s[SU2HIP_SUM_PHI2] += mod;
+      s[SU2HIP_SUM_PHI4] += mod * mod;
+      if (p->triplet) s[SU2HIP_SUM_PHI2SIGMA2] += mod * tripletsq(SIG(o, i));
+    }
+    if (p->triplet) {
+      double tm = tripletsq(SIG(o, i));
+      s[SU2HIP_SUM_SIGMA2] += tm;
+      s[SU2HIP_SUM_SIGMA4] += tm * tm;
+      for (int d = 0; d < D; d++) s[SU2HIP_SUM_HOP_SIGMA] += hop_triplet_fwd(o, i, d);
+      double ch = magcharge_cube(o, i);
+      s[SU2HIP_SUM_MAGCHARGE] += ch;
+      s[SU2HIP_SUM_MAGCHARGE_ABS] += fabs(ch);
+    }
+    if (p->singlet) {
+      double S = o->S[i];
+      for (int d = 0; d < D; d++) {
+        double Sn = o->S[NX(o, i, d)];
+        s[SU2HIP_SUM_S_COV] += S * S - S * Sn;
+      }
+      s[SU2HIP_SUM_S] += S;
+      s[SU2HIP_SUM_S2] += S * S;
+      s[SU2HIP_SUM_S3] += S * S * S;
+      s[SU2HIP_SUM_S4] += S * S * S * S;
+      if (p->nhiggs == 1) {
+        double mod = doubletsq(PHI(o, i));
+        s[SU2HIP_SUM_SPHISQ] += mod * S;
+        s[SU2HIP_SUM_S2PHISQ] += mod * S * S;
+      }
+    }
+  }
+  memcpy(sums, s, sizeof(s));
+}
+
+/* ------------------------------------------------------------------ */
+/* gradient flow (src/gradflow.c) */
+
+/* grad_force_link (src/gradflow.c:190-258) and grad_force_triplet
+ * (:267-330) for every site (calc_gradient, :335-354) */
+void orc_flow_force(const orc_t *o, double *force, double *tforce) {
+  const su2hip_params *p = &o->p;
+  for (long i = 0; i < o->vol; i++) {
+    for (int d = 0; d < D; d++) {
+      double s[4], m[4];
+      orc_wilson_staple(o, i, d, s);
+      for (int a = 0; a < 4; a++) s[a] *= -p->betasu2 / (8.0);
+      if (p->triplet) {
+        /* s <- s - Sigma(x+d) U^+ Sigma(x), Sigma = a.sigma/2 */
+        double a2[4] = {0.0, SIG(o, NX(o, i, d))[0], SIG(o, NX(o, i, d))[1],
+                        SIG(o, NX(o, i, d))[2]};
+        double a1[4] = {0.0, SIG(o, i)[0], SIG(o, i)[1], SIG(o, i)[2]};
+        double t[4], r[4];
+        qmul_nc(a2, LNK(o, i, d), t);
+        qmul(t, a1, r);
+        for (int a = 0; a < 4; a++) s[a] += r[a] / 4.0;
+      }
+      qmul(LNK(o, i, d), s, m);
+      double *f = force + ((size_t)i * D + d) * 4;
+      f[0] = 0.0;
+      for (int a = 1; a < 4; a++) {
+        f[a] = 2.0 * m[a];
+        f[a] *= 4.0 / p->betasu2;
+      }
+    }
+    if (p->triplet && tforce) {
+      double res[3];
+      orc_triplet_staple(o, i, res);
+      const double *tr = SIG(o, i);
+      double tsq = tripletsq(tr);
+      for (int a = 0; a < 3; a++) {
+        res[a] += (2.0 * D + p->msq_triplet + 2.0 * p->b4 * tsq) * tr[a];
+        if (p->nhiggs) res[a] += p->a2 * doubletsq(PHI(o, i)) * tr[a];
+      }
+      for (int a = 0; a < 3; a++) tforce[(size_t)i * 3 + a] = -1.0 * res[a];
+    }
+  }
+}
+
+/* V <- exp(i z.sigma) V (flow_gauge's per-link step, src/gradflow.c:370-395);
+ * a zero vector leaves the link unchanged (the reference returns from the
+ * whole update there, src/gradflow.c:377-381: a deliberate divergence). */
+static void expmul(const double *z, double *v) {
+  double mod = z[0] * z[0] + z[1] * z[1] + z[2] * z[2];
+  if (mod <= 0.0) return;
+  mod = sqrt(mod);
+  double u[4];
+  u[0] = cos(mod);
+  for (int a = 0; a < 3; a++) u[a + 1] = sin(mod) * z[a] / mod;
+  qmul(u, v, v);
+}
+
+void orc_flow_step(orc_t *o, int scheme, double dt) {
+  size_t V = (size_t)o->vol;
+  double *F = (double *)malloc(sizeof(double) * V * D * 4);
+  double *TF = o->p.triplet ? (double *)malloc(sizeof(double) * V * 3) : NULL;
+  if (scheme == SU2HIP_FLOW_EULER) {
+    orc_flow_force(o, F, TF);
+    for (size_t l = 0; l < V * D; l++) {
+      double z[3];
+      for (int a = 0; a < 3; a++) z[a] = dt * F[l * 4 + a + 1];
+      expmul(z, o->U + l * 4);
+    }
+    if (TF)
+      for (size_t l = 0; l < V * 3; l++) o->sig[l] += dt * TF[l];
+  } else {
+    /* Luscher's RK3 (arXiv:1006.4518 appendix C):
+     *   W1 = exp(Z0/4) W0, W2 = exp(8/9 Z1 - 17/36 Z0) W1,
+     *   W3 = exp(3/4 Z2 - 8/9 Z1 + 17/36 Z0) W2,  Zk = dt F(Wk),
+     * in 2N-storage form X <- c_k Zk - d_k X, W <- exp(X) W with
+     * (c,d) = (1/4,0), (8/9,17/9), (3/4,1). */
+    static const double c[3] = {0.25, 8.0 / 9.0, 0.75};
+    static const double dd[3] = {0.0, 17.0 / 9.0, 1.0};
+    double *Z = (double *)calloc(V * D * 3, sizeof(double));
+    double *ZT = TF ? (double *)calloc(V * 3, sizeof(double)) : NULL;
+    for (int st = 0; st < 3; st++) {
+      orc_flow_force(o, F, TF);
+      for (size_t l = 0; l < V * D; l++) {
+        double *z = Z + l * 3;
+        for (int a = 0; a < 3; a++) z[a] = c[st] * dt * F[l * 4 + a + 1] - dd[st] * z[a];
+        expmul(z, o->U + l * 4);
+      }
+      if (TF)
+        for (size_t l = 0; l < V * 3; l++) {
+          ZT[l] = c[st] * dt * TF[l] - dd[st] * ZT[l];
+          o->sig[l] += ZT[l];
+        }
+    }
+    free(Z);
+    free(ZT);
+  }
+  free(F);
+  free(TF);
+}
+
+/* remove_counterterms (src/gradflow.c:424-459): the triplet's tree-level
+ * mass for the flow (2-loop lattice counterterm constants of the reference) */
+static double triplet_flow_msq(const su2hip_params *p) {
+  double Sg = 3.17591153625, zeta = 0.08849, delta = 1.942130, rho = -0.313964;
+  double k1 = 0.958382, k2 = 0.25 * Sg * Sg - 0.5 * delta - 0.25;
+  double k3 = 0.751498, k4 = 1.204295;
+  double gsq = 4.0 / p->betasu2, RG = gsq, b4 = p->b4;
+  double ct = -(4.0 * gsq + 5.0 * b4) * Sg / (4.0 * M_PI);
+  ct += -1.0 / (16.0 * M_PI * M_PI) *
+        ((20.0 * b4 * gsq - 10.0 * b4 * b4) * (log(6.0 / (RG)) + zeta) +
+         20.0 * b4 * gsq * (0.25 * Sg * Sg - delta) +
+         2.0 * gsq * gsq *
+             (1.25 * Sg * Sg + M_PI / 3.0 * Sg - 6.0 * delta - 6.0 * rho + 4.0 * k1 - k2 -
+              k3 - 3.0 * k4));
+  return p->msq_triplet - ct;
+}
+
+/* grad_flow (src/gradflow.c:46-173) on a copy; returns the number of rows */
+int orc_flow(orc_t *o, int scheme, double dt, double t_max, int meas_interval,
+             double *rows, int max_rows) {
+  size_t V = (size_t)o->vol;
+  double *U0 = (double *)malloc(sizeof(double) * V * D * 4);
+  double *S0 = (double *)malloc(sizeof(double) * V * 3);
+  memcpy(U0, o->U, sizeof(double) * V * D * 4);
+  memcpy(S0, o->sig, sizeof(double) * V * 3);
+  double msq = o->p.msq_triplet;
+  if (o->p.triplet) o->p.msq_triplet = triplet_flow_msq(&o->p);
+  int nr = 0;
+  double t = 0.0;
+#define ROW()                                                \
+  do {                                                       \
+    if (nr < max_rows) {                                     \
+      rows[(size_t)nr * (1 + SU2HIP_NSUMS)] = t;             \
+      orc_measure(o, rows + (size_t)nr * (1 + SU2HIP_NSUMS) + 1); \
+    }                                                        \
+    nr++;                                                    \
+  } while (0)
+  ROW();
+  int iter = 1;
+  while (t < t_max) {
+    if (t + dt > t_max) dt = t_max - t;
+    orc_flow_step(o, scheme, dt);
+    t += dt;
+    if (iter % meas_interval == 0) ROW();
+    iter++;
+  }
+  if ((iter - 1) % meas_interval != 0) ROW();
+#undef ROW
+  o->p.msq_triplet = msq;
+  memcpy(o->U, U0, sizeof(double) * V * D * 4);
+  memcpy(o->sig, S0, sizeof(double) * V * 3);
+  free(U0);
+  free(S0);
+  return nr;
+}

@@ -1,0 +1,220 @@
+"""ctypes mirror of include/su2hip.h (the C ABI of libsu2hip.so).
+
+The structures here are byte-for-byte the C structs; `load()` opens the
+in-tree library and raises if it was not built (there is no CPU fallback:
+the sweep only runs as HIP kernels).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+LIBPATH = os.path.join(LIBDIR, "libsu2hip.so")
+
+METROPOLIS, HEATBATH, OVERRELAX = 1, 2, 3
+EVEN, ODD = 0, 1
+PHISQ, SIGMASQ, PHI2MINUSSIGMA2 = 1, 2, 3
+READONLY, FAST, SLOW = 0, 1, 2
+FP64, FP32 = 64, 32
+FLOW_EULER, FLOW_RK3 = 1, 3
+
+(K_LINK_HEATBATH, K_LINK_METROPOLIS, K_LINK_OVERRELAX, K_U1_METROPOLIS,
+ K_DOUBLET_OVERRELAX, K_DOUBLET_METROPOLIS, K_TRIPLET_OVERRELAX,
+ K_TRIPLET_METROPOLIS, K_SINGLET_OVERRELAX, K_SINGLET_METROPOLIS, K_MEASURE,
+ K_MUCA, K_FLOW, K_AUX, NKINDS) = range(15)
+
+KIND_NAMES = ["link_heatbath", "link_metropolis", "link_overrelax",
+              "u1_metropolis", "doublet_overrelax", "doublet_metropolis",
+              "triplet_overrelax", "triplet_metropolis", "singlet_overrelax",
+              "singlet_metropolis", "measure", "muca", "flow", "aux"]
+
+SUM_NAMES = ["action", "wilson", "u1wilson", "cov_phi", "phi2", "phi4",
+             "hop_sigma", "sigma2", "sigma4", "phi2sigma2", "magcharge",
+             "magcharge_abs", "s_cov", "s", "s2", "s3", "s4", "sphisq",
+             "s2phisq"]
+NSUMS = len(SUM_NAMES)
+SUM = {n: i for i, n in enumerate(SUM_NAMES)}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("L", C.c_int32 * 3),
+        ("nhiggs", C.c_int32), ("u1", C.c_int32), ("triplet", C.c_int32),
+        ("singlet", C.c_int32), ("precision", C.c_int32),
+        ("algorithm_su2link", C.c_int32), ("algorithm_u1link", C.c_int32),
+        ("algorithm_su2doublet", C.c_int32),
+        ("algorithm_su2triplet", C.c_int32), ("algorithm_singlet", C.c_int32),
+        ("update_links", C.c_int32), ("link_overrelax", C.c_int32),
+        ("scalar_sweeps", C.c_int32), ("update_su2doublet", C.c_int32),
+        ("update_su2triplet", C.c_int32), ("update_singlet", C.c_int32),
+        ("reunitarize", C.c_int32), ("reserved", C.c_int32),
+        ("betasu2", C.c_double), ("betau1", C.c_double), ("r_u1", C.c_double),
+        ("msq_phi", C.c_double), ("lambda_phi", C.c_double),
+        ("msq_triplet", C.c_double), ("a2", C.c_double), ("b4", C.c_double),
+        ("b1_s", C.c_double), ("msq_s", C.c_double), ("b3_s", C.c_double),
+        ("b4_s", C.c_double), ("a1_s", C.c_double), ("a2_s", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "total_su2link", "accepted_su2link", "total_u1link",
+        "accepted_u1link", "total_doublet", "accepted_doublet",
+        "total_overrelax_doublet", "acc_overrelax_doublet", "total_triplet",
+        "accepted_triplet", "total_overrelax_triplet", "acc_overrelax_triplet",
+        "total_singlet", "accepted_singlet", "total_overrelax_singlet",
+        "acc_overrelax_singlet", "total_muca", "accepted_muca",
+        "total_link_or", "accepted_link_or")]
+
+    def asdict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Muca(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32), ("orderparam", C.c_int32),
+        ("mode", C.c_int32), ("checks_per_sweep", C.c_int32),
+        ("bins", C.c_int32), ("last_max", C.c_int32),
+        ("do_acceptance", C.c_int32), ("muca_count", C.c_int32),
+        ("update_interval", C.c_int32), ("reserved", C.c_int32),
+        ("delta", C.c_double), ("min", C.c_double), ("max", C.c_double),
+        ("wrk_min", C.c_double), ("wrk_max", C.c_double),
+        ("param_value", C.c_double * 2),
+    ]
+
+
+def make_params(L=(8, 8, 8), nhiggs=1, u1=0, triplet=0, singlet=0,
+                precision=FP64, **kw):
+    """Params with the reference's sample-config defaults for the rest."""
+    p = Params()
+    p.L[:] = list(L)
+    p.nhiggs, p.u1, p.triplet, p.singlet = nhiggs, u1, triplet, singlet
+    p.precision = precision
+    p.algorithm_su2link = HEATBATH
+    p.algorithm_u1link = METROPOLIS
+    p.algorithm_su2doublet = OVERRELAX
+    p.algorithm_su2triplet = OVERRELAX
+    p.algorithm_singlet = OVERRELAX
+    p.update_links = 1
+    p.link_overrelax = 0
+    p.scalar_sweeps = 4
+    p.update_su2doublet = 1
+    p.update_su2triplet = 1
+    p.update_singlet = 1
+    p.reunitarize = 1 if precision == FP32 else 0
+    p.betasu2 = 8.0
+    p.betau1 = 8.0
+    p.r_u1 = 1.0
+    p.msq_phi = -0.25
+    p.lambda_phi = 0.02
+    p.msq_triplet = -0.5
+    p.a2 = 2.7
+    p.b4 = 0.12
+    p.b1_s, p.msq_s, p.b3_s, p.b4_s, p.a1_s, p.a2_s = 0.3, -0.1, 0.2, 0.3, 0.3, 0.25
+    p.seed = 1234
+    for k, v in kw.items():
+        if k == "L":
+            continue
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def volume(p):
+    return int(p.L[0]) * int(p.L[1]) * int(p.L[2])
+
+
+def field_arrays(p):
+    """Zeroed host arrays in the reference layout for the fields of p."""
+    V = volume(p)
+    return {
+        "su2link": np.zeros((V, 3, 4)),
+        "u1link": np.zeros((V, 3)) if p.u1 else None,
+        "doublet": np.zeros((V, 4)) if p.nhiggs else None,
+        "triplet": np.zeros((V, 3)) if p.triplet else None,
+        "singlet": np.zeros(V) if p.singlet else None,
+    }
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+_lib = None
+
+
+def load():
+    """Open the in-tree libsu2hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(
+            "libsu2hip.so is not built (%s); run __graft_entry__.build()" % LIBPATH)
+    lib = C.CDLL(LIBPATH)
+    P = C.POINTER
+    dp = P(C.c_double)
+    h = C.c_void_p
+    sig = {
+        "su2hip_error_string": (C.c_char_p, [C.c_int]),
+        "su2hip_last_error": (C.c_char_p, []),
+        "su2hip_create": (C.c_int, [C.c_int, P(Params), C.c_int, C.c_uint64, P(h)]),
+        "su2hip_destroy": (C.c_int, [h]),
+        "su2hip_footprint": (C.c_int, [P(Params), C.c_int, P(C.c_uint64)]),
+        "su2hip_add": (C.c_int, [h, P(Params), P(C.c_int)]),
+        "su2hip_remove": (C.c_int, [h, C.c_int]),
+        "su2hip_slots": (C.c_int, [h, P(C.c_int), P(C.c_int)]),
+        "su2hip_slot_active": (C.c_int, [h, C.c_int, P(C.c_int)]),
+        "su2hip_set_fields": (C.c_int, [h, C.c_int, dp, dp, dp, dp, dp]),
+        "su2hip_get_fields": (C.c_int, [h, C.c_int, dp, dp, dp, dp, dp]),
+        "su2hip_cold_start": (C.c_int, [h, C.c_int, C.c_double, C.c_double, C.c_double]),
+        "su2hip_hot_start": (C.c_int, [h, C.c_int]),
+        "su2hip_update": (C.c_int, [h, C.c_int]),
+        "su2hip_set_budget": (C.c_int, [h, C.c_int, C.c_int64]),
+        "su2hip_get_budgets": (C.c_int, [h, P(C.c_int64)]),
+        "su2hip_pass": (C.c_int, [h, C.c_int, C.c_int, C.c_int]),
+        "su2hip_use_graph": (C.c_int, [h, C.c_int]),
+        "su2hip_sync": (C.c_int, [h]),
+        "su2hip_get_counters": (C.c_int, [h, C.c_int, P(Counters)]),
+        "su2hip_reset_counters": (C.c_int, [h, C.c_int]),
+        "su2hip_rng_get": (C.c_int, [h, C.c_int, P(C.c_uint64), P(C.c_uint64)]),
+        "su2hip_rng_set": (C.c_int, [h, C.c_int, C.c_uint64, C.c_uint64]),
+        "su2hip_measure": (C.c_int, [h, C.c_int, dp]),
+        "su2hip_measure_all": (C.c_int, [h, dp]),
+        "su2hip_measure_columns": (C.c_int, [P(Params), dp, C.c_double, C.c_double, dp, P(C.c_int)]),
+        "su2hip_muca_set": (C.c_int, [h, C.c_int, P(Muca), dp, dp]),
+        "su2hip_muca_get": (C.c_int, [h, C.c_int, P(Muca), dp, dp, P(C.c_int32)]),
+        "su2hip_muca_weight_of": (C.c_int, [h, C.c_int, C.c_double, dp]),
+        "su2hip_flow": (C.c_int, [h, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, P(C.c_int)]),
+        "su2hip_flow_force": (C.c_int, [h, C.c_int, dp]),
+        "su2hip_set_timing": (C.c_int, [h, C.c_int]),
+        "su2hip_kernel_times": (C.c_int, [h, dp, P(C.c_int64)]),
+        "su2hip_reset_timing": (C.c_int, [h]),
+        "su2hip_copy_probe": (C.c_int, [C.c_uint64, C.c_int, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class Su2hipError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        lib = load()
+        raise Su2hipError("su2hip error %d (%s): %s" % (
+            rc, lib.su2hip_error_string(rc).decode(),
+            lib.su2hip_last_error().decode()))
+    return rc

@@ -1,0 +1,30 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+    config.addinivalue_line("markers", "slow: longer statistical runs")
+
+
+def gpu_present():
+    try:
+        import ctypes
+        lib = ctypes.CDLL("libamdhip64.so")
+        n = ctypes.c_int(0)
+        return lib.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
+        return False
+
+
+@pytest.fixture(scope="session")
+def su2lib():
+    """The built HIP library; on a GPU box its absence is an error."""
+    from su2higgs_amd import abi
+    return abi.load()

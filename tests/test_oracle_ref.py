@@ -1,0 +1,186 @@
+"""The oracle restatement against the reference compiled from its own sources
+(oracle/_ref, built by `make -C oracle ref`) on the same inputs and the same
+Philox draws.  CPU only; skipped where the reference build is absent."""
+import numpy as np
+import pytest
+
+import orc
+from su2higgs_amd.abi import (FP64, HEATBATH, METROPOLIS, OVERRELAX, K_DOUBLET_METROPOLIS,
+                              K_DOUBLET_OVERRELAX, K_LINK_HEATBATH, K_LINK_METROPOLIS,
+                              K_SINGLET_METROPOLIS, K_SINGLET_OVERRELAX, K_TRIPLET_METROPOLIS,
+                              K_TRIPLET_OVERRELAX, K_U1_METROPOLIS, make_params)
+
+MODELS = {
+    "h": dict(nhiggs=1),
+    "hu": dict(nhiggs=1, u1=1),
+    "t": dict(nhiggs=0, triplet=1),
+    "ht": dict(nhiggs=1, triplet=1),
+    "hsu": dict(nhiggs=1, u1=1, singlet=1),
+}
+
+
+def need_ref(tag):
+    if not orc.ref_available(tag):
+        pytest.skip("reference build oracle/_ref/libref_%s.so absent" % tag)
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors for philox4x32_10
+    assert orc.philox([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert orc.philox([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert orc.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_uniform_mapping():
+    # draw k uses block k>>1, words (0,1) for even k and (2,3) for odd k,
+    # top 53 bits of the 64-bit word
+    w = orc.philox([3, 17, 5, 0], [1234, 0])
+    u0 = orc.uniform(1234, 5, 17, 6)
+    u1 = orc.uniform(1234, 5, 17, 7)
+    assert u0 == ((w[1] << 32 | w[0]) >> 11) * 2.0 ** -53
+    assert u1 == ((w[3] << 32 | w[2]) >> 11) * 2.0 ** -53
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def setup(tag, L=(4, 4, 6), **kw):
+    p = make_params(L=L, **MODELS[tag], **kw)
+    o = orc.Oracle(p)
+    o.hot_start()
+    f = o.get_fields()
+    r = orc.Ref(p)
+    r.set_fields(f)
+    return p, o, r, f
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_layout_tables(tag):
+    need_ref(tag)
+    p, o, r, _ = setup(tag, L=(4, 6, 8))
+    nx, pv, nat = o.tables()
+    rnx, rpv = r.tables()
+    assert (nx == rnx).all() and (pv == rpv).all()
+    # even sites first, natural order preserved inside a parity
+    x = nat % 4
+    y = (nat // 4) % 6
+    z = nat // 24
+    par = (x + y + z) % 2
+    V = o.V
+    assert (par[:V // 2] == 0).all() and (par[V // 2:] == 1).all()
+    assert (np.diff(nat[:V // 2]) > 0).all() and (np.diff(nat[V // 2:]) > 0).all()
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_deterministic_pieces(tag):
+    need_ref(tag)
+    p, o, r, _ = setup(tag)
+    for i in range(o.V):
+        for d in range(3):
+            a = o.vec("orc_link_staple", 4, i, d)
+            b = r.vec("ref_link_staple", 4, i, d)
+            assert rel(a, b) < 1e-12
+            a = o.vec("orc_wilson_staple", 4, i, d)
+            b = r.vec("ref_wilson_staple", 4, i, d)
+            assert rel(a, b) < 1e-12
+            assert rel(o.lib.orc_localact_su2link(o.h, i, d), r.lib.ref_localact_su2link(i, d)) < 1e-12
+            if p.u1:
+                assert rel(o.lib.orc_localact_u1link(o.h, i, d), r.lib.ref_localact_u1link(i, d)) < 1e-12
+        assert rel(o.lib.orc_action_local(o.h, i), r.lib.ref_action_local(i)) < 1e-12
+        if p.nhiggs:
+            a = o.vec("orc_doublet_staple", 4, i)
+            b = r.vec("ref_doublet_staple", 4, i)
+            assert rel(a, b) < 1e-12
+            assert rel(o.lib.orc_localact_doublet(o.h, i), r.lib.ref_localact_doublet(i)) < 1e-12
+        if p.triplet:
+            assert rel(o.lib.orc_localact_triplet(o.h, i), r.lib.ref_localact_triplet(i)) < 1e-12
+        if p.singlet:
+            assert rel(o.lib.orc_localact_singlet(o.h, i), r.lib.ref_localact_singlet(i)) < 1e-12
+
+
+def kinds_for(p):
+    ks = [(K_LINK_HEATBATH, True), (K_LINK_METROPOLIS, True)]
+    if p.u1:
+        ks.append((K_U1_METROPOLIS, True))
+    if p.nhiggs:
+        ks += [(K_DOUBLET_OVERRELAX, False), (K_DOUBLET_METROPOLIS, False)]
+    if p.triplet:
+        ks += [(K_TRIPLET_OVERRELAX, False), (K_TRIPLET_METROPOLIS, False)]
+    if p.singlet:
+        ks += [(K_SINGLET_OVERRELAX, False), (K_SINGLET_METROPOLIS, False)]
+    return ks
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_site_updates(tag):
+    """Every per-site update, site by site, same draws, same result."""
+    need_ref(tag)
+    p, o, r, f = setup(tag)
+    pas = 11
+    for kind, has_dir in kinds_for(p):
+        for i in range(0, o.V, 5):
+            d = (i // 5) % 3 if has_dir else 0
+            a = o.site(kind, i, d, pas)
+            b = r.site(kind, i, d, pas)
+            assert a == b, (kind, i)
+            pas += 1
+        fo, fr = o.get_fields(), r.get_fields()
+        for k in orc.FIELDS:
+            assert np.abs(fo[k] - fr[k]).max() < 1e-12, (kind, k)
+        # resynchronise rounding before the next kind
+        r.set_fields(fo)
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+@pytest.mark.parametrize("variant", ["or", "metro"])
+def test_full_sweeps(tag, variant):
+    """The reference's own update_lattice against orc_update."""
+    need_ref(tag)
+    kw = {}
+    if variant == "metro":
+        kw = dict(algorithm_su2link=METROPOLIS, algorithm_su2doublet=METROPOLIS,
+                  algorithm_su2triplet=METROPOLIS, algorithm_singlet=METROPOLIS,
+                  scalar_sweeps=2, update_links=2)
+    p, o, r, f = setup(tag, **kw)
+    o.rng_set(99, 0)
+    r.rng_set(99, 0)
+    o.update(3)
+    r.update(3)
+    fo, fr = o.get_fields(), r.get_fields()
+    for k in orc.FIELDS:
+        assert np.abs(fo[k] - fr[k]).max() < 1e-11, k
+    co, cr = o.counters(), r.counters()
+    for k in cr:
+        if k in ("total_link_or", "accepted_link_or"):
+            continue
+        assert co[k] == cr[k], k
+    assert o.rng_get()[1] == r.lib.ref_rng_pass()
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_measure_sums(tag):
+    need_ref(tag)
+    p, o, r, _ = setup(tag, L=(6, 4, 4))
+    a, b = o.measure(), r.measure()
+    scale = np.maximum(np.abs(b), 1.0)
+    assert (np.abs(a - b) / scale).max() < 1e-12
+
+
+def test_polysolve3_against_reference():
+    need_ref("h")
+    lib = orc.ref_lib("h")
+    olib = orc.oracle_lib()
+    rng = np.random.default_rng(5)
+    for _ in range(2000):
+        a = rng.uniform(0.001, 0.05)
+        b = rng.normal() * a
+        c = rng.uniform(0.5, 5.0)
+        d = rng.normal() * 5
+        x = olib.orc_polysolve3(a, b, c, d)
+        y = lib.ref_polysolve3(a, b, c, d)
+        assert x == y or abs(x - y) < 1e-12 * max(1.0, abs(y))
+        # and it is a root
+        assert abs(((a * x + b) * x + c) * x + d) < 1e-8 * (abs(d) + c * abs(x) + 1)
