@@ -1,0 +1,1474 @@
+// su2hip_api.cpp -- the C ABI of libsu2hip.so (include/su2hip.h): lattice
+// slots in HBM, the pass schedule of one compound sweep, hipGraph replay,
+// measurement, multicanonical set-up, gradient flow and timing.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "su2hip_internal.h"
+
+using namespace su2;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPTRY(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      return fail(SU2HIP_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_));       \
+  } while (0)
+
+namespace {
+
+struct Slot {
+  bool used = false;
+  su2hip_params p{};
+  LatDesc hd{};
+  void *mem = nullptr;
+  size_t bytes = 0;
+  su2hip_counters cnt{};
+  void *bu_mem = nullptr;
+  size_t bu_bytes = 0;
+  void *muca_mem = nullptr;
+  size_t muca_bytes = 0;
+  MucaDev hm{};
+  void *flow_mem = nullptr;
+  size_t flow_bytes = 0;
+};
+
+struct Step {
+  int kind, parity, dir, pass_off;
+  bool muca;
+};
+
+struct Ev {
+  int kind;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct su2hip_batch {
+  int max_slots = 0;
+  uint64_t hbm_budget = 0, used = 0;
+  int model = 0, fp64 = 1;
+  su2hip_params tmpl{};
+  std::vector<Slot> slots;
+  hipStream_t stream = nullptr;
+  LatDesc *d_desc = nullptr;
+  MucaDev *d_muca = nullptr;
+  BlockMap *d_blk = nullptr;
+  int nblk = 0, nblk_cap = 0;
+  std::vector<BlockMap> h_blk;
+  long long *d_cnt = nullptr;
+  double *d_partial = nullptr;
+  size_t partial_cap = 0;
+  double *d_sums = nullptr;
+  double *d_stage = nullptr;
+  size_t stage_cap = 0;
+  bool map_dirty = true;
+  std::vector<Step> sched;
+  int passes = 0, muca_steps = 1;
+  bool sched_dirty = true;
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  bool timing = false;
+  std::vector<Ev> pending;
+  std::vector<hipEvent_t> pool;
+  double kms[SU2HIP_NKINDS] = {};
+  long long kcount[SU2HIP_NKINDS] = {};
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+
+static size_t wsize(const su2hip_params &p) { return p.precision == SU2HIP_FP64 ? 8 : 4; }
+static int vol_of(const su2hip_params &p) { return p.L[0] * p.L[1] * p.L[2]; }
+
+static size_t field_words(const su2hip_params &p) {
+  size_t Vh = (size_t)vol_of(p) / 2;
+  size_t w = 24 * Vh;
+  if (p.u1) w += 6 * Vh;
+  if (p.nhiggs) w += 8 * Vh;
+  if (p.triplet) w += 6 * Vh;
+  if (p.singlet) w += 2 * Vh;
+  return w;
+}
+
+static size_t flow_words(const su2hip_params &p) {
+  size_t Vh = (size_t)vol_of(p) / 2;
+  size_t w = 48 * Vh + 18 * Vh;
+  if (p.triplet) w += 12 * Vh + 6 * Vh;
+  return w;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static bool supported_model(int m) {
+  return m == 0 || m == M_NH || m == (M_NH | M_U1) || m == (M_NH | M_SG) ||
+         m == (M_NH | M_U1 | M_SG) || m == M_TR || m == (M_NH | M_TR);
+}
+
+static int validate(const su2hip_params *p) {
+  if (!p) return fail(SU2HIP_EINVAL, "params is NULL");
+  for (int d = 0; d < 3; d++)
+    if (p->L[d] < 2 || (p->L[d] & 1))
+      return fail(SU2HIP_EINVAL, "extent L%d = %d must be even and >= 2", d + 1, p->L[d]);
+  long long V = (long long)p->L[0] * p->L[1] * p->L[2];
+  if (V > (1ll << 28)) return fail(SU2HIP_ERANGE, "volume %lld too large", V);
+  if (p->nhiggs < 0 || p->nhiggs > 1)
+    return fail(SU2HIP_EUNSUPPORTED, "nhiggs = %d (0 or 1 supported)", p->nhiggs);
+  if (!supported_model(model_bits(*p)))
+    return fail(SU2HIP_EUNSUPPORTED,
+                "field content nhiggs=%d u1=%d triplet=%d singlet=%d is not supported",
+                p->nhiggs, p->u1, p->triplet, p->singlet);
+  if (p->precision != SU2HIP_FP64 && p->precision != SU2HIP_FP32)
+    return fail(SU2HIP_EINVAL, "precision %d (64 or 32)", p->precision);
+  if (p->algorithm_su2link != SU2HIP_HEATBATH && p->algorithm_su2link != SU2HIP_METROPOLIS)
+    return fail(SU2HIP_EINVAL, "algorithm_su2link %d", p->algorithm_su2link);
+  if (p->u1 && p->algorithm_u1link != SU2HIP_METROPOLIS)
+    return fail(SU2HIP_EINVAL, "algorithm_u1link %d", p->algorithm_u1link);
+  int sc[3] = {p->algorithm_su2doublet, p->algorithm_su2triplet, p->algorithm_singlet};
+  int on[3] = {p->nhiggs, p->triplet, p->singlet};
+  for (int k = 0; k < 3; k++)
+    if (on[k] && sc[k] != SU2HIP_OVERRELAX && sc[k] != SU2HIP_METROPOLIS)
+      return fail(SU2HIP_EINVAL, "scalar algorithm %d", sc[k]);
+  if (p->update_links < 0 || p->link_overrelax < 0 || p->scalar_sweeps < 0 ||
+      p->update_su2doublet < 0 || p->update_su2triplet < 0 || p->update_singlet < 0)
+    return fail(SU2HIP_EINVAL, "negative update count");
+  if (p->betasu2 == 0.0) return fail(SU2HIP_EINVAL, "betasu2 must be nonzero");
+  return SU2HIP_OK;
+}
+
+// lattices of one batch share the pass schedule and the arithmetic
+static bool compatible(const su2hip_params &a, const su2hip_params &b) {
+  return a.nhiggs == b.nhiggs && a.u1 == b.u1 && a.triplet == b.triplet &&
+         a.singlet == b.singlet && a.precision == b.precision &&
+         a.algorithm_su2link == b.algorithm_su2link && a.algorithm_u1link == b.algorithm_u1link &&
+         a.algorithm_su2doublet == b.algorithm_su2doublet &&
+         a.algorithm_su2triplet == b.algorithm_su2triplet &&
+         a.algorithm_singlet == b.algorithm_singlet && a.update_links == b.update_links &&
+         a.link_overrelax == b.link_overrelax && a.scalar_sweeps == b.scalar_sweeps &&
+         a.update_su2doublet == b.update_su2doublet &&
+         a.update_su2triplet == b.update_su2triplet && a.update_singlet == b.update_singlet &&
+         a.reunitarize == b.reunitarize;
+}
+
+static LaunchCtx ctx_of(su2hip_batch *b) {
+  LaunchCtx c;
+  c.stream = b->stream;
+  c.desc = b->d_desc;
+  c.muca = b->d_muca;
+  c.blk = b->d_blk;
+  c.nblk = b->nblk;
+  c.cnt = b->d_cnt;
+  c.partial = b->d_partial;
+  c.sums = b->d_sums;
+  c.max_slots = b->max_slots;
+  c.model = b->model;
+  c.fp64 = b->fp64;
+  return c;
+}
+
+static void fill_desc(Slot &s) {
+  const su2hip_params &p = s.p;
+  LatDesc &L = s.hd;
+  size_t Vh = (size_t)vol_of(p) / 2, w = wsize(p);
+  char *base = (char *)s.mem;
+  size_t off = 0;
+  auto take = [&](size_t words) {
+    void *q = base + off;
+    off += align256(words * w);
+    return q;
+  };
+  L.U = take(24 * Vh);
+  L.A = p.u1 ? take(6 * Vh) : nullptr;
+  L.F = p.nhiggs ? take(8 * Vh) : nullptr;
+  L.G = p.triplet ? take(6 * Vh) : nullptr;
+  L.S = p.singlet ? take(2 * Vh) : nullptr;
+  L.B = s.bu_mem;
+  L.L0 = p.L[0];
+  L.L1 = p.L[1];
+  L.L2 = p.L[2];
+  L.h0 = p.L[0] / 2;
+  L.Vh = (int)Vh;
+  L.vol = vol_of(p);
+  L.div_h0 = make_fastdiv((uint32_t)L.h0);
+  L.div_L1 = make_fastdiv((uint32_t)L.L1);
+  L.active = 1;
+  L.key0 = (uint32_t)p.seed;
+  L.key1 = (uint32_t)(p.seed >> 32);
+  L.beta = p.betasu2;
+  L.betau1 = p.betau1;
+  L.r_u1 = p.r_u1;
+  L.msq = p.msq_phi;
+  L.lambda = p.lambda_phi;
+  L.msq_t = p.msq_triplet;
+  L.a2 = p.a2;
+  L.b4 = p.b4;
+  L.b1_s = p.b1_s;
+  L.msq_s = p.msq_s;
+  L.b3_s = p.b3_s;
+  L.b4_s = p.b4_s;
+  L.a1_s = p.a1_s;
+  L.a2_s = p.a2_s;
+}
+
+static size_t slot_bytes(const su2hip_params &p) {
+  size_t Vh = (size_t)vol_of(p) / 2, w = wsize(p);
+  size_t b = align256(24 * Vh * w);
+  if (p.u1) b += align256(6 * Vh * w);
+  if (p.nhiggs) b += align256(8 * Vh * w);
+  if (p.triplet) b += align256(6 * Vh * w);
+  if (p.singlet) b += align256(2 * Vh * w);
+  return b;
+}
+
+// pull the device-owned descriptor fields (pass counter, budget)
+static int pull_desc(su2hip_batch *b) {
+  std::vector<LatDesc> tmp(b->max_slots);
+  HIPTRY(hipMemcpyAsync(tmp.data(), b->d_desc, sizeof(LatDesc) * b->max_slots,
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->max_slots; i++)
+    if (b->slots[i].used) {
+      b->slots[i].hd.pass = tmp[i].pass;
+      b->slots[i].hd.budget = tmp[i].budget;
+    }
+  return SU2HIP_OK;
+}
+
+static int push_desc(su2hip_batch *b) {
+  std::vector<LatDesc> tmp(b->max_slots);
+  for (int i = 0; i < b->max_slots; i++) {
+    if (b->slots[i].used) {
+      tmp[i] = b->slots[i].hd;
+    } else {
+      memset(&tmp[i], 0, sizeof(LatDesc));
+      tmp[i].active = 0;
+    }
+  }
+  HIPTRY(hipMemcpyAsync(b->d_desc, tmp.data(), sizeof(LatDesc) * b->max_slots,
+                        hipMemcpyHostToDevice, b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return SU2HIP_OK;
+}
+
+static void add_counts(su2hip_counters &c, int kind, long long acc, long long tot) {
+  switch (kind) {
+    case SU2HIP_K_LINK_HEATBATH:
+    case SU2HIP_K_LINK_METROPOLIS:
+      c.accepted_su2link += acc;
+      c.total_su2link += tot;
+      break;
+    case SU2HIP_K_LINK_OVERRELAX:
+      c.accepted_link_or += acc;
+      c.total_link_or += tot;
+      break;
+    case SU2HIP_K_U1_METROPOLIS:
+      c.accepted_u1link += acc;
+      c.total_u1link += tot;
+      break;
+    case SU2HIP_K_DOUBLET_OVERRELAX:
+      c.acc_overrelax_doublet += acc;
+      c.total_overrelax_doublet += tot;
+      break;
+    case SU2HIP_K_DOUBLET_METROPOLIS:
+      c.accepted_doublet += acc;
+      c.total_doublet += tot;
+      break;
+    case SU2HIP_K_TRIPLET_OVERRELAX:
+      c.acc_overrelax_triplet += acc;
+      c.total_overrelax_triplet += tot;
+      break;
+    case SU2HIP_K_TRIPLET_METROPOLIS:
+      c.accepted_triplet += acc;
+      c.total_triplet += tot;
+      break;
+    case SU2HIP_K_SINGLET_OVERRELAX:
+      c.acc_overrelax_singlet += acc;
+      c.total_overrelax_singlet += tot;
+      break;
+    case SU2HIP_K_SINGLET_METROPOLIS:
+      c.accepted_singlet += acc;
+      c.total_singlet += tot;
+      break;
+  }
+}
+
+// fold the per-block device counters into the slots' host counters
+static int fold_counts(su2hip_batch *b) {
+  if (!b->d_cnt || b->nblk == 0) return SU2HIP_OK;
+  size_t n = (size_t)NCOUNT * 2 * b->nblk;
+  std::vector<long long> h(n);
+  HIPTRY(hipMemcpyAsync(h.data(), b->d_cnt, n * sizeof(long long), hipMemcpyDeviceToHost,
+                        b->stream));
+  HIPTRY(hipMemsetAsync(b->d_cnt, 0, n * sizeof(long long), b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  for (int k = 0; k < NCOUNT; k++)
+    for (int i = 0; i < b->nblk; i++) {
+      long long a = h[(size_t)(k * 2) * b->nblk + i], t = h[(size_t)(k * 2 + 1) * b->nblk + i];
+      if (a || t) add_counts(b->slots[b->h_blk[i].lat].cnt, k, a, t);
+    }
+  return SU2HIP_OK;
+}
+
+// Block table: logical chunks lattice by lattice, then physical workgroup
+// ids dealt so that XCD j (workgroups j, j+8, ...) gets a contiguous run of
+// chunks and its L2 sees neighbouring rows.
+static int rebuild_map(su2hip_batch *b) {
+  if (!b->map_dirty) return SU2HIP_OK;
+  int rc = fold_counts(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  std::vector<BlockMap> logical;
+  for (int i = 0; i < b->max_slots; i++) {
+    Slot &s = b->slots[i];
+    if (!s.used) continue;
+    int nb = (s.hd.Vh + BLK - 1) / BLK;
+    s.hd.blk_base = (int)logical.size();
+    s.hd.nblk = nb;
+    for (int k = 0; k < nb; k++) logical.push_back(BlockMap{i, k * BLK});
+  }
+  int n = (int)logical.size();
+  std::vector<BlockMap> phys(n);
+  const int X = 8;
+  int base = 0;
+  for (int j = 0; j < X; j++) {
+    int cnt = n / X + (j < n % X ? 1 : 0);
+    for (int k = 0; k < cnt; k++) phys[j + X * k] = logical[base + k];
+    base += cnt;
+  }
+  if (n > b->nblk_cap) {
+    if (b->d_blk) HIPTRY(hipFree(b->d_blk));
+    if (b->d_cnt) HIPTRY(hipFree(b->d_cnt));
+    b->d_blk = nullptr;
+    b->d_cnt = nullptr;
+    int cap = std::max(n, 64);
+    if (hipMalloc(&b->d_blk, sizeof(BlockMap) * cap) != hipSuccess ||
+        hipMalloc(&b->d_cnt, sizeof(long long) * NCOUNT * 2 * cap) != hipSuccess)
+      return fail(SU2HIP_ENOMEM, "block table allocation failed");
+    b->nblk_cap = cap;
+  }
+  size_t need = (size_t)std::max(n, 1) * SU2HIP_NSUMS * 2;
+  if (need > b->partial_cap) {
+    if (b->d_partial) HIPTRY(hipFree(b->d_partial));
+    b->d_partial = nullptr;
+    if (hipMalloc(&b->d_partial, need * sizeof(double)) != hipSuccess)
+      return fail(SU2HIP_ENOMEM, "partial-sum buffer allocation failed");
+    b->partial_cap = need;
+  }
+  b->nblk = n;
+  b->h_blk = phys;
+  if (n) {
+    HIPTRY(hipMemcpyAsync(b->d_blk, phys.data(), sizeof(BlockMap) * n, hipMemcpyHostToDevice,
+                          b->stream));
+    HIPTRY(hipMemsetAsync(b->d_cnt, 0, sizeof(long long) * NCOUNT * 2 * n, b->stream));
+  }
+  rc = push_desc(b);
+  if (rc) return rc;
+  b->map_dirty = false;
+  b->graph_valid = false;
+  b->sched_dirty = true;
+  return SU2HIP_OK;
+}
+
+static bool muca_on_field(const MucaDev &m, int field) {
+  if (!m.enabled || !m.do_acceptance) return false;
+  if (field == 1) return m.orderparam == SU2HIP_PHISQ || m.orderparam == SU2HIP_PHI2MINUSSIGMA2;
+  if (field == 2) return m.orderparam == SU2HIP_SIGMASQ || m.orderparam == SU2HIP_PHI2MINUSSIGMA2;
+  return false;
+}
+
+static int muca_nsteps(int Vh, int cps) {
+  int iv = Vh / cps;
+  if (iv <= 0) iv = 1;
+  int nfull = Vh / iv;
+  return nfull + (((Vh % iv) != 0) || ((Vh % cps) != 0) ? 1 : 0);
+}
+
+static int scalar_round(int alg, bool last, int kor, int kme) {
+  if (last) return alg == SU2HIP_OVERRELAX ? kme : -1;
+  return alg == SU2HIP_OVERRELAX ? kor : kme;
+}
+
+// The pass sequence of update_lattice (src/update.c:270-430), with the link
+// overrelaxation rounds after each heatbath/Metropolis link round.
+static void build_sched(su2hip_batch *b) {
+  if (!b->sched_dirty) return;
+  const su2hip_params &p = b->tmpl;
+  bool muca_f[3] = {false, false, false};
+  int steps = 1;
+  for (auto &s : b->slots) {
+    if (!s.used) continue;
+    for (int f = 1; f <= 2; f++)
+      if (muca_on_field(s.hm, f)) {
+        muca_f[f] = true;
+        steps = std::max(steps, muca_nsteps(s.hd.Vh, std::max(1, s.hm.cps)));
+      }
+  }
+  b->muca_steps = steps;
+  std::vector<Step> &v = b->sched;
+  v.clear();
+  int off = 0;
+  auto add = [&](int kind, int par, int dir, bool muca) {
+    v.push_back(Step{kind, par, dir, off++, muca});
+  };
+  int lk = p.algorithm_su2link == SU2HIP_HEATBATH ? SU2HIP_K_LINK_HEATBATH : SU2HIP_K_LINK_METROPOLIS;
+  for (int k = 0; k < p.update_links; k++) {
+    for (int d = 0; d < 3; d++)
+      for (int par = 0; par < 2; par++) add(lk, par, d, false);
+    for (int r = 0; r < p.link_overrelax; r++)
+      for (int d = 0; d < 3; d++)
+        for (int par = 0; par < 2; par++) add(SU2HIP_K_LINK_OVERRELAX, par, d, false);
+    if (p.u1)
+      for (int d = 0; d < 3; d++)
+        for (int par = 0; par < 2; par++) add(SU2HIP_K_U1_METROPOLIS, par, d, false);
+  }
+  int ns = p.scalar_sweeps + 1;
+  for (int s = 0; s < ns; s++) {
+    bool last = s >= ns - 1;
+    if (p.nhiggs) {
+      int kd = scalar_round(p.algorithm_su2doublet, last, SU2HIP_K_DOUBLET_OVERRELAX,
+                            SU2HIP_K_DOUBLET_METROPOLIS);
+      if (kd >= 0)
+        for (int k = 0; k < p.update_su2doublet; k++)
+          for (int par = 0; par < 2; par++) add(kd, par, 0, muca_f[1]);
+    }
+    if (p.triplet) {
+      int kt = scalar_round(p.algorithm_su2triplet, last, SU2HIP_K_TRIPLET_OVERRELAX,
+                            SU2HIP_K_TRIPLET_METROPOLIS);
+      if (kt >= 0)
+        for (int k = 0; k < p.update_su2triplet; k++)
+          for (int par = 0; par < 2; par++) add(kt, par, 0, muca_f[2]);
+    }
+    if (p.singlet) {
+      int ks = scalar_round(p.algorithm_singlet, last, SU2HIP_K_SINGLET_OVERRELAX,
+                            SU2HIP_K_SINGLET_METROPOLIS);
+      if (ks >= 0)
+        for (int k = 0; k < p.update_singlet; k++)
+          for (int par = 0; par < 2; par++) add(ks, par, 0, false);
+    }
+  }
+  b->passes = off;
+  b->sched_dirty = false;
+  b->graph_valid = false;
+}
+
+static hipEvent_t ev_get(su2hip_batch *b) {
+  if (!b->pool.empty()) {
+    hipEvent_t e = b->pool.back();
+    b->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// run fn() bracketed by timing events when timing is on
+template <typename Fn>
+static hipError_t timed(su2hip_batch *b, int kind, Fn &&fn) {
+  if (!b->timing) return fn();
+  hipEvent_t a = ev_get(b), e = ev_get(b);
+  if (!a || !e) return hipErrorOutOfMemory;
+  hipError_t r = hipEventRecord(a, b->stream);
+  if (r != hipSuccess) return r;
+  r = fn();
+  if (r != hipSuccess) return r;
+  r = hipEventRecord(e, b->stream);
+  b->pending.push_back(Ev{kind, a, e});
+  return r;
+}
+
+static hipError_t enqueue_step(su2hip_batch *b, const Step &st) {
+  LaunchCtx c = ctx_of(b);
+  int reunit = b->tmpl.reunitarize && !b->fp64;
+  if (!st.muca)
+    return timed(b, st.kind, [&] {
+      return launch_pass(c, st.kind, st.parity, st.dir, st.pass_off, -1, reunit);
+    });
+  for (int k = 0; k < b->muca_steps; k++) {
+    hipError_t e = timed(b, st.kind, [&] {
+      return launch_pass(c, st.kind, st.parity, st.dir, st.pass_off, k, reunit);
+    });
+    if (e != hipSuccess) return e;
+    e = timed(b, SU2HIP_K_MUCA,
+              [&] { return launch_muca_accept(c, st.kind, st.parity, st.pass_off, k); });
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+static hipError_t enqueue_sweep(su2hip_batch *b) {
+  for (const Step &st : b->sched) {
+    hipError_t e = enqueue_step(b, st);
+    if (e != hipSuccess) return e;
+  }
+  LaunchCtx c = ctx_of(b);
+  return timed(b, SU2HIP_K_AUX, [&] { return launch_end_sweep(c, b->passes); });
+}
+
+static int prepare(su2hip_batch *b) {
+  int rc = rebuild_map(b);
+  if (rc) return rc;
+  build_sched(b);
+  return SU2HIP_OK;
+}
+
+static int ensure_stage(su2hip_batch *b, size_t doubles) {
+  if (doubles <= b->stage_cap) return SU2HIP_OK;
+  if (b->d_stage) HIPTRY(hipFree(b->d_stage));
+  b->d_stage = nullptr;
+  if (hipMalloc(&b->d_stage, doubles * sizeof(double)) != hipSuccess)
+    return fail(SU2HIP_ENOMEM, "staging buffer of %zu doubles", doubles);
+  b->stage_cap = doubles;
+  return SU2HIP_OK;
+}
+
+static int check_slot(su2hip_batch *b, int slot) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  if (slot < 0 || slot >= b->max_slots) return fail(SU2HIP_ERANGE, "slot %d out of range", slot);
+  if (!b->slots[slot].used) return fail(SU2HIP_ERANGE, "slot %d is empty", slot);
+  return SU2HIP_OK;
+}
+
+static int alloc_slot(su2hip_batch *b, int i, const su2hip_params &p) {
+  size_t bytes = slot_bytes(p);
+  if (b->hbm_budget && b->used + bytes > b->hbm_budget)
+    return fail(SU2HIP_ENOMEM, "HBM budget: %llu + %zu bytes exceeds %llu",
+                (unsigned long long)b->used, bytes, (unsigned long long)b->hbm_budget);
+  Slot &s = b->slots[i];
+  s = Slot();
+  s.p = p;
+  if (hipMalloc(&s.mem, bytes) != hipSuccess) {
+    s.mem = nullptr;
+    return fail(SU2HIP_ENOMEM, "device allocation of %zu bytes failed", bytes);
+  }
+  s.bytes = bytes;
+  s.used = true;
+  b->used += bytes;
+  fill_desc(s);
+  s.hd.budget = -1;
+  s.hd.pass = 0;
+  s.hm = MucaDev();
+  b->map_dirty = true;
+  return SU2HIP_OK;
+}
+
+static void free_slot(su2hip_batch *b, int i) {
+  Slot &s = b->slots[i];
+  if (s.mem) (void)hipFree(s.mem);
+  if (s.bu_mem) (void)hipFree(s.bu_mem);
+  if (s.muca_mem) (void)hipFree(s.muca_mem);
+  if (s.flow_mem) (void)hipFree(s.flow_mem);
+  b->used -= s.bytes + s.bu_bytes + s.flow_bytes;
+  s = Slot();
+  b->map_dirty = true;
+}
+
+static int push_muca(su2hip_batch *b, int slot) {
+  HIPTRY(hipMemcpyAsync(b->d_muca + slot, &b->slots[slot].hm, sizeof(MucaDev),
+                        hipMemcpyHostToDevice, b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return SU2HIP_OK;
+}
+
+static int pull_muca(su2hip_batch *b, int slot) {
+  HIPTRY(hipMemcpyAsync(&b->slots[slot].hm, b->d_muca + slot, sizeof(MucaDev),
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return SU2HIP_OK;
+}
+
+static int orderparam_refresh(su2hip_batch *b) {
+  bool any = false;
+  for (auto &s : b->slots)
+    if (s.used && s.hm.enabled) any = true;
+  if (!any) return SU2HIP_OK;
+  int rc = prepare(b);
+  if (rc) return rc;
+  HIPTRY(launch_orderparam_init(ctx_of(b)));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return SU2HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char *su2hip_error_string(int code) {
+  switch (code) {
+    case SU2HIP_OK: return "ok";
+    case SU2HIP_EINVAL: return "invalid argument";
+    case SU2HIP_ENOMEM: return "out of memory or HBM budget";
+    case SU2HIP_EDEVICE: return "HIP runtime error";
+    case SU2HIP_ERANGE: return "index or size out of range";
+    case SU2HIP_EUNSUPPORTED: return "unsupported model";
+    case SU2HIP_ESTATE: return "invalid state";
+    case SU2HIP_EIO: return "I/O or parse error";
+  }
+  return "unknown error";
+}
+
+const char *su2hip_last_error(void) { return g_err.c_str(); }
+
+int su2hip_footprint(const su2hip_params *p, int with_flow, uint64_t *bytes) {
+  int rc = validate(p);
+  if (rc) return rc;
+  if (!bytes) return fail(SU2HIP_EINVAL, "bytes is NULL");
+  size_t b = slot_bytes(*p);
+  size_t Vh = (size_t)vol_of(*p) / 2;
+  if (p->nhiggs || p->triplet) b += align256((p->nhiggs ? 8 : 6) * Vh * wsize(*p));
+  if (with_flow) b += align256(flow_words(*p) * wsize(*p));
+  *bytes = b;
+  return SU2HIP_OK;
+}
+
+int su2hip_create(int n, const su2hip_params *params, int max_slots, uint64_t hbm_budget,
+                  su2hip_batch_t **out) {
+  if (!out) return fail(SU2HIP_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (n < 1 || !params) return fail(SU2HIP_EINVAL, "need at least one lattice");
+  if (max_slots < n) max_slots = n;
+  for (int i = 0; i < n; i++) {
+    int rc = validate(&params[i]);
+    if (rc) return rc;
+    if (!compatible(params[0], params[i]))
+      return fail(SU2HIP_EINVAL, "lattice %d: field content, precision or schedule differs from lattice 0", i);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(SU2HIP_EDEVICE, "no HIP device");
+  su2hip_batch *b = new su2hip_batch();
+  b->max_slots = max_slots;
+  b->hbm_budget = hbm_budget;
+  b->tmpl = params[0];
+  b->model = model_bits(params[0]);
+  b->fp64 = params[0].precision == SU2HIP_FP64;
+  b->slots.resize(max_slots);
+  int rc = SU2HIP_OK;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&b->d_desc, sizeof(LatDesc) * max_slots) != hipSuccess ||
+      hipMalloc(&b->d_muca, sizeof(MucaDev) * max_slots) != hipSuccess ||
+      hipMalloc(&b->d_sums, sizeof(double) * SU2HIP_NSUMS * max_slots) != hipSuccess) {
+    su2hip_destroy(b);
+    return fail(SU2HIP_ENOMEM, "handle allocation failed");
+  }
+  std::vector<MucaDev> zm(max_slots);
+  (void)hipMemcpy(b->d_muca, zm.data(), sizeof(MucaDev) * max_slots, hipMemcpyHostToDevice);
+  for (int i = 0; i < n && !rc; i++) rc = alloc_slot(b, i, params[i]);
+  if (!rc) rc = push_desc(b);
+  if (!rc) rc = prepare(b);
+  for (int i = 0; i < n && !rc; i++) rc = su2hip_cold_start(b, i, 0.0, 0.0, 0.0);
+  if (rc) {
+    std::string keep = g_err;
+    su2hip_destroy(b);
+    g_err = keep;
+    return rc;
+  }
+  *out = b;
+  return SU2HIP_OK;
+}
+
+int su2hip_destroy(su2hip_batch_t *b) {
+  if (!b) return SU2HIP_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (int i = 0; i < b->max_slots; i++)
+    if (b->slots[i].used) free_slot(b, i);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  for (auto &e : b->pending) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  for (auto e : b->pool) (void)hipEventDestroy(e);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_muca) (void)hipFree(b->d_muca);
+  if (b->d_blk) (void)hipFree(b->d_blk);
+  if (b->d_cnt) (void)hipFree(b->d_cnt);
+  if (b->d_partial) (void)hipFree(b->d_partial);
+  if (b->d_sums) (void)hipFree(b->d_sums);
+  if (b->d_stage) (void)hipFree(b->d_stage);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return SU2HIP_OK;
+}
+
+int su2hip_add(su2hip_batch_t *b, const su2hip_params *p, int *slot) {
+  if (!b || !slot) return fail(SU2HIP_EINVAL, "NULL argument");
+  int rc = validate(p);
+  if (rc) return rc;
+  if (!compatible(b->tmpl, *p))
+    return fail(SU2HIP_EINVAL, "field content, precision or schedule differs from the batch");
+  int i = 0;
+  while (i < b->max_slots && b->slots[i].used) i++;
+  if (i == b->max_slots) return fail(SU2HIP_ENOMEM, "all %d slots in use", b->max_slots);
+  rc = rebuild_map(b);  // fold counters of the current table first
+  if (rc) return rc;
+  rc = alloc_slot(b, i, *p);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = su2hip_cold_start(b, i, 0.0, 0.0, 0.0);
+  if (rc) return rc;
+  *slot = i;
+  return SU2HIP_OK;
+}
+
+int su2hip_remove(su2hip_batch_t *b, int slot) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  HIPTRY(hipStreamSynchronize(b->stream));
+  rc = fold_counts(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  free_slot(b, slot);
+  MucaDev z{};
+  b->slots[slot].hm = z;
+  rc = push_muca(b, slot);
+  if (rc) return rc;
+  return prepare(b);
+}
+
+int su2hip_slots(su2hip_batch_t *b, int *max_slots, int *active) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  int n = 0;
+  for (auto &s : b->slots) n += s.used;
+  if (max_slots) *max_slots = b->max_slots;
+  if (active) *active = n;
+  return SU2HIP_OK;
+}
+
+int su2hip_slot_active(su2hip_batch_t *b, int slot, int *active) {
+  if (!b || !active) return fail(SU2HIP_EINVAL, "NULL argument");
+  if (slot < 0 || slot >= b->max_slots) return fail(SU2HIP_ERANGE, "slot %d out of range", slot);
+  *active = b->slots[slot].used;
+  return SU2HIP_OK;
+}
+
+int su2hip_set_fields(su2hip_batch_t *b, int slot, const double *su2link, const double *u1link,
+                      const double *doublet, const double *triplet, const double *singlet) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  const su2hip_params &p = b->slots[slot].p;
+  size_t V = vol_of(p);
+  rc = ensure_stage(b, V * 12);
+  if (rc) return rc;
+  const double *src[5] = {su2link, u1link, doublet, triplet, singlet};
+  int have[5] = {1, p.u1, p.nhiggs, p.triplet, p.singlet};
+  int dofs[5] = {12, 3, 4, 3, 1};
+  LaunchCtx c = ctx_of(b);
+  for (int f = 0; f < 5; f++) {
+    if (!src[f]) continue;
+    if (!have[f]) return fail(SU2HIP_EINVAL, "field %d is not part of this model", f);
+    HIPTRY(hipMemcpyAsync(b->d_stage, src[f], V * dofs[f] * sizeof(double), hipMemcpyHostToDevice,
+                          b->stream));
+    HIPTRY(launch_to_device_layout(c, slot, f, b->d_stage));
+    HIPTRY(hipStreamSynchronize(b->stream));
+  }
+  return orderparam_refresh(b);
+}
+
+int su2hip_get_fields(su2hip_batch_t *b, int slot, double *su2link, double *u1link,
+                      double *doublet, double *triplet, double *singlet) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  const su2hip_params &p = b->slots[slot].p;
+  size_t V = vol_of(p);
+  rc = ensure_stage(b, V * 12);
+  if (rc) return rc;
+  double *dst[5] = {su2link, u1link, doublet, triplet, singlet};
+  int have[5] = {1, p.u1, p.nhiggs, p.triplet, p.singlet};
+  int dofs[5] = {12, 3, 4, 3, 1};
+  LaunchCtx c = ctx_of(b);
+  for (int f = 0; f < 5; f++) {
+    if (!dst[f]) continue;
+    if (!have[f]) return fail(SU2HIP_EINVAL, "field %d is not part of this model", f);
+    HIPTRY(launch_to_host_layout(c, slot, f, b->d_stage));
+    HIPTRY(hipMemcpyAsync(dst[f], b->d_stage, V * dofs[f] * sizeof(double), hipMemcpyDeviceToHost,
+                          b->stream));
+    HIPTRY(hipStreamSynchronize(b->stream));
+  }
+  return SU2HIP_OK;
+}
+
+int su2hip_cold_start(su2hip_batch_t *b, int slot, double phi0, double sigma0, double singlet0) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  HIPTRY(launch_cold(ctx_of(b), slot, phi0, sigma0, singlet0));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return orderparam_refresh(b);
+}
+
+int su2hip_hot_start(su2hip_batch_t *b, int slot) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  HIPTRY(launch_hot(ctx_of(b), slot));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return orderparam_refresh(b);
+}
+
+int su2hip_update(su2hip_batch_t *b, int n_sweeps) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  if (n_sweeps < 0) return fail(SU2HIP_EINVAL, "n_sweeps < 0");
+  int rc = prepare(b);
+  if (rc) return rc;
+  if (b->nblk == 0 || n_sweeps == 0) return SU2HIP_OK;
+  if (b->use_graph && !b->timing) {
+    if (!b->graph_valid) {
+      if (b->gexec) HIPTRY(hipGraphExecDestroy(b->gexec));
+      b->gexec = nullptr;
+      hipGraph_t g = nullptr;
+      HIPTRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = enqueue_sweep(b);
+      hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(SU2HIP_EDEVICE, "graph capture failed: %s",
+                    hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
+      b->graph_valid = true;
+    }
+    for (int i = 0; i < n_sweeps; i++) HIPTRY(hipGraphLaunch(b->gexec, b->stream));
+    return SU2HIP_OK;
+  }
+  for (int i = 0; i < n_sweeps; i++) {
+    hipError_t e = enqueue_sweep(b);
+    if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "sweep launch: %s", hipGetErrorString(e));
+  }
+  return SU2HIP_OK;
+}
+
+int su2hip_set_budget(su2hip_batch_t *b, int slot, int64_t sweeps) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  b->slots[slot].hd.budget = sweeps < 0 ? -1 : sweeps;
+  return push_desc(b);
+}
+
+int su2hip_get_budgets(su2hip_batch_t *b, int64_t *budgets) {
+  if (!b || !budgets) return fail(SU2HIP_EINVAL, "NULL argument");
+  int rc = prepare(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  for (int i = 0; i < b->max_slots; i++) budgets[i] = b->slots[i].used ? b->slots[i].hd.budget : -2;
+  return SU2HIP_OK;
+}
+
+int su2hip_pass(su2hip_batch_t *b, int kind, int parity, int dir) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  if (kind < 0 || kind >= NCOUNT) return fail(SU2HIP_EINVAL, "kind %d", kind);
+  if (parity < 0 || parity > 1 || dir < 0 || dir > 2) return fail(SU2HIP_EINVAL, "parity/dir");
+  int rc = prepare(b);
+  if (rc) return rc;
+  int field = (kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS)   ? 1
+              : (kind == SU2HIP_K_TRIPLET_OVERRELAX || kind == SU2HIP_K_TRIPLET_METROPOLIS) ? 2
+                                                                                             : 0;
+  bool muca = false;
+  for (auto &s : b->slots)
+    if (s.used && field && muca_on_field(s.hm, field)) muca = true;
+  hipError_t e = enqueue_step(b, Step{kind, parity, dir, 0, muca});
+  if (e == hipErrorInvalidValue)
+    return fail(SU2HIP_EINVAL, "kind %d is not part of this model", kind);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "pass launch: %s", hipGetErrorString(e));
+  // one pass number consumed, budget untouched
+  rc = pull_desc(b);
+  if (rc) return rc;
+  for (auto &s : b->slots)
+    if (s.used && s.hd.budget != 0) s.hd.pass += 1;
+  return push_desc(b);
+}
+
+int su2hip_use_graph(su2hip_batch_t *b, int enable) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  b->use_graph = enable != 0;
+  return SU2HIP_OK;
+}
+
+int su2hip_sync(su2hip_batch_t *b) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return SU2HIP_OK;
+}
+
+int su2hip_get_counters(su2hip_batch_t *b, int slot, su2hip_counters *c) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!c) return fail(SU2HIP_EINVAL, "NULL counters");
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = fold_counts(b);
+  if (rc) return rc;
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  *c = b->slots[slot].cnt;
+  c->total_muca = b->slots[slot].hm.total;
+  c->accepted_muca = b->slots[slot].hm.accepted;
+  return SU2HIP_OK;
+}
+
+int su2hip_reset_counters(su2hip_batch_t *b, int slot) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = fold_counts(b);
+  if (rc) return rc;
+  b->slots[slot].cnt = su2hip_counters();
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  b->slots[slot].hm.total = 0;
+  b->slots[slot].hm.accepted = 0;
+  return push_muca(b, slot);
+}
+
+int su2hip_rng_get(su2hip_batch_t *b, int slot, uint64_t *seed, uint64_t *pass) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  const LatDesc &L = b->slots[slot].hd;
+  if (seed) *seed = (uint64_t)L.key0 | ((uint64_t)L.key1 << 32);
+  if (pass) *pass = L.pass;
+  return SU2HIP_OK;
+}
+
+int su2hip_rng_set(su2hip_batch_t *b, int slot, uint64_t seed, uint64_t pass) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  LatDesc &L = b->slots[slot].hd;
+  L.key0 = (uint32_t)seed;
+  L.key1 = (uint32_t)(seed >> 32);
+  L.pass = pass;
+  b->slots[slot].p.seed = seed;
+  return push_desc(b);
+}
+
+int su2hip_measure_all(su2hip_batch_t *b, double *sums) {
+  if (!b || !sums) return fail(SU2HIP_EINVAL, "NULL argument");
+  int rc = prepare(b);
+  if (rc) return rc;
+  LaunchCtx c = ctx_of(b);
+  hipError_t e = timed(b, SU2HIP_K_MEASURE, [&] { return launch_measure(c); });
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "measure: %s", hipGetErrorString(e));
+  HIPTRY(hipMemcpyAsync(sums, b->d_sums, sizeof(double) * SU2HIP_NSUMS * b->max_slots,
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return SU2HIP_OK;
+}
+
+int su2hip_measure(su2hip_batch_t *b, int slot, double *sums) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!sums) return fail(SU2HIP_EINVAL, "NULL sums");
+  std::vector<double> all((size_t)SU2HIP_NSUMS * b->max_slots);
+  rc = su2hip_measure_all(b, all.data());
+  if (rc) return rc;
+  memcpy(sums, all.data() + (size_t)slot * SU2HIP_NSUMS, sizeof(double) * SU2HIP_NSUMS);
+  return SU2HIP_OK;
+}
+
+// column values in the order of measure()'s fprintf (src/measure.c:252-306)
+int su2hip_measure_columns(const su2hip_params *p, const double *s, double weight,
+                           double muca_param, double *cols, int *ncols) {
+  if (!p || !s || !cols || !ncols) return fail(SU2HIP_EINVAL, "NULL argument");
+  double vol = (double)((long)p->L[0] * p->L[1] * p->L[2]);
+  int n = 0;
+  cols[n++] = weight;
+  cols[n++] = muca_param;
+  cols[n++] = s[SU2HIP_SUM_ACTION];
+  cols[n++] = s[SU2HIP_SUM_WILSON] / (vol * p->betasu2);
+  if (p->u1) cols[n++] = s[SU2HIP_SUM_U1WILSON] / (vol * p->betau1);
+  if (p->nhiggs) {
+    cols[n++] = s[SU2HIP_SUM_COV_PHI] / (vol * 3);
+    cols[n++] = s[SU2HIP_SUM_PHI2] / vol;
+    cols[n++] = s[SU2HIP_SUM_PHI4] / vol;
+  }
+  if (p->triplet) {
+    cols[n++] = s[SU2HIP_SUM_HOP_SIGMA] / (vol * 3);
+    cols[n++] = s[SU2HIP_SUM_SIGMA2] / vol;
+    cols[n++] = s[SU2HIP_SUM_SIGMA4] / vol;
+    if (p->nhiggs) cols[n++] = s[SU2HIP_SUM_PHI2SIGMA2] / vol;
+    cols[n++] = s[SU2HIP_SUM_MAGCHARGE];
+    cols[n++] = s[SU2HIP_SUM_MAGCHARGE_ABS] / (2.0 * M_PI * sqrt(p->betasu2));
+  }
+  if (p->singlet) {
+    cols[n++] = s[SU2HIP_SUM_S_COV] / (vol * 3);
+    cols[n++] = s[SU2HIP_SUM_S] / vol;
+    cols[n++] = s[SU2HIP_SUM_S2] / vol;
+    cols[n++] = s[SU2HIP_SUM_S3] / vol;
+    cols[n++] = s[SU2HIP_SUM_S4] / vol;
+    if (p->nhiggs == 1) {
+      cols[n++] = s[SU2HIP_SUM_SPHISQ] / vol;
+      cols[n++] = s[SU2HIP_SUM_S2PHISQ] / vol;
+    }
+  }
+  *ncols = n;
+  return SU2HIP_OK;
+}
+
+int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m, const double *pos,
+                    const double *W) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!m) return fail(SU2HIP_EINVAL, "NULL muca");
+  Slot &s = b->slots[slot];
+  if (m->enabled) {
+    if (!pos || !W) return fail(SU2HIP_EINVAL, "pos/W required");
+    if (m->bins < 2) return fail(SU2HIP_EINVAL, "bins = %d", m->bins);
+    if (m->checks_per_sweep < 1) return fail(SU2HIP_EINVAL, "checks_per_sweep = %d", m->checks_per_sweep);
+    if (m->mode < SU2HIP_READONLY || m->mode > SU2HIP_SLOW) return fail(SU2HIP_EINVAL, "mode %d", m->mode);
+    int op = m->orderparam;
+    bool ok = (op == SU2HIP_PHISQ && s.p.nhiggs) || (op == SU2HIP_SIGMASQ && s.p.triplet) ||
+              (op == SU2HIP_PHI2MINUSSIGMA2 && s.p.nhiggs && s.p.triplet);
+    if (!ok) return fail(SU2HIP_EINVAL, "order parameter %d needs a field this model lacks", op);
+  }
+  HIPTRY(hipStreamSynchronize(b->stream));
+  if (s.muca_mem) {
+    HIPTRY(hipFree(s.muca_mem));
+    s.muca_mem = nullptr;
+  }
+  MucaDev h{};
+  if (m->enabled) {
+    // backup of the order-parameter field (dofs of the doublet or triplet)
+    size_t Vh = (size_t)vol_of(s.p) / 2;
+    size_t bub = align256((s.p.nhiggs ? 8 : 6) * Vh * wsize(s.p));
+    if (!s.bu_mem) {
+      if (b->hbm_budget && b->used + bub > b->hbm_budget)
+        return fail(SU2HIP_ENOMEM, "HBM budget exceeded by the multicanonical backup");
+      if (hipMalloc(&s.bu_mem, bub) != hipSuccess) return fail(SU2HIP_ENOMEM, "backup allocation");
+      s.bu_bytes = bub;
+      b->used += bub;
+      s.hd.B = s.bu_mem;
+    }
+    int n = m->bins + 2;
+    size_t bytes = sizeof(double) * n * 6 + sizeof(int) * n + sizeof(long long) * n * 2 + 1024;
+    if (hipMalloc(&s.muca_mem, bytes) != hipSuccess) return fail(SU2HIP_ENOMEM, "weight arrays");
+    s.muca_bytes = bytes;
+    char *q = (char *)s.muca_mem;
+    auto take = [&](size_t sz) {
+      void *r = q;
+      q += (sz + 15) & ~(size_t)15;
+      return r;
+    };
+    h.pos = (double *)take(sizeof(double) * n);
+    h.W = (double *)take(sizeof(double) * n);
+    h.slope = (double *)take(sizeof(double) * n);
+    h.b = (double *)take(sizeof(double) * n);
+    h.hgram = (double *)take(sizeof(double) * n * 2);
+    h.hits = (int *)take(sizeof(int) * n);
+    h.gsum = (long long *)take(sizeof(long long) * n);
+    h.nsum = (long long *)take(sizeof(long long) * n);
+    std::vector<double> slope(n), bb(n), hg(2 * n, 0.0);
+    std::vector<int> hits(n, 0);
+    std::vector<long long> gs(n, 5), ns(n, 1);
+    // linearize_weight (src/multicanonical.c:337-346)
+    slope[0] = 0;
+    slope[n - 1] = 0;
+    bb[0] = W[0];
+    bb[n - 1] = W[n - 1];
+    for (int i = 1; i < n - 1; i++) {
+      slope[i] = (W[i + 1] - W[i]) / (pos[i + 1] - pos[i]);
+      bb[i] = W[i] - slope[i] * pos[i];
+    }
+    if (m->mode == SU2HIP_SLOW)
+      for (int i = 0; i < n; i++) hg[i] = exp(W[i]);
+    HIPTRY(hipMemcpy(h.pos, pos, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(h.W, W, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(h.slope, slope.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(h.b, bb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(h.hgram, hg.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(h.hits, hits.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(h.gsum, gs.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(h.nsum, ns.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    h.enabled = 1;
+    h.orderparam = m->orderparam;
+    h.mode = m->mode;
+    h.cps = m->checks_per_sweep;
+    h.bins = m->bins;
+    h.last_max = m->last_max;
+    h.do_acceptance = m->do_acceptance;
+    h.muca_count = m->muca_count;
+    h.update_interval = m->update_interval > 0 ? m->update_interval
+                                               : (m->mode == SU2HIP_SLOW ? 5000 : 100);
+    h.delta = m->delta;
+    h.min = m->min;
+    h.max = m->max;
+    h.wrk_min = m->wrk_min;
+    h.wrk_max = m->wrk_max;
+  }
+  s.hm = h;
+  rc = push_muca(b, slot);
+  if (rc) return rc;
+  b->sched_dirty = true;
+  b->graph_valid = false;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = push_desc(b);
+  if (rc) return rc;
+  return orderparam_refresh(b);
+}
+
+int su2hip_muca_get(su2hip_batch_t *b, int slot, su2hip_muca *m, double *pos, double *W,
+                    int32_t *hits) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!m) return fail(SU2HIP_EINVAL, "NULL muca");
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  const MucaDev &h = b->slots[slot].hm;
+  memset(m, 0, sizeof(*m));
+  m->enabled = h.enabled;
+  m->orderparam = h.orderparam;
+  m->mode = h.mode;
+  m->checks_per_sweep = h.cps;
+  m->bins = h.bins;
+  m->last_max = h.last_max;
+  m->do_acceptance = h.do_acceptance;
+  m->muca_count = h.muca_count;
+  m->update_interval = h.update_interval;
+  m->delta = h.delta;
+  m->min = h.min;
+  m->max = h.max;
+  m->wrk_min = h.wrk_min;
+  m->wrk_max = h.wrk_max;
+  m->param_value[0] = h.pv[0];
+  m->param_value[1] = h.pv[1];
+  if (h.enabled) {
+    int n = h.bins + 2;
+    if (pos) HIPTRY(hipMemcpy(pos, h.pos, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (W) HIPTRY(hipMemcpy(W, h.W, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (hits) HIPTRY(hipMemcpy(hits, h.hits, sizeof(int) * n, hipMemcpyDeviceToHost));
+  }
+  return SU2HIP_OK;
+}
+
+int su2hip_muca_weight_of(su2hip_batch_t *b, int slot, double v, double *weight) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!weight) return fail(SU2HIP_EINVAL, "NULL weight");
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  const MucaDev &h = b->slots[slot].hm;
+  if (!h.enabled) return fail(SU2HIP_ESTATE, "no weight installed");
+  int n = h.bins + 2;
+  std::vector<double> pos(n), bb(n), sl(n);
+  HIPTRY(hipMemcpy(pos.data(), h.pos, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIPTRY(hipMemcpy(bb.data(), h.b, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIPTRY(hipMemcpy(sl.data(), h.slope, sizeof(double) * n, hipMemcpyDeviceToHost));
+  // get_weight / whichbin (src/multicanonical.c:350-379, :671-704)
+  if (v >= h.max) {
+    *weight = bb[h.bins + 1];
+    return SU2HIP_OK;
+  }
+  if (v < h.min) {
+    *weight = bb[0];
+    return SU2HIP_OK;
+  }
+  int lo = 1, hi = h.bins, bin = h.bins;
+  while (lo <= hi) {
+    int k = lo + (hi - lo) / 2;
+    if ((v < pos[k + 1] && v > pos[k]) || fabs(v - pos[k]) < 1e-12) {
+      bin = k;
+      break;
+    }
+    if (v > pos[k])
+      lo = k + 1;
+    else
+      hi = k - 1;
+  }
+  *weight = bin >= h.bins + 1 ? bb[h.bins + 1] : bb[bin] + sl[bin] * v;
+  return SU2HIP_OK;
+}
+
+// remove_counterterms (src/gradflow.c:424-459): tree-level triplet mass
+static double triplet_flow_msq(const su2hip_params &p) {
+  double Sg = 3.17591153625, zeta = 0.08849, delta = 1.942130, rho = -0.313964;
+  double k1 = 0.958382, k2 = 0.25 * Sg * Sg - 0.5 * delta - 0.25;
+  double k3 = 0.751498, k4 = 1.204295;
+  double gsq = 4.0 / p.betasu2, RG = gsq, b4 = p.b4;
+  double ct = -(4.0 * gsq + 5.0 * b4) * Sg / (4.0 * M_PI);
+  ct += -1.0 / (16.0 * M_PI * M_PI) *
+        ((20.0 * b4 * gsq - 10.0 * b4 * b4) * (log(6.0 / (RG)) + zeta) +
+         20.0 * b4 * gsq * (0.25 * Sg * Sg - delta) +
+         2.0 * gsq * gsq *
+             (1.25 * Sg * Sg + M_PI / 3.0 * Sg - 6.0 * delta - 6.0 * rho + 4.0 * k1 - k2 - k3 -
+              3.0 * k4));
+  return p.msq_triplet - ct;
+}
+
+// a one-lattice launch context for `slot` with the given descriptor array
+static int single_ctx(su2hip_batch *b, int slot, LatDesc *ddesc, BlockMap *dblk, int nb,
+                      LaunchCtx &c) {
+  c = ctx_of(b);
+  c.desc = ddesc;
+  c.blk = dblk;
+  c.nblk = nb;
+  (void)slot;
+  return SU2HIP_OK;
+}
+
+int su2hip_flow_force(su2hip_batch_t *b, int slot, double *force) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!force) return fail(SU2HIP_EINVAL, "NULL force");
+  rc = prepare(b);
+  if (rc) return rc;
+  const Slot &s = b->slots[slot];
+  size_t V = vol_of(s.p);
+  rc = ensure_stage(b, V * 12);
+  if (rc) return rc;
+  int nb = (s.hd.Vh + BLK - 1) / BLK;
+  std::vector<BlockMap> bm(nb);
+  for (int k = 0; k < nb; k++) bm[k] = BlockMap{slot, k * BLK};
+  BlockMap *dblk = nullptr;
+  if (hipMalloc(&dblk, sizeof(BlockMap) * nb) != hipSuccess) return fail(SU2HIP_ENOMEM, "block table");
+  hipError_t e = hipMemcpy(dblk, bm.data(), sizeof(BlockMap) * nb, hipMemcpyHostToDevice);
+  LaunchCtx c;
+  single_ctx(b, slot, b->d_desc, dblk, nb, c);
+  if (e == hipSuccess) e = launch_flow_force(c, slot, b->d_stage);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(force, b->d_stage, sizeof(double) * V * 12, hipMemcpyDeviceToHost);
+  (void)hipFree(dblk);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "flow force: %s", hipGetErrorString(e));
+  return SU2HIP_OK;
+}
+
+int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max,
+                int meas_interval, double *rows, int max_rows, int *nrows) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (scheme != SU2HIP_FLOW_EULER && scheme != SU2HIP_FLOW_RK3)
+    return fail(SU2HIP_EINVAL, "flow scheme %d", scheme);
+  if (!(dt > 0) || meas_interval < 1) return fail(SU2HIP_EINVAL, "dt/meas_interval");
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  Slot &s = b->slots[slot];
+  const su2hip_params &p = s.p;
+  size_t Vh = (size_t)vol_of(p) / 2, w = wsize(p);
+  if (!s.flow_mem) {
+    size_t fb = align256(flow_words(p) * w) + 4096;
+    if (b->hbm_budget && b->used + fb > b->hbm_budget)
+      return fail(SU2HIP_ENOMEM, "HBM budget exceeded by the flow buffers");
+    if (hipMalloc(&s.flow_mem, fb) != hipSuccess) return fail(SU2HIP_ENOMEM, "flow buffers");
+    s.flow_bytes = fb;
+    b->used += fb;
+  }
+  char *q = (char *)s.flow_mem;
+  auto take = [&](size_t words) {
+    void *r = q;
+    q += align256(words * w);
+    return r;
+  };
+  void *Ub[2] = {take(24 * Vh), take(24 * Vh)};
+  void *Z = take(18 * Vh);
+  void *Gb[2] = {nullptr, nullptr}, *ZG = nullptr;
+  if (p.triplet) {
+    Gb[0] = take(6 * Vh);
+    Gb[1] = take(6 * Vh);
+    ZG = take(6 * Vh);
+  }
+  HIPTRY(hipMemcpyAsync(Ub[0], s.hd.U, 24 * Vh * w, hipMemcpyDeviceToDevice, b->stream));
+  if (p.triplet) HIPTRY(hipMemcpyAsync(Gb[0], s.hd.G, 6 * Vh * w, hipMemcpyDeviceToDevice, b->stream));
+  double msq_t = p.triplet ? triplet_flow_msq(p) : p.msq_triplet;
+  // two descriptor arrays (links in buffer 0 / 1), only `slot` active
+  int nb = (int)((Vh + BLK - 1) / BLK);
+  std::vector<LatDesc> hd(2 * (size_t)b->max_slots);
+  for (int k = 0; k < 2; k++)
+    for (int i = 0; i < b->max_slots; i++) {
+      LatDesc &L = hd[(size_t)k * b->max_slots + i];
+      if (i == slot) {
+        L = s.hd;
+        L.U = Ub[k];
+        if (p.triplet) L.G = Gb[k];
+        L.msq_t = msq_t;
+        L.blk_base = 0;
+        L.nblk = nb;
+        L.active = 1;
+      } else {
+        memset(&L, 0, sizeof(L));
+      }
+    }
+  std::vector<BlockMap> bm(nb);
+  for (int k = 0; k < nb; k++) bm[k] = BlockMap{slot, k * BLK};
+  LatDesc *ddesc = nullptr;
+  BlockMap *dblk = nullptr;
+  if (hipMalloc(&ddesc, sizeof(LatDesc) * hd.size()) != hipSuccess ||
+      hipMalloc(&dblk, sizeof(BlockMap) * nb) != hipSuccess) {
+    if (ddesc) (void)hipFree(ddesc);
+    return fail(SU2HIP_ENOMEM, "flow descriptors");
+  }
+  hipError_t e = hipMemcpyAsync(ddesc, hd.data(), sizeof(LatDesc) * hd.size(), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dblk, bm.data(), sizeof(BlockMap) * nb, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  int cur = 0, nr = 0;
+  double t = 0.0;
+  std::vector<double> sums((size_t)SU2HIP_NSUMS * b->max_slots);
+  auto row = [&]() -> hipError_t {
+    LaunchCtx c;
+    single_ctx(b, slot, ddesc + (size_t)cur * b->max_slots, dblk, nb, c);
+    hipError_t r = timed(b, SU2HIP_K_MEASURE, [&] { return launch_measure(c); });
+    if (r != hipSuccess) return r;
+    r = hipMemcpyAsync(sums.data(), b->d_sums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost,
+                       b->stream);
+    if (r != hipSuccess) return r;
+    r = hipStreamSynchronize(b->stream);
+    if (r != hipSuccess) return r;
+    if (nr < max_rows && rows) {
+      double *o = rows + (size_t)nr * (1 + SU2HIP_NSUMS);
+      o[0] = t;
+      memcpy(o + 1, sums.data() + (size_t)slot * SU2HIP_NSUMS, sizeof(double) * SU2HIP_NSUMS);
+    }
+    nr++;
+    return hipSuccess;
+  };
+  if (e == hipSuccess) e = row();
+  int iter = 1;
+  while (e == hipSuccess && t < t_max) {
+    // grad_flow's time loop (src/gradflow.c:91-141)
+    if (t + dt > t_max) dt = t_max - t;
+    int stages = scheme == SU2HIP_FLOW_EULER ? 1 : 3;
+    for (int st = 0; st < stages && e == hipSuccess; st++) {
+      LaunchCtx c;
+      single_ctx(b, slot, ddesc + (size_t)cur * b->max_slots, dblk, nb, c);
+      int stage = scheme == SU2HIP_FLOW_EULER ? 0 : st + 1;
+      e = timed(b, SU2HIP_K_FLOW, [&] {
+        return launch_flow_stage(c, slot, Ub[cur], Ub[1 - cur], Gb[cur], Gb[1 - cur], Z, ZG, dt,
+                                 msq_t, stage);
+      });
+      cur = 1 - cur;
+    }
+    t += dt;
+    if (e == hipSuccess && iter % meas_interval == 0) e = row();
+    iter++;
+  }
+  if (e == hipSuccess && (iter - 1) % meas_interval != 0) e = row();
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(ddesc);
+  (void)hipFree(dblk);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "flow: %s", hipGetErrorString(e));
+  if (nrows) *nrows = nr;
+  return SU2HIP_OK;
+}
+
+int su2hip_set_timing(su2hip_batch_t *b, int enable) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  b->timing = enable != 0;
+  return SU2HIP_OK;
+}
+
+static int resolve_events(su2hip_batch *b) {
+  if (b->pending.empty()) return SU2HIP_OK;
+  HIPTRY(hipStreamSynchronize(b->stream));
+  for (auto &ev : b->pending) {
+    float ms = 0.f;
+    HIPTRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+    b->kms[ev.kind] += ms;
+    b->kcount[ev.kind]++;
+    b->pool.push_back(ev.a);
+    b->pool.push_back(ev.b);
+  }
+  b->pending.clear();
+  return SU2HIP_OK;
+}
+
+int su2hip_kernel_times(su2hip_batch_t *b, double *ms, int64_t *launches) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  int rc = resolve_events(b);
+  if (rc) return rc;
+  for (int k = 0; k < SU2HIP_NKINDS; k++) {
+    if (ms) ms[k] = b->kms[k];
+    if (launches) launches[k] = b->kcount[k];
+  }
+  return SU2HIP_OK;
+}
+
+int su2hip_reset_timing(su2hip_batch_t *b) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  int rc = resolve_events(b);
+  if (rc) return rc;
+  for (int k = 0; k < SU2HIP_NKINDS; k++) {
+    b->kms[k] = 0;
+    b->kcount[k] = 0;
+  }
+  return SU2HIP_OK;
+}
+
+int su2hip_copy_probe(uint64_t bytes, int repeats, double *ms) {
+  if (!ms || bytes < 16 || repeats < 1) return fail(SU2HIP_EINVAL, "copy probe arguments");
+  size_t n = bytes / sizeof(double);
+  double *a = nullptr, *c = nullptr;
+  if (hipMalloc(&a, n * sizeof(double)) != hipSuccess) return fail(SU2HIP_ENOMEM, "probe src");
+  if (hipMalloc(&c, n * sizeof(double)) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(SU2HIP_ENOMEM, "probe dst");
+  }
+  hipStream_t s;
+  hipEvent_t e0, e1;
+  HIPTRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIPTRY(hipEventCreate(&e0));
+  HIPTRY(hipEventCreate(&e1));
+  HIPTRY(hipMemsetAsync(a, 0, n * sizeof(double), s));
+  HIPTRY(launch_copy_probe(a, c, n, s));
+  HIPTRY(hipEventRecord(e0, s));
+  for (int r = 0; r < repeats; r++) HIPTRY(launch_copy_probe(a, c, n, s));
+  HIPTRY(hipEventRecord(e1, s));
+  HIPTRY(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPTRY(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / repeats;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  (void)hipFree(a);
+  (void)hipFree(c);
+  return SU2HIP_OK;
+}
+
+}  // extern "C"

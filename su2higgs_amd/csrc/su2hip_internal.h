@@ -1,0 +1,143 @@
+// su2hip_internal.h -- device data structures shared by the kernels and the
+// C ABI layer of libsu2hip.so.
+//
+// Device layout of one lattice (structure of arrays, checkerboarded).  With
+// Vh = V/2 sites per parity and s the index of a site inside its parity
+// (s = natural_index >> 1 for even L0, which is exactly the reference's
+// "even sites first, order preserved" site order, src/layout.c:826-851):
+//   links    U[d][c][p][s]   d<3, c<4    (24 Vh words)
+//   U(1)     A[d][p][s]                  (6 Vh)
+//   doublet  F[c][p][s]      c<4         (8 Vh)
+//   triplet  G[c][p][s]      c<3         (6 Vh)
+//   singlet  S[p][s]                     (2 Vh)
+//   backup   B[c][p][s]      dofs of the multicanonical field
+// so a wavefront reads each component of 64 consecutive sites of one parity
+// as one contiguous segment.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/su2hip.h"
+
+namespace su2 {
+
+constexpr int BLK = 256;          // threads per workgroup in every site kernel
+constexpr int NCOUNT = 10;        // counted pass kinds (SU2HIP_K_LINK_HEATBATH .. SINGLET_METROPOLIS)
+constexpr uint32_t MUCA_SITE = 0xFFFFFFFFu;
+constexpr uint64_t HOTSTART_PASS = 0xFFFFFFFFFFFFFFFFull;
+
+// model bits
+enum : int { M_NH = 1, M_U1 = 2, M_TR = 4, M_SG = 8 };
+
+inline int model_bits(const su2hip_params &p) {
+  return (p.nhiggs ? M_NH : 0) | (p.u1 ? M_U1 : 0) | (p.triplet ? M_TR : 0) |
+         (p.singlet ? M_SG : 0);
+}
+
+// fast division by a runtime constant (Granlund-Montgomery, round-up
+// variant): q = (umulhi(m, n) + n) >> l for n < 2^31
+struct FastDiv {
+  uint32_t m, l, d;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  f.l = l;
+  f.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+  return f;
+}
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t t = __umulhi(f.m, n);
+#else
+  uint32_t t = (uint32_t)(((uint64_t)f.m * n) >> 32);
+#endif
+  return (uint32_t)(((uint64_t)t + n) >> f.l);
+}
+
+// Per-lattice descriptor, resident in device memory.
+struct LatDesc {
+  void *U, *A, *F, *G, *S, *B;  // field arrays (element type = precision)
+  int L0, L1, L2, h0;           // extents, h0 = L0/2
+  int Vh, vol;                  // sites per parity, volume
+  FastDiv div_h0, div_L1;
+  int active;
+  int pad0;
+  int64_t budget;               // sweeps left (-1 unlimited, 0 = skip)
+  uint64_t pass;                // Philox pass counter at the start of the sweep
+  uint32_t key0, key1;          // Philox key
+  int blk_base;                 // first logical block of this lattice
+  int nblk;                     // logical blocks of this lattice
+  double beta, betau1, r_u1, msq, lambda, msq_t, a2, b4;
+  double b1_s, msq_s, b3_s, b4_s, a1_s, a2_s;
+};
+
+// Multicanonical state of one lattice on the device.
+struct MucaDev {
+  int enabled, orderparam, mode, cps;
+  int bins, last_max, do_acceptance, muca_count;
+  int update_interval, rejected;
+  uint32_t check;
+  int pad;
+  double delta, min, max, wrk_min, wrk_max;
+  double pv[2];
+  long long total, accepted;
+  double *pos, *W, *slope, *b, *hgram;
+  int *hits;
+  long long *gsum, *nsum;
+};
+
+// One logical chunk of BLK sites of one lattice.
+struct BlockMap {
+  int lat;
+  int s0;
+};
+
+// ---- launch interface (implemented in su2hip_kernels.hip) ----
+struct LaunchCtx {
+  hipStream_t stream;
+  LatDesc *desc;       // device, [max_slots]
+  MucaDev *muca;       // device, [max_slots]
+  const BlockMap *blk; // device, [nblk]
+  int nblk;
+  long long *cnt;      // device, [NCOUNT][2][nblk]
+  double *partial;     // device, [nblk][SU2HIP_NSUMS]
+  double *sums;        // device, [max_slots][SU2HIP_NSUMS]
+  int max_slots;
+  int model;
+  int fp64;
+};
+
+// One checkerboard pass.  muca_step < 0: plain pass; otherwise the pass is
+// step `muca_step` of a multicanonical parity sweep.
+hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir,
+                       int pass_off, int muca_step, int reunit);
+// multicanonical global accept/reject after a scalar pass step
+hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int parity,
+                              int pass_off, int muca_step);
+// sweep epilogue: pass counter += passes, budget -= 1
+hipError_t launch_end_sweep(const LaunchCtx &c, int passes);
+hipError_t launch_measure(const LaunchCtx &c);
+hipError_t launch_orderparam_init(const LaunchCtx &c);
+hipError_t launch_cold(const LaunchCtx &c, int slot, double phi0, double sigma0,
+                       double singlet0);
+hipError_t launch_hot(const LaunchCtx &c, int slot);
+hipError_t launch_to_device_layout(const LaunchCtx &c, int slot, int field,
+                                   const double *host_layout_dev);
+hipError_t launch_to_host_layout(const LaunchCtx &c, int slot, int field,
+                                 double *host_layout_dev);
+hipError_t launch_copy_probe(const double *src, double *dst, size_t n,
+                             hipStream_t s);
+// gradient flow on one lattice (slot) whose descriptor in c.desc points at
+// the flow buffers: stage 0 = Euler, 1..3 = RK3 stages
+hipError_t launch_flow_stage(const LaunchCtx &c, int slot, const void *Uin,
+                             void *Uout, const void *Gin, void *Gout, void *Z,
+                             void *ZG, double dt, double msq_t, int stage);
+hipError_t launch_flow_force(const LaunchCtx &c, int slot, double *force);
+
+}  // namespace su2

@@ -1,0 +1,1496 @@
+// su2hip_kernels.hip -- HIP kernels of the checkerboard update sweep for
+// gfx950 (MI355X), and their launchers.
+//
+// One thread updates one site of one parity; a workgroup of BLK threads owns
+// BLK consecutive sites (in the device site order) of one lattice, and the
+// block table maps workgroups to (lattice, first site) for every lattice of
+// the batch, in an XCD-aware order built by the host.  Every per-site
+// routine names the reference function it restates (file:line under the
+// reference's src/).  The SU(2) algebra is quaternion products; nothing here
+// is a dense contraction, so MFMA is not used and the kernels are bound by
+// HBM traffic and FP math.
+#include <type_traits>
+
+#include "su2hip_device.h"
+
+namespace su2 {
+
+// ------------------------------------------------------------------------
+// math wrappers (float / double overloads)
+
+template <typename T> __device__ __forceinline__ T msqrt(T x) { if constexpr (sizeof(T) == 8) return ::sqrt(x); else return ::sqrtf(x); }
+template <typename T> __device__ __forceinline__ T mlog(T x) { if constexpr (sizeof(T) == 8) return ::log(x); else return ::logf(x); }
+template <typename T> __device__ __forceinline__ T mexp(T x) { if constexpr (sizeof(T) == 8) return ::exp(x); else return ::expf(x); }
+template <typename T> __device__ __forceinline__ T mcos(T x) { if constexpr (sizeof(T) == 8) return ::cos(x); else return ::cosf(x); }
+template <typename T> __device__ __forceinline__ T msin(T x) { if constexpr (sizeof(T) == 8) return ::sin(x); else return ::sinf(x); }
+
+constexpr double PI = 3.14159265358979323846;
+
+template <int M, int KIND>
+constexpr bool kind_valid() {
+  if (KIND == SU2HIP_K_LINK_HEATBATH || KIND == SU2HIP_K_LINK_METROPOLIS ||
+      KIND == SU2HIP_K_LINK_OVERRELAX)
+    return true;
+  if (KIND == SU2HIP_K_U1_METROPOLIS) return (M & M_U1) != 0;
+  if (KIND == SU2HIP_K_DOUBLET_OVERRELAX || KIND == SU2HIP_K_DOUBLET_METROPOLIS)
+    return (M & M_NH) != 0;
+  if (KIND == SU2HIP_K_TRIPLET_OVERRELAX || KIND == SU2HIP_K_TRIPLET_METROPOLIS)
+    return (M & M_TR) != 0;
+  if (KIND == SU2HIP_K_SINGLET_OVERRELAX || KIND == SU2HIP_K_SINGLET_METROPOLIS)
+    return (M & M_SG) != 0;
+  return false;
+}
+
+// ------------------------------------------------------------------------
+// local pieces of the action
+
+// su2link_staple (src/staples.c:134-193) with su2staple_wilson (:84-115):
+// -beta/2 sum_nu [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+
+// U_mu(x-nu)^+ U_nu(x-nu)] - 1/2 Phi(x+mu) exp(-i alpha sigma_3) Phi(x)^+
+template <typename T>
+__device__ __forceinline__ Q<T> wilson_staple(const LatDesc &L, const T *U, const Site &x, int mu) {
+  Q<T> W{(T)0, (T)0, (T)0, (T)0};
+  Site xp = shift(L, x, mu, +1);
+#pragma unroll
+  for (int nu = 0; nu < 3; nu++) {
+    if (nu == mu) continue;
+    Site xn = shift(L, x, nu, +1), xm = shift(L, x, nu, -1), xpm = shift(L, xp, nu, -1);
+    Q<T> up = qmul(qmul(U_at(L, U, xp, nu), qconj(U_at(L, U, xn, mu))), qconj(U_at(L, U, x, nu)));
+    W = qadd(W, up);
+    Q<T> lo = qmul(qmul(qconj(U_at(L, U, xpm, nu)), qconj(U_at(L, U, xm, mu))), U_at(L, U, xm, nu));
+    W = qadd(W, lo);
+  }
+  return W;
+}
+
+template <typename T, int M>
+__device__ __forceinline__ Q<T> link_staple(const LatDesc &L, const T *U, const T *A, const T *F,
+                                            const Site &x, int mu) {
+  Q<T> V = qscale(wilson_staple(L, U, x, mu), (T)(-0.5 * L.beta));
+  if constexpr ((M & M_NH) != 0) {
+    Site xp = shift(L, x, mu, +1);
+    Q<T> b = F_at(L, F, xp);
+    if constexpr ((M & M_U1) != 0) b = qmul(b, u1twist<T>(A_at(L, A, x, mu)));
+    Q<T> t = qmul(b, qconj(F_at(L, F, x)));
+    V = qadd(V, qscale(t, (T)-0.5));
+  }
+  return V;
+}
+
+// staple_doublet (src/staples.c:243-302)
+template <typename T, int M>
+__device__ __forceinline__ Q<T> doublet_staple(const LatDesc &L, const T *U, const T *A, const T *F,
+                                               const Site &x) {
+  Q<T> s{(T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);
+    Q<T> t = qmul(U_at(L, U, x, d), F_at(L, F, xp));
+    if constexpr ((M & M_U1) != 0) t = qmul(t, u1twist<T>(A_at(L, A, x, d)));
+    s = qadd(s, qscale(t, (T)-1));
+    Q<T> t2 = qmul(qconj(U_at(L, U, xm, d)), F_at(L, F, xm));
+    if constexpr ((M & M_U1) != 0) t2 = qmul(t2, qconj(u1twist<T>(A_at(L, A, xm, d))));
+    s = qadd(s, qscale(t2, (T)-1));
+  }
+  return s;
+}
+
+// hopping staple of the triplet (src/overrelax.c:143-180)
+template <typename T>
+__device__ __forceinline__ V3<T> triplet_staple(const LatDesc &L, const T *U, const T *G, const Site &x) {
+  V3<T> s{(T)0, (T)0, (T)0};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);
+    V3<T> r = rotate(U_at(L, U, x, d), G_at(L, G, xp));
+    s.x -= r.x;
+    s.y -= r.y;
+    s.z -= r.z;
+    r = rotate(qconj(U_at(L, U, xm, d)), G_at(L, G, xm));
+    s.x -= r.x;
+    s.y -= r.y;
+    s.z -= r.z;
+  }
+  return s;
+}
+
+// hopping_triplet_forward (src/su2u1.c:605-617) with link u
+template <typename T>
+__device__ __forceinline__ T hop_triplet(const Q<T> &u, const V3<T> &a, const V3<T> &b) {
+  V3<T> r = rotate(u, b);
+  return -(a.x * r.x + a.y * r.y + a.z * r.z);
+}
+
+// polysolve3 (src/overrelax.c:14-34): real root by Cardano's formula for a
+// negative discriminant, else b/a as the reference returns
+__device__ __forceinline__ double polysolve3(double a, double b, double c, double d) {
+  double disc = 18.0 * a * b * c * d - 4.0 * b * b * b * d + b * b * c * c -
+                4.0 * a * c * c * c - 27.0 * a * a * d * d;
+  double d1 = 2.0 * b * b * b - 9.0 * a * b * c + 27.0 * a * a * d;
+  if (disc < 0.0) {
+    double A = 0.5 * (d1 + a * ::sqrt(-27.0 * disc));
+    A = ::cbrt(A);
+    return -1.0 * (b + A + (b * b - 3.0 * a * c) / A) / (3.0 * a);
+  }
+  return b / a;
+}
+
+// Cartesian (X,Y) overrelaxation of src/overrelax.c:54-128 given the staple
+// norm F, X, |Y|^2 and the local B, C: returns the accept probability and
+// the new X in nX.
+__device__ __forceinline__ double xy_or(double F, double X, double Ysq, double B, double C,
+                                        double &nX) {
+  double al = C;
+  double be = C * X;
+  double cc = B + 2.0 * Ysq * C;
+  double ga = (cc + C * X * X);
+  double de = (F + cc * X + C * X * X * X);
+  nX = polysolve3(al, be, ga, de);
+  double dV = F + 2.0 * B * X + 4.0 * C * (X * X * X + Ysq * X);
+  double dVn = F + 2.0 * B * nX + 4.0 * C * (nX * nX * nX + Ysq * nX);
+  return ::fabs(dV / dVn);
+}
+
+// accept rule of every Metropolis update (src/metropolis.c:31-40): a draw
+// is taken only when the action grows
+template <typename T>
+__device__ __forceinline__ int metro_accept(T diff, Rng &r) {
+  if (diff < (T)0) return 1;
+  if (diff > (T)0 && mexp(-diff) > uni<T>(r)) return 1;
+  return 0;
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> normalise(const Q<T> &q) {
+  T n = (T)1 / msqrt(qdot(q, q));
+  return qscale(q, n);
+}
+
+// ------------------------------------------------------------------------
+// per-site updates; each returns 1 if accepted and stores the new value
+
+struct Fields {
+  void *U, *A, *F, *G, *S, *B;
+};
+
+template <typename T, int M, int KIND>
+__device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int par, int s, int mu,
+                                           Rng &r, int reunit) {
+  T *U = (T *)L.U;
+  T *A = (T *)L.A;
+  T *F = (T *)L.F;
+  T *G = (T *)L.G;
+  T *S = (T *)L.S;
+  const int Vh = L.Vh;
+
+  if constexpr (KIND == SU2HIP_K_LINK_HEATBATH) {
+    // heatbath_su2link (src/heatbath.c:15-124), Kennedy-Pendleton
+    Q<T> old{};
+    T oldact = 0;
+    V3<T> a1{}, a2{};
+    if constexpr ((M & M_TR) != 0) {
+      old = ldU(U, Vh, mu, par, s);
+      a1 = ldG(G, Vh, par, s);
+      a2 = G_at(L, G, shift(L, x, mu, +1));
+      oldact = hop_triplet(old, a1, a2);
+    }
+    Q<T> V = link_staple<T, M>(L, U, A, F, x, mu);
+    T a = msqrt(qdot(V, V));
+    V = qscale(V, (T)-1 / a);
+    a *= (T)2;
+    T r1, r2, r3, d, a0;
+    int loop = 1;
+    do {
+      r1 = -mlog((T)1 - uni<T>(r)) / a;
+      r2 = -mlog((T)1 - uni<T>(r)) / a;
+      r3 = mcos((T)(2.0 * PI) * uni<T>(r));
+      r3 *= r3;
+      d = r1 * r3 + r2;
+      r3 = (T)1 - uni<T>(r);
+      loop++;
+    } while (r3 * r3 > (T)1 - (T)0.5 * d && loop <= 200);
+    a0 = (loop > 200) ? (T)1e-9 : (T)1 - d;
+    T rad = (T)1 - a0 * a0;
+    if (rad < (T)0) return 0;
+    rad = msqrt(rad);
+    d = (T)2;
+    while (d > (T)1 || d == (T)0) {
+      r1 = (T)2 * uni<T>(r) - (T)1;
+      r2 = (T)2 * uni<T>(r) - (T)1;
+      r3 = (T)2 * uni<T>(r) - (T)1;
+      d = r1 * r1 + r2 * r2 + r3 * r3;
+    }
+    d = msqrt(d);
+    Q<T> nu{a0, r1 * (rad / d), r2 * (rad / d), r3 * (rad / d)};
+    nu = qmul(nu, qconj(V));
+    if constexpr ((M & M_TR) != 0) {
+      T diff = oldact - hop_triplet(nu, a1, a2);
+      if (!(uni<T>(r) < mexp(diff))) return 0;
+    }
+    stU(U, Vh, mu, par, s, nu);
+    return 1;
+  } else if constexpr (KIND == SU2HIP_K_LINK_METROPOLIS) {
+    // metro_su2link (src/metropolis.c:15-41), random_su2link (src/init.c:43-58);
+    // Delta S = 2 Re[(U'-U) V]_0 + Delta(triplet hop), from the staple
+    Q<T> old = ldU(U, Vh, mu, par, s);
+    Q<T> V = link_staple<T, M>(L, U, A, F, x, mu);
+    Q<T> R;
+    R.a = (T)10 * (uni<T>(r) - (T)0.5);
+    R.b = (T)1 * (uni<T>(r) - (T)0.5);
+    R.c = (T)1 * (uni<T>(r) - (T)0.5);
+    R.d = (T)1 * (uni<T>(r) - (T)0.5);
+    T n = msqrt(qdot(R, R));
+    R = Q<T>{R.a / n, R.b / n, R.c / n, R.d / n};
+    Q<T> nu = qmul(old, R);
+    if (reunit) nu = normalise(nu);
+    Q<T> du{nu.a - old.a, nu.b - old.b, nu.c - old.c, nu.d - old.d};
+    T diff = (T)2 * qmul(du, V).a;
+    if constexpr ((M & M_TR) != 0) {
+      V3<T> a1 = ldG(G, Vh, par, s), a2 = G_at(L, G, shift(L, x, mu, +1));
+      diff += hop_triplet(nu, a1, a2) - hop_triplet(old, a1, a2);
+    }
+    if (!metro_accept(diff, r)) return 0;
+    stU(U, Vh, mu, par, s, nu);
+    return 1;
+  } else if constexpr (KIND == SU2HIP_K_LINK_OVERRELAX) {
+    // SU(2) link overrelaxation, U' = W^+ U^+ W^+ with W = staple/|staple|
+    // (no reference counterpart); with a triplet a Metropolis test on its
+    // hopping term as in src/heatbath.c:107-120
+    Q<T> old = ldU(U, Vh, mu, par, s);
+    Q<T> V = link_staple<T, M>(L, U, A, F, x, mu);
+    T a = msqrt(qdot(V, V));
+    if (a == (T)0) return 1;
+    Q<T> w = qconj(Q<T>{V.a / a, V.b / a, V.c / a, V.d / a});
+    Q<T> nu = qmul(qmul(w, qconj(old)), w);
+    if (reunit) nu = normalise(nu);
+    if constexpr ((M & M_TR) != 0) {
+      V3<T> a1 = ldG(G, Vh, par, s), a2 = G_at(L, G, shift(L, x, mu, +1));
+      T diff = hop_triplet(old, a1, a2) - hop_triplet(nu, a1, a2);
+      if (!(uni<T>(r) < mexp(diff))) return 0;
+    }
+    stU(U, Vh, mu, par, s, nu);
+    return 1;
+  } else if constexpr (KIND == SU2HIP_K_U1_METROPOLIS) {
+    // metro_u1link (src/metropolis.c:48-70) with localact_u1link
+    // (src/su2u1.c:336-354) evaluated as a difference
+    T al = ldA(A, Vh, mu, par, s);
+    T na = al + (T)0.95 * (uni<T>(r) - (T)0.5);
+    T rr = (T)L.r_u1;
+    Site xp = shift(L, x, mu, +1);
+    T dS = 0;
+#pragma unroll
+    for (int nu = 0; nu < 3; nu++) {
+      if (nu == mu) continue;
+      Site xn = shift(L, x, nu, +1), xm = shift(L, x, nu, -1), xpm = shift(L, xp, nu, -1);
+      T rest_up = A_at(L, A, xp, nu) - A_at(L, A, xn, mu) - A_at(L, A, x, nu);
+      T rest_dn = A_at(L, A, xm, mu) + A_at(L, A, xpm, nu) - A_at(L, A, xm, nu);
+      dS += mcos(rr * (al + rest_up)) - mcos(rr * (na + rest_up));
+      dS += mcos(rr * (rest_dn - al)) - mcos(rr * (rest_dn - na));
+    }
+    dS *= (T)L.betau1;
+    if constexpr ((M & M_NH) != 0) {
+      // hop = -(q0 cos a + q3 sin a), q = Phi(x)^+ U_mu(x) Phi(x+mu)
+      Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F, xp));
+      dS -= q.a * (mcos(na) - mcos(al)) + q.d * (msin(na) - msin(al));
+    }
+    if (!metro_accept(dS, r)) return 0;
+    A[(size_t)(mu * 2 + par) * Vh + s] = na;
+    return 1;
+  } else if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX ||
+                       KIND == SU2HIP_K_DOUBLET_METROPOLIS) {
+    Q<T> f = ldF(F, Vh, par, s);
+    Q<T> st = doublet_staple<T, M>(L, U, A, F, x);
+    double B0 = 0.5 * L.msq + 1.0 * 3;
+    if constexpr ((M & M_TR) != 0) {
+      V3<T> g = ldG(G, Vh, par, s);
+      B0 += 0.5 * L.a2 * (0.5 * ((double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z));
+    }
+    if constexpr ((M & M_SG) != 0) {
+      double Sx = ldS(S, Vh, par, s);
+      B0 += 0.25 * L.a1_s * Sx + 0.25 * L.a2_s * Sx * Sx;
+    }
+    double C = 0.25 * L.lambda;
+    Q<T> nf;
+    int acc;
+    if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX) {
+      // overrelax_doublet (src/overrelax.c:54-128)
+      double s0 = st.a, s1 = st.b, s2 = st.c, s3 = st.d;
+      double Fn = ::sqrt(s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3);
+      double X = ((double)f.a * s0 + (double)f.b * s1 + (double)f.c * s2 + (double)f.d * s3) / Fn;
+      double y0 = f.a - X * s0 / Fn, y1 = f.b - X * s1 / Fn, y2 = f.c - X * s2 / Fn,
+             y3 = f.d - X * s3 / Fn;
+      double Ysq = y0 * y0 + y1 * y1 + y2 * y2 + y3 * y3;
+      double nX;
+      double p = xy_or(Fn, X, Ysq, B0, C, nX);
+      acc = p >= rng_u53(r);
+      if (!acc) return 0;
+      double k = (nX + X);
+      nf = Q<T>{(T)(-1.0 * f.a + k * s0 / Fn), (T)(-1.0 * f.b + k * s1 / Fn),
+                (T)(-1.0 * f.c + k * s2 / Fn), (T)(-1.0 * f.d + k * s3 / Fn)};
+    } else {
+      // metro_doublet (src/metropolis.c:77-106); Delta S of
+      // localact_doublet = (f'-f).s + B(|f'|^2-|f|^2) + C(|f'|^4-|f|^4)
+      nf = Q<T>{f.a + (T)1 * (uni<T>(r) - (T)0.5), f.b + (T)1 * (uni<T>(r) - (T)0.5),
+                f.c + (T)1 * (uni<T>(r) - (T)0.5), f.d + (T)1 * (uni<T>(r) - (T)0.5)};
+      T o2 = qdot(f, f), n2 = qdot(nf, nf);
+      T diff = (nf.a - f.a) * st.a + (nf.b - f.b) * st.b + (nf.c - f.c) * st.c +
+               (nf.d - f.d) * st.d + (T)B0 * (n2 - o2) + (T)C * (n2 * n2 - o2 * o2);
+      if (!metro_accept(diff, r)) return 0;
+    }
+    stF(F, Vh, par, s, nf);
+    return 1;
+  } else if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX ||
+                       KIND == SU2HIP_K_TRIPLET_METROPOLIS) {
+    V3<T> g = ldG(G, Vh, par, s);
+    V3<T> st = triplet_staple<T>(L, U, G, x);
+    double C = 0.25 * L.b4;
+    V3<T> ng;
+    if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX) {
+      // overrelax_triplet (src/overrelax.c:137-236); the portal term is not
+      // in B, as in the reference (src/overrelax.c:202)
+      double s0 = st.x, s1 = st.y, s2 = st.z;
+      double Fn = ::sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+      double X = ((double)g.x * s0 + (double)g.y * s1 + (double)g.z * s2) / Fn;
+      double y0 = g.x - X * s0 / Fn, y1 = g.y - X * s1 / Fn, y2 = g.z - X * s2 / Fn;
+      double Ysq = y0 * y0 + y1 * y1 + y2 * y2;
+      double B0 = 0.5 * L.msq_t + 1.0 * 3;
+      double nX;
+      double p = xy_or(Fn, X, Ysq, B0, C, nX);
+      if (!(p >= rng_u53(r))) return 0;
+      double k = (nX + X);
+      ng = V3<T>{(T)(-1.0 * g.x + k * s0 / Fn), (T)(-1.0 * g.y + k * s1 / Fn),
+                 (T)(-1.0 * g.z + k * s2 / Fn)};
+    } else {
+      // metro_triplet (src/metropolis.c:113-143) with localact_triplet
+      // (src/su2u1.c:657-679) as a difference
+      double B0 = 0.5 * L.msq_t + 1.0 * 3;
+      if constexpr ((M & M_NH) != 0) {
+        Q<T> f = ldF(F, Vh, par, s);
+        B0 += 0.5 * L.a2 * (0.5 * (double)qdot(f, f));
+      }
+      ng = V3<T>{g.x + (T)1 * (uni<T>(r) - (T)0.5), g.y + (T)1 * (uni<T>(r) - (T)0.5),
+                 g.z + (T)1 * (uni<T>(r) - (T)0.5)};
+      T o2 = g.x * g.x + g.y * g.y + g.z * g.z, n2 = ng.x * ng.x + ng.y * ng.y + ng.z * ng.z;
+      T diff = (ng.x - g.x) * st.x + (ng.y - g.y) * st.y + (ng.z - g.z) * st.z +
+               (T)B0 * (n2 - o2) + (T)C * (n2 * n2 - o2 * o2);
+      if (!metro_accept(diff, r)) return 0;
+    }
+    stG(G, Vh, par, s, ng);
+    return 1;
+  } else if constexpr (KIND == SU2HIP_K_SINGLET_OVERRELAX ||
+                       KIND == SU2HIP_K_SINGLET_METROPOLIS) {
+    double Sx = ldS(S, Vh, par, s);
+    double nb = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      nb += (double)S_at(L, S, shift(L, x, d, +1)) + (double)S_at(L, S, shift(L, x, d, -1));
+    double ph = 0.0;
+    if constexpr ((M & M_NH) != 0) {
+      Q<T> f = ldF(F, Vh, par, s);
+      ph = 0.5 * (double)qdot(f, f);
+    }
+    double nS;
+    if constexpr (KIND == SU2HIP_K_SINGLET_OVERRELAX) {
+      // overrelax_singlet (src/overrelax.c:243-285)
+      double c1 = L.b1_s - nb;
+      double c2 = 3 + 0.5 * L.msq_s;
+      double c3 = 1.0 / 3.0 * L.b3_s;
+      double c4 = 0.25 * L.b4_s;
+      if constexpr ((M & M_NH) != 0) {
+        c1 += 0.5 * L.a1_s * ph;
+        c2 += 0.5 * L.a2_s * ph;
+      }
+      double d0 = c1 + c2 * Sx + c3 * Sx * Sx + c4 * Sx * Sx * Sx;
+      double d1 = c2 + c3 * Sx + c4 * Sx * Sx;
+      double d2 = c3 + c4 * Sx;
+      double d3 = c4;
+      nS = polysolve3(d3, d2, d1, d0);
+      double dV = c1 + 2.0 * c2 * Sx + 3.0 * c3 * Sx * Sx + 4.0 * c4 * Sx * Sx * Sx;
+      double dVn = c1 + 2.0 * c2 * nS + 3.0 * c3 * nS * nS + 4.0 * c4 * nS * nS * nS;
+      if (!(::fabs(dV / dVn) >= rng_u53(r))) return 0;
+    } else {
+      // metro_singlet (src/metropolis.c:148-167), localact_singlet
+      // (src/su2u1.c:688-723) as a difference
+      T u = uni<T>(r);
+      nS = (double)(T)(Sx + (T)1 * (u - (T)0.5));
+      auto pot = [&](double v) {
+        double pv = L.b1_s * v + 0.5 * L.msq_s * v * v + 1.0 / 3.0 * L.b3_s * v * v * v +
+                    0.25 * L.b4_s * v * v * v * v;
+        if constexpr ((M & M_NH) != 0) pv += 0.5 * L.a1_s * v * ph + 0.5 * L.a2_s * v * v * ph;
+        return pv;
+      };
+      double diff = 3.0 * (nS * nS - Sx * Sx) - (nS - Sx) * nb + (pot(nS) - pot(Sx));
+      if (!metro_accept((T)diff, r)) return 0;
+    }
+    S[(size_t)par * Vh + s] = (T)nS;
+    return 1;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------
+// multicanonical helpers (src/multicanonical.c, src/update.c:82-156)
+
+__device__ __forceinline__ bool muca_field_on(const MucaDev &m, int kind) {
+  if (!m.enabled || !m.do_acceptance) return false;
+  bool dbl = kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS;
+  bool trp = kind == SU2HIP_K_TRIPLET_OVERRELAX || kind == SU2HIP_K_TRIPLET_METROPOLIS;
+  if (dbl) return m.orderparam == SU2HIP_PHISQ || m.orderparam == SU2HIP_PHI2MINUSSIGMA2;
+  if (trp) return m.orderparam == SU2HIP_SIGMASQ || m.orderparam == SU2HIP_PHI2MINUSSIGMA2;
+  return false;
+}
+
+// Sites [lo,hi) of step `step` of a parity sweep with checks_per_sweep cps
+// (a check after every `iv` sites, plus a final check when Vh % cps != 0,
+// src/update.c:105-153); returns false if the lattice has no such step.
+__device__ __forceinline__ bool muca_step_range(int Vh, int cps, int step, int &lo, int &hi) {
+  int iv = Vh / cps;
+  if (iv <= 0) iv = 1;
+  int nfull = Vh / iv;
+  int extra = (Vh % iv != 0) || (Vh % cps != 0);
+  if (step >= nfull + extra) return false;
+  lo = step * iv;
+  hi = step < nfull ? lo + iv : Vh;
+  return true;
+}
+
+template <typename T, int M>
+__device__ __forceinline__ double orderparam_at(const LatDesc &L, int op, int par, int s) {
+  double v = 0.0;
+  if constexpr ((M & M_NH) != 0) {
+    if (op == SU2HIP_PHISQ || op == SU2HIP_PHI2MINUSSIGMA2) {
+      Q<T> f = ldF((const T *)L.F, L.Vh, par, s);
+      v += 0.5 * ((double)f.a * f.a + (double)f.b * f.b + (double)f.c * f.c + (double)f.d * f.d);
+    }
+  }
+  if constexpr ((M & M_TR) != 0) {
+    if (op == SU2HIP_SIGMASQ || op == SU2HIP_PHI2MINUSSIGMA2) {
+      V3<T> g = ldG((const T *)L.G, L.Vh, par, s);
+      double t = 0.5 * ((double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z);
+      v = (op == SU2HIP_SIGMASQ) ? t : v - t;
+    }
+  }
+  return v;
+}
+
+// ------------------------------------------------------------------------
+// the pass kernel
+
+template <typename T, int M, int KIND>
+__global__ __launch_bounds__(BLK) void k_pass(const LatDesc *__restrict__ lats,
+                                              const MucaDev *__restrict__ mucas,
+                                              const BlockMap *__restrict__ blk, int par, int mu,
+                                              int pass_off, int step, int reunit,
+                                              long long *__restrict__ cnt, int nblk,
+                                              double *__restrict__ partial) {
+  __shared__ double lds[BLK / 64];
+  const BlockMap bm = blk[blockIdx.x];
+  const LatDesc &L = lats[bm.lat];
+  if (!L.active || L.budget == 0) return;
+  const int s = bm.s0 + (int)threadIdx.x;
+  const bool valid = s < L.Vh;
+  int lo = 0, hi = L.Vh;
+  bool muca = false;
+  int op = 0;
+  if (step >= 0) {
+    const MucaDev &m = mucas[bm.lat];
+    muca = muca_field_on(m, KIND);
+    if (muca) {
+      if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
+      op = m.orderparam;
+    } else if (step > 0) {
+      return;
+    }
+  }
+  const bool upd = valid && s >= lo && s < hi;
+  int acc = 0;
+  if (upd) {
+    if (muca) {
+      // backup of every site of the step, accepted locally or not: a
+      // rejected global check restores the whole step from it
+      if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX || KIND == SU2HIP_K_DOUBLET_METROPOLIS)
+        stF((T *)L.B, L.Vh, par, s, ldF((const T *)L.F, L.Vh, par, s));
+      if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX || KIND == SU2HIP_K_TRIPLET_METROPOLIS)
+        stG((T *)L.B, L.Vh, par, s, ldG((const T *)L.G, L.Vh, par, s));
+    }
+    Site x = site_of(L, par, s);
+    Rng r;
+    rng_init(r, L.key0, L.key1, L.pass + (uint64_t)pass_off, (uint32_t)(par * L.Vh + s));
+    acc = site_update<T, M, KIND>(L, x, par, s, mu, r, reunit);
+  }
+  int nacc = __syncthreads_count(acc);
+  int ntot = __syncthreads_count(upd);
+  if (threadIdx.x == 0) {
+    cnt[(size_t)(KIND * 2 + 0) * nblk + blockIdx.x] += nacc;
+    cnt[(size_t)(KIND * 2 + 1) * nblk + blockIdx.x] += ntot;
+  }
+  if (muca) {
+    double v = valid ? orderparam_at<T, M>(L, op, par, s) : 0.0;
+    double t = block_sum(v, lds);
+    if (threadIdx.x == 0) partial[L.blk_base + bm.s0 / BLK] = t;
+  }
+}
+
+// get_weight (src/multicanonical.c:350-379) with whichbin (:671-704)
+__device__ int muca_whichbin(const MucaDev &m, double val) {
+  if (val < m.min) return 0;
+  if (val >= m.max) return m.bins + 1;
+  int lo = 1, hi = m.bins;
+  while (lo <= hi) {
+    int bin = lo + (hi - lo) / 2;
+    double cur = m.pos[bin], nxt = m.pos[bin + 1];
+    if ((val < nxt && val > cur) || (::fabs(val - cur) < 1e-12)) return bin;
+    if (val > cur)
+      lo = bin + 1;
+    else
+      hi = bin - 1;
+  }
+  return m.bins;
+}
+
+__device__ double muca_weight(const MucaDev &m, double val) {
+  int n = m.bins;
+  if (val >= m.max) return m.b[n + 1];
+  if (val < m.min) return m.b[0];
+  int bin = muca_whichbin(m, val);
+  if (bin >= n + 1) return m.b[n + 1];
+  return m.b[bin] + m.slope[bin] * val;
+}
+
+// linearize_weight (src/multicanonical.c:337-346)
+__device__ void muca_linearize(MucaDev &m) {
+  int n = m.bins;
+  m.slope[0] = 0;
+  m.slope[n + 1] = 0;
+  m.b[0] = m.W[0];
+  m.b[n + 1] = m.W[n + 1];
+  for (int i = 1; i < n + 1; i++) {
+    m.slope[i] = (m.W[i + 1] - m.W[i]) / (m.pos[i + 1] - m.pos[i]);
+    m.b[i] = m.W[i] - m.slope[i] * m.pos[i];
+  }
+}
+
+// muca_accumulate_hits (src/multicanonical.c:451-498)
+__device__ void muca_hits(MucaDev &m, double val) {
+  int n = m.bins;
+  int bin = muca_whichbin(m, val);
+  if (bin == 0) {
+    double dmin = m.pos[2] - m.pos[1];
+    if (val < m.pos[1] - 0.5 * dmin) return;
+    bin += 1;
+  } else if (bin < n + 1) {
+    if (val - m.pos[bin] > m.pos[bin + 1] - val) bin = bin + 1;
+  } else {
+    double dmax = m.pos[n + 1] - m.pos[n];
+    if (val > m.pos[n + 1] + 0.5 * dmax) return;
+  }
+  if (m.mode == SU2HIP_FAST) {
+    m.hits[bin] += 5;
+    if (bin > 1) m.hits[bin - 1] += 3;
+    if (bin > 2) m.hits[bin - 2] += 1;
+    if (bin < n) m.hits[bin + 1] += 3;
+    if (bin < n - 1) m.hits[bin + 2] += 1;
+  } else if (m.mode == SU2HIP_SLOW) {
+    m.hits[bin]++;
+    m.hgram[bin] += ::exp(muca_weight(m, val));
+  }
+}
+
+// update_weight (src/multicanonical.c:504-557), FAST recursion
+__device__ void muca_update_fast(MucaDev &m) {
+  int n = m.bins, first = -1;
+  double delta = 0.0;
+  for (int i = 0; i < n + 2; i++) {
+    if (m.pos[i] >= m.wrk_min && m.wrk_max >= m.pos[i]) {
+      if (first < 0) first = i;
+      delta = (m.hits[i] - m.hits[first]) * m.delta / ((double)n);
+      m.W[i] += delta;
+    } else if (m.wrk_max < m.pos[i]) {
+      m.W[i] += delta;
+    }
+  }
+  m.W[0] = m.W[1];
+  m.W[n + 1] = m.W[n];
+  muca_linearize(m);
+  int tunnel = 0;
+  if (m.last_max && m.hits[1] > 0) {
+    m.last_max = 0;
+    tunnel = 1;
+  } else if (!m.last_max && m.hits[n] > 0) {
+    m.last_max = 1;
+    tunnel = 1;
+  }
+  if (tunnel) m.delta /= 1.5;
+  for (int i = 0; i < n + 2; i++) m.hits[i] = 0;
+}
+
+// update_weight_slow (src/multicanonical.c:561-666), hep-lat/9804019; the
+// scratch copy of W lives in hgram's second half
+__device__ void muca_update_slow(MucaDev &m) {
+  int n = m.bins;
+  double *wold = m.hgram + (n + 2);
+  for (int i = 0; i < n + 2; i++) wold[i] = m.W[i];
+  double dmin = m.pos[2] - m.pos[1];
+  m.hgram[1] /= dmin;
+  for (int i = 2; i < n + 1; i++) m.hgram[i] /= (m.pos[i + 1] - m.pos[i - 1]);
+  int first = -1, last = -1;
+  for (int i = 1; i < n + 1; i++) {
+    if (m.pos[i] >= m.wrk_min && m.wrk_max >= m.pos[i]) {
+      if (first < 0) {
+        first = i;
+        continue;
+      }
+      m.nsum[i] += m.hits[i];
+      int g;
+      if (m.hits[i] >= 8 && m.hits[i - 1] >= 8) {
+        g = m.hits[i] + m.hits[i - 1];
+      } else {
+        m.W[i] = wold[i] - wold[i - 1] + m.W[i - 1];
+        continue;
+      }
+      long long gnew = m.gsum[i] + g;
+      double ln = m.hgram[i - 1] / m.hgram[i];
+      ln = -1.0 * g * ::log(ln);
+      m.W[i] = m.W[i - 1] + ((wold[i] - wold[i - 1]) * m.gsum[i] + ln) / ((double)gnew);
+      m.gsum[i] = gnew;
+    } else if (m.wrk_max < m.pos[i]) {
+      if (last < 0) last = i - 1;
+      m.W[i] += m.W[last] - wold[last];
+    }
+  }
+  for (int i = 0; i < n + 2; i++) {
+    m.hits[i] = 0;
+    m.hgram[i] = 0.0;
+  }
+  m.W[0] = m.W[1];
+  m.W[n + 1] = m.W[n];
+  for (int i = 0; i < n + 2; i++) wold[i] = m.W[i];
+  double corr = 0.0;
+  dmin = (first == 1) ? m.pos[first + 1] - m.pos[first] : m.pos[first + 1] - m.pos[first - 1];
+  for (int i = first + 1; i < n + 1; i++) {
+    if (i <= last) {
+      double d1 = m.pos[i + 1] - m.pos[i - 1];
+      corr = 2.0 * ::log((dmin / d1) * m.nsum[i] / m.nsum[first]);
+    }
+    m.W[i] += corr;
+  }
+  m.W[0] = m.W[1];
+  m.W[n + 1] = m.W[n];
+  muca_linearize(m);
+  for (int i = 0; i < n + 2; i++) m.W[i] = wold[i];
+}
+
+// muca_check (src/update.c:13-30) + calc_orderparam's reduction
+// (src/multicanonical.c:709-751) + multicanonical_acceptance (:385-445);
+// one workgroup per lattice, the sequential part in thread 0
+__global__ __launch_bounds__(BLK) void k_muca_accept(const LatDesc *__restrict__ lats,
+                                                     MucaDev *__restrict__ mucas,
+                                                     const double *__restrict__ partial, int kind,
+                                                     int par, int pass_off, int step) {
+  __shared__ double lds[BLK / 64];
+  const LatDesc &L = lats[blockIdx.x];
+  MucaDev &m = mucas[blockIdx.x];
+  if (!L.active || L.budget == 0 || !muca_field_on(m, kind)) return;
+  int lo, hi;
+  if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
+  // fixed-order sum of this lattice's block partials
+  double v = 0.0;
+  for (int i = threadIdx.x; i < L.nblk; i += BLK) v += partial[L.blk_base + i];
+  double tot = block_sum(v, lds);
+  if (threadIdx.x == 0) {
+    double newp = tot / L.vol;
+    double old = m.pv[0] + m.pv[1];
+    m.pv[par] = newp;
+    double nw = newp + m.pv[1 - par];
+    double diff = muca_weight(m, nw) - muca_weight(m, old);
+    Rng r;
+    rng_init(r, L.key0, L.key1, L.pass + (uint64_t)pass_off, MUCA_SITE);
+    r.k = (uint32_t)step;
+    int accept = ::exp(-(diff)) > rng_u53(r);
+    if (m.mode != SU2HIP_READONLY) {
+      if (!accept) nw = old;
+      muca_hits(m, nw);
+      m.muca_count++;
+      if (m.muca_count % m.update_interval == 0) {
+        if (m.mode == SU2HIP_FAST)
+          muca_update_fast(m);
+        else if (m.mode == SU2HIP_SLOW)
+          muca_update_slow(m);
+        m.muca_count = 0;
+      }
+    }
+    if (!accept) m.pv[par] = old - m.pv[1 - par];
+    m.rejected = !accept;
+    m.accepted += accept;
+    m.total++;
+  }
+}
+
+// undo a rejected step: copy the backup of the step's sites back
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_muca_restore(const LatDesc *__restrict__ lats,
+                                                      const MucaDev *__restrict__ mucas,
+                                                      const BlockMap *__restrict__ blk, int kind,
+                                                      int par, int step) {
+  const BlockMap bm = blk[blockIdx.x];
+  const LatDesc &L = lats[bm.lat];
+  const MucaDev &m = mucas[bm.lat];
+  if (!L.active || L.budget == 0 || !muca_field_on(m, kind) || !m.rejected) return;
+  int lo, hi;
+  if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
+  int s = bm.s0 + (int)threadIdx.x;
+  if (s >= L.Vh || s < lo || s >= hi) return;
+  bool dbl = kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS;
+  int dofs = dbl ? 4 : 3;
+  T *dst = (T *)(dbl ? L.F : L.G);
+  const T *src = (const T *)L.B;
+  for (int c = 0; c < dofs; c++) {
+    size_t o = (size_t)(c * 2 + par) * L.Vh + s;
+    dst[o] = src[o];
+  }
+}
+
+// order parameter of both parities (calc_orderparam at start-up,
+// src/main.c:212-217): partial sums of parity p in partial[p*nblk + ...]
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_orderparam(const LatDesc *__restrict__ lats,
+                                                    const MucaDev *__restrict__ mucas,
+                                                    const BlockMap *__restrict__ blk, int nblk,
+                                                    double *__restrict__ partial) {
+  __shared__ double lds[BLK / 64];
+  const BlockMap bm = blk[blockIdx.x];
+  const LatDesc &L = lats[bm.lat];
+  const MucaDev &m = mucas[bm.lat];
+  if (!L.active || !m.enabled) return;
+  int s = bm.s0 + (int)threadIdx.x;
+  for (int par = 0; par < 2; par++) {
+    double v = s < L.Vh ? orderparam_at<T, M>(L, m.orderparam, par, s) : 0.0;
+    double t = block_sum(v, lds);
+    if (threadIdx.x == 0) partial[(size_t)par * nblk + L.blk_base + bm.s0 / BLK] = t;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_orderparam_final(const LatDesc *__restrict__ lats,
+                                                          MucaDev *__restrict__ mucas, int nblk,
+                                                          const double *__restrict__ partial) {
+  __shared__ double lds[BLK / 64];
+  const LatDesc &L = lats[blockIdx.x];
+  MucaDev &m = mucas[blockIdx.x];
+  if (!L.active || !m.enabled) return;
+  for (int par = 0; par < 2; par++) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < L.nblk; i += BLK) v += partial[(size_t)par * nblk + L.blk_base + i];
+    double tot = block_sum(v, lds);
+    if (threadIdx.x == 0) m.pv[par] = tot / L.vol;
+  }
+}
+
+// end of a compound sweep: advance the pass counter and the budget
+__global__ void k_end_sweep(LatDesc *lats, int n, int passes) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  LatDesc &L = lats[i];
+  if (!L.active || L.budget == 0) return;
+  L.pass += (uint64_t)passes;
+  if (L.budget > 0) L.budget--;
+}
+
+// ------------------------------------------------------------------------
+// measure() site loop (src/measure.c:110-174) in FP64
+
+struct Cm {
+  double re[4], im[4];
+};
+
+__device__ __forceinline__ Cm cm_mul(const Cm &a, const Cm &b, bool dag) {
+  Cm bb = b;
+  if (dag) {
+    bb.re[0] = b.re[0];
+    bb.im[0] = -b.im[0];
+    bb.re[1] = b.re[2];
+    bb.im[1] = -b.im[2];
+    bb.re[2] = b.re[1];
+    bb.im[2] = -b.im[1];
+    bb.re[3] = b.re[3];
+    bb.im[3] = -b.im[3];
+  }
+  Cm t;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      double sr = 0, si = 0;
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        double ar = a.re[2 * i + k], ai = a.im[2 * i + k];
+        double br = bb.re[2 * k + j], bi = bb.im[2 * k + j];
+        sr += ar * br - ai * bi;
+        si += ar * bi + ai * br;
+      }
+      t.re[2 * i + j] = sr;
+      t.im[2 * i + j] = si;
+    }
+  return t;
+}
+
+template <typename T>
+__device__ __forceinline__ Cm halfproj(const V3<T> &g) {
+  double a0 = g.x, a1 = g.y, a2 = g.z;
+  double md = ::sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+  double n0 = a0 / md, n1 = a1 / md, n2 = a2 / md;
+  Cm m;
+  m.re[0] = 0.5 * (1 + n2);
+  m.im[0] = 0.0;
+  m.re[1] = 0.5 * n0;
+  m.im[1] = -0.5 * n1;
+  m.re[2] = 0.5 * n0;
+  m.im[2] = 0.5 * n1;
+  m.re[3] = 0.5 * (1 - n2);
+  m.im[3] = 0.0;
+  return m;
+}
+
+// project_u1 (src/magfield.c:115-184)
+template <typename T>
+__device__ Cm project_u1(const LatDesc &L, const T *U, const T *G, const Site &x, int d) {
+  Cm pl = halfproj(G_at(L, G, x)), pr = halfproj(G_at(L, G, shift(L, x, d, +1)));
+  Q<T> q = U_at(L, U, x, d);
+  Cm u;
+  u.re[0] = q.a;
+  u.im[0] = q.d;
+  u.re[1] = q.c;
+  u.im[1] = q.b;
+  u.re[2] = -q.c;
+  u.im[2] = q.b;
+  u.re[3] = q.a;
+  u.im[3] = -q.d;
+  return cm_mul(cm_mul(pl, u, false), pr, false);
+}
+
+// alpha_proj + magfield + magcharge_cube (src/magfield.c:189-280)
+template <typename T>
+__device__ double magfield(const LatDesc &L, const T *U, const T *G, const Site &x, int dir) {
+  double res = 0.0;
+  for (int d1 = 0; d1 < 3; d1++) {
+    if (d1 == dir) continue;
+    for (int d2 = d1 + 1; d2 < 3; d2++) {
+      if (d2 == dir) continue;
+      Cm u1 = project_u1(L, U, G, x, d1);
+      Cm u2 = project_u1(L, U, G, shift(L, x, d1, +1), d2);
+      Cm u3 = project_u1(L, U, G, shift(L, x, d2, +1), d1);
+      Cm u4 = project_u1(L, U, G, x, d2);
+      u1 = cm_mul(u1, u2, false);
+      u1 = cm_mul(u1, u3, true);
+      u1 = cm_mul(u1, u4, true);
+      double al = ::atan2(u1.im[0] + u1.im[3], u1.re[0] + u1.re[3]) * ::sqrt(L.beta);
+      if (d1 > dir)
+        res += al;
+      else if (d2 > dir)
+        res -= al;
+      else
+        res += al;
+    }
+  }
+  return res;
+}
+
+template <typename T>
+__device__ __forceinline__ Q<double> qd(const Q<T> &q) {
+  return Q<double>{(double)q.a, (double)q.b, (double)q.c, (double)q.d};
+}
+
+template <typename T, int M>
+__device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
+  const T *U = (const T *)L.U, *A = (const T *)L.A, *F = (const T *)L.F, *G = (const T *)L.G,
+          *S = (const T *)L.S;
+  Site x = site_of(L, par, s);
+  // local_su2wilson (src/su2u1.c:143-157)
+  double wl = 0.0;
+#pragma unroll
+  for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+    for (int d2 = 0; d2 < d1; d2++) {
+      Q<double> t = qmul(qmul(qmul(qd(U_at(L, U, x, d2)), qd(U_at(L, U, shift(L, x, d2, +1), d1))),
+                              qconj(qd(U_at(L, U, shift(L, x, d1, +1), d2)))),
+                         qconj(qd(U_at(L, U, x, d1))));
+      wl += (1.0 - 0.5 * (2.0 * t.a));
+    }
+  wl *= L.beta;
+  double action = wl;
+  acc[SU2HIP_SUM_WILSON] += wl;
+  if constexpr ((M & M_U1) != 0) {
+    // local_u1wilson (src/su2u1.c:316-328)
+    double u = 0.0;
+#pragma unroll
+    for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+      for (int d2 = 0; d2 < d1; d2++) {
+        double pl = (double)A_at(L, A, x, d2) + (double)A_at(L, A, shift(L, x, d2, +1), d1) -
+                    (double)A_at(L, A, shift(L, x, d1, +1), d2) - (double)A_at(L, A, x, d1);
+        u += (1.0 - ::cos(L.r_u1 * pl));
+      }
+    u *= L.betau1;
+    action += u;
+    acc[SU2HIP_SUM_U1WILSON] += u;
+  }
+  double mod = 0.0, tmod = 0.0;
+  if constexpr ((M & M_NH) != 0) {
+    Q<double> f = qd(F_at(L, F, x));
+    mod = 0.5 * qdot(f, f);
+    // covariant_doublet (src/su2u1.c:477-488)
+    double cov = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      Q<double> t = qmul(qmul(qconj(f), qd(U_at(L, U, x, d))), qd(F_at(L, F, shift(L, x, d, +1))));
+      double hop;
+      if constexpr ((M & M_U1) != 0) {
+        double al = A_at(L, A, x, d);
+        hop = -(t.a * ::cos(al) + t.d * ::sin(al));
+      } else {
+        hop = -t.a;
+      }
+      cov += 2.0 * mod + hop;
+    }
+    acc[SU2HIP_SUM_COV_PHI] += cov;
+    acc[SU2HIP_SUM_PHI2] += mod;
+    acc[SU2HIP_SUM_PHI4] += mod * mod;
+    action += cov + L.msq * mod + L.lambda * mod * mod;
+  }
+  if constexpr ((M & M_TR) != 0) {
+    V3<T> gt = G_at(L, G, x);
+    V3<double> g{(double)gt.x, (double)gt.y, (double)gt.z};
+    tmod = 0.5 * (g.x * g.x + g.y * g.y + g.z * g.z);
+    double hop = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      V3<T> nb = G_at(L, G, shift(L, x, d, +1));
+      V3<double> r = rotate(qd(U_at(L, U, x, d)), V3<double>{(double)nb.x, (double)nb.y, (double)nb.z});
+      hop += -(g.x * r.x + g.y * r.y + g.z * r.z);
+    }
+    acc[SU2HIP_SUM_SIGMA2] += tmod;
+    acc[SU2HIP_SUM_SIGMA4] += tmod * tmod;
+    acc[SU2HIP_SUM_HOP_SIGMA] += hop;
+    // covariant_triplet + triplet potential (src/su2u1.c:558-565, :643-651)
+    action += 3 * 2.0 * tmod + hop + L.msq_t * tmod + L.b4 * tmod * tmod;
+    if constexpr ((M & M_NH) != 0) {
+      action += L.a2 * mod * tmod;
+      acc[SU2HIP_SUM_PHI2SIGMA2] += mod * tmod;
+    }
+    double ch = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      ch += magfield(L, U, G, shift(L, x, d, +1), d) - magfield(L, U, G, x, d);
+    acc[SU2HIP_SUM_MAGCHARGE] += ch;
+    acc[SU2HIP_SUM_MAGCHARGE_ABS] += ::fabs(ch);
+  }
+  if constexpr ((M & M_SG) != 0) {
+    double Sx = S_at(L, S, x);
+    double kin = 3 * Sx * Sx, cov = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double Sn = S_at(L, S, shift(L, x, d, +1));
+      kin -= Sx * Sn;
+      cov += Sx * Sx - Sx * Sn;
+    }
+    double pot = L.b1_s * Sx + 0.5 * L.msq_s * Sx * Sx + 1.0 / 3.0 * L.b3_s * Sx * Sx * Sx +
+                 0.25 * L.b4_s * Sx * Sx * Sx * Sx;
+    if constexpr ((M & M_NH) != 0) {
+      pot += 0.5 * L.a1_s * Sx * mod + 0.5 * L.a2_s * Sx * Sx * mod;
+      acc[SU2HIP_SUM_SPHISQ] += mod * Sx;
+      acc[SU2HIP_SUM_S2PHISQ] += mod * Sx * Sx;
+    }
+    action += kin + pot;
+    acc[SU2HIP_SUM_S_COV] += cov;
+    acc[SU2HIP_SUM_S] += Sx;
+    acc[SU2HIP_SUM_S2] += Sx * Sx;
+    acc[SU2HIP_SUM_S3] += Sx * Sx * Sx;
+    acc[SU2HIP_SUM_S4] += Sx * Sx * Sx * Sx;
+  }
+  acc[SU2HIP_SUM_ACTION] += action;
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_measure(const LatDesc *__restrict__ lats,
+                                                 const BlockMap *__restrict__ blk,
+                                                 double *__restrict__ partial) {
+  __shared__ double lds[BLK / 64];
+  const BlockMap bm = blk[blockIdx.x];
+  const LatDesc &L = lats[bm.lat];
+  if (!L.active) return;
+  int s = bm.s0 + (int)threadIdx.x;
+  double acc[SU2HIP_NSUMS];
+#pragma unroll
+  for (int j = 0; j < SU2HIP_NSUMS; j++) acc[j] = 0.0;
+  if (s < L.Vh) {
+    measure_site<T, M>(L, 0, s, acc);
+    measure_site<T, M>(L, 1, s, acc);
+  }
+  double *out = partial + (size_t)(L.blk_base + bm.s0 / BLK) * SU2HIP_NSUMS;
+#pragma unroll
+  for (int j = 0; j < SU2HIP_NSUMS; j++) {
+    double t = block_sum(acc[j], lds);
+    if (threadIdx.x == 0) out[j] = t;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_measure_final(const LatDesc *__restrict__ lats,
+                                                       const double *__restrict__ partial,
+                                                       double *__restrict__ sums) {
+  __shared__ double lds[BLK / 64];
+  const LatDesc &L = lats[blockIdx.x];
+  double *out = sums + (size_t)blockIdx.x * SU2HIP_NSUMS;
+  if (!L.active) {
+    if (threadIdx.x < SU2HIP_NSUMS) out[threadIdx.x] = 0.0;
+    return;
+  }
+  for (int j = 0; j < SU2HIP_NSUMS; j++) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < L.nblk; i += BLK)
+      v += partial[(size_t)(L.blk_base + i) * SU2HIP_NSUMS + j];
+    double t = block_sum(v, lds);
+    if (threadIdx.x == 0) out[j] = t;
+  }
+}
+
+// ------------------------------------------------------------------------
+// start-up fields and layout conversion
+
+// setfields (src/init.c:92-112)
+template <typename T, int M>
+__global__ void k_cold(LatDesc L, double phi0, double sigma0, double singlet0) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.Vh) return;
+  for (int par = 0; par < 2; par++) {
+    for (int d = 0; d < 3; d++) {
+      stU((T *)L.U, L.Vh, d, par, s, Q<T>{(T)1, (T)0, (T)0, (T)0});
+      if constexpr ((M & M_U1) != 0) ((T *)L.A)[(size_t)(d * 2 + par) * L.Vh + s] = 0;
+    }
+    if constexpr ((M & M_NH) != 0) stF((T *)L.F, L.Vh, par, s, Q<T>{(T)phi0, (T)0, (T)0, (T)0});
+    if constexpr ((M & M_TR) != 0) stG((T *)L.G, L.Vh, par, s, V3<T>{(T)sigma0, (T)0, (T)0});
+    if constexpr ((M & M_SG) != 0) ((T *)L.S)[(size_t)par * L.Vh + s] = (T)singlet0;
+  }
+}
+
+__device__ __forceinline__ void gauss2(Rng &r, double &z0, double &z1) {
+  double u1 = rng_u53(r), u2 = rng_u53(r);
+  double rr = ::sqrt(-2.0 * ::log(1.0 - u1));
+  double t = 2.0 * PI * u2;
+  z0 = rr * ::cos(t);
+  z1 = rr * ::sin(t);
+}
+
+// hot start, the draw order of the oracle's orc_hot_start
+template <typename T, int M>
+__global__ void k_hot(LatDesc L) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.Vh) return;
+  for (int par = 0; par < 2; par++) {
+    Rng r;
+    rng_init(r, L.key0, L.key1, HOTSTART_PASS, (uint32_t)(par * L.Vh + s));
+    for (int d = 0; d < 3; d++) {
+      double z[4];
+      gauss2(r, z[0], z[1]);
+      gauss2(r, z[2], z[3]);
+      double n = ::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3]);
+      stU((T *)L.U, L.Vh, d, par, s, Q<T>{(T)(z[0] / n), (T)(z[1] / n), (T)(z[2] / n), (T)(z[3] / n)});
+    }
+    if constexpr ((M & M_U1) != 0)
+      for (int d = 0; d < 3; d++)
+        ((T *)L.A)[(size_t)(d * 2 + par) * L.Vh + s] = (T)(PI * (2.0 * rng_u53(r) - 1.0));
+    if constexpr ((M & M_NH) != 0) {
+      double z[4];
+      gauss2(r, z[0], z[1]);
+      gauss2(r, z[2], z[3]);
+      stF((T *)L.F, L.Vh, par, s, Q<T>{(T)z[0], (T)z[1], (T)z[2], (T)z[3]});
+    }
+    if constexpr ((M & M_TR) != 0) {
+      double z[4];
+      gauss2(r, z[0], z[1]);
+      gauss2(r, z[2], z[3]);
+      stG((T *)L.G, L.Vh, par, s, V3<T>{(T)z[0], (T)z[1], (T)z[2]});
+    }
+    if constexpr ((M & M_SG) != 0) {
+      double z0, z1;
+      gauss2(r, z0, z1);
+      ((T *)L.S)[(size_t)par * L.Vh + s] = (T)z0;
+    }
+  }
+}
+
+// reference host layout (site i = par*Vh + s, components innermost)
+// <-> device SoA; field 0 links, 1 U(1), 2 doublet, 3 triplet, 4 singlet
+__host__ __device__ inline int field_dofs(int field) {
+  return field == 0 ? 12 : (field == 1 ? 3 : (field == 2 ? 4 : (field == 3 ? 3 : 1)));
+}
+
+__device__ __forceinline__ size_t soa_index(int field, int Vh, int par, int s, int k) {
+  if (field == 0) {
+    int d = k >> 2, c = k & 3;
+    return (size_t)((d * 4 + c) * 2 + par) * Vh + s;
+  }
+  return (size_t)(k * 2 + par) * Vh + s;
+}
+
+template <typename T>
+__global__ void k_to_device(LatDesc L, int field, const double *__restrict__ h) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.Vh) return;
+  void *dst = field == 0 ? L.U : (field == 1 ? L.A : (field == 2 ? L.F : (field == 3 ? L.G : L.S)));
+  int n = field_dofs(field);
+  for (int par = 0; par < 2; par++) {
+    size_t i = (size_t)par * L.Vh + s;
+    for (int k = 0; k < n; k++) ((T *)dst)[soa_index(field, L.Vh, par, s, k)] = (T)h[i * n + k];
+  }
+}
+
+template <typename T>
+__global__ void k_to_host(LatDesc L, int field, double *__restrict__ h) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.Vh) return;
+  const void *src = field == 0 ? L.U : (field == 1 ? L.A : (field == 2 ? L.F : (field == 3 ? L.G : L.S)));
+  int n = field_dofs(field);
+  for (int par = 0; par < 2; par++) {
+    size_t i = (size_t)par * L.Vh + s;
+    for (int k = 0; k < n; k++) h[i * n + k] = (double)((const T *)src)[soa_index(field, L.Vh, par, s, k)];
+  }
+}
+
+__global__ void k_copy(const double2 *__restrict__ a, double2 *__restrict__ b, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) b[i] = a[i];
+}
+
+// ------------------------------------------------------------------------
+// gradient flow (src/gradflow.c), links double-buffered, one thread per
+// site of both parities
+
+// grad_force_link (src/gradflow.c:190-258): F^a = 2 (4/beta) (U s)_a with
+// s = -beta/8 * Wilson staple (+ the triplet's -Sigma(x+d) U^+ Sigma(x))
+template <typename T, int M>
+__device__ __forceinline__ V3<T> link_force(const LatDesc &L, const T *U, const T *G, const Site &x,
+                                           int d) {
+  Q<T> st = qscale(wilson_staple(L, U, x, d), (T)(-L.beta / 8.0));
+  Q<T> u = U_at(L, U, x, d);
+  if constexpr ((M & M_TR) != 0) {
+    V3<T> g2 = G_at(L, G, shift(L, x, d, +1)), g1 = G_at(L, G, x);
+    Q<T> t = qmul(qmul(Q<T>{(T)0, g2.x, g2.y, g2.z}, qconj(u)), Q<T>{(T)0, g1.x, g1.y, g1.z});
+    st = qadd(st, qscale(t, (T)0.25));
+  }
+  Q<T> m = qmul(u, st);
+  T k = (T)(4.0 / L.beta);
+  return V3<T>{(T)2 * m.b * k, (T)2 * m.c * k, (T)2 * m.d * k};
+}
+
+// grad_force_triplet (src/gradflow.c:267-330)
+template <typename T, int M>
+__device__ __forceinline__ V3<T> triplet_force(const LatDesc &L, const T *U, const T *F, const T *G,
+                                              const Site &x, double msq_t) {
+  V3<T> res = triplet_staple<T>(L, U, G, x);
+  V3<T> g = G_at(L, G, x);
+  T tsq = (T)0.5 * (g.x * g.x + g.y * g.y + g.z * g.z);
+  T k = (T)(2.0 * 3 + msq_t) + (T)2 * (T)L.b4 * tsq;
+  if constexpr ((M & M_NH) != 0) {
+    Q<T> f = F_at(L, F, x);
+    k += (T)L.a2 * (T)0.5 * qdot(f, f);
+  }
+  return V3<T>{-(res.x + k * g.x), -(res.y + k * g.y), -(res.z + k * g.z)};
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> expmul(const V3<T> &z, const Q<T> &v) {
+  T mod = z.x * z.x + z.y * z.y + z.z * z.z;
+  if (mod <= (T)0) return v;
+  mod = msqrt(mod);
+  T sn = msin(mod) / mod;
+  Q<T> u{mcos(mod), sn * z.x, sn * z.y, sn * z.z};
+  return qmul(u, v);
+}
+
+// stage 0: Euler (flow_gauge, src/gradflow.c:361-401); stages 1..3: RK3
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_flow(const LatDesc *__restrict__ lats,
+                                              const BlockMap *__restrict__ blk, const T *__restrict__ Uin,
+                                              T *__restrict__ Uout, const T *__restrict__ Gin,
+                                              T *__restrict__ Gout, T *__restrict__ Z,
+                                              T *__restrict__ ZG, double dt, double msq_t, int stage) {
+  const BlockMap bm = blk[blockIdx.x];
+  const LatDesc &L = lats[bm.lat];
+  int s = bm.s0 + (int)threadIdx.x;
+  if (s >= L.Vh) return;
+  const double cs[4] = {1.0, 0.25, 8.0 / 9.0, 0.75};
+  const double ds[4] = {0.0, 0.0, 17.0 / 9.0, 1.0};
+  T c = (T)(cs[stage] * dt), dd = (T)ds[stage];
+  for (int par = 0; par < 2; par++) {
+    Site x = site_of(L, par, s);
+    for (int d = 0; d < 3; d++) {
+      V3<T> f = link_force<T, M>(L, Uin, Gin, x, d);
+      V3<T> z{c * f.x, c * f.y, c * f.z};
+      if (stage > 0) {
+        size_t o = (size_t)((d * 3) * 2 + par) * L.Vh + s, st = (size_t)2 * L.Vh;
+        if (stage > 1) {
+          z.x -= dd * Z[o];
+          z.y -= dd * Z[o + st];
+          z.z -= dd * Z[o + 2 * st];
+        }
+        Z[o] = z.x;
+        Z[o + st] = z.y;
+        Z[o + 2 * st] = z.z;
+      }
+      stU(Uout, L.Vh, d, par, s, expmul(z, ldU(Uin, L.Vh, d, par, s)));
+    }
+    if constexpr ((M & M_TR) != 0) {
+      V3<T> f = triplet_force<T, M>(L, Uin, (const T *)L.F, Gin, x, msq_t);
+      V3<T> z{c * f.x, c * f.y, c * f.z};
+      if (stage > 0) {
+        size_t o = (size_t)par * L.Vh + s, st = (size_t)2 * L.Vh;
+        if (stage > 1) {
+          z.x -= dd * ZG[o];
+          z.y -= dd * ZG[o + st];
+          z.z -= dd * ZG[o + 2 * st];
+        }
+        ZG[o] = z.x;
+        ZG[o + st] = z.y;
+        ZG[o + 2 * st] = z.z;
+      }
+      V3<T> g = ldG(Gin, L.Vh, par, s);
+      stG(Gout, L.Vh, par, s, V3<T>{g.x + z.x, g.y + z.y, g.z + z.z});
+    }
+  }
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_flow_force(const LatDesc *__restrict__ lats,
+                                                    const BlockMap *__restrict__ blk,
+                                                    double *__restrict__ out) {
+  const BlockMap bm = blk[blockIdx.x];
+  const LatDesc &L = lats[bm.lat];
+  int s = bm.s0 + (int)threadIdx.x;
+  if (s >= L.Vh) return;
+  for (int par = 0; par < 2; par++) {
+    Site x = site_of(L, par, s);
+    size_t i = (size_t)par * L.Vh + s;
+    for (int d = 0; d < 3; d++) {
+      V3<T> f = link_force<T, M>(L, (const T *)L.U, (const T *)L.G, x, d);
+      double *o = out + (i * 3 + d) * 4;
+      o[0] = 0.0;
+      o[1] = f.x;
+      o[2] = f.y;
+      o[3] = f.z;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// dispatch
+
+template <typename Fn>
+static hipError_t with_model(int model, Fn &&fn) {
+  switch (model) {
+    case 0: return fn(std::integral_constant<int, 0>{});
+    case M_NH: return fn(std::integral_constant<int, M_NH>{});
+    case M_NH | M_U1: return fn(std::integral_constant<int, M_NH | M_U1>{});
+    case M_NH | M_SG: return fn(std::integral_constant<int, M_NH | M_SG>{});
+    case M_NH | M_U1 | M_SG: return fn(std::integral_constant<int, M_NH | M_U1 | M_SG>{});
+    case M_TR: return fn(std::integral_constant<int, M_TR>{});
+    case M_NH | M_TR: return fn(std::integral_constant<int, M_NH | M_TR>{});
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename Fn>
+static hipError_t with_prec(int fp64, Fn &&fn) {
+  if (fp64) return fn(double(0));
+  return fn(float(0));
+}
+
+template <typename T, int M, int KIND>
+static hipError_t pass_one(const LaunchCtx &c, int parity, int dir, int pass_off, int step,
+                           int reunit) {
+  if constexpr (kind_valid<M, KIND>()) {
+    hipLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.muca,
+                       c.blk, parity, dir, pass_off, step, reunit, c.cnt, c.nblk, c.partial);
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir, int pass_off,
+                       int muca_step, int reunit) {
+  if (c.nblk == 0) return hipSuccess;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      switch (kind) {
+        case SU2HIP_K_LINK_HEATBATH: return pass_one<T, M, SU2HIP_K_LINK_HEATBATH>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_LINK_METROPOLIS: return pass_one<T, M, SU2HIP_K_LINK_METROPOLIS>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_LINK_OVERRELAX: return pass_one<T, M, SU2HIP_K_LINK_OVERRELAX>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_U1_METROPOLIS: return pass_one<T, M, SU2HIP_K_U1_METROPOLIS>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_DOUBLET_OVERRELAX: return pass_one<T, M, SU2HIP_K_DOUBLET_OVERRELAX>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_DOUBLET_METROPOLIS: return pass_one<T, M, SU2HIP_K_DOUBLET_METROPOLIS>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_TRIPLET_OVERRELAX: return pass_one<T, M, SU2HIP_K_TRIPLET_OVERRELAX>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_TRIPLET_METROPOLIS: return pass_one<T, M, SU2HIP_K_TRIPLET_METROPOLIS>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_SINGLET_OVERRELAX: return pass_one<T, M, SU2HIP_K_SINGLET_OVERRELAX>(c, parity, dir, pass_off, muca_step, reunit);
+        case SU2HIP_K_SINGLET_METROPOLIS: return pass_one<T, M, SU2HIP_K_SINGLET_METROPOLIS>(c, parity, dir, pass_off, muca_step, reunit);
+      }
+      return hipErrorInvalidValue;
+    });
+  });
+}
+
+hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int parity, int pass_off,
+                              int muca_step) {
+  hipLaunchKernelGGL(k_muca_accept, dim3(c.max_slots), dim3(BLK), 0, c.stream, c.desc, c.muca,
+                     c.partial, kind, parity, pass_off, muca_step);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || c.nblk == 0) return e;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    hipLaunchKernelGGL(k_muca_restore<T>, dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.muca,
+                       c.blk, kind, parity, muca_step);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_orderparam_init(const LaunchCtx &c) {
+  if (c.nblk == 0) return hipSuccess;
+  hipError_t e = with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_orderparam<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc,
+                         c.muca, c.blk, c.nblk, c.partial);
+      return hipGetLastError();
+    });
+  });
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_orderparam_final, dim3(c.max_slots), dim3(BLK), 0, c.stream, c.desc,
+                     c.muca, c.nblk, c.partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_end_sweep(const LaunchCtx &c, int passes) {
+  int nb = (c.max_slots + 63) / 64;
+  hipLaunchKernelGGL(k_end_sweep, dim3(nb), dim3(64), 0, c.stream, c.desc, c.max_slots, passes);
+  return hipGetLastError();
+}
+
+hipError_t launch_measure(const LaunchCtx &c) {
+  if (c.nblk > 0) {
+    hipError_t e = with_prec(c.fp64, [&](auto t) {
+      using T = decltype(t);
+      return with_model(c.model, [&](auto m) -> hipError_t {
+        constexpr int M = decltype(m)::value;
+        hipLaunchKernelGGL((k_measure<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.blk,
+                           c.partial);
+        return hipGetLastError();
+      });
+    });
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_measure_final, dim3(c.max_slots), dim3(BLK), 0, c.stream, c.desc,
+                     c.partial, c.sums);
+  return hipGetLastError();
+}
+
+static LatDesc host_desc_of(const LaunchCtx &c, int slot, hipError_t &e) {
+  LatDesc L;
+  e = hipMemcpyAsync(&L, c.desc + slot, sizeof(LatDesc), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  return L;
+}
+
+hipError_t launch_cold(const LaunchCtx &c, int slot, double phi0, double sigma0, double singlet0) {
+  hipError_t e;
+  LatDesc L = host_desc_of(c, slot, e);
+  if (e != hipSuccess) return e;
+  int nb = (L.Vh + BLK - 1) / BLK;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_cold<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L, phi0, sigma0, singlet0);
+      return hipGetLastError();
+    });
+  });
+}
+
+hipError_t launch_hot(const LaunchCtx &c, int slot) {
+  hipError_t e;
+  LatDesc L = host_desc_of(c, slot, e);
+  if (e != hipSuccess) return e;
+  int nb = (L.Vh + BLK - 1) / BLK;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_hot<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L);
+      return hipGetLastError();
+    });
+  });
+}
+
+hipError_t launch_to_device_layout(const LaunchCtx &c, int slot, int field, const double *h) {
+  hipError_t e;
+  LatDesc L = host_desc_of(c, slot, e);
+  if (e != hipSuccess) return e;
+  int nb = (L.Vh + BLK - 1) / BLK;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    hipLaunchKernelGGL(k_to_device<T>, dim3(nb), dim3(BLK), 0, c.stream, L, field, h);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_to_host_layout(const LaunchCtx &c, int slot, int field, double *h) {
+  hipError_t e;
+  LatDesc L = host_desc_of(c, slot, e);
+  if (e != hipSuccess) return e;
+  int nb = (L.Vh + BLK - 1) / BLK;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    hipLaunchKernelGGL(k_to_host<T>, dim3(nb), dim3(BLK), 0, c.stream, L, field, h);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_copy_probe(const double *src, double *dst, size_t n, hipStream_t s) {
+  size_t n2 = n / 2;
+  int nb = 256 * 16;
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, s, (const double2 *)src, (double2 *)dst, n2);
+  return hipGetLastError();
+}
+
+hipError_t launch_flow_stage(const LaunchCtx &c, int slot, const void *Uin, void *Uout,
+                             const void *Gin, void *Gout, void *Z, void *ZG, double dt,
+                             double msq_t, int stage) {
+  (void)slot;
+  if (c.nblk == 0) return hipSuccess;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_flow<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.blk,
+                         (const T *)Uin, (T *)Uout, (const T *)Gin, (T *)Gout, (T *)Z, (T *)ZG, dt,
+                         msq_t, stage);
+      return hipGetLastError();
+    });
+  });
+}
+
+hipError_t launch_flow_force(const LaunchCtx &c, int slot, double *force) {
+  (void)slot;
+  if (c.nblk == 0) return hipSuccess;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_flow_force<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.blk,
+                         force);
+      return hipGetLastError();
+    });
+  });
+}
+
+}  // namespace su2

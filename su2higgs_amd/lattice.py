@@ -1,0 +1,163 @@
+"""Python handle over libsu2hip.so: a batch of lattices on one GPU stream.
+
+Every method is one C ABI call (include/su2hip.h); arrays are numpy in the
+reference's host layout (double[V][3][4] links, double[V][4] doublet, ...).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import NSUMS, Counters, Muca, Params, check, ptr, volume
+
+
+class Batch:
+    def __init__(self, params, max_slots=None, hbm_budget=0):
+        if isinstance(params, Params):
+            params = [params]
+        self.lib = abi.load()
+        n = len(params)
+        arr = (Params * n)(*params)
+        h = C.c_void_p()
+        check(self.lib.su2hip_create(n, arr, max_slots or n, hbm_budget, C.byref(h)))
+        self.h = h
+        self.params = {i: params[i] for i in range(n)}
+        self.max_slots = max_slots or n
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.su2hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # slots ----------------------------------------------------------------
+    def add(self, p):
+        s = C.c_int()
+        check(self.lib.su2hip_add(self.h, C.byref(p), C.byref(s)))
+        self.params[s.value] = p
+        return s.value
+
+    def remove(self, slot):
+        check(self.lib.su2hip_remove(self.h, slot))
+        self.params.pop(slot, None)
+
+    # fields ---------------------------------------------------------------
+    def set_fields(self, slot, f):
+        """Upload the fields this lattice has (others in f are ignored)."""
+        have = abi.field_arrays(self.params[slot])
+        check(self.lib.su2hip_set_fields(self.h, slot, *[
+            ptr(np.ascontiguousarray(f[k], dtype=np.float64)) if have[k] is not None and f.get(k) is not None
+            else None for k in ("su2link", "u1link", "doublet", "triplet", "singlet")]))
+
+    def get_fields(self, slot):
+        f = abi.field_arrays(self.params[slot])
+        check(self.lib.su2hip_get_fields(self.h, slot, *[ptr(f[k]) for k in
+                                                         ("su2link", "u1link", "doublet", "triplet", "singlet")]))
+        return f
+
+    def cold_start(self, slot, phi0=0.0, sigma0=0.0, singlet0=0.0):
+        check(self.lib.su2hip_cold_start(self.h, slot, phi0, sigma0, singlet0))
+
+    def hot_start(self, slot):
+        check(self.lib.su2hip_hot_start(self.h, slot))
+
+    # sweeps ---------------------------------------------------------------
+    def update(self, n=1):
+        check(self.lib.su2hip_update(self.h, n))
+
+    def pass_(self, kind, parity, d=0):
+        check(self.lib.su2hip_pass(self.h, kind, parity, d))
+
+    def sync(self):
+        check(self.lib.su2hip_sync(self.h))
+
+    def use_graph(self, on=True):
+        check(self.lib.su2hip_use_graph(self.h, 1 if on else 0))
+
+    def set_budget(self, slot, n):
+        check(self.lib.su2hip_set_budget(self.h, slot, n))
+
+    def budgets(self):
+        b = (C.c_int64 * self.max_slots)()
+        check(self.lib.su2hip_get_budgets(self.h, b))
+        return list(b)
+
+    def counters(self, slot):
+        c = Counters()
+        check(self.lib.su2hip_get_counters(self.h, slot, C.byref(c)))
+        return c.asdict()
+
+    def reset_counters(self, slot):
+        check(self.lib.su2hip_reset_counters(self.h, slot))
+
+    def rng_get(self, slot):
+        s, q = C.c_uint64(), C.c_uint64()
+        check(self.lib.su2hip_rng_get(self.h, slot, C.byref(s), C.byref(q)))
+        return s.value, q.value
+
+    def rng_set(self, slot, seed, pas):
+        check(self.lib.su2hip_rng_set(self.h, slot, seed, pas))
+
+    # measurement --------------------------------------------------------------
+    def measure(self, slot):
+        s = np.zeros(NSUMS)
+        check(self.lib.su2hip_measure(self.h, slot, ptr(s)))
+        return s
+
+    def measure_all(self):
+        s = np.zeros((self.max_slots, NSUMS))
+        check(self.lib.su2hip_measure_all(self.h, ptr(s)))
+        return s
+
+    # multicanonical -----------------------------------------------------------
+    def muca_set(self, slot, m, pos, W):
+        check(self.lib.su2hip_muca_set(self.h, slot, C.byref(m),
+                                       ptr(np.ascontiguousarray(pos, dtype=np.float64)),
+                                       ptr(np.ascontiguousarray(W, dtype=np.float64))))
+
+    def muca_get(self, slot):
+        m = Muca()
+        check(self.lib.su2hip_muca_get(self.h, slot, C.byref(m), None, None, None))
+        n = m.bins + 2 if m.enabled else 0
+        pos, W, hits = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int32)
+        if n:
+            check(self.lib.su2hip_muca_get(self.h, slot, C.byref(m), ptr(pos), ptr(W),
+                                           hits.ctypes.data_as(C.POINTER(C.c_int32))))
+        return m, pos, W, hits
+
+    # flow ---------------------------------------------------------------------
+    def flow(self, slot, scheme, dt, tmax, meas, max_rows=100000):
+        rows = np.zeros((max_rows, 1 + NSUMS))
+        n = C.c_int()
+        check(self.lib.su2hip_flow(self.h, slot, scheme, dt, tmax, meas, ptr(rows), max_rows, C.byref(n)))
+        return rows[:min(n.value, max_rows)]
+
+    def flow_force(self, slot):
+        F = np.zeros((volume(self.params[slot]), 3, 4))
+        check(self.lib.su2hip_flow_force(self.h, slot, ptr(F)))
+        return F
+
+    # timing -------------------------------------------------------------------
+    def set_timing(self, on=True):
+        check(self.lib.su2hip_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = np.zeros(abi.NKINDS)
+        n = (C.c_int64 * abi.NKINDS)()
+        check(self.lib.su2hip_kernel_times(self.h, ptr(ms), n))
+        return ms, np.array(list(n))
+
+    def reset_timing(self):
+        check(self.lib.su2hip_reset_timing(self.h))
+
+
+def copy_probe(nbytes, repeats=20):
+    lib = abi.load()
+    ms = C.c_double()
+    check(lib.su2hip_copy_probe(nbytes, repeats, C.byref(ms)))
+    return ms.value

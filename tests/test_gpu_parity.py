@@ -1,0 +1,261 @@
+"""HIP path against the oracle on the same inputs and Philox draws (FP64).
+
+Tolerances: deterministic quantities 1e-12 relative (the north-star bar for
+staples, action density, overrelaxation and flow steps); after stochastic
+passes the fields agree to rounding (1e-11 absolute on O(1) values) because
+every accept/reject decision is the same; counters are exact."""
+import numpy as np
+import pytest
+
+import orc
+from su2higgs_amd import abi
+from su2higgs_amd.abi import (FP64, HEATBATH, METROPOLIS, OVERRELAX, K_DOUBLET_METROPOLIS,
+                              K_DOUBLET_OVERRELAX, K_LINK_HEATBATH, K_LINK_METROPOLIS,
+                              K_LINK_OVERRELAX, K_SINGLET_METROPOLIS, K_SINGLET_OVERRELAX,
+                              K_TRIPLET_METROPOLIS, K_TRIPLET_OVERRELAX, K_U1_METROPOLIS,
+                              make_params)
+from su2higgs_amd.lattice import Batch
+
+pytestmark = pytest.mark.gpu
+
+MODELS = {
+    "g": dict(nhiggs=0),
+    "h": dict(nhiggs=1),
+    "hu": dict(nhiggs=1, u1=1),
+    "hs": dict(nhiggs=1, singlet=1),
+    "hsu": dict(nhiggs=1, u1=1, singlet=1),
+    "t": dict(nhiggs=0, triplet=1),
+    "ht": dict(nhiggs=1, triplet=1),
+}
+FIELDS = ("su2link", "u1link", "doublet", "triplet", "singlet")
+
+
+def pair(tag, L=(4, 6, 8), **kw):
+    p = make_params(L=L, **MODELS[tag], **kw)
+    o = orc.Oracle(p)
+    o.hot_start()
+    f = o.get_fields()
+    b = Batch(p)
+    b.set_fields(0, f)
+    return p, o, b
+
+
+def maxdiff(fo, fg):
+    d = 0.0
+    for k in FIELDS:
+        if fg.get(k) is not None:
+            d = max(d, np.abs(fo[k] - fg[k]).max())
+    return d
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_roundtrip_and_hot_start(tag):
+    p, o, b = pair(tag)
+    f = o.get_fields()
+    g = b.get_fields(0)
+    for k in FIELDS:
+        if g.get(k) is not None:
+            assert np.array_equal(g[k], f[k]), k
+    b.hot_start(0)
+    g = b.get_fields(0)
+    assert maxdiff(f, g) < 1e-13
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_measure_sums(tag):
+    p, o, b = pair(tag)
+    a, g = o.measure(), b.measure(0)
+    scale = np.maximum(np.abs(a), 1.0)
+    assert (np.abs(a - g) / scale).max() < 1e-12
+
+
+def kinds(p):
+    ks = [(K_LINK_HEATBATH, 3), (K_LINK_METROPOLIS, 3), (K_LINK_OVERRELAX, 3)]
+    if p.u1:
+        ks.append((K_U1_METROPOLIS, 3))
+    if p.nhiggs:
+        ks += [(K_DOUBLET_OVERRELAX, 1), (K_DOUBLET_METROPOLIS, 1)]
+    if p.triplet:
+        ks += [(K_TRIPLET_OVERRELAX, 1), (K_TRIPLET_METROPOLIS, 1)]
+    if p.singlet:
+        ks += [(K_SINGLET_OVERRELAX, 1), (K_SINGLET_METROPOLIS, 1)]
+    return ks
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_passes_site_by_site(tag):
+    """Every pass kind, both parities: same accepts, same fields."""
+    p, o, b = pair(tag)
+    for kind, ndir in kinds(p):
+        for d in range(ndir):
+            for par in (0, 1):
+                o.pass_(kind, par, d)
+                b.pass_(kind, par, d)
+        fo, fg = o.get_fields(), b.get_fields(0)
+        assert maxdiff(fo, fg) < 1e-11, kind
+        # resynchronise rounding before the next kind
+        b.set_fields(0, fo)
+    co, cg = o.counters(), b.counters(0)
+    for k in co:
+        assert co[k] == cg[k], k
+    assert o.rng_get()[1] == b.rng_get(0)[1]
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_compound_sweeps(tag):
+    p, o, b = pair(tag, link_overrelax=2)
+    o.update(2)
+    b.update(2)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-10
+    co, cg = o.counters(), b.counters(0)
+    for k in co:
+        assert co[k] == cg[k], k
+    assert o.rng_get()[1] == b.rng_get(0)[1]
+
+
+def test_graph_replay_matches_eager():
+    p = make_params(L=(8, 8, 8), nhiggs=1, link_overrelax=2)
+    b1, b2 = Batch(p), Batch(p)
+    b1.hot_start(0)
+    b2.hot_start(0)
+    b2.use_graph(True)
+    for _ in range(3):
+        b1.update(2)
+        b2.update(2)
+        b1.sync()
+        b2.sync()
+        f1, f2 = b1.get_fields(0), b2.get_fields(0)
+        for k in FIELDS:
+            if f1.get(k) is not None:
+                assert np.array_equal(f1[k], f2[k])
+    assert b1.counters(0) == b2.counters(0)
+
+
+def test_batch_same_alone_and_in_mix():
+    """Lattices of different volume swept in one launch give bitwise the
+    results each gives alone."""
+    ps = [make_params(L=L, nhiggs=1, seed=100 + i, link_overrelax=1, msq_phi=-0.3 + 0.05 * i)
+          for i, L in enumerate([(4, 4, 4), (6, 4, 8), (8, 8, 8), (4, 6, 6)])]
+    mix = Batch(ps)
+    for i in range(len(ps)):
+        mix.hot_start(i)
+    mix.update(3)
+    for i, p in enumerate(ps):
+        alone = Batch(p)
+        alone.hot_start(0)
+        alone.update(3)
+        fa, fm = alone.get_fields(0), mix.get_fields(i)
+        for k in FIELDS:
+            if fa.get(k) is not None:
+                assert np.array_equal(fa[k], fm[k]), (i, k)
+        assert np.array_equal(alone.measure(0), mix.measure(i))
+
+
+def test_batch_matches_oracle():
+    ps = [make_params(L=L, nhiggs=1, seed=7 + i) for i, L in enumerate([(4, 4, 4), (6, 4, 8)])]
+    mix = Batch(ps)
+    for i, p in enumerate(ps):
+        o = orc.Oracle(p)
+        o.hot_start()
+        mix.set_fields(i, o.get_fields())
+        o.update(2)
+        ps[i] = (p, o)
+    mix.update(2)
+    for i, (p, o) in enumerate(ps):
+        assert maxdiff(o.get_fields(), mix.get_fields(i)) < 1e-10
+
+
+def test_budgets():
+    ps = [make_params(L=(4, 4, 4), nhiggs=1, seed=i) for i in range(3)]
+    b = Batch(ps)
+    for i in range(3):
+        b.hot_start(i)
+    f1 = b.get_fields(1)
+    b.set_budget(0, 2)
+    b.set_budget(1, 0)
+    b.update(3)
+    bud = b.budgets()
+    assert bud[0] == 0 and bud[1] == 0 and bud[2] == -1
+    # lattice 0 advanced exactly 2 sweeps: compare with the oracle
+    o = orc.Oracle(ps[0])
+    o.hot_start()
+    o.update(2)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-10
+    # lattice 1 (budget 0) was not touched at all
+    assert maxdiff(f1, b.get_fields(1)) == 0.0
+
+
+def test_flow_force_and_euler():
+    p = make_params(L=(4, 6, 8), nhiggs=1)
+    o = orc.Oracle(p)
+    o.hot_start()
+    b = Batch(p)
+    b.set_fields(0, o.get_fields())
+    F, _ = o.flow_force()
+    G = b.flow_force(0)
+    assert np.abs(F - G).max() < 1e-12 * max(1.0, np.abs(F).max())
+    ro = o.flow(abi.FLOW_EULER, 0.01, 0.05, 2)
+    rg = b.flow(0, abi.FLOW_EULER, 0.01, 0.05, 2)
+    assert ro.shape == rg.shape
+    assert np.abs(ro - rg).max() / np.abs(ro).max() < 1e-12
+    ro = o.flow(abi.FLOW_RK3, 0.02, 0.1, 1)
+    rg = b.flow(0, abi.FLOW_RK3, 0.02, 0.1, 1)
+    assert np.abs(ro - rg).max() / np.abs(ro).max() < 1e-12
+
+
+def muca_weight(bins=40, lo=0.0, hi=2.0, seed=3, slope=5.0):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros(bins + 2)
+    W = np.zeros(bins + 2)
+    dbin = (hi - lo) / bins
+    for i in range(bins + 1):
+        pos[i + 1] = lo + i * dbin
+    slopes = rng.uniform(-slope, slope, bins)
+    for i in range(1, bins + 1):
+        W[i + 1] = W[i] + slopes[i - 1] * dbin
+    pos[0] = lo - 1000.0 * (hi - lo)
+    W[0] = W[1]
+    W[bins + 1] = W[bins]
+    return pos, W
+
+
+@pytest.mark.parametrize("mode", [abi.READONLY, abi.FAST])
+@pytest.mark.parametrize("tag,op,cps,slope", [("h", abi.PHISQ, 1, 5.0), ("t", abi.SIGMASQ, 1, 5.0),
+                                               ("h", abi.PHISQ, 3, 5.0),
+                                               ("ht", abi.PHI2MINUSSIGMA2, 2, 5.0),
+                                               ("h", abi.PHISQ, 3, 400.0),
+                                               ("t", abi.SIGMASQ, 2, 400.0)])
+def test_muca_sweeps(tag, op, cps, slope, mode):
+    p, o, b = pair(tag, L=(4, 4, 6))
+    m = abi.Muca()
+    m.enabled = 1
+    m.orderparam = op
+    m.mode = mode
+    m.checks_per_sweep = cps
+    m.bins = 40
+    m.min, m.max = 0.0, 2.0
+    m.wrk_min, m.wrk_max = 0.0, 2.0
+    m.do_acceptance = 1
+    m.delta = 0.3
+    m.update_interval = 5
+    lo, hi = (-1.0, 1.0) if op == abi.PHI2MINUSSIGMA2 else (0.0, 2.0)
+    m.min, m.max, m.wrk_min, m.wrk_max = lo, hi, lo, hi
+    pos, W = muca_weight(lo=lo, hi=hi, slope=slope)
+    o.muca_set(m, pos, W)
+    b.muca_set(0, m, pos, W)
+    mo, _, _, _ = o.muca_get()
+    mg, _, _, _ = b.muca_get(0)
+    assert abs(mo.param_value[0] - mg.param_value[0]) < 1e-13
+    o.update(3)
+    b.update(3)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-10
+    co, cg = o.counters(), b.counters(0)
+    for k in co:
+        assert co[k] == cg[k], k
+    if slope > 100:
+        assert 0 < co["accepted_muca"] < co["total_muca"]  # the restore path ran
+    mo, po, Wo, ho = o.muca_get()
+    mg, pg, Wg, hg = b.muca_get(0)
+    assert np.abs(Wo - Wg).max() < 1e-12
+    assert (ho == hg).all()
+    assert abs(mo.param_value[1] - mg.param_value[1]) < 1e-13
