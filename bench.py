@@ -1,0 +1,339 @@
+#!/usr/bin/env python3
+"""Throughput of the lattice update sweep on one MI355X.
+
+A step is one compound sweep of the 96^3 SU(2)xU(1)+Higgs lattice
+(betaG=9, betaY=27, r=1, msq=-0.20, lambda=0.015, cold start phi=(1,0,0,0)):
+1 link heatbath + 4 link overrelaxation rounds + U(1) Metropolis +
+4 Higgs overrelaxation + 1 Higgs Metropolis, i.e. 46 checkerboard passes and
+23 V variable updates.  `value` is FP32-mixed site-updates/s (FP32 fields and
+staples, FP64 reductions), timed with the fields resident in HBM; the same
+line reports the plain FP64 rate, the launch-by-launch rate, a batched scan
+(96 lattices of 16^3..48^3 in one launch per pass), per-kernel HIP-event
+times, the HBM roofline of the dominant kernel and the reference's CPU
+build timed on this host.
+
+    python bench.py [--gpus 1] [--steps 500] [--warmup 20]
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_params(precision, L=96):
+    from su2higgs_amd.abi import FP32, make_params
+    return make_params(L=(L, L, L), nhiggs=1, u1=1, precision=precision, betasu2=9.0,
+                       betau1=27.0, r_u1=1.0, msq_phi=-0.20, lambda_phi=0.015, link_overrelax=4,
+                       scalar_sweeps=4, update_su2doublet=1, update_links=1,
+                       reunitarize=1 if precision == FP32 else 0, seed=20240096)
+
+
+def passes_and_updates(p):
+    """(passes per sweep, variable updates per sweep / V)."""
+    links = p.update_links * (1 + p.link_overrelax + (1 if p.u1 else 0)) * 6
+    scal = 0
+    if p.nhiggs:
+        scal += (p.scalar_sweeps + (1 if p.algorithm_su2doublet == 3 else 0)) * p.update_su2doublet * 2
+    return links + scal, (links + scal) / 2
+
+
+def algo_words(kind, p):
+    """Compulsory HBM words per pass / V (each datum read or written once)."""
+    from su2higgs_amd import abi
+    nh, u1, tr, sg = p.nhiggs, p.u1, p.triplet, p.singlet
+    if kind in (abi.K_LINK_HEATBATH, abi.K_LINK_METROPOLIS, abi.K_LINK_OVERRELAX):
+        w = 2 + 8 + 2 + 4 * nh + 0.5 * u1 + 3 * tr  # U_mu(1-p), U_nu, write, phi, alpha_mu(p), Sigma
+        if kind != abi.K_LINK_HEATBATH:
+            w += 2  # the old link U_mu(p)
+        return w
+    if kind == abi.K_U1_METROPOLIS:
+        return 3 + 4 * nh + 2 + 0.5
+    if kind in (abi.K_DOUBLET_OVERRELAX, abi.K_DOUBLET_METROPOLIS):
+        return 4 + 12 + 2 + 3 * u1 + 1.5 * tr + 0.5 * sg
+    if kind in (abi.K_TRIPLET_OVERRELAX, abi.K_TRIPLET_METROPOLIS):
+        return 3 + 12 + 1.5 + 2 * nh
+    if kind in (abi.K_SINGLET_OVERRELAX, abi.K_SINGLET_METROPOLIS):
+        return 1 + 0.5 + 2 * nh
+    return None
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/*.json)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        e = d.get(workload, {}).get(kernel)
+        return None if e is None else float(e["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100):
+    """Timed region: `steps` compound sweeps replayed as a hipGraph on the
+    library's stream (wall clock between stream synchronisations).  Kernel
+    region: `timing_steps` sweeps launched one by one with a HIP event pair
+    around every launch on the same stream (per-kernel durations)."""
+    from su2higgs_amd import abi
+    from su2higgs_amd.lattice import Batch
+    p = workload_params(precision)
+    V = abi.volume(p)
+    npass, upd = passes_and_updates(p)
+    b = Batch(p)
+    b.cold_start(0, phi0=1.0)
+    b.use_graph(True)
+    b.update(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.update(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    res = {"precision": precision, "V": V, "passes_per_sweep": npass, "updates_per_sweep_over_V": upd,
+           "wall_s": wall, "ms_per_sweep": 1e3 * wall / steps,
+           "site_updates_per_s": upd * V * steps / wall}
+    b.use_graph(False)
+    b.set_timing(True)
+    ts = max(1, min(steps, timing_steps))
+    b.reset_timing()
+    e0 = time.perf_counter()
+    b.update(ts)
+    b.sync()
+    res["eager_site_updates_per_s"] = upd * V * ts / (time.perf_counter() - e0)
+    ms, n = b.kernel_times()
+    b.set_timing(False)
+    res["kernel_ms"] = {abi.KIND_NAMES[k]: float(ms[k]) for k in range(abi.NKINDS) if n[k]}
+    res["kernel_launches"] = {abi.KIND_NAMES[k]: int(n[k]) for k in range(abi.NKINDS) if n[k]}
+    res["timing_sweeps"] = ts
+    if measure_obs:
+        b.use_graph(True)
+        plaq, phi2 = [], []
+        for _ in range(40):
+            b.update(5)
+            s = b.measure(0)
+            plaq.append(1.0 - s[abi.SUM["wilson"]] / (V * p.betasu2) / 3.0)
+            phi2.append(s[abi.SUM["phi2"]] / V)
+        plaq, phi2 = np.array(plaq), np.array(phi2)
+        res["observables"] = {
+            "plaquette": [float(plaq.mean()), float(plaq.std(ddof=1) / np.sqrt(len(plaq)))],
+            "phi2": [float(phi2.mean()), float(phi2.std(ddof=1) / np.sqrt(len(phi2)))],
+            "after_sweeps": warmup + steps + ts, "samples": len(plaq)}
+    res["p"] = p
+    b.close()
+    return res
+
+
+def roofline(res):
+    from su2higgs_amd import abi
+    p = res["p"]
+    w = 8 if p.precision == abi.FP64 else 4
+    kms = res["kernel_ms"]
+    kind = max((k for k in kms if k not in ("measure", "aux", "muca")), key=lambda k: kms[k])
+    kid = abi.KIND_NAMES.index(kind)
+    avg_ms = kms[kind] / res["kernel_launches"][kind]
+    words = algo_words(kid, p)
+    bytes_per_launch = words * res["V"] * w
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    workload = "96^3 SU(2)xU(1)+Higgs fp%d" % (64 if p.precision == abi.FP64 else 32)
+    return {"bound": "hbm", "kernel": kind, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(kind, workload),
+            "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
+            "words_per_site_per_pass": words}
+
+
+def batched_scan(steps):
+    """96 independent SU(2)+Higgs lattices, volumes drawn from
+    {16^3,24^3,32^3,48^3}, msq spaced across [-0.30,-0.15], FP32, one launch per
+    pass for the whole batch, compound sweeps replayed as a hipGraph."""
+    from su2higgs_amd import abi
+    from su2higgs_amd.abi import FP32, make_params
+    from su2higgs_amd.lattice import Batch
+    rng = np.random.default_rng(96)
+    Ls = rng.choice([16, 24, 32, 48], size=96)
+    ps = [make_params(L=(int(L),) * 3, nhiggs=1, precision=FP32, betasu2=8.0, lambda_phi=0.02,
+                      msq_phi=-0.30 + 0.15 * k / 95, link_overrelax=4, scalar_sweeps=4,
+                      reunitarize=1, seed=1000 + k) for k, L in enumerate(Ls)]
+    b = Batch(ps)
+    for i in range(len(ps)):
+        b.hot_start(i)
+    b.use_graph(True)
+    b.update(3)
+    b.sync()
+    t0 = time.perf_counter()
+    b.update(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    Vtot = sum(abi.volume(p) for p in ps)
+    _, upd = passes_and_updates(ps[0])
+    b.close()
+    return {"lattices": 96, "total_sites": int(Vtot), "sweeps": steps,
+            "ms_per_sweep": 1e3 * wall / steps, "site_updates_per_s": upd * Vtot * steps / wall}
+
+
+REF_CONFIG = """dim 3
+L1 {L}
+L2 {L}
+L3 {L}
+reset 1
+iterations {it}
+n_thermalize 0
+interval 1000000
+checkpoint 1000000
+run_checks 0
+resultsfile measure
+latticefile lattice
+betasu2 9.0
+betau1 27.0
+r_u1 1
+msq -0.20
+lambda 0.015
+phi0 1.0
+algorithm_su2link heatbath
+algorithm_u1link metropolis
+algorithm_su2doublet overrelax
+scalar_sweeps 4
+update_doublet 1
+update_links 1
+multicanonical 0
+measure_local 0
+"""
+
+
+def cpu_baseline():
+    """The reference's serial CPU build (oracle/_ref/su2_hu, -DNHIGGS=1 -DU1)
+    on this host: wall(4 sweeps) - wall(1 sweep) of 96^3, cold start, the
+    reference's own mix (heatbath + U(1) Metropolis + 4 OR + 1 Metropolis =
+    11 V variable updates per sweep; it has no link overrelaxation)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "su2_hu")
+    if not os.path.exists(exe):
+        return oracle_baseline()
+    walls = {}
+    d = tempfile.mkdtemp(prefix="su2ref_")
+    try:
+        for it in (1, 4):
+            with open(os.path.join(d, "config"), "w") as f:
+                f.write(REF_CONFIG.format(L=96, it=it))
+            for fn in ("lattice", "measure", "labels"):
+                try:
+                    os.remove(os.path.join(d, fn))
+                except OSError:
+                    pass
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "config"], cwd=d, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL, timeout=300)
+            walls[it] = time.perf_counter() - t0
+            if r.returncode != 0:
+                return oracle_baseline()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    per_sweep = (walls[4] - walls[1]) / 3
+    V = 96 ** 3
+    return {"value": 11 * V / per_sweep, "unit": "site-updates/s", "cores": 1, "kind": "reference",
+            "sample": "reference serial build (gcc -O3 -fcommon -DNHIGGS=1 -DU1), 96^3 cold start, "
+                      "wall(4 sweeps)-wall(1 sweep) = %.2f s/sweep, 11 V variable updates per sweep"
+                      % per_sweep}
+
+
+def oracle_baseline():
+    """Fallback: the FP64 oracle restatement (same mix as the GPU) on 32^3."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    p = workload_params(64, L=32)
+    o = orc.Oracle(p)
+    o.cold_start(phi0=1.0)
+    o.update(1)
+    t0 = time.perf_counter()
+    o.update(2)
+    per = (time.perf_counter() - t0) / 2
+    _, upd = passes_and_updates(p)
+    return {"value": upd * 32 ** 3 / per, "unit": "site-updates/s", "cores": 1, "kind": "port",
+            "sample": "oracle restatement, 32^3, 2 sweeps of the GPU mix (%.2f s/sweep)" % per}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(lattices would be sharded across GPUs with no communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    from su2higgs_amd import abi
+    log("[bench] FP32-mixed 96^3, %d warmup + %d timed sweeps" % (args.warmup, args.steps))
+    r32 = run_precision(abi.FP32, args.steps, args.warmup)
+    log("[bench] fp32 %.3f ms/sweep, %.3e site-updates/s" % (r32["ms_per_sweep"], r32["site_updates_per_s"]))
+    s64 = max(1, min(args.steps, 200))
+    log("[bench] FP64 plain 96^3, %d timed sweeps" % s64)
+    r64 = run_precision(abi.FP64, s64, max(1, args.warmup // 2))
+    log("[bench] fp64 %.3f ms/sweep, %.3e site-updates/s" % (r64["ms_per_sweep"], r64["site_updates_per_s"]))
+    roof = roofline(r32)
+    roof64 = roofline(r64)
+    batched = None
+    if not args.no_batch:
+        log("[bench] batched scan, 96 lattices")
+        batched = batched_scan(max(5, min(args.steps // 10, 50)))
+        log("[bench] batched %.3e site-updates/s" % batched["site_updates_per_s"])
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline()
+        log("[bench] cpu %.3e site-updates/s (%s)" % (cpu["value"], cpu["kind"]))
+    o32, o64 = r32.get("observables", {}), r64.get("observables", {})
+    out = {
+        "metric": "site-updates/s/GPU, FP32-mixed batched vs FP64 plain; p50/p99 scan latency",
+        "value": r32["site_updates_per_s"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_sweep"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: cold start phi=(1,0,0,0), Philox4x32-10 draws",
+        "config": {"workload": "96^3 SU(2)xU(1)+fundamental Higgs, betaG=9 betaY=27 msq=-0.20 "
+                               "lambda=0.015, sweep = 1 link heatbath + 4 link overrelax + U(1) "
+                               "Metropolis + 4 Higgs overrelax + 1 Higgs Metropolis",
+                   "L": 96, "passes_per_sweep": r32["passes_per_sweep"],
+                   "variable_updates_per_sweep": "%g V" % r32["updates_per_sweep_over_V"],
+                   "precision": "FP32 fields/staples, FP64 reductions"},
+        "fp64_value": r64["site_updates_per_s"],
+        "fp64_ms_per_step": r64["ms_per_sweep"],
+        "fp32_over_fp64": r32["site_updates_per_s"] / r64["site_updates_per_s"],
+        "eager_value": r32["eager_site_updates_per_s"],
+        "fp64_eager_value": r64["eager_site_updates_per_s"],
+        "batched_scan": batched,
+        "observables": {"fp32": o32, "fp64": o64},
+        "roofline": roof,
+        "roofline_fp64": roof64,
+        "kernel_ms_fp32": r32["kernel_ms"],
+        "kernel_launches_fp32": r32["kernel_launches"],
+        "kernel_ms_fp64": r64["kernel_ms"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
