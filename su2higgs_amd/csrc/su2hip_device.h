@@ -143,10 +143,12 @@ __device__ __forceinline__ V3<T> rotate(const Q<T> &u, const V3<T> &v) {
 }
 
 // ------------------------------------------------------------------------
-// geometry
+// geometry: a site carries its coordinates, parity and index in its parity;
+// unit steps update the index incrementally (periodic wrap)
 
 struct Site {
   int x, y, z;
+  int par, s;
 };
 
 __device__ __forceinline__ Site site_of(const LatDesc &L, int par, int s) {
@@ -158,65 +160,89 @@ __device__ __forceinline__ Site site_of(const LatDesc &L, int par, int s) {
   c.y = y;
   c.z = (int)z;
   c.x = 2 * xh + ((par + y + (int)z) & 1);
+  c.par = par;
+  c.s = s;
   return c;
 }
 
-__device__ __forceinline__ int ext(const LatDesc &L, int d) {
-  return d == 0 ? L.L0 : (d == 1 ? L.L1 : L.L2);
-}
-
-__device__ __forceinline__ Site shift(const LatDesc &L, Site c, int d, int sign) {
-  int n = ext(L, d);
+__device__ __forceinline__ Site shift(const LatDesc &L, const Site &c, int d, int sign) {
+  Site r = c;
+  r.par = c.par ^ 1;
   if (d == 0) {
-    c.x += sign;
-    c.x = c.x < 0 ? n - 1 : (c.x >= n ? 0 : c.x);
+    int nx = c.x + sign;
+    nx = nx < 0 ? L.L0 - 1 : (nx >= L.L0 ? 0 : nx);
+    r.x = nx;
+    r.s = c.s - (c.x >> 1) + (nx >> 1);
   } else if (d == 1) {
-    c.y += sign;
-    c.y = c.y < 0 ? n - 1 : (c.y >= n ? 0 : c.y);
+    if (sign > 0) {
+      bool w = c.y + 1 == L.L1;
+      r.y = w ? 0 : c.y + 1;
+      r.s = w ? c.s - L.h0 * (L.L1 - 1) : c.s + L.h0;
+    } else {
+      bool w = c.y == 0;
+      r.y = w ? L.L1 - 1 : c.y - 1;
+      r.s = w ? c.s + L.h0 * (L.L1 - 1) : c.s - L.h0;
+    }
   } else {
-    c.z += sign;
-    c.z = c.z < 0 ? n - 1 : (c.z >= n ? 0 : c.z);
+    int plane = L.h0 * L.L1;
+    if (sign > 0) {
+      bool w = c.z + 1 == L.L2;
+      r.z = w ? 0 : c.z + 1;
+      r.s = w ? c.s - plane * (L.L2 - 1) : c.s + plane;
+    } else {
+      bool w = c.z == 0;
+      r.z = w ? L.L2 - 1 : c.z - 1;
+      r.s = w ? c.s + plane * (L.L2 - 1) : c.s - plane;
+    }
   }
-  return c;
-}
-
-__device__ __forceinline__ int par_of(const Site &c) { return (c.x + c.y + c.z) & 1; }
-__device__ __forceinline__ int s_of(const LatDesc &L, const Site &c) {
-  return (c.x >> 1) + L.h0 * (c.y + L.L1 * c.z);
+  return r;
 }
 
 // ------------------------------------------------------------------------
-// field access (layout in su2hip_internal.h)
+// field access (layout in su2hip_internal.h): links and doublet as
+// 4-component site vectors (one 16-byte load per site in FP32), the others
+// component-major
+
+template <typename T>
+__device__ __forceinline__ Q<T> ld4(const T *p);
+template <>
+__device__ __forceinline__ Q<float> ld4<float>(const float *p) {
+  float4 v = *reinterpret_cast<const float4 *>(p);
+  return Q<float>{v.x, v.y, v.z, v.w};
+}
+template <>
+__device__ __forceinline__ Q<double> ld4<double>(const double *p) {
+  double2 a = *reinterpret_cast<const double2 *>(p);
+  double2 b = *reinterpret_cast<const double2 *>(p + 2);
+  return Q<double>{a.x, a.y, b.x, b.y};
+}
+template <typename T>
+__device__ __forceinline__ void st4(T *p, const Q<T> &q);
+template <>
+__device__ __forceinline__ void st4<float>(float *p, const Q<float> &q) {
+  *reinterpret_cast<float4 *>(p) = make_float4(q.a, q.b, q.c, q.d);
+}
+template <>
+__device__ __forceinline__ void st4<double>(double *p, const Q<double> &q) {
+  *reinterpret_cast<double2 *>(p) = make_double2(q.a, q.b);
+  *reinterpret_cast<double2 *>(p + 2) = make_double2(q.c, q.d);
+}
 
 template <typename T>
 __device__ __forceinline__ Q<T> ldU(const T *__restrict__ U, int Vh, int d, int par, int s) {
-  const T *b = U + (size_t)(d * 8 + par) * Vh + s;
-  size_t st = (size_t)2 * Vh;
-  return Q<T>{b[0], b[st], b[2 * st], b[3 * st]};
+  return ld4(U + ((size_t)(d * 2 + par) * Vh + s) * 4);
 }
 template <typename T>
 __device__ __forceinline__ void stU(T *__restrict__ U, int Vh, int d, int par, int s, const Q<T> &q) {
-  T *b = U + (size_t)(d * 8 + par) * Vh + s;
-  size_t st = (size_t)2 * Vh;
-  b[0] = q.a;
-  b[st] = q.b;
-  b[2 * st] = q.c;
-  b[3 * st] = q.d;
+  st4(U + ((size_t)(d * 2 + par) * Vh + s) * 4, q);
 }
 template <typename T>
 __device__ __forceinline__ Q<T> ldF(const T *__restrict__ F, int Vh, int par, int s) {
-  const T *b = F + (size_t)par * Vh + s;
-  size_t st = (size_t)2 * Vh;
-  return Q<T>{b[0], b[st], b[2 * st], b[3 * st]};
+  return ld4(F + ((size_t)par * Vh + s) * 4);
 }
 template <typename T>
 __device__ __forceinline__ void stF(T *__restrict__ F, int Vh, int par, int s, const Q<T> &q) {
-  T *b = F + (size_t)par * Vh + s;
-  size_t st = (size_t)2 * Vh;
-  b[0] = q.a;
-  b[st] = q.b;
-  b[2 * st] = q.c;
-  b[3 * st] = q.d;
+  st4(F + ((size_t)par * Vh + s) * 4, q);
 }
 template <typename T>
 __device__ __forceinline__ V3<T> ldG(const T *__restrict__ G, int Vh, int par, int s) {
@@ -241,26 +267,26 @@ __device__ __forceinline__ T ldS(const T *__restrict__ S, int Vh, int par, int s
   return S[(size_t)par * Vh + s];
 }
 
-// load by coordinates
+// load by site
 template <typename T>
 __device__ __forceinline__ Q<T> U_at(const LatDesc &L, const T *U, const Site &c, int d) {
-  return ldU(U, L.Vh, d, par_of(c), s_of(L, c));
+  return ldU(U, L.Vh, d, c.par, c.s);
 }
 template <typename T>
 __device__ __forceinline__ Q<T> F_at(const LatDesc &L, const T *F, const Site &c) {
-  return ldF(F, L.Vh, par_of(c), s_of(L, c));
+  return ldF(F, L.Vh, c.par, c.s);
 }
 template <typename T>
 __device__ __forceinline__ V3<T> G_at(const LatDesc &L, const T *G, const Site &c) {
-  return ldG(G, L.Vh, par_of(c), s_of(L, c));
+  return ldG(G, L.Vh, c.par, c.s);
 }
 template <typename T>
 __device__ __forceinline__ T A_at(const LatDesc &L, const T *A, const Site &c, int d) {
-  return ldA(A, L.Vh, d, par_of(c), s_of(L, c));
+  return ldA(A, L.Vh, d, c.par, c.s);
 }
 template <typename T>
 __device__ __forceinline__ T S_at(const LatDesc &L, const T *S, const Site &c) {
-  return ldS(S, L.Vh, par_of(c), s_of(L, c));
+  return ldS(S, L.Vh, c.par, c.s);
 }
 
 // ------------------------------------------------------------------------

@@ -1,18 +1,19 @@
 // su2hip_internal.h -- device data structures shared by the kernels and the
 // C ABI layer of libsu2hip.so.
 //
-// Device layout of one lattice (structure of arrays, checkerboarded).  With
-// Vh = V/2 sites per parity and s the index of a site inside its parity
-// (s = natural_index >> 1 for even L0, which is exactly the reference's
-// "even sites first, order preserved" site order, src/layout.c:826-851):
-//   links    U[d][c][p][s]   d<3, c<4    (24 Vh words)
+// Device layout of one lattice (checkerboarded).  With Vh = V/2 sites per
+// parity and s the index of a site inside its parity (s = natural_index >> 1
+// for even L0, which is exactly the reference's "even sites first, order
+// preserved" site order, src/layout.c:826-851):
+//   links    U[d][p][s][c]   d<3, c<4    (24 Vh words; a 4-vector per link)
+//   doublet  F[p][s][c]      c<4         (8 Vh; a 4-vector per site)
 //   U(1)     A[d][p][s]                  (6 Vh)
-//   doublet  F[c][p][s]      c<4         (8 Vh)
 //   triplet  G[c][p][s]      c<3         (6 Vh)
 //   singlet  S[p][s]                     (2 Vh)
-//   backup   B[c][p][s]      dofs of the multicanonical field
-// so a wavefront reads each component of 64 consecutive sites of one parity
-// as one contiguous segment.
+//   backup   B: the doublet's or the triplet's layout
+// so the 64 lanes of a wavefront, which own 64 consecutive s of one parity,
+// read one contiguous 1 KiB (FP32) segment per link or doublet load and one
+// contiguous segment per component of the other fields.
 #pragma once
 
 #include <hip/hip_runtime.h>

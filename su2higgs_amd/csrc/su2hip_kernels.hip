@@ -213,15 +213,30 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     T rad = (T)1 - a0 * a0;
     if (rad < (T)0) return 0;
     rad = msqrt(rad);
-    d = (T)2;
-    while (d > (T)1 || d == (T)0) {
-      r1 = (T)2 * uni<T>(r) - (T)1;
-      r2 = (T)2 * uni<T>(r) - (T)1;
-      r3 = (T)2 * uni<T>(r) - (T)1;
-      d = r1 * r1 + r2 * r2 + r3 * r3;
+    Q<T> nu;
+    if constexpr (sizeof(T) == 8) {
+      // isotropic direction by cube rejection, the reference's draw order
+      // (src/heatbath.c:82-93)
+      d = (T)2;
+      while (d > (T)1 || d == (T)0) {
+        r1 = (T)2 * uni<T>(r) - (T)1;
+        r2 = (T)2 * uni<T>(r) - (T)1;
+        r3 = (T)2 * uni<T>(r) - (T)1;
+        d = r1 * r1 + r2 * r2 + r3 * r3;
+      }
+      d = msqrt(d);
+      nu = Q<T>{a0, r1 * (rad / d), r2 * (rad / d), r3 * (rad / d)};
+    } else {
+      // FP32: the same uniform direction on S^2 without rejection
+      // (cos(theta) uniform in [-1,1], phi uniform), so a wavefront does
+      // not wait for its slowest lane
+      T cz = (T)2 * uni<T>(r) - (T)1;
+      T ph = (T)(2.0 * PI) * uni<T>(r);
+      T sn, cs;
+      sincosf(ph, &sn, &cs);
+      T sr = msqrt(fmaxf((T)1 - cz * cz, (T)0)) * rad;
+      nu = Q<T>{a0, sr * cs, sr * sn, rad * cz};
     }
-    d = msqrt(d);
-    Q<T> nu{a0, r1 * (rad / d), r2 * (rad / d), r3 * (rad / d)};
     nu = qmul(nu, qconj(V));
     if constexpr ((M & M_TR) != 0) {
       T diff = oldact - hop_triplet(nu, a1, a2);
@@ -275,6 +290,11 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     // (src/su2u1.c:336-354) evaluated as a difference
     T al = ldA(A, Vh, mu, par, s);
     T na = al + (T)0.95 * (uni<T>(r) - (T)0.5);
+    if constexpr (sizeof(T) == 4) {
+      // FP32: keep the angle in [-pi, pi] so it keeps its precision; the
+      // action is 2 pi periodic in it for an integer charge r_u1
+      if (L.r_u1 == ::rint(L.r_u1)) na -= (T)(2.0 * PI) * rintf(na * (T)(0.5 / PI));
+    }
     T rr = (T)L.r_u1;
     Site xp = shift(L, x, mu, +1);
     T dS = 0;
@@ -741,13 +761,10 @@ __global__ __launch_bounds__(BLK) void k_muca_restore(const LatDesc *__restrict_
   int s = bm.s0 + (int)threadIdx.x;
   if (s >= L.Vh || s < lo || s >= hi) return;
   bool dbl = kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS;
-  int dofs = dbl ? 4 : 3;
-  T *dst = (T *)(dbl ? L.F : L.G);
-  const T *src = (const T *)L.B;
-  for (int c = 0; c < dofs; c++) {
-    size_t o = (size_t)(c * 2 + par) * L.Vh + s;
-    dst[o] = src[o];
-  }
+  if (dbl)
+    stF((T *)L.F, L.Vh, par, s, ldF((const T *)L.B, L.Vh, par, s));
+  else
+    stG((T *)L.G, L.Vh, par, s, ldG((const T *)L.B, L.Vh, par, s));
 }
 
 // order parameter of both parities (calc_orderparam at start-up,
@@ -1126,8 +1143,9 @@ __host__ __device__ inline int field_dofs(int field) {
 __device__ __forceinline__ size_t soa_index(int field, int Vh, int par, int s, int k) {
   if (field == 0) {
     int d = k >> 2, c = k & 3;
-    return (size_t)((d * 4 + c) * 2 + par) * Vh + s;
+    return ((size_t)(d * 2 + par) * Vh + s) * 4 + c;
   }
+  if (field == 2) return ((size_t)par * Vh + s) * 4 + k;
   return (size_t)(k * 2 + par) * Vh + s;
 }
 

@@ -1,0 +1,93 @@
+"""FP32-mixed sweeps: FP32 fields and staples with FP64 reductions.
+
+There is no per-site oracle for FP32 (different draw mapping and rounding);
+these tests pin it by invariants and by agreement with the FP64 path within
+statistical errors."""
+import numpy as np
+import pytest
+
+from su2higgs_amd import abi
+from su2higgs_amd.abi import FP32, FP64, make_params
+from su2higgs_amd.lattice import Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fp32_roundtrip_rounds_to_float():
+    p = make_params(L=(8, 8, 8), nhiggs=1, u1=1, precision=FP32)
+    b = Batch(p)
+    rng = np.random.default_rng(1)
+    f = abi.field_arrays(p)
+    f["su2link"][:] = rng.normal(size=f["su2link"].shape)
+    f["doublet"][:] = rng.normal(size=f["doublet"].shape)
+    f["u1link"][:] = rng.normal(size=f["u1link"].shape)
+    b.set_fields(0, f)
+    g = b.get_fields(0)
+    for k in ("su2link", "doublet", "u1link"):
+        assert np.array_equal(g[k], f[k].astype(np.float32).astype(np.float64))
+
+
+def test_fp32_links_stay_unitary_and_angles_wrapped():
+    p = make_params(L=(16, 16, 16), nhiggs=1, u1=1, precision=FP32, link_overrelax=4,
+                    betasu2=9.0, betau1=27.0, msq_phi=-0.2, lambda_phi=0.015)
+    b = Batch(p)
+    b.hot_start(0)
+    b.use_graph(True)
+    b.update(200)
+    f = b.get_fields(0)
+    n = np.sqrt((f["su2link"] ** 2).sum(-1))
+    assert np.abs(n - 1).max() < 5e-6
+    assert np.abs(f["u1link"]).max() <= np.pi * (1 + 1e-6)
+
+
+@pytest.mark.parametrize("model", [dict(nhiggs=1), dict(nhiggs=1, u1=1)])
+def test_fp32_observables_match_fp64(model):
+    """Plaquette and <phi^+phi> of FP32 and FP64 chains agree within
+    statistical errors (5 sigma of the combined error, autocorrelations
+    estimated by blocking)."""
+    res = {}
+    for prec in (FP64, FP32):
+        p = make_params(L=(16, 16, 16), precision=prec, link_overrelax=4, betasu2=8.0,
+                        msq_phi=-0.25, lambda_phi=0.02, betau1=27.0, seed=77 + prec, **model)
+        b = Batch(p)
+        b.cold_start(0, phi0=1.0)
+        b.use_graph(True)
+        b.update(100)
+        V = abi.volume(p)
+        pl, ph = [], []
+        for _ in range(400):
+            b.update(1)
+            s = b.measure(0)
+            pl.append(1 - s[abi.SUM["wilson"]] / (V * p.betasu2) / 3)
+            ph.append(s[abi.SUM["phi2"]] / V)
+        res[prec] = (np.array(pl), np.array(ph))
+
+    def mean_err(x, nb=20):
+        blocks = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
+        return blocks.mean(), blocks.std(ddof=1) / np.sqrt(nb)
+
+    for j in range(2):
+        m64, e64 = mean_err(res[FP64][j])
+        m32, e32 = mean_err(res[FP32][j])
+        assert abs(m64 - m32) < 5 * np.hypot(e64, e32) + 1e-6, (j, m64, e64, m32, e32)
+
+
+def test_fp32_heatbath_distribution():
+    """a0 = (U W)_0 / |W| of freshly heatbathed links follows
+    sqrt(1-a0^2) exp(2 k a0) (Kennedy-Pendleton); checked on a cold lattice
+    where every staple has the same norm."""
+    from scipy import stats
+    p = make_params(L=(16, 16, 16), nhiggs=0, precision=FP32, betasu2=1.5)
+    b = Batch(p)
+    b.cold_start(0)
+    b.pass_(abi.K_LINK_HEATBATH, 0, 0)
+    f = b.get_fields(0)
+    # even sites, direction 0: staple of a cold lattice = -beta/2 * 4 * 1
+    a0 = f["su2link"][: abi.volume(p) // 2, 0, 0]
+    k = 0.5 * p.betasu2 * 4  # |staple|; weight exp(+2k a0) after the sign flip
+    xs = np.linspace(-1, 1, 20001)
+    pdf = np.sqrt(1 - xs ** 2) * np.exp(2 * k * (xs - 1))
+    cdf = np.cumsum(pdf)
+    cdf /= cdf[-1]
+    stat = stats.kstest(a0, lambda x: np.interp(x, xs, cdf))
+    assert stat.pvalue > 1e-4, stat
