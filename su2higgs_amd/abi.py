@@ -198,6 +198,7 @@ def load():
         "su2hip_kernel_times": (C.c_int, [h, dp, P(C.c_int64)]),
         "su2hip_reset_timing": (C.c_int, [h]),
         "su2hip_copy_probe": (C.c_int, [C.c_uint64, C.c_int, dp]),
+        "su2hip_time_pass": (C.c_int, [h, C.c_int, C.c_int, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -1440,6 +1440,34 @@ int su2hip_reset_timing(su2hip_batch_t *b) {
   return SU2HIP_OK;
 }
 
+int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms) {
+  if (!b || !ms) return fail(SU2HIP_EINVAL, "NULL argument");
+  if (kind < 0 || kind >= NCOUNT || repeats < 1) return fail(SU2HIP_EINVAL, "kind/repeats");
+  int rc = prepare(b);
+  if (rc) return rc;
+  int ndir = (kind <= SU2HIP_K_U1_METROPOLIS) ? 3 : 1;
+  hipEvent_t e0, e1;
+  HIPTRY(hipEventCreate(&e0));
+  HIPTRY(hipEventCreate(&e1));
+  LaunchCtx c = ctx_of(b);
+  int reunit = b->tmpl.reunitarize && !b->fp64;
+  hipError_t e = hipEventRecord(e0, b->stream);
+  for (int i = 0; i < repeats && e == hipSuccess; i++)
+    e = launch_pass(c, kind, i & 1, (i >> 1) % ndir, i, -1, reunit);
+  if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e == hipSuccess) e = launch_end_sweep(c, repeats);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e == hipErrorInvalidValue) return fail(SU2HIP_EINVAL, "kind %d is not part of this model", kind);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "time_pass: %s", hipGetErrorString(e));
+  *ms = t / repeats;
+  return SU2HIP_OK;
+}
+
 int su2hip_copy_probe(uint64_t bytes, int repeats, double *ms) {
   if (!ms || bytes < 16 || repeats < 1) return fail(SU2HIP_EINVAL, "copy probe arguments");
   size_t n = bytes / sizeof(double);

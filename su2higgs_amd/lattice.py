@@ -155,6 +155,12 @@ class Batch:
     def reset_timing(self):
         check(self.lib.su2hip_reset_timing(self.h))
 
+    def time_pass(self, kind, repeats=100):
+        """Average ms of back-to-back launches of one pass kind."""
+        ms = C.c_double()
+        check(self.lib.su2hip_time_pass(self.h, kind, repeats, C.byref(ms)))
+        return ms.value
+
 
 def copy_probe(nbytes, repeats=20):
     lib = abi.load()
