@@ -1,0 +1,88 @@
+/* su2hip_io.h -- host-side file formats of the sweep, part of libsu2hip.so:
+ * the reference's `config` key-value file, the binary `lattice` checkpoint,
+ * the multicanonical `weight` file and the `labels` / `measure` /
+ * `measure_flow` text outputs, byte for byte.  Pure C, no GPU calls.
+ * All functions return 0 or a negative SU2HIP_E* code
+ * (su2hip_last_error() has the message); none calls exit().
+ */
+#ifndef SU2HIP_IO_H
+#define SU2HIP_IO_H
+
+#include <stdint.h>
+#include <stdio.h>
+
+#include "su2hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SU2HIP_PATHLEN 200
+
+/* Everything a run reads from `config` (src/parameters.c:157-402), plus the
+ * keys the reference does not know (model, seed, start, link_overrelax,
+ * precision, flow_scheme), which it ignores when it reads the same file. */
+typedef struct su2hip_run_config {
+  su2hip_params p;
+  int32_t gradflow;          /* field content includes the gradient-flow keys */
+  int32_t reset, run_checks, multicanonical, measure_local;
+  int64_t iterations, interval, checkpoint, n_thermalize;
+  double phi0, sigma0, singlet0;
+  int32_t hot_start;         /* start hot (no reference counterpart) */
+  int32_t do_flow, flow_interval, flow_meas_interval, flow_scheme;
+  double flow_dt, flow_t_max;
+  su2hip_muca muca;          /* bins/min/max/mode/checks/orderparam/delta from config */
+  char resultsfile[SU2HIP_PATHLEN];
+  char latticefile[SU2HIP_PATHLEN];
+  char weightfile[SU2HIP_PATHLEN];
+} su2hip_run_config;
+
+/* Parse `path` the way get_parameters + get_weight_parameters do
+ * (case-insensitive labels, `#` comments, first match searching on from the
+ * previous match and wrapping once, strtol/strtod conversion, unknown
+ * algorithm names -> metropolis).  `model` overrides the config's `model` key
+ * when non-NULL: '+'-separated tokens among higgs, u1, triplet, singlet,
+ * gradflow; the default is the reference Makefile's build, higgs+singlet+u1. */
+int su2hip_config_read(const char *path, const char *model, su2hip_run_config *out);
+/* Re-read iterations, interval and checkpoint (read_updated_parameters,
+ * src/parameters.c:460-494). */
+int su2hip_config_reread(const char *path, su2hip_run_config *cfg);
+
+/* Binary checkpoint (save_lattice/load_lattice, src/checkpoint.c:84-214):
+ * int size(=1), int dim(=3), int L[3], long iter, double total_time,
+ * double comms_time, then su2link, u1link, doublet, triplet, singlet for the
+ * fields of p, each in the reference host layout. */
+int su2hip_lattice_write(const char *path, const su2hip_params *p, int64_t iter,
+                         double total_time, double comms_time, const double *su2link,
+                         const double *u1link, const double *doublet,
+                         const double *triplet, const double *singlet);
+/* Checks size, dim and extents against p (the reference dies on a mismatch,
+ * src/checkpoint.c:154-179; here SU2HIP_EIO). */
+int su2hip_lattice_read(const char *path, const su2hip_params *p, int64_t *iter,
+                        double *total_time, double *comms_time, double *su2link,
+                        double *u1link, double *doublet, double *triplet, double *singlet);
+
+/* Weight file (load_weight / save_weight, src/multicanonical.c:82-288).
+ * On input m holds the config's bins/min/max/mode/delta/orderparam/
+ * checks_per_sweep; pos/W must have room for max_bins+2 values.  A missing
+ * file gives a flat weight (an error for READONLY, as in the reference). */
+int su2hip_weight_load(const char *path, su2hip_muca *m, double *pos, double *W,
+                       int max_bins);
+int su2hip_weight_save(const char *path, const su2hip_muca *m, const double *pos,
+                       const double *W);
+
+/* `labels` (print_labels, src/measure.c:15-64) for the field content of p. */
+int su2hip_labels_write(const char *path, const su2hip_params *p);
+/* One `measure` line (src/measure.c:252-306): "%g " per column, newline. */
+int su2hip_measure_format(const su2hip_params *p, const double *sums, double weight,
+                          double muca_param, char *buf, int buflen);
+
+/* Acceptance summary (print_acceptance, src/checkpoint.c:17-78) to `out`. */
+int su2hip_print_acceptance(FILE *out, const su2hip_run_config *cfg,
+                            const su2hip_counters *c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

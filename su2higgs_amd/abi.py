@@ -86,6 +86,24 @@ class Muca(C.Structure):
     ]
 
 
+class RunConfig(C.Structure):
+    """su2hip_run_config of include/su2hip_io.h."""
+    _fields_ = [
+        ("p", Params), ("gradflow", C.c_int32), ("reset", C.c_int32),
+        ("run_checks", C.c_int32), ("multicanonical", C.c_int32),
+        ("measure_local", C.c_int32), ("iterations", C.c_int64),
+        ("interval", C.c_int64), ("checkpoint", C.c_int64),
+        ("n_thermalize", C.c_int64), ("phi0", C.c_double),
+        ("sigma0", C.c_double), ("singlet0", C.c_double),
+        ("hot_start", C.c_int32), ("do_flow", C.c_int32),
+        ("flow_interval", C.c_int32), ("flow_meas_interval", C.c_int32),
+        ("flow_scheme", C.c_int32), ("flow_dt", C.c_double),
+        ("flow_t_max", C.c_double), ("muca", Muca),
+        ("resultsfile", C.c_char * 200), ("latticefile", C.c_char * 200),
+        ("weightfile", C.c_char * 200),
+    ]
+
+
 def make_params(L=(8, 8, 8), nhiggs=1, u1=0, triplet=0, singlet=0,
                 precision=FP64, **kw):
     """Params with the reference's sample-config defaults for the rest."""
@@ -199,6 +217,17 @@ def load():
         "su2hip_reset_timing": (C.c_int, [h]),
         "su2hip_copy_probe": (C.c_int, [C.c_uint64, C.c_int, dp]),
         "su2hip_time_pass": (C.c_int, [h, C.c_int, C.c_int, dp]),
+        # host I/O layer (include/su2hip_io.h)
+        "su2hip_config_read": (C.c_int, [C.c_char_p, C.c_char_p, P(RunConfig)]),
+        "su2hip_config_reread": (C.c_int, [C.c_char_p, P(RunConfig)]),
+        "su2hip_lattice_write": (C.c_int, [C.c_char_p, P(Params), C.c_int64, C.c_double, C.c_double,
+                                           dp, dp, dp, dp, dp]),
+        "su2hip_lattice_read": (C.c_int, [C.c_char_p, P(Params), P(C.c_int64), dp, dp,
+                                          dp, dp, dp, dp, dp]),
+        "su2hip_weight_load": (C.c_int, [C.c_char_p, P(Muca), dp, dp, C.c_int]),
+        "su2hip_weight_save": (C.c_int, [C.c_char_p, P(Muca), dp, dp]),
+        "su2hip_labels_write": (C.c_int, [C.c_char_p, P(Params)]),
+        "su2hip_measure_format": (C.c_int, [P(Params), dp, C.c_double, C.c_double, C.c_char_p, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

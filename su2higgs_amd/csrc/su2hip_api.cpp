@@ -633,6 +633,12 @@ const char *su2hip_error_string(int code) {
 
 const char *su2hip_last_error(void) { return g_err.c_str(); }
 
+// used by the host I/O layer (su2hip_io.c) to report its errors the same way
+int su2hip_internal_fail(int code, const char *msg) {
+  g_err = msg ? msg : "";
+  return code;
+}
+
 int su2hip_footprint(const su2hip_params *p, int with_flow, uint64_t *bytes) {
   int rc = validate(p);
   if (rc) return rc;

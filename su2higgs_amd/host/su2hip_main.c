@@ -1,0 +1,296 @@
+/* su2hip_main.c -- the `su2hip <config>` driver: the reference's main()
+ * (src/main.c:10-421) over the C ABI of libsu2hip.so.  Reads the same
+ * config, writes the same labels / measure / lattice / weight /
+ * measure_flow files; the sweeps run on the GPU.
+ *
+ *   su2hip [--model higgs+u1] [--fp32] [--no-graph] <config>
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+#include "../../include/su2hip.h"
+#include "../../include/su2hip_io.h"
+
+#define MAXBINS 100000
+
+static double now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void die(const char *what, int rc) {
+  fprintf(stderr, "su2hip: %s failed: %s (%s)\n", what, su2hip_error_string(rc), su2hip_last_error());
+  exit(1);
+}
+
+#define TRY(what, x)          \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_) die(what, rc_);  \
+  } while (0)
+
+typedef struct {
+  double *link, *u1, *db, *tr, *sg;
+} hostfields;
+
+static void alloc_fields(const su2hip_params *p, hostfields *h) {
+  long V = (long)p->L[0] * p->L[1] * p->L[2];
+  h->link = calloc(V * 12, sizeof(double));
+  h->u1 = p->u1 ? calloc(V * 3, sizeof(double)) : NULL;
+  h->db = p->nhiggs ? calloc(V * 4, sizeof(double)) : NULL;
+  h->tr = p->triplet ? calloc(V * 3, sizeof(double)) : NULL;
+  h->sg = p->singlet ? calloc(V, sizeof(double)) : NULL;
+  if (!h->link) {
+    fprintf(stderr, "su2hip: host allocation failed\n");
+    exit(1);
+  }
+}
+
+/* save_lattice (src/checkpoint.c:84-122) plus the Philox position in
+ * <latticefile>.rng for an exact resume */
+static void checkpoint(su2hip_batch_t *b, const su2hip_run_config *cfg, hostfields *h, long iter,
+                       double total) {
+  TRY("get_fields", su2hip_get_fields(b, 0, h->link, h->u1, h->db, h->tr, h->sg));
+  TRY("lattice_write", su2hip_lattice_write(cfg->latticefile, &cfg->p, iter, total, 0.0, h->link,
+                                            h->u1, h->db, h->tr, h->sg));
+  uint64_t seed, pass;
+  TRY("rng_get", su2hip_rng_get(b, 0, &seed, &pass));
+  char rp[SU2HIP_PATHLEN + 8];
+  snprintf(rp, sizeof(rp), "%s.rng", cfg->latticefile);
+  FILE *f = fopen(rp, "w");
+  if (f) {
+    fprintf(f, "%llu %llu\n", (unsigned long long)seed, (unsigned long long)pass);
+    fclose(f);
+  }
+  printf("Wrote fields to %s.\n", cfg->latticefile);
+}
+
+static void save_weight(su2hip_batch_t *b, const su2hip_run_config *cfg, double *pos, double *W) {
+  if (!cfg->multicanonical) return;
+  su2hip_muca m;
+  TRY("muca_get", su2hip_muca_get(b, 0, &m, pos, W, NULL));
+  TRY("weight_save", su2hip_weight_save(cfg->weightfile, &m, pos, W));
+}
+
+/* weight and order parameter columns of a measure line (src/measure.c:180-188) */
+static void muca_columns(su2hip_batch_t *b, const su2hip_run_config *cfg, double *weight,
+                         double *param) {
+  double w = 0.0, r = 0.0;
+  if (cfg->multicanonical) {
+    su2hip_muca m;
+    TRY("muca_get", su2hip_muca_get(b, 0, &m, NULL, NULL, NULL));
+    r = m.param_value[0] + m.param_value[1];
+    TRY("muca_weight_of", su2hip_muca_weight_of(b, 0, r, &w));
+  }
+  *weight = -1.0 * w;
+  *param = r;
+}
+
+/* grad_flow (src/gradflow.c:46-173): rows appended to measure_flow */
+static void flow(su2hip_batch_t *b, const su2hip_run_config *cfg, int flow_id) {
+  int maxrows = (int)(cfg->flow_t_max / cfg->flow_dt) / (cfg->flow_meas_interval > 0 ? cfg->flow_meas_interval : 1) + 8;
+  double *rows = malloc(sizeof(double) * (size_t)maxrows * (1 + SU2HIP_NSUMS));
+  int n = 0;
+  TRY("flow", su2hip_flow(b, 0, cfg->flow_scheme, cfg->flow_dt, cfg->flow_t_max, cfg->flow_meas_interval,
+                          rows, maxrows, &n));
+  double weight, param;
+  muca_columns(b, cfg, &weight, &param);
+  FILE *f = fopen("measure_flow", "a");
+  if (!f) {
+    fprintf(stderr, "su2hip: cannot open measure_flow\n");
+    exit(1);
+  }
+  fprintf(f, "\n =========== Flow id: %d ===========\n", flow_id);
+  /* grad_flow's time loop, to know which row is the unaligned last one */
+  double t = 0.0, dt = cfg->flow_dt;
+  int iter = 1, r = 0;
+  char line[4096];
+  TRY("measure_format", su2hip_measure_format(&cfg->p, rows + 1, weight, param, line, sizeof(line)));
+  fprintf(f, "%.6lf %s", rows[0], line);
+  r = 1;
+  while (t < cfg->flow_t_max) {
+    if (t + dt > cfg->flow_t_max) dt = cfg->flow_t_max - t;
+    t += dt;
+    if (iter % cfg->flow_meas_interval == 0 && r < n) {
+      double *row = rows + (size_t)r * (1 + SU2HIP_NSUMS);
+      TRY("measure_format", su2hip_measure_format(&cfg->p, row + 1, weight, param, line, sizeof(line)));
+      fprintf(f, "%.6lf %s", row[0], line);
+      r++;
+    }
+    iter++;
+  }
+  if ((iter - 1) % cfg->flow_meas_interval != 0 && r < n) {
+    double *row = rows + (size_t)r * (1 + SU2HIP_NSUMS);
+    TRY("measure_format", su2hip_measure_format(&cfg->p, row + 1, weight, param, line, sizeof(line)));
+    fprintf(f, "%g %s", row[0], line);
+  }
+  fclose(f);
+  free(rows);
+}
+
+int main(int argc, char **argv) {
+  const char *model = NULL, *path = NULL;
+  int fp32 = 0, graph = 1;
+  for (int i = 1; i < argc; i++) {
+    if (!strcmp(argv[i], "--model") && i + 1 < argc)
+      model = argv[++i];
+    else if (!strcmp(argv[i], "--fp32"))
+      fp32 = 1;
+    else if (!strcmp(argv[i], "--no-graph"))
+      graph = 0;
+    else
+      path = argv[i];
+  }
+  if (!path) {
+    printf("Usage: su2hip [--model higgs+u1+triplet+singlet+gradflow] [--fp32] [--no-graph] <config file>\n");
+    return 0;
+  }
+  su2hip_run_config cfg;
+  TRY("config", su2hip_config_read(path, model, &cfg));
+  su2hip_params *p = &cfg.p;
+  if (fp32) {
+    p->precision = SU2HIP_FP32;
+    p->reunitarize = 1;
+  }
+  if (p->seed == 0) {
+    /* the reference seeds from the clock (src/main.c:64-68) */
+    long seed = time(NULL);
+    seed = seed ^ (seed << 26) ^ (seed << 9);
+    p->seed = (uint64_t)seed;
+  }
+  printf("Seed: %llu\n", (unsigned long long)p->seed);
+  printf("Volume %d x %d x %d, total %ld\n", p->L[0], p->L[1], p->L[2], (long)p->L[0] * p->L[1] * p->L[2]);
+  printf("iterations %ld, measurement interval %ld, checkpoint interval %ld\n", (long)cfg.iterations,
+         (long)cfg.interval, (long)cfg.checkpoint);
+  printf("Will perform %d gauge sweeps (+%d overrelaxation rounds), %d scalar sweeps per iteration, %s\n",
+         p->update_links, p->link_overrelax, p->scalar_sweeps, p->precision == SU2HIP_FP32 ? "FP32-mixed" : "FP64");
+
+  su2hip_batch_t *b = NULL;
+  TRY("create", su2hip_create(1, p, 1, 0, &b));
+  TRY("use_graph", su2hip_use_graph(b, graph));
+  hostfields h;
+  alloc_fields(p, &h);
+  long iter = 1;
+  int64_t file_iter = 0;
+  double total_time = 0.0;
+  if (access(cfg.latticefile, R_OK) == 0) {
+    printf("\nLoading latticefile: %s\n", cfg.latticefile);
+    double tt, ct;
+    TRY("lattice_read", su2hip_lattice_read(cfg.latticefile, p, &file_iter, &tt, &ct, h.link, h.u1, h.db,
+                                            h.tr, h.sg));
+    TRY("set_fields", su2hip_set_fields(b, 0, h.link, h.u1, h.db, h.tr, h.sg));
+    char rp[SU2HIP_PATHLEN + 8];
+    snprintf(rp, sizeof(rp), "%s.rng", cfg.latticefile);
+    FILE *f = fopen(rp, "r");
+    unsigned long long seed, pass;
+    if (f && fscanf(f, "%llu %llu", &seed, &pass) == 2 && !cfg.reset) {
+      TRY("rng_set", su2hip_rng_set(b, 0, seed, pass));
+      printf("Resuming the Philox stream of %s\n", rp);
+    }
+    if (f) fclose(f);
+    printf("Fields loaded succesfully.\n");
+  } else {
+    if (cfg.hot_start) {
+      printf("No latticefile found; starting with a hot configuration.\n");
+      TRY("hot_start", su2hip_hot_start(b, 0));
+    } else {
+      printf("No latticefile found; starting with cold configuration.\n");
+      TRY("cold_start", su2hip_cold_start(b, 0, cfg.phi0, cfg.sigma0, cfg.singlet0));
+    }
+    cfg.reset = 1;
+  }
+  TRY("labels", su2hip_labels_write("labels", p));
+
+  double *pos = NULL, *W = NULL;
+  int mode = SU2HIP_READONLY;
+  if (cfg.multicanonical) {
+    pos = calloc(MAXBINS + 2, sizeof(double));
+    W = calloc(MAXBINS + 2, sizeof(double));
+    printf("\nLoading weightfile: %s\n", cfg.weightfile);
+    TRY("weight_load", su2hip_weight_load(cfg.weightfile, &cfg.muca, pos, W, MAXBINS));
+    mode = cfg.muca.mode;
+    TRY("muca_set", su2hip_muca_set(b, 0, &cfg.muca, pos, W));
+    printf("Using weight function with %d bins in range %lf, %lf\n", cfg.muca.bins, cfg.muca.min, cfg.muca.max);
+  }
+
+  if (cfg.reset) {
+    /* thermalise without modifying the weight (src/main.c:253-282) */
+    su2hip_muca m;
+    if (cfg.multicanonical && mode != SU2HIP_READONLY) {
+      TRY("muca_get", su2hip_muca_get(b, 0, &m, pos, W, NULL));
+      m.mode = SU2HIP_READONLY;
+      TRY("muca_set", su2hip_muca_set(b, 0, &m, pos, W));
+    }
+    printf("\nThermalizing %ld iterations\n", (long)cfg.n_thermalize);
+    double t0 = now();
+    TRY("update", su2hip_update(b, (int)cfg.n_thermalize));
+    TRY("sync", su2hip_sync(b));
+    printf("Thermalization done, took %lf seconds.\n", now() - t0);
+    TRY("reset_counters", su2hip_reset_counters(b, 0));
+    if (cfg.multicanonical && mode != SU2HIP_READONLY) {
+      TRY("muca_get", su2hip_muca_get(b, 0, &m, pos, W, NULL));
+      m.mode = mode;
+      /* init_last_max (src/multicanonical.c:826-831) */
+      double val = m.param_value[0] + m.param_value[1];
+      m.last_max = fabs(val - m.wrk_min) >= fabs(val - m.wrk_max);
+      m.muca_count = 0;
+      TRY("muca_set", su2hip_muca_set(b, 0, &m, pos, W));
+    }
+    iter = 1;
+    printf("\nStarting new simulation!\n");
+  } else {
+    iter = file_iter + 1;
+    printf("\nContinuing from iteration %ld!\n", iter - 1);
+  }
+
+  FILE *res = fopen(cfg.resultsfile, "a");
+  if (!res) {
+    fprintf(stderr, "su2hip: cannot open %s\n", cfg.resultsfile);
+    return 1;
+  }
+  double start = now();
+  int flow_id = 1;
+  char line[4096];
+  while (iter <= cfg.iterations) {
+    if (iter % cfg.interval == 0) {
+      double sums[SU2HIP_NSUMS], weight, param;
+      TRY("measure", su2hip_measure(b, 0, sums));
+      muca_columns(b, &cfg, &weight, &param);
+      TRY("measure_format", su2hip_measure_format(p, sums, weight, param, line, sizeof(line)));
+      fputs(line, res);
+      fflush(res);
+    }
+    if (cfg.gradflow && cfg.do_flow && iter % cfg.flow_interval == 0) flow(b, &cfg, flow_id++);
+    TRY("update", su2hip_update(b, 1));
+    if (iter % cfg.checkpoint == 0) {
+      TRY("sync", su2hip_sync(b));
+      total_time += now() - start;
+      start = now();
+      su2hip_counters c;
+      TRY("counters", su2hip_get_counters(b, 0, &c));
+      printf("\nCheckpointing at iteration %lu. Total time: %.1lfs, %.2lf%% comms.\n", (unsigned long)iter,
+             total_time, 0.0);
+      su2hip_print_acceptance(stdout, &cfg, &c);
+      fflush(stdout);
+      checkpoint(b, &cfg, &h, iter, total_time);
+      save_weight(b, &cfg, pos, W);
+      TRY("config_reread", su2hip_config_reread(path, &cfg));
+    }
+    iter++;
+  }
+  fclose(res);
+  TRY("sync", su2hip_sync(b));
+  total_time += now() - start;
+  checkpoint(b, &cfg, &h, iter, total_time);
+  save_weight(b, &cfg, pos, W);
+  printf("\nReached end! Total time taken: %.1lfs, of which %.2lf%% comms. time per iteration: %.6lfs \n",
+         total_time, 0.0, cfg.iterations > 0 ? total_time / cfg.iterations : 0.0);
+  su2hip_destroy(b);
+  return 0;
+}

@@ -1,0 +1,321 @@
+"""Host I/O layer (include/su2hip_io.h) against the reference build: the
+config reader, the lattice checkpoint bytes, labels, measure lines and the
+weight file.  CPU only (no GPU call is made)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from su2higgs_amd import abi
+from su2higgs_amd.abi import Muca, Params, RunConfig, make_params, ptr
+
+MODELS = {
+    "h": dict(nhiggs=1),
+    "hu": dict(nhiggs=1, u1=1),
+    "t": dict(nhiggs=0, triplet=1),
+    "ht": dict(nhiggs=1, triplet=1),
+    "hsu": dict(nhiggs=1, u1=1, singlet=1),
+}
+MODEL_KEY = {"h": "higgs", "hu": "higgs+u1", "t": "triplet", "ht": "higgs+triplet",
+             "hsu": "higgs+u1+singlet"}
+
+CONFIG = """### test config for the su2higgs config reader
+dim 3
+
+L1 {L1}
+L2 6
+L3 8
+reset 1
+iterations 20
+n_thermalize 3
+interval 2
+checkpoint 50
+run_checks 1
+resultsfile measure
+latticefile lattice
+
+# couplings
+betasu2 8.5
+betau1 7.0
+r_u1 1
+msq -0.194549221785
+lambda 0.016095347348
+msq_triplet -0.5032354703283564
+b4 0.11666666666666665
+a2 2.6953060227675745
+msq_s -0.1
+b1_s 3
+b3_s 2
+b4_s 0.3
+a1_s 3
+a2_s 0.25
+phi0 0.2
+sigma0 0.3
+singlet0 0.4
+algorithm_su2link heatbath
+algorithm_u1link metropolis
+algorithm_su2doublet overrelax
+algorithm_su2triplet METROPOLIS
+algorithm_singlet overrelax
+scalar_sweeps 5
+update_singlet 1
+update_doublet 2
+update_triplet 1
+update_links 1
+multicanonical {muca}
+bins 80
+min 0.418
+max 3.57
+weightfile weight
+checks_per_sweep 1
+orderparam {op}
+muca_mode 1
+muca_delta 0.3
+do_flow 0
+flow_meas_interval 10
+flow_dt 0.05
+flow_interval 1000
+flow_t_max 10
+measure_local 0
+"""
+
+
+def need_ref(tag):
+    if not orc.ref_available(tag):
+        pytest.skip("reference build oracle/_ref/libref_%s.so absent" % tag)
+
+
+@pytest.fixture
+def lib():
+    return abi.load()
+
+
+def read_config(lib, path, model=None):
+    rc = RunConfig()
+    code = lib.su2hip_config_read(path.encode(), model.encode() if model else None, C.byref(rc))
+    return code, rc
+
+
+def ref_config(tag, path):
+    r = orc.ref_lib(tag)
+    p = Params()
+    longs = (C.c_long * 16)()
+    m = Muca()
+    lat = C.create_string_buffer(256)
+    res = C.create_string_buffer(256)
+    r.ref_get_parameters(path.encode(), C.byref(p), longs, C.byref(m), lat, res)
+    return p, list(longs), m, lat.value.decode()
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+@pytest.mark.parametrize("muca", [0, 1])
+def test_config_matches_reference(lib, tmp_path, tag, muca):
+    need_ref(tag)
+    op = "Sigmasq" if tag == "t" else "phisq"
+    path = str(tmp_path / "config")
+    with open(path, "w") as f:
+        f.write(CONFIG.format(L1=4, muca=muca, op=op))
+    os.chdir(tmp_path)
+    code, rc = read_config(lib, path, MODEL_KEY[tag])
+    assert code == 0, lib.su2hip_last_error()
+    p, longs, m, lat = ref_config(tag, path)
+    q = rc.p
+    assert list(q.L) == list(p.L)
+    for k in ("algorithm_su2link", "algorithm_su2doublet", "algorithm_su2triplet", "update_links",
+              "scalar_sweeps", "update_su2doublet", "update_su2triplet"):
+        if k == "algorithm_su2doublet" and not q.nhiggs:
+            continue
+        if k in ("algorithm_su2triplet", "update_su2triplet") and not q.triplet:
+            continue
+        assert getattr(q, k) == getattr(p, k), k
+    for k in ("betasu2", "msq_phi", "lambda_phi", "msq_triplet", "b4"):
+        if k in ("msq_phi", "lambda_phi") and not q.nhiggs:
+            continue
+        if k in ("msq_triplet", "b4") and not q.triplet:
+            continue
+        assert getattr(q, k) == getattr(p, k), k
+    if q.nhiggs and q.triplet:
+        assert q.a2 == p.a2
+    if q.u1:
+        assert (q.betau1, q.r_u1, q.algorithm_u1link) == (p.betau1, p.r_u1, p.algorithm_u1link)
+    if q.singlet:
+        for k in ("update_singlet", "algorithm_singlet", "b1_s", "msq_s", "b3_s", "b4_s", "a1_s", "a2_s"):
+            assert getattr(q, k) == getattr(p, k), k
+    assert [rc.iterations, rc.interval, rc.checkpoint, rc.n_thermalize, rc.reset, rc.multicanonical,
+            rc.run_checks, rc.measure_local] == longs[:8]
+    assert rc.latticefile.decode() == lat
+    if muca:
+        for k in ("orderparam", "mode", "checks_per_sweep", "bins", "min", "max", "delta"):
+            assert getattr(rc.muca, k) == getattr(m, k), k
+
+
+def test_config_lookup_semantics(lib, tmp_path):
+    """Duplicate keys: each lookup searches on from the previous match and
+    wraps once (src/parameters.c:30-92), so the result depends on order."""
+    need_ref("h")
+    base = CONFIG.format(L1=4, muca=0, op="phisq")
+    # betasu2 is looked up right after scalar_sweeps, which sits near the
+    # end of the file: the search finds the duplicate appended at the end
+    # before it wraps round to the first occurrence
+    txt = base + "# betasu2 1.0\nBETASU2 9.9\n"
+    path = str(tmp_path / "config")
+    with open(path, "w") as f:
+        f.write(txt)
+    os.chdir(tmp_path)
+    code, rc = read_config(lib, path, "higgs")
+    assert code == 0
+    p, *_ = ref_config("h", path)
+    assert rc.p.betasu2 == p.betasu2 == 9.9
+
+
+def test_config_missing_key_is_an_error(lib, tmp_path):
+    txt = CONFIG.format(L1=4, muca=0, op="phisq").replace("lambda 0.016095347348\n", "")
+    path = str(tmp_path / "config")
+    with open(path, "w") as f:
+        f.write(txt)
+    code, rc = read_config(lib, path, "higgs")
+    assert code == -7 and b"lambda" in lib.su2hip_last_error()
+
+
+def test_config_odd_extent_is_an_error(lib, tmp_path):
+    path = str(tmp_path / "config")
+    with open(path, "w") as f:
+        f.write(CONFIG.format(L1=5, muca=0, op="phisq"))
+    code, _ = read_config(lib, path, "higgs")
+    assert code != 0
+
+
+def test_reference_sample_config(lib, tmp_path):
+    """The sample config shipped with the reference, default build flags."""
+    src = "/root/reference/config"
+    if not os.path.exists(src) or not orc.ref_available("hsu"):
+        pytest.skip("reference sample config or build absent")
+    os.chdir(tmp_path)
+    code, rc = read_config(lib, src, None)
+    assert code == 0, lib.su2hip_last_error()
+    p, longs, m, lat = ref_config("hsu", src)
+    assert list(rc.p.L) == list(p.L) and rc.p.betasu2 == p.betasu2 and rc.p.b3_s == p.b3_s
+    assert rc.iterations == longs[0] and rc.latticefile.decode() == lat
+
+
+def hot_fields(p):
+    o = orc.Oracle(p)
+    o.hot_start()
+    return o.get_fields()
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_checkpoint_bytes_match_reference(lib, tmp_path, tag):
+    need_ref(tag)
+    p = make_params(L=(4, 6, 8), **MODELS[tag])
+    f = hot_fields(p)
+    r = orc.Ref(p)
+    r.set_fields(f)
+    a, b = str(tmp_path / "ref_lattice"), str(tmp_path / "our_lattice")
+    r.lib.ref_save_lattice(a.encode(), 17)
+    have = abi.field_arrays(p)
+    args = [ptr(f[k]) if have[k] is not None else None for k in orc.FIELDS]
+    assert lib.su2hip_lattice_write(b.encode(), C.byref(p), 17, 0.0, 0.0, *args) == 0
+    assert open(a, "rb").read() == open(b, "rb").read()
+    # read back the reference's file
+    g = abi.field_arrays(p)
+    it = C.c_int64()
+    tt, ct = C.c_double(), C.c_double()
+    assert lib.su2hip_lattice_read(a.encode(), C.byref(p), C.byref(it), C.byref(tt), C.byref(ct),
+                                   *[ptr(g[k]) for k in orc.FIELDS]) == 0
+    assert it.value == 17
+    for k in orc.FIELDS:
+        if g[k] is not None:
+            assert np.array_equal(g[k], f[k])
+    # and the reference reads ours
+    r2 = orc.Ref(p)
+    assert r2.lib.ref_load_lattice(b.encode()) == 17
+    h = r2.get_fields()
+    for k in orc.FIELDS:
+        if g[k] is not None:
+            assert np.array_equal(h[k], f[k])
+
+
+def test_checkpoint_rejects_mismatch_and_truncation(lib, tmp_path):
+    p = make_params(L=(4, 6, 8), nhiggs=1)
+    f = hot_fields(p)
+    path = str(tmp_path / "lattice")
+    assert lib.su2hip_lattice_write(path.encode(), C.byref(p), 3, 0.0, 0.0, ptr(f["su2link"]), None,
+                                    ptr(f["doublet"]), None, None) == 0
+    q = make_params(L=(4, 6, 6), nhiggs=1)
+    g = abi.field_arrays(q)
+    assert lib.su2hip_lattice_read(path.encode(), C.byref(q), None, None, None,
+                                   ptr(g["su2link"]), None, ptr(g["doublet"]), None, None) == -7
+    data = open(path, "rb").read()
+    with open(path, "wb") as fh:
+        fh.write(data[:-100])
+    g = abi.field_arrays(p)
+    assert lib.su2hip_lattice_read(path.encode(), C.byref(p), None, None, None,
+                                   ptr(g["su2link"]), None, ptr(g["doublet"]), None, None) == -7
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_labels_match_reference(lib, tmp_path, tag):
+    need_ref(tag)
+    p = make_params(L=(4, 4, 4), **MODELS[tag])
+    os.chdir(tmp_path)
+    orc.ref_lib(tag).ref_print_labels()
+    ref = open("labels").read()
+    assert lib.su2hip_labels_write(b"ours", C.byref(p)) == 0
+    assert open("ours").read() == ref
+
+
+@pytest.mark.parametrize("tag", list(MODELS))
+def test_measure_line_matches_reference(lib, tag):
+    need_ref(tag)
+    p = make_params(L=(4, 6, 8), **MODELS[tag])
+    r = orc.Ref(p)
+    r.set_fields(hot_fields(p))
+    sums = r.measure()
+    buf = C.create_string_buffer(4096)
+    assert lib.su2hip_measure_format(C.byref(p), ptr(sums), -0.0, 0.0, buf, 4096) == 0
+    assert buf.value.decode() == r.measure_text()
+
+
+def write_weight(path, bins, lo, hi, seed=3, delta=0.3, last_max=0):
+    rng = np.random.default_rng(seed)
+    dbin = (hi - lo) / bins
+    W = [0.0]
+    for s in rng.uniform(-5, 5, bins):
+        W.append(W[-1] + s * dbin)
+    with open(path, "w") as f:
+        f.write("%d %.12f %d %f %f\n" % (bins, delta, last_max, lo, hi))
+        for i in range(bins + 1):
+            f.write("%f %f\n" % (lo + i * dbin, W[i]))
+
+
+@pytest.mark.parametrize("mode", [abi.READONLY, abi.FAST])
+@pytest.mark.parametrize("exists", [True, False])
+def test_weight_file_matches_reference(lib, tmp_path, mode, exists):
+    need_ref("h")
+    if mode == abi.READONLY and not exists:
+        pytest.skip("read-only runs need a weight file")
+    os.chdir(tmp_path)
+    p = make_params(L=(4, 4, 6), nhiggs=1)
+    r = orc.Ref(p)
+    r.set_fields(hot_fields(p))
+    m = Muca()
+    m.enabled, m.orderparam, m.mode, m.checks_per_sweep = 1, abi.PHISQ, mode, 1
+    m.bins, m.min, m.max, m.delta = 40, 0.0, 2.0, 0.3
+    if exists:
+        write_weight("w_ref", 40, 0.0, 2.0)
+        write_weight("w_our", 40, 0.0, 2.0)
+    r.lib.ref_muca_load(b"w_ref", C.byref(m))
+    mr = Muca()
+    pr, Wr = np.zeros(42), np.zeros(42)
+    r.lib.ref_muca_get(C.byref(mr), ptr(pr), ptr(Wr), None)
+    mo = Muca.from_buffer_copy(m)
+    po, Wo = np.zeros(42), np.zeros(42)
+    assert lib.su2hip_weight_load(b"w_our", C.byref(mo), ptr(po), ptr(Wo), 40) == 0
+    assert np.array_equal(po, pr) and np.array_equal(Wo, Wr)
+    for k in ("bins", "min", "max", "wrk_min", "wrk_max", "delta", "last_max", "update_interval"):
+        assert getattr(mo, k) == getattr(mr, k), k
+    if mode != abi.READONLY:
+        assert open("w_our").read() == open("w_ref").read()
