@@ -187,7 +187,8 @@ const char *su2hip_last_error(void);
 /* ---- handle lifecycle --------------------------------------------------
  * Replaces the reference's layout()+alloc_fields() (src/su2.h:310,335;
  * src/layout.c:657-704, src/alloc.c:103-125) for n lattices at once.
- * max_slots >= n reserves room for lattices added later (serving);
+ * max_slots >= n reserves room for lattices added later (serving); n = 0
+ * makes an empty handle whose template is params[0];
  * hbm_budget (bytes, 0 = unlimited) caps the field memory of all slots. */
 int su2hip_create(int n, const su2hip_params *params, int max_slots,
                   uint64_t hbm_budget, su2hip_batch_t **out);

@@ -18,6 +18,7 @@ EVEN, ODD = 0, 1
 PHISQ, SIGMASQ, PHI2MINUSSIGMA2 = 1, 2, 3
 READONLY, FAST, SLOW = 0, 1, 2
 FP64, FP32 = 64, 32
+OK, EINVAL, ENOMEM, EDEVICE, ERANGE, EUNSUPPORTED, ESTATE, EIO = 0, -1, -2, -3, -4, -5, -6, -7
 FLOW_EULER, FLOW_RK3 = 1, 3
 
 (K_LINK_HEATBATH, K_LINK_METROPOLIS, K_LINK_OVERRELAX, K_U1_METROPOLIS,
@@ -102,6 +103,49 @@ class RunConfig(C.Structure):
         ("resultsfile", C.c_char * 200), ("latticefile", C.c_char * 200),
         ("weightfile", C.c_char * 200),
     ]
+
+
+class Request(C.Structure):
+    """su2hip_request (include/su2hip_serve.h)."""
+    _fields_ = [
+        ("id", C.c_int64), ("chain", C.c_int64), ("sweeps", C.c_int64),
+        ("arrival", C.c_double), ("betasu2", C.c_double), ("msq_phi", C.c_double),
+        ("lambda_phi", C.c_double), ("seed", C.c_uint64), ("L", C.c_int32),
+        ("flow", C.c_int32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("id", C.c_int64), ("status", C.c_int32), ("slot", C.c_int32),
+        ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+        ("sums", C.c_double * NSUMS), ("flow_t", C.c_double),
+        ("flow_sums", C.c_double * NSUMS), ("pass_", C.c_uint64), ("sweeps_done", C.c_int64),
+    ]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [
+        ("tmpl", Params), ("hbm_budget", C.c_uint64), ("max_slots", C.c_int32),
+        ("tick_sweeps", C.c_int32), ("use_graph", C.c_int32), ("realtime", C.c_int32),
+        ("keep_chains", C.c_int32), ("flow_scheme", C.c_int32), ("flow_dt", C.c_double),
+        ("flow_t_max", C.c_double), ("phi0", C.c_double), ("sigma0", C.c_double),
+        ("singlet0", C.c_double),
+    ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("completed", C.c_int64), ("failed", C.c_int64), ("makespan", C.c_double),
+        ("p50", C.c_double), ("p99", C.c_double), ("mean", C.c_double),
+        ("max_wait", C.c_double), ("site_updates", C.c_double),
+        ("site_updates_per_s", C.c_double), ("busy", C.c_double), ("ticks", C.c_int64),
+        ("peak_resident", C.c_int64), ("peak_bytes", C.c_uint64), ("flows", C.c_int64),
+        ("evictions", C.c_int64),
+    ]
+
+    def asdict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 def make_params(L=(8, 8, 8), nhiggs=1, u1=0, triplet=0, singlet=0,
@@ -228,6 +272,12 @@ def load():
         "su2hip_weight_save": (C.c_int, [C.c_char_p, P(Muca), dp, dp]),
         "su2hip_labels_write": (C.c_int, [C.c_char_p, P(Params)]),
         "su2hip_measure_format": (C.c_int, [P(Params), dp, C.c_double, C.c_double, C.c_char_p, C.c_int]),
+        "su2hip_print_acceptance": (C.c_int, [C.c_void_p, P(RunConfig), P(Counters)]),
+        # request loop (include/su2hip_serve.h)
+        "su2hip_serve_defaults": (None, [P(ServeConfig)]),
+        "su2hip_serve_updates_per_sweep": (C.c_double, [P(Params)]),
+        "su2hip_serve_stream": (C.c_int, [C.c_uint64, C.c_double, C.c_int, C.c_int, P(Request)]),
+        "su2hip_serve": (C.c_int, [P(ServeConfig), P(Request), C.c_int, P(Result), P(ServeStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -646,7 +646,7 @@ int su2hip_footprint(const su2hip_params *p, int with_flow, uint64_t *bytes) {
   size_t b = slot_bytes(*p);
   size_t Vh = (size_t)vol_of(*p) / 2;
   if (p->nhiggs || p->triplet) b += align256((p->nhiggs ? 8 : 6) * Vh * wsize(*p));
-  if (with_flow) b += align256(flow_words(*p) * wsize(*p));
+  if (with_flow) b += align256(flow_words(*p) * wsize(*p)) + 4096;  // as su2hip_flow allocates
   *bytes = b;
   return SU2HIP_OK;
 }
@@ -655,7 +655,14 @@ int su2hip_create(int n, const su2hip_params *params, int max_slots, uint64_t hb
                   su2hip_batch_t **out) {
   if (!out) return fail(SU2HIP_EINVAL, "out is NULL");
   *out = nullptr;
-  if (n < 1 || !params) return fail(SU2HIP_EINVAL, "need at least one lattice");
+  // n = 0 makes an empty handle whose lattices come later through su2hip_add;
+  // params[0] still fixes the field content, precision and schedule
+  if (n < 0 || !params) return fail(SU2HIP_EINVAL, "need the parameters of lattice 0");
+  if (n == 0) {
+    int rc = validate(params);
+    if (rc) return rc;
+    if (max_slots < 1) return fail(SU2HIP_EINVAL, "an empty handle needs max_slots >= 1");
+  }
   if (max_slots < n) max_slots = n;
   for (int i = 0; i < n; i++) {
     int rc = validate(&params[i]);
