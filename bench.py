@@ -119,13 +119,14 @@ def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100):
     res["kernel_ms"] = {abi.KIND_NAMES[k]: float(ms[k]) for k in range(abi.NKINDS) if n[k]}
     res["kernel_launches"] = {abi.KIND_NAMES[k]: int(n[k]) for k in range(abi.NKINDS) if n[k]}
     res["timing_sweeps"] = ts
-    # average launch duration of every pass kind of the sweep, each over 200
-    # back-to-back launches between one HIP event pair on the library stream
-    res["kernel_avg_ms"] = {}
+    # average duration of every pass kind of the sweep over 200 back-to-back
+    # launches on the library stream: kernel time from the start/stop events
+    # each dispatch stamps, and the span per launch including launch gaps
+    res["kernel_avg_ms"], res["kernel_span_ms"] = {}, {}
     for name in res["kernel_launches"]:
         k = abi.KIND_NAMES.index(name)
         if k < abi.K_MEASURE:
-            res["kernel_avg_ms"][name] = b.time_pass(k, 200)
+            res["kernel_avg_ms"][name], res["kernel_span_ms"][name] = b.time_pass(k, 200)
     if measure_obs:
         b.use_graph(True)
         plaq, phi2 = [], []
@@ -156,11 +157,52 @@ def roofline(res):
     bytes_per_launch = words * res["V"] * w
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     workload = "96^3 SU(2)xU(1)+Higgs fp%d" % (64 if p.precision == abi.FP64 else 32)
+    span_ms = res["kernel_span_ms"][kind]
     return {"bound": "hbm", "kernel": kind, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": load_traffic(kind, workload),
             "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
+            "avg_span_ms": span_ms,
+            "frac_with_launch_gaps": bytes_per_launch / (span_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "words_per_site_per_pass": words}
+
+
+def serving(n, rate=20.0, seed=2026):
+    """Scan serving: a seeded Poisson stream (rate req/s) of SU(2)+Higgs scan
+    points, L in {24..64}, betaG in [6,12], msq in [-0.3,-0.1], lambda in
+    [0.005,0.03], 100..1000 sweeps, every 10th request 64^3 with RK3 flow
+    dt 0.01 to t 4.0; FP32-mixed sweeps, admission by HBM footprint,
+    continuous batching (su2hip_serve, realtime)."""
+    from su2higgs_amd import serve
+    reqs = serve.stream(seed, rate, n, flow_every=10)
+    cfg = serve.config()
+    res, st = serve.serve(cfg, reqs)
+    flow_lat = [r.t_done - r.t_arrival for r, q in zip(res, reqs) if q.flow and r.status == 0]
+    return {"requests": n, "rate_per_s": rate, "seed": seed, "completed": st["completed"],
+            "failed": st["failed"], "p50_s": st["p50"], "p99_s": st["p99"], "mean_s": st["mean"],
+            "max_wait_s": st["max_wait"], "makespan_s": st["makespan"],
+            "last_arrival_s": reqs[n - 1].arrival if n else 0.0,
+            "site_updates_per_s": st["site_updates_per_s"], "busy_frac": st["busy"] / st["makespan"],
+            "ticks": st["ticks"], "peak_resident": st["peak_resident"],
+            "peak_bytes": st["peak_bytes"], "hbm_cap_bytes": cfg.hbm_budget, "flows": st["flows"],
+            "flow_request_p50_s": float(np.median(flow_lat)) if flow_lat else None,
+            "tick_sweeps": cfg.tick_sweeps, "precision": "FP32 sweeps, FP64 sums"}
+
+
+def multicanonical():
+    """48^3 SU(2)+adjoint Higgs, multicanonical in Sigmasq with the READONLY
+    weight of 200 bins on [0,2] (seeded slopes in [-5,5]), one global check
+    per parity half-sweep, 500 + 2000 sweeps, FP32-mixed (FP64 order
+    parameter, weight and accept), through the su2hip driver; compared with
+    the reference's own chains of the same run (tests/golden)."""
+    from su2higgs_amd import muca48
+    if not os.path.exists(muca48.FIXTURE):
+        return None
+    ours = muca48.run(fp32=True)
+    c = muca48.compare(ours, muca48.reference())
+    c["wall_s"] = ours["wall_s"]
+    c["sweeps_per_s_incl_io"] = 2500 / ours["wall_s"]
+    return c
 
 
 def batched_scan(steps):
@@ -279,6 +321,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-muca", action="store_true", help="skip the multicanonical run")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests of the serving stream (0 skips it)")
     ap.add_argument("--no-obs", action="store_true", help="skip the observable comparison")
     ap.add_argument("--probe-bytes", type=int, default=0,
                     help="only run the streaming copy kernel over this many bytes (HBM counter calibration)")
@@ -308,8 +353,21 @@ def main():
     batched = None
     if not args.no_batch:
         log("[bench] batched scan, 96 lattices")
-        batched = batched_scan(max(5, min(args.steps // 10, 50)))
+        batched = batched_scan(max(5, 2 * args.steps))
         log("[bench] batched %.3e site-updates/s" % batched["site_updates_per_s"])
+    served = None
+    if args.serve_requests > 0:
+        log("[bench] serving %d requests at 20/s" % args.serve_requests)
+        served = serving(args.serve_requests)
+        log("[bench] serving p50 %.3f s p99 %.3f s, %.3e site-updates/s"
+            % (served["p50_s"], served["p99_s"], served["site_updates_per_s"]))
+    muca = None
+    if not args.no_muca:
+        log("[bench] multicanonical 48^3 adjoint, 2500 sweeps")
+        muca = multicanonical()
+        if muca:
+            log("[bench] muca <R> %.6f +- %.6f vs reference %.6f +- %.6f (z %.2f)"
+                % (muca["mean"], muca["err"], muca["ref_mean"], muca["ref_err"], muca["z"]))
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline")
@@ -341,11 +399,15 @@ def main():
         "eager_value": r32["eager_site_updates_per_s"],
         "fp64_eager_value": r64["eager_site_updates_per_s"],
         "batched_scan": batched,
+        "serving": served,
+        "multicanonical": muca,
         "observables": {"fp32": o32, "fp64": o64},
         "roofline": roof,
         "roofline_fp64": roof64,
         "kernel_avg_ms_fp32": r32["kernel_avg_ms"],
         "kernel_avg_ms_fp64": r64["kernel_avg_ms"],
+        "kernel_span_ms_fp32": r32["kernel_span_ms"],
+        "kernel_span_ms_fp64": r64["kernel_span_ms"],
         "kernel_ms_fp32": r32["kernel_ms"],
         "kernel_launches_fp32": r32["kernel_launches"],
         "kernel_ms_fp64": r64["kernel_ms"],

@@ -302,12 +302,16 @@ int su2hip_flow_force(su2hip_batch_t *b, int slot, double *force);
 int su2hip_set_timing(su2hip_batch_t *b, int enable);
 int su2hip_kernel_times(su2hip_batch_t *b, double *ms, int64_t *launches);
 int su2hip_reset_timing(su2hip_batch_t *b);
-/* Average duration (ms) of `repeats` back-to-back launches of one pass kind
- * over every active lattice, bracketed by one HIP event pair on the handle's
- * stream (directions and parities cycle as in a sweep; the fields are
- * updated and each lattice consumes `repeats` pass numbers and one sweep of
- * its budget). */
-int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms);
+/* Durations of `repeats` back-to-back launches of one pass kind over every
+ * active lattice on the handle's stream: *ms_kernel is the average kernel
+ * duration (start and end stamped on each dispatch by hipExtLaunchKernel's
+ * events, the interval a kernel profiler reports), *ms_span the average
+ * share of the whole span including the gaps between dependent launches.
+ * Directions and parities cycle as in a sweep; the fields are updated and
+ * each lattice consumes `repeats` pass numbers and one sweep of its budget.
+ * Either output may be NULL. */
+int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel,
+                     double *ms_span);
 /* Copy `bytes` of device memory with the library's streaming copy kernel and
  * return its duration (ms): the calibration kernel for HBM counters. */
 int su2hip_copy_probe(uint64_t bytes, int repeats, double *ms);

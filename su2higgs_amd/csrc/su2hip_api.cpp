@@ -1453,31 +1453,45 @@ int su2hip_reset_timing(su2hip_batch_t *b) {
   return SU2HIP_OK;
 }
 
-int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms) {
-  if (!b || !ms) return fail(SU2HIP_EINVAL, "NULL argument");
+int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel,
+                     double *ms_span) {
+  if (!b || (!ms_kernel && !ms_span)) return fail(SU2HIP_EINVAL, "NULL argument");
   if (kind < 0 || kind >= NCOUNT || repeats < 1) return fail(SU2HIP_EINVAL, "kind/repeats");
   int rc = prepare(b);
   if (rc) return rc;
   int ndir = (kind <= SU2HIP_K_U1_METROPOLIS) ? 3 : 1;
-  hipEvent_t e0, e1;
-  HIPTRY(hipEventCreate(&e0));
-  HIPTRY(hipEventCreate(&e1));
+  // one start/stop pair per launch, stamped by the kernel dispatch itself
+  std::vector<hipEvent_t> ev(2 * (size_t)repeats + 2, nullptr);
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
   LaunchCtx c = ctx_of(b);
   int reunit = b->tmpl.reunitarize && !b->fp64;
-  hipError_t e = hipEventRecord(e0, b->stream);
-  for (int i = 0; i < repeats && e == hipSuccess; i++)
+  hipEvent_t s0 = ev[2 * (size_t)repeats], s1 = ev[2 * (size_t)repeats + 1];
+  if (e == hipSuccess) e = hipEventRecord(s0, b->stream);
+  for (int i = 0; i < repeats && e == hipSuccess; i++) {
+    c.ev_start = ev[2 * (size_t)i];
+    c.ev_stop = ev[2 * (size_t)i + 1];
     e = launch_pass(c, kind, i & 1, (i >> 1) % ndir, i, -1, reunit);
-  if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-  if (e == hipSuccess) e = hipEventSynchronize(e1);
-  float t = 0.f;
-  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  }
+  if (e == hipSuccess) e = hipEventRecord(s1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(s1);
+  double sum = 0.0;
+  float span = 0.f;
+  for (int i = 0; i < repeats && e == hipSuccess; i++) {
+    float t = 0.f;
+    e = hipEventElapsedTime(&t, ev[2 * (size_t)i], ev[2 * (size_t)i + 1]);
+    sum += t;
+  }
+  if (e == hipSuccess) e = hipEventElapsedTime(&span, s0, s1);
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  c.ev_start = c.ev_stop = nullptr;
   if (e == hipSuccess) e = launch_end_sweep(c, repeats);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   if (e == hipErrorInvalidValue) return fail(SU2HIP_EINVAL, "kind %d is not part of this model", kind);
   if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "time_pass: %s", hipGetErrorString(e));
-  *ms = t / repeats;
+  if (ms_kernel) *ms_kernel = sum / repeats;
+  if (ms_span) *ms_span = (double)span / repeats;
   return SU2HIP_OK;
 }
 

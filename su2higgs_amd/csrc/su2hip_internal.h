@@ -112,6 +112,9 @@ struct LaunchCtx {
   int max_slots;
   int model;
   int fp64;
+  // when set, pass kernels are launched with hipExtLaunchKernel and these
+  // events take the kernel's own start and end timestamps
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 };
 
 // One checkerboard pass.  muca_step < 0: plain pass; otherwise the pass is

@@ -9,6 +9,7 @@
 // reference's src/).  The SU(2) algebra is quaternion products; nothing here
 // is a dense contraction, so MFMA is not used and the kernels are bound by
 // HBM traffic and FP math.
+#include <hip/hip_ext.h>
 #include <type_traits>
 
 #include "su2hip_device.h"
@@ -1326,8 +1327,15 @@ template <typename T, int M, int KIND>
 static hipError_t pass_one(const LaunchCtx &c, int parity, int dir, int pass_off, int step,
                            int reunit) {
   if constexpr (kind_valid<M, KIND>()) {
-    hipLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.muca,
-                       c.blk, parity, dir, pass_off, step, reunit, c.cnt, c.nblk, c.partial);
+    if (c.ev_start || c.ev_stop) {
+      hipExtLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.ev_start,
+                            c.ev_stop, 0, c.desc, c.muca, c.blk, parity, dir, pass_off, step,
+                            reunit, c.cnt, c.nblk, c.partial);
+    } else {
+      hipLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc,
+                         c.muca, c.blk, parity, dir, pass_off, step, reunit, c.cnt, c.nblk,
+                         c.partial);
+    }
     return hipGetLastError();
   }
   return hipErrorInvalidValue;

@@ -156,10 +156,11 @@ class Batch:
         check(self.lib.su2hip_reset_timing(self.h))
 
     def time_pass(self, kind, repeats=100):
-        """Average ms of back-to-back launches of one pass kind."""
-        ms = C.c_double()
-        check(self.lib.su2hip_time_pass(self.h, kind, repeats, C.byref(ms)))
-        return ms.value
+        """(average kernel ms, average ms of the back-to-back span) of
+        `repeats` launches of one pass kind."""
+        k, s = C.c_double(), C.c_double()
+        check(self.lib.su2hip_time_pass(self.h, kind, repeats, C.byref(k), C.byref(s)))
+        return k.value, s.value
 
 
 def copy_probe(nbytes, repeats=20):
