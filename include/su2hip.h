@@ -312,6 +312,25 @@ int su2hip_reset_timing(su2hip_batch_t *b);
  * Either output may be NULL. */
 int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel,
                      double *ms_span);
+/* ---- launch shape and autotuning -----------------------------------------
+ * Pass kernels run one thread per site of one parity in workgroups of
+ * 64, 128 or 256 threads, each over a contiguous chunk of that many sites
+ * of one lattice (chunks dealt to the 8 XCDs in contiguous runs).  The
+ * chunk of a pass kind is the one set here, or with 0 (default) the widest
+ * that still gives 4 workgroups per CU for the batch.  Results do not depend
+ * on the chunk.  Multicanonical steps always use 256. */
+int su2hip_set_chunk(su2hip_batch_t *b, int kind, int chunk);
+/* The chunk a pass kind launches with now. */
+int su2hip_get_chunk(su2hip_batch_t *b, int kind, int *chunk);
+/* Time every pass kind of the batch's sweep with each chunk (kernel
+ * durations of `repeats` back-to-back launches, as su2hip_time_pass) and
+ * keep the fastest; 256 stays unless a smaller chunk is 2% faster.  Fields,
+ * RNG positions, budgets and counters are restored afterwards (the handle
+ * needs one more copy of its lattices' memory meanwhile).  table, if not
+ * NULL, receives the milliseconds as table[kind * 3 + {0,1,2}] for chunks
+ * 64, 128, 256 (entries of kinds not in the sweep are left alone). */
+int su2hip_autotune(su2hip_batch_t *b, int repeats, double *table);
+
 /* Copy `bytes` of device memory with the library's streaming copy kernel and
  * return its duration (ms): the calibration kernel for HBM counters. */
 int su2hip_copy_probe(uint64_t bytes, int repeats, double *ms);

@@ -260,6 +260,9 @@ def load():
         "su2hip_kernel_times": (C.c_int, [h, dp, P(C.c_int64)]),
         "su2hip_reset_timing": (C.c_int, [h]),
         "su2hip_copy_probe": (C.c_int, [C.c_uint64, C.c_int, dp]),
+        "su2hip_set_chunk": (C.c_int, [h, C.c_int, C.c_int]),
+        "su2hip_get_chunk": (C.c_int, [h, C.c_int, P(C.c_int)]),
+        "su2hip_autotune": (C.c_int, [h, C.c_int, dp]),
         "su2hip_time_pass": (C.c_int, [h, C.c_int, C.c_int, dp, dp]),
         # host I/O layer (include/su2hip_io.h)
         "su2hip_config_read": (C.c_int, [C.c_char_p, C.c_char_p, P(RunConfig)]),

@@ -63,6 +63,19 @@ struct Ev {
   hipEvent_t a, b;
 };
 
+// Block table of pass kernels launched with a smaller workgroup (64 or 128
+// sites per chunk), with its own per-block counters.
+struct AltMap {
+  int chunk = 0;
+  BlockMap *d_blk = nullptr;
+  long long *d_cnt = nullptr;
+  int nblk = 0, cap = 0;
+  std::vector<BlockMap> h;
+};
+
+constexpr int NCHUNK = 3;
+constexpr int CHUNKS[NCHUNK] = {64, 128, 256};
+
 }  // namespace
 
 struct su2hip_batch {
@@ -95,6 +108,8 @@ struct su2hip_batch {
   std::vector<hipEvent_t> pool;
   double kms[SU2HIP_NKINDS] = {};
   long long kcount[SU2HIP_NKINDS] = {};
+  AltMap alt[2];                  // chunks 64 and 128; 256 is the main table
+  int tune[SU2HIP_NKINDS] = {};   // chunk per pass kind, 0 = chosen by size
 };
 
 // ---------------------------------------------------------------------------
@@ -185,6 +200,28 @@ static LaunchCtx ctx_of(su2hip_batch *b) {
   c.max_slots = b->max_slots;
   c.model = b->model;
   c.fp64 = b->fp64;
+  return c;
+}
+
+// workgroup width of a pass kind: the tuned one, else the widest that still
+// gives at least 4 workgroups per CU (256 CUs), else 64
+static int chunk_for(su2hip_batch *b, int kind) {
+  if (kind >= 0 && kind < SU2HIP_NKINDS && b->tune[kind]) return b->tune[kind];
+  if (b->nblk >= 1024) return 256;
+  if (b->alt[1].nblk >= 1024) return 128;
+  return 64;
+}
+
+static LaunchCtx ctx_for(su2hip_batch *b, int kind) {
+  LaunchCtx c = ctx_of(b);
+  int ch = chunk_for(b, kind);
+  for (int k = 0; k < 2; k++)
+    if (ch == b->alt[k].chunk) {
+      c.blk = b->alt[k].d_blk;
+      c.nblk = b->alt[k].nblk;
+      c.cnt = b->alt[k].d_cnt;
+      c.block = ch;
+    }
   return c;
 }
 
@@ -314,20 +351,71 @@ static void add_counts(su2hip_counters &c, int kind, long long acc, long long to
   }
 }
 
-// fold the per-block device counters into the slots' host counters
-static int fold_counts(su2hip_batch *b) {
-  if (!b->d_cnt || b->nblk == 0) return SU2HIP_OK;
-  size_t n = (size_t)NCOUNT * 2 * b->nblk;
+// fold per-block device counters of one block table into the slots' host
+// counters and clear them
+static int fold_table(su2hip_batch *b, long long *d_cnt, int nblk, const std::vector<BlockMap> &hb) {
+  if (!d_cnt || nblk == 0) return SU2HIP_OK;
+  size_t n = (size_t)NCOUNT * 2 * nblk;
   std::vector<long long> h(n);
-  HIPTRY(hipMemcpyAsync(h.data(), b->d_cnt, n * sizeof(long long), hipMemcpyDeviceToHost,
-                        b->stream));
-  HIPTRY(hipMemsetAsync(b->d_cnt, 0, n * sizeof(long long), b->stream));
+  HIPTRY(hipMemcpyAsync(h.data(), d_cnt, n * sizeof(long long), hipMemcpyDeviceToHost, b->stream));
+  HIPTRY(hipMemsetAsync(d_cnt, 0, n * sizeof(long long), b->stream));
   HIPTRY(hipStreamSynchronize(b->stream));
   for (int k = 0; k < NCOUNT; k++)
-    for (int i = 0; i < b->nblk; i++) {
-      long long a = h[(size_t)(k * 2) * b->nblk + i], t = h[(size_t)(k * 2 + 1) * b->nblk + i];
-      if (a || t) add_counts(b->slots[b->h_blk[i].lat].cnt, k, a, t);
+    for (int i = 0; i < nblk; i++) {
+      long long a = h[(size_t)(k * 2) * nblk + i], t = h[(size_t)(k * 2 + 1) * nblk + i];
+      if (a || t) add_counts(b->slots[hb[i].lat].cnt, k, a, t);
     }
+  return SU2HIP_OK;
+}
+
+static int fold_counts(su2hip_batch *b) {
+  int rc = fold_table(b, b->d_cnt, b->nblk, b->h_blk);
+  for (int k = 0; k < 2 && !rc; k++) rc = fold_table(b, b->alt[k].d_cnt, b->alt[k].nblk, b->alt[k].h);
+  return rc;
+}
+
+// logical chunks lattice by lattice, dealt to physical workgroups so that
+// XCD j (workgroups j, j+8, ...) gets a contiguous run of chunks
+static std::vector<BlockMap> deal_to_xcds(const std::vector<BlockMap> &logical) {
+  int n = (int)logical.size();
+  std::vector<BlockMap> phys(n);
+  const int X = 8;
+  int base = 0;
+  for (int j = 0; j < X; j++) {
+    int cnt = n / X + (j < n % X ? 1 : 0);
+    for (int k = 0; k < cnt; k++) phys[j + X * k] = logical[base + k];
+    base += cnt;
+  }
+  return phys;
+}
+
+static int build_alt(su2hip_batch *b, AltMap &m, int chunk) {
+  m.chunk = chunk;
+  std::vector<BlockMap> logical;
+  for (int i = 0; i < b->max_slots; i++) {
+    Slot &s = b->slots[i];
+    if (!s.used) continue;
+    for (int k = 0; k * chunk < s.hd.Vh; k++) logical.push_back(BlockMap{i, k * chunk});
+  }
+  int n = (int)logical.size();
+  if (n > m.cap) {
+    if (m.d_blk) HIPTRY(hipFree(m.d_blk));
+    if (m.d_cnt) HIPTRY(hipFree(m.d_cnt));
+    m.d_blk = nullptr;
+    m.d_cnt = nullptr;
+    int cap = std::max(n, 256);
+    if (hipMalloc(&m.d_blk, sizeof(BlockMap) * cap) != hipSuccess ||
+        hipMalloc(&m.d_cnt, sizeof(long long) * NCOUNT * 2 * cap) != hipSuccess)
+      return fail(SU2HIP_ENOMEM, "block table allocation failed");
+    m.cap = cap;
+  }
+  m.h = deal_to_xcds(logical);
+  m.nblk = n;
+  if (n) {
+    HIPTRY(hipMemcpyAsync(m.d_blk, m.h.data(), sizeof(BlockMap) * n, hipMemcpyHostToDevice,
+                          b->stream));
+    HIPTRY(hipMemsetAsync(m.d_cnt, 0, sizeof(long long) * NCOUNT * 2 * n, b->stream));
+  }
   return SU2HIP_OK;
 }
 
@@ -350,14 +438,7 @@ static int rebuild_map(su2hip_batch *b) {
     for (int k = 0; k < nb; k++) logical.push_back(BlockMap{i, k * BLK});
   }
   int n = (int)logical.size();
-  std::vector<BlockMap> phys(n);
-  const int X = 8;
-  int base = 0;
-  for (int j = 0; j < X; j++) {
-    int cnt = n / X + (j < n % X ? 1 : 0);
-    for (int k = 0; k < cnt; k++) phys[j + X * k] = logical[base + k];
-    base += cnt;
-  }
+  std::vector<BlockMap> phys = deal_to_xcds(logical);
   if (n > b->nblk_cap) {
     if (b->d_blk) HIPTRY(hipFree(b->d_blk));
     if (b->d_cnt) HIPTRY(hipFree(b->d_cnt));
@@ -384,6 +465,8 @@ static int rebuild_map(su2hip_batch *b) {
                           b->stream));
     HIPTRY(hipMemsetAsync(b->d_cnt, 0, sizeof(long long) * NCOUNT * 2 * n, b->stream));
   }
+  for (int k = 0; k < 2 && !rc; k++) rc = build_alt(b, b->alt[k], CHUNKS[k]);
+  if (rc) return rc;
   rc = push_desc(b);
   if (rc) return rc;
   b->map_dirty = false;
@@ -503,10 +586,12 @@ static hipError_t timed(su2hip_batch *b, int kind, Fn &&fn) {
 static hipError_t enqueue_step(su2hip_batch *b, const Step &st) {
   LaunchCtx c = ctx_of(b);
   int reunit = b->tmpl.reunitarize && !b->fp64;
-  if (!st.muca)
+  if (!st.muca) {
+    LaunchCtx ct = ctx_for(b, st.kind);
     return timed(b, st.kind, [&] {
-      return launch_pass(c, st.kind, st.parity, st.dir, st.pass_off, -1, reunit);
+      return launch_pass(ct, st.kind, st.parity, st.dir, st.pass_off, -1, reunit);
     });
+  }
   for (int k = 0; k < b->muca_steps; k++) {
     hipError_t e = timed(b, st.kind, [&] {
       return launch_pass(c, st.kind, st.parity, st.dir, st.pass_off, k, reunit);
@@ -722,6 +807,10 @@ int su2hip_destroy(su2hip_batch_t *b) {
   if (b->d_partial) (void)hipFree(b->d_partial);
   if (b->d_sums) (void)hipFree(b->d_sums);
   if (b->d_stage) (void)hipFree(b->d_stage);
+  for (auto &m : b->alt) {
+    if (m.d_blk) (void)hipFree(m.d_blk);
+    if (m.d_cnt) (void)hipFree(m.d_cnt);
+  }
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return SU2HIP_OK;
@@ -1453,8 +1542,8 @@ int su2hip_reset_timing(su2hip_batch_t *b) {
   return SU2HIP_OK;
 }
 
-int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel,
-                     double *ms_span) {
+static int time_pass_impl(su2hip_batch *b, int kind, int repeats, double *ms_kernel,
+                          double *ms_span) {
   if (!b || (!ms_kernel && !ms_span)) return fail(SU2HIP_EINVAL, "NULL argument");
   if (kind < 0 || kind >= NCOUNT || repeats < 1) return fail(SU2HIP_EINVAL, "kind/repeats");
   int rc = prepare(b);
@@ -1464,7 +1553,7 @@ int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel
   std::vector<hipEvent_t> ev(2 * (size_t)repeats + 2, nullptr);
   hipError_t e = hipSuccess;
   for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
-  LaunchCtx c = ctx_of(b);
+  LaunchCtx c = ctx_for(b, kind);
   int reunit = b->tmpl.reunitarize && !b->fp64;
   hipEvent_t s0 = ev[2 * (size_t)repeats], s1 = ev[2 * (size_t)repeats + 1];
   if (e == hipSuccess) e = hipEventRecord(s0, b->stream);
@@ -1485,7 +1574,7 @@ int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel
   if (e == hipSuccess) e = hipEventElapsedTime(&span, s0, s1);
   for (hipEvent_t x : ev)
     if (x) (void)hipEventDestroy(x);
-  c.ev_start = c.ev_stop = nullptr;
+  c = ctx_of(b);
   if (e == hipSuccess) e = launch_end_sweep(c, repeats);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   if (e == hipErrorInvalidValue) return fail(SU2HIP_EINVAL, "kind %d is not part of this model", kind);
@@ -1493,6 +1582,105 @@ int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel
   if (ms_kernel) *ms_kernel = sum / repeats;
   if (ms_span) *ms_span = (double)span / repeats;
   return SU2HIP_OK;
+}
+
+int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel,
+                     double *ms_span) {
+  return time_pass_impl(b, kind, repeats, ms_kernel, ms_span);
+}
+
+int su2hip_set_chunk(su2hip_batch_t *b, int kind, int chunk) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  if (kind < 0 || kind >= NCOUNT) return fail(SU2HIP_EINVAL, "kind %d is not a pass kind", kind);
+  if (chunk != 0 && chunk != 64 && chunk != 128 && chunk != 256)
+    return fail(SU2HIP_EINVAL, "chunk %d (0, 64, 128 or 256)", chunk);
+  b->tune[kind] = chunk;
+  b->graph_valid = false;
+  return SU2HIP_OK;
+}
+
+int su2hip_get_chunk(su2hip_batch_t *b, int kind, int *chunk) {
+  if (!b || !chunk) return fail(SU2HIP_EINVAL, "NULL argument");
+  if (kind < 0 || kind >= NCOUNT) return fail(SU2HIP_EINVAL, "kind %d is not a pass kind", kind);
+  int rc = prepare(b);
+  if (rc) return rc;
+  *chunk = chunk_for(b, kind);
+  return SU2HIP_OK;
+}
+
+int su2hip_autotune(su2hip_batch_t *b, int repeats, double *table) {
+  if (!b || repeats < 1) return fail(SU2HIP_EINVAL, "NULL handle or repeats < 1");
+  int rc = prepare(b);
+  if (rc) return rc;
+  if (b->nblk == 0) return SU2HIP_OK;
+  // the pass kinds of this batch's sweep
+  bool in_sweep[NCOUNT] = {};
+  for (const Step &st : b->sched) in_sweep[st.kind] = true;
+  // snapshot: fields, pass counters, budgets and counters come back as they were
+  HIPTRY(hipStreamSynchronize(b->stream));
+  rc = fold_counts(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  std::vector<void *> save(b->max_slots, nullptr);
+  std::vector<LatDesc> hd(b->max_slots);
+  std::vector<su2hip_counters> cnt(b->max_slots);
+  auto release = [&] {
+    for (void *q : save)
+      if (q) (void)hipFree(q);
+  };
+  for (int i = 0; i < b->max_slots; i++) {
+    Slot &sl = b->slots[i];
+    if (!sl.used) continue;
+    hd[i] = sl.hd;
+    cnt[i] = sl.cnt;
+    if (hipMalloc(&save[i], sl.bytes) != hipSuccess) {
+      save[i] = nullptr;
+      release();
+      return fail(SU2HIP_ENOMEM, "autotune snapshot of %zu bytes", sl.bytes);
+    }
+    hipError_t e = hipMemcpyAsync(save[i], sl.mem, sl.bytes, hipMemcpyDeviceToDevice, b->stream);
+    if (e != hipSuccess) {
+      release();
+      return fail(SU2HIP_EDEVICE, "autotune snapshot: %s", hipGetErrorString(e));
+    }
+  }
+  int saved_tune[SU2HIP_NKINDS];
+  memcpy(saved_tune, b->tune, sizeof(saved_tune));
+  for (int k = 0; k < NCOUNT && !rc; k++) {
+    if (!in_sweep[k]) continue;
+    double best = 0.0;
+    int pick = 256;
+    for (int c = 0; c < NCHUNK && !rc; c++) {
+      b->tune[k] = CHUNKS[c];
+      double ms = 0.0;
+      rc = time_pass_impl(b, k, repeats, &ms, nullptr);
+      if (table) table[k * NCHUNK + c] = ms;
+      // a smaller workgroup has to win by 2% to replace 256
+      if (!rc && (c == 0 || ms < best * (CHUNKS[c] == 256 ? 1.02 : 1.0))) {
+        best = ms;
+        pick = CHUNKS[c];
+      }
+    }
+    b->tune[k] = rc ? saved_tune[k] : pick;
+  }
+  // restore the state
+  int rc2 = fold_counts(b);
+  for (int i = 0; i < b->max_slots && !rc2; i++) {
+    Slot &sl = b->slots[i];
+    if (!sl.used) continue;
+    if (hipMemcpyAsync(sl.mem, save[i], sl.bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+      rc2 = fail(SU2HIP_EDEVICE, "autotune restore");
+    sl.hd.pass = hd[i].pass;
+    sl.hd.budget = hd[i].budget;
+    sl.cnt = cnt[i];
+  }
+  if (!rc2) rc2 = push_desc(b);
+  if (!rc2 && hipStreamSynchronize(b->stream) != hipSuccess) rc2 = fail(SU2HIP_EDEVICE, "autotune sync");
+  release();
+  rc2 = rc2 ? rc2 : orderparam_refresh(b);
+  b->graph_valid = false;
+  return rc ? rc : rc2;
 }
 
 int su2hip_copy_probe(uint64_t bytes, int repeats, double *ms) {

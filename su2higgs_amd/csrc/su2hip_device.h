@@ -125,7 +125,11 @@ __device__ __forceinline__ Q<T> u1twist(T alpha) {
   if constexpr (sizeof(T) == 8) {
     sincos(alpha, &s, &c);
   } else {
-    sincosf(alpha, &s, &c);
+    // FP32 sweeps: the hardware sine/cosine on [-1/2, 1/2] revolutions
+    float t = alpha * 0.159154943091895336f;
+    t -= rintf(t);
+    s = __builtin_amdgcn_sinf(t);
+    c = __builtin_amdgcn_cosf(t);
   }
   return Q<T>{c, (T)0, (T)0, -s};
 }

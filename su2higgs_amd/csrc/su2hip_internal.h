@@ -115,6 +115,9 @@ struct LaunchCtx {
   // when set, pass kernels are launched with hipExtLaunchKernel and these
   // events take the kernel's own start and end timestamps
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  // threads per workgroup of pass kernels = sites per block-table chunk
+  // (64, 128 or 256); blk/nblk/cnt then belong to the table of that chunk
+  int block = BLK;
 };
 
 // One checkerboard pass.  muca_step < 0: plain pass; otherwise the pass is

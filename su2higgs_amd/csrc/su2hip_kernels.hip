@@ -25,6 +25,28 @@ template <typename T> __device__ __forceinline__ T mexp(T x) { if constexpr (siz
 template <typename T> __device__ __forceinline__ T mcos(T x) { if constexpr (sizeof(T) == 8) return ::cos(x); else return ::cosf(x); }
 template <typename T> __device__ __forceinline__ T msin(T x) { if constexpr (sizeof(T) == 8) return ::sin(x); else return ::sinf(x); }
 
+// Transcendentals of the sweep updates.  FP64 keeps the accurate library
+// functions above (the reference's arithmetic, compared with it to 1e-12).
+// FP32 uses the hardware instructions -- v_log_f32, v_exp_f32, v_sqrt_f32,
+// v_rsq_f32, v_rcp_f32, and v_sin_f32/v_cos_f32 on an argument reduced to
+// [-1/2, 1/2] revolutions -- whose errors of a few ulp sit far below what
+// the FP32 mode is checked to (observables within statistical errors).
+constexpr float LN2F = 0.693147180559945309f;
+constexpr float LOG2EF = 1.442695040888963407f;
+constexpr float INV2PIF = 0.159154943091895336f;
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * LN2F; }
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * LOG2EF); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// cos and sin of 2 pi t
+__device__ __forceinline__ float fcos2pi(float t) { return __builtin_amdgcn_cosf(t - rintf(t)); }
+__device__ __forceinline__ float fsin2pi(float t) { return __builtin_amdgcn_sinf(t - rintf(t)); }
+__device__ __forceinline__ float fcos(float x) { return fcos2pi(x * INV2PIF); }
+__device__ __forceinline__ float fsin(float x) { return fsin2pi(x * INV2PIF); }
+// exp for an accept test: accurate in FP64, the hardware exp in FP32
+template <typename T> __device__ __forceinline__ T sexp(T x) { if constexpr (sizeof(T) == 8) return ::exp(x); else return fexp(x); }
+
 constexpr double PI = 3.14159265358979323846;
 
 template <int M, int KIND>
@@ -152,18 +174,48 @@ __device__ __forceinline__ double xy_or(double F, double X, double Ysq, double B
   return ::fabs(dV / dVn);
 }
 
+// FP32 version of xy_or: the same move, with the cubic's real root found by
+// Newton's method from the harmonic reflection -X - F/cc (the root for
+// C = 0).  With cc > 0 the cubic is increasing, so it has one real root
+// and Newton converges; otherwise the FP64 Cardano path decides, with the
+// reference's fallback.
+__device__ __forceinline__ float xy_or_f(float F, float X, float Ysq, float B, float C, float &nX) {
+  float cc = B + 2.f * Ysq * C;
+  float al = C, be = C * X, ga = cc + C * X * X, de = F + cc * X + C * X * X * X;
+  float x;
+  if (cc > 0.f) {
+    x = -X - F * frcp(cc);
+#pragma unroll 1
+    for (int it = 0; it < 8; it++) {
+      float g = ((al * x + be) * x + ga) * x + de;
+      float dg = (3.f * al * x + 2.f * be) * x + ga;
+      float dx = g * frcp(dg);
+      x -= dx;
+      if (fabsf(dx) <= 2.5e-7f * (1.f + fabsf(x))) break;
+    }
+  } else {
+    x = (float)polysolve3(al, be, ga, de);
+  }
+  nX = x;
+  float dV = F + 2.f * B * X + 4.f * C * (X * X * X + Ysq * X);
+  float dVn = F + 2.f * B * x + 4.f * C * (x * x * x + Ysq * x);
+  return fabsf(dV * frcp(dVn));
+}
+
 // accept rule of every Metropolis update (src/metropolis.c:31-40): a draw
 // is taken only when the action grows
 template <typename T>
 __device__ __forceinline__ int metro_accept(T diff, Rng &r) {
   if (diff < (T)0) return 1;
-  if (diff > (T)0 && mexp(-diff) > uni<T>(r)) return 1;
+  if (diff > (T)0 && sexp(-diff) > uni<T>(r)) return 1;
   return 0;
 }
 
 template <typename T>
 __device__ __forceinline__ Q<T> normalise(const Q<T> &q) {
-  T n = (T)1 / msqrt(qdot(q, q));
+  T n;
+  if constexpr (sizeof(T) == 8) n = 1.0 / ::sqrt(qdot(q, q));
+  else n = frsqrt(qdot(q, q));
   return qscale(q, n);
 }
 
@@ -196,24 +248,39 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       oldact = hop_triplet(old, a1, a2);
     }
     Q<T> V = link_staple<T, M>(L, U, A, F, x, mu);
-    T a = msqrt(qdot(V, V));
-    V = qscale(V, (T)-1 / a);
-    a *= (T)2;
     T r1, r2, r3, d, a0;
     int loop = 1;
-    do {
-      r1 = -mlog((T)1 - uni<T>(r)) / a;
-      r2 = -mlog((T)1 - uni<T>(r)) / a;
-      r3 = mcos((T)(2.0 * PI) * uni<T>(r));
-      r3 *= r3;
-      d = r1 * r3 + r2;
-      r3 = (T)1 - uni<T>(r);
-      loop++;
-    } while (r3 * r3 > (T)1 - (T)0.5 * d && loop <= 200);
+    if constexpr (sizeof(T) == 8) {
+      T a = ::sqrt(qdot(V, V));
+      V = qscale(V, (T)-1 / a);
+      a *= (T)2;
+      do {
+        r1 = -::log((T)1 - uni<T>(r)) / a;
+        r2 = -::log((T)1 - uni<T>(r)) / a;
+        r3 = ::cos((T)(2.0 * PI) * uni<T>(r));
+        r3 *= r3;
+        d = r1 * r3 + r2;
+        r3 = (T)1 - uni<T>(r);
+        loop++;
+      } while (r3 * r3 > (T)1 - (T)0.5 * d && loop <= 200);
+    } else {
+      // the same Kennedy-Pendleton loop on the hardware log/cos
+      T ia = frsqrt(qdot(V, V));
+      V = qscale(V, -ia);
+      ia *= (T)0.5;  // 1 / (2 |V|)
+      do {
+        r1 = -flog((T)1 - uni<T>(r)) * ia;
+        r2 = -flog((T)1 - uni<T>(r)) * ia;
+        r3 = fcos2pi(uni<T>(r));
+        r3 *= r3;
+        d = r1 * r3 + r2;
+        r3 = (T)1 - uni<T>(r);
+        loop++;
+      } while (r3 * r3 > (T)1 - (T)0.5 * d && loop <= 200);
+    }
     a0 = (loop > 200) ? (T)1e-9 : (T)1 - d;
     T rad = (T)1 - a0 * a0;
     if (rad < (T)0) return 0;
-    rad = msqrt(rad);
     Q<T> nu;
     if constexpr (sizeof(T) == 8) {
       // isotropic direction by cube rejection, the reference's draw order
@@ -225,23 +292,24 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
         r3 = (T)2 * uni<T>(r) - (T)1;
         d = r1 * r1 + r2 * r2 + r3 * r3;
       }
-      d = msqrt(d);
+      rad = ::sqrt(rad);
+      d = ::sqrt(d);
       nu = Q<T>{a0, r1 * (rad / d), r2 * (rad / d), r3 * (rad / d)};
     } else {
       // FP32: the same uniform direction on S^2 without rejection
       // (cos(theta) uniform in [-1,1], phi uniform), so a wavefront does
       // not wait for its slowest lane
+      rad = fsqrt(rad);
       T cz = (T)2 * uni<T>(r) - (T)1;
-      T ph = (T)(2.0 * PI) * uni<T>(r);
-      T sn, cs;
-      sincosf(ph, &sn, &cs);
-      T sr = msqrt(fmaxf((T)1 - cz * cz, (T)0)) * rad;
+      T ph = uni<T>(r);
+      T sn = fsin2pi(ph), cs = fcos2pi(ph);
+      T sr = fsqrt(fmaxf((T)1 - cz * cz, (T)0)) * rad;
       nu = Q<T>{a0, sr * cs, sr * sn, rad * cz};
     }
     nu = qmul(nu, qconj(V));
     if constexpr ((M & M_TR) != 0) {
       T diff = oldact - hop_triplet(nu, a1, a2);
-      if (!(uni<T>(r) < mexp(diff))) return 0;
+      if (!(uni<T>(r) < sexp(diff))) return 0;
     }
     stU(U, Vh, mu, par, s, nu);
     return 1;
@@ -255,8 +323,12 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     R.b = (T)1 * (uni<T>(r) - (T)0.5);
     R.c = (T)1 * (uni<T>(r) - (T)0.5);
     R.d = (T)1 * (uni<T>(r) - (T)0.5);
-    T n = msqrt(qdot(R, R));
-    R = Q<T>{R.a / n, R.b / n, R.c / n, R.d / n};
+    if constexpr (sizeof(T) == 8) {
+      T n = ::sqrt(qdot(R, R));
+      R = Q<T>{R.a / n, R.b / n, R.c / n, R.d / n};
+    } else {
+      R = qscale(R, frsqrt(qdot(R, R)));
+    }
     Q<T> nu = qmul(old, R);
     if (reunit) nu = normalise(nu);
     Q<T> du{nu.a - old.a, nu.b - old.b, nu.c - old.c, nu.d - old.d};
@@ -274,15 +346,22 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     // hopping term as in src/heatbath.c:107-120
     Q<T> old = ldU(U, Vh, mu, par, s);
     Q<T> V = link_staple<T, M>(L, U, A, F, x, mu);
-    T a = msqrt(qdot(V, V));
-    if (a == (T)0) return 1;
-    Q<T> w = qconj(Q<T>{V.a / a, V.b / a, V.c / a, V.d / a});
+    Q<T> w;
+    if constexpr (sizeof(T) == 8) {
+      T a = ::sqrt(qdot(V, V));
+      if (a == (T)0) return 1;
+      w = qconj(Q<T>{V.a / a, V.b / a, V.c / a, V.d / a});
+    } else {
+      T n2 = qdot(V, V);
+      if (n2 == (T)0) return 1;
+      w = qconj(qscale(V, frsqrt(n2)));
+    }
     Q<T> nu = qmul(qmul(w, qconj(old)), w);
     if (reunit) nu = normalise(nu);
     if constexpr ((M & M_TR) != 0) {
       V3<T> a1 = ldG(G, Vh, par, s), a2 = G_at(L, G, shift(L, x, mu, +1));
       T diff = hop_triplet(old, a1, a2) - hop_triplet(nu, a1, a2);
-      if (!(uni<T>(r) < mexp(diff))) return 0;
+      if (!(uni<T>(r) < sexp(diff))) return 0;
     }
     stU(U, Vh, mu, par, s, nu);
     return 1;
@@ -299,20 +378,43 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     T rr = (T)L.r_u1;
     Site xp = shift(L, x, mu, +1);
     T dS = 0;
+    if constexpr (sizeof(T) == 8) {
 #pragma unroll
-    for (int nu = 0; nu < 3; nu++) {
-      if (nu == mu) continue;
-      Site xn = shift(L, x, nu, +1), xm = shift(L, x, nu, -1), xpm = shift(L, xp, nu, -1);
-      T rest_up = A_at(L, A, xp, nu) - A_at(L, A, xn, mu) - A_at(L, A, x, nu);
-      T rest_dn = A_at(L, A, xm, mu) + A_at(L, A, xpm, nu) - A_at(L, A, xm, nu);
-      dS += mcos(rr * (al + rest_up)) - mcos(rr * (na + rest_up));
-      dS += mcos(rr * (rest_dn - al)) - mcos(rr * (rest_dn - na));
-    }
-    dS *= (T)L.betau1;
-    if constexpr ((M & M_NH) != 0) {
-      // hop = -(q0 cos a + q3 sin a), q = Phi(x)^+ U_mu(x) Phi(x+mu)
-      Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F, xp));
-      dS -= q.a * (mcos(na) - mcos(al)) + q.d * (msin(na) - msin(al));
+      for (int nu = 0; nu < 3; nu++) {
+        if (nu == mu) continue;
+        Site xn = shift(L, x, nu, +1), xm = shift(L, x, nu, -1), xpm = shift(L, xp, nu, -1);
+        T rest_up = A_at(L, A, xp, nu) - A_at(L, A, xn, mu) - A_at(L, A, x, nu);
+        T rest_dn = A_at(L, A, xm, mu) + A_at(L, A, xpm, nu) - A_at(L, A, xm, nu);
+        dS += ::cos(rr * (al + rest_up)) - ::cos(rr * (na + rest_up));
+        dS += ::cos(rr * (rest_dn - al)) - ::cos(rr * (rest_dn - na));
+      }
+      dS *= (T)L.betau1;
+      if constexpr ((M & M_NH) != 0) {
+        // hop = -(q0 cos a + q3 sin a), q = Phi(x)^+ U_mu(x) Phi(x+mu)
+        Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F, xp));
+        dS -= q.a * (::cos(na) - ::cos(al)) + q.d * (::sin(na) - ::sin(al));
+      }
+    } else {
+      // the same difference through cos A - cos B = -2 sin((A+B)/2) sin((A-B)/2)
+      // with m = (al+na)/2, h = (na-al)/2: one sine of the small step h, the
+      // rest of the accuracy of the hardware sine; exactly antisymmetric
+      // under exchanging al and na
+      T m = (T)0.5 * (al + na), h = (T)0.5 * (na - al);
+      T sum = 0;
+#pragma unroll
+      for (int nu = 0; nu < 3; nu++) {
+        if (nu == mu) continue;
+        Site xn = shift(L, x, nu, +1), xm = shift(L, x, nu, -1), xpm = shift(L, xp, nu, -1);
+        T rest_up = A_at(L, A, xp, nu) - A_at(L, A, xn, mu) - A_at(L, A, x, nu);
+        T rest_dn = A_at(L, A, xm, mu) + A_at(L, A, xpm, nu) - A_at(L, A, xm, nu);
+        sum += fsin(rr * (m + rest_up)) - fsin(rr * (rest_dn - m));
+      }
+      dS = (T)2 * (T)L.betau1 * fsin(rr * h) * sum;
+      if constexpr ((M & M_NH) != 0) {
+        Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F, xp));
+        // -(q0 (cos na - cos al) + q3 (sin na - sin al))
+        dS -= (T)2 * fsin(h) * (q.d * fcos(m) - q.a * fsin(m));
+      }
     }
     if (!metro_accept(dS, r)) return 0;
     A[(size_t)(mu * 2 + par) * Vh + s] = na;
@@ -333,7 +435,20 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     double C = 0.25 * L.lambda;
     Q<T> nf;
     int acc;
-    if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX) {
+    if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX && sizeof(T) == 4) {
+      // overrelax_doublet in FP32
+      T n2 = qdot(st, st);
+      T iF = frsqrt(n2);
+      T Fn = n2 * iF;
+      Q<T> e = qscale(st, iF);
+      T X = qdot(f, e);
+      Q<T> y{f.a - X * e.a, f.b - X * e.b, f.c - X * e.c, f.d - X * e.d};
+      T nX;
+      T p = xy_or_f(Fn, X, qdot(y, y), (T)B0, (T)C, nX);
+      if (!(p >= uni<T>(r))) return 0;
+      T k = nX + X;
+      nf = Q<T>{-f.a + k * e.a, -f.b + k * e.b, -f.c + k * e.c, -f.d + k * e.d};
+    } else if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX) {
       // overrelax_doublet (src/overrelax.c:54-128)
       double s0 = st.a, s1 = st.b, s2 = st.c, s3 = st.d;
       double Fn = ::sqrt(s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3);
@@ -366,7 +481,20 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     V3<T> st = triplet_staple<T>(L, U, G, x);
     double C = 0.25 * L.b4;
     V3<T> ng;
-    if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX) {
+    if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX && sizeof(T) == 4) {
+      // overrelax_triplet in FP32 (no portal term in B, src/overrelax.c:202)
+      T n2 = st.x * st.x + st.y * st.y + st.z * st.z;
+      T iF = frsqrt(n2);
+      T Fn = n2 * iF;
+      V3<T> e{st.x * iF, st.y * iF, st.z * iF};
+      T X = g.x * e.x + g.y * e.y + g.z * e.z;
+      V3<T> y{g.x - X * e.x, g.y - X * e.y, g.z - X * e.z};
+      T nX;
+      T p = xy_or_f(Fn, X, y.x * y.x + y.y * y.y + y.z * y.z, (T)(0.5 * L.msq_t + 3.0), (T)C, nX);
+      if (!(p >= uni<T>(r))) return 0;
+      T k = nX + X;
+      ng = V3<T>{-g.x + k * e.x, -g.y + k * e.y, -g.z + k * e.z};
+    } else if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX) {
       // overrelax_triplet (src/overrelax.c:137-236); the portal term is not
       // in B, as in the reference (src/overrelax.c:202)
       double s0 = st.x, s1 = st.y, s2 = st.z;
@@ -1327,12 +1455,15 @@ template <typename T, int M, int KIND>
 static hipError_t pass_one(const LaunchCtx &c, int parity, int dir, int pass_off, int step,
                            int reunit) {
   if constexpr (kind_valid<M, KIND>()) {
+    // multicanonical steps use the 256-site table: their order-parameter
+    // partials are indexed by 256-site chunk
+    if (step >= 0 && c.block != BLK) return hipErrorInvalidValue;
     if (c.ev_start || c.ev_stop) {
-      hipExtLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.ev_start,
-                            c.ev_stop, 0, c.desc, c.muca, c.blk, parity, dir, pass_off, step,
-                            reunit, c.cnt, c.nblk, c.partial);
+      hipExtLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream,
+                            c.ev_start, c.ev_stop, 0, c.desc, c.muca, c.blk, parity, dir, pass_off,
+                            step, reunit, c.cnt, c.nblk, c.partial);
     } else {
-      hipLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc,
+      hipLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream, c.desc,
                          c.muca, c.blk, parity, dir, pass_off, step, reunit, c.cnt, c.nblk,
                          c.partial);
     }

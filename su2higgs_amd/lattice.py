@@ -155,6 +155,22 @@ class Batch:
     def reset_timing(self):
         check(self.lib.su2hip_reset_timing(self.h))
 
+    def set_chunk(self, kind, chunk):
+        check(self.lib.su2hip_set_chunk(self.h, kind, chunk))
+
+    def chunk(self, kind):
+        c = C.c_int()
+        check(self.lib.su2hip_get_chunk(self.h, kind, C.byref(c)))
+        return c.value
+
+    def autotune(self, repeats=50):
+        """Pick the fastest workgroup chunk per pass kind; returns
+        {kind name: (ms at 64, 128, 256)} for the kinds of the sweep."""
+        t = np.full(abi.NKINDS * 3, np.nan)
+        check(self.lib.su2hip_autotune(self.h, repeats, ptr(t)))
+        return {abi.KIND_NAMES[k]: tuple(t[3 * k:3 * k + 3]) for k in range(abi.NKINDS)
+                if not np.isnan(t[3 * k])}
+
     def time_pass(self, kind, repeats=100):
         """(average kernel ms, average ms of the back-to-back span) of
         `repeats` launches of one pass kind."""

@@ -42,7 +42,7 @@ def test_fp32_links_stay_unitary_and_angles_wrapped():
 
 @pytest.mark.parametrize("model", [dict(nhiggs=1), dict(nhiggs=1, u1=1)])
 def test_fp32_observables_match_fp64(model):
-    """Plaquette and <phi^+phi> of FP32 and FP64 chains agree within
+    """Plaquette, <phi^+phi> (and the U(1) plaquette) of FP32 and FP64 chains agree within
     statistical errors (5 sigma of the combined error, autocorrelations
     estimated by blocking)."""
     res = {}
@@ -54,19 +54,20 @@ def test_fp32_observables_match_fp64(model):
         b.use_graph(True)
         b.update(100)
         V = abi.volume(p)
-        pl, ph = [], []
+        pl, ph, pu = [], [], []
         for _ in range(400):
             b.update(1)
             s = b.measure(0)
             pl.append(1 - s[abi.SUM["wilson"]] / (V * p.betasu2) / 3)
             ph.append(s[abi.SUM["phi2"]] / V)
-        res[prec] = (np.array(pl), np.array(ph))
+            pu.append(s[abi.SUM["u1wilson"]] / (V * p.betau1))
+        res[prec] = (np.array(pl), np.array(ph)) + ((np.array(pu),) if p.u1 else ())
 
     def mean_err(x, nb=20):
         blocks = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
         return blocks.mean(), blocks.std(ddof=1) / np.sqrt(nb)
 
-    for j in range(2):
+    for j in range(len(res[FP64])):
         m64, e64 = mean_err(res[FP64][j])
         m32, e32 = mean_err(res[FP32][j])
         assert abs(m64 - m32) < 5 * np.hypot(e64, e32) + 1e-6, (j, m64, e64, m32, e32)
@@ -91,3 +92,48 @@ def test_fp32_heatbath_distribution():
     cdf /= cdf[-1]
     stat = stats.kstest(a0, lambda x: np.interp(x, xs, cdf))
     assert stat.pvalue > 1e-4, stat
+
+
+MOVES = [
+    ("doublet", dict(nhiggs=1), abi.K_DOUBLET_OVERRELAX, 0, "doublet"),
+    ("doublet_u1", dict(nhiggs=1, u1=1), abi.K_DOUBLET_OVERRELAX, 1, "doublet"),
+    ("triplet", dict(nhiggs=0, triplet=1), abi.K_TRIPLET_OVERRELAX, 0, "triplet"),
+    ("link_or_u1", dict(nhiggs=1, u1=1), abi.K_LINK_OVERRELAX, 1, "su2link"),
+]
+
+
+@pytest.mark.parametrize("name,model,kind,parity,field", MOVES, ids=[m[0] for m in MOVES])
+def test_fp32_overrelaxation_is_the_fp64_move(name, model, kind, parity, field):
+    """One FP32 overrelaxation pass against the FP64 oracle's pass on the
+    same (float-rounded) fields: the deterministic move agrees to FP32
+    accuracy wherever both accept (the accept draws differ; acceptance is
+    close to 1)."""
+    import orc
+    p64 = make_params(L=(8, 8, 12), precision=FP64, msq_phi=-0.2, lambda_phi=0.02,
+                      msq_triplet=-0.3, b4=0.1, betau1=9.0, seed=5, **model)
+    o = orc.Oracle(p64)
+    o.hot_start()
+    f = {k: (v.astype(np.float32).astype(np.float64) if v is not None else None)
+         for k, v in o.get_fields().items()}
+    o.set_fields(f)
+    p32 = make_params(L=(8, 8, 12), precision=FP32, msq_phi=-0.2, lambda_phi=0.02,
+                      msq_triplet=-0.3, b4=0.1, betau1=9.0, seed=5, reunitarize=0, **model)
+    b = Batch(p32)
+    b.set_fields(0, f)
+    b.pass_(kind, parity, 1)
+    g = b.get_fields(0)[field]
+    o.pass_(kind, parity, 1)
+    h = o.get_fields()[field]
+    V = abi.volume(p64)
+    sl = slice(parity * V // 2, (parity + 1) * V // 2)
+    g, h, f0 = g[sl].reshape(V // 2, -1), h[sl].reshape(V // 2, -1), f[field][sl].reshape(V // 2, -1)
+    scale = np.abs(h).max(1) + 1.0
+    same = np.abs(g - h).max(1) <= 2e-4 * scale
+    moved = np.abs(h - f0).max(1) > 1e-3 * scale
+    assert moved.mean() > 0.9          # the oracle's move changes almost every variable
+    assert same.mean() > 0.9, same.mean()
+    # where they differ, one of the two rejected and kept the old value
+    # (the two accept draws are independent; acceptance is 0.95-1 here)
+    diff = ~same
+    kept = (np.abs(g - f0).max(1) == 0) | (np.abs(h - f0).max(1) == 0)
+    assert kept[diff].all()
