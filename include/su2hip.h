@@ -278,6 +278,12 @@ int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m,
 /* Read back the weight state (pos/W/hits may be NULL). */
 int su2hip_muca_get(su2hip_batch_t *b, int slot, su2hip_muca *m, double *pos,
                     double *W, int32_t *hits);
+/* gsum/nsum of the SLOW recursion (bins+2 entries each; either may be NULL).
+ * su2hip_muca_set starts them at 5 and 1 as load_weight_params does without
+ * a file (src/multicanonical.c:296-303). */
+int su2hip_muca_set_sums(su2hip_batch_t *b, int slot, const int64_t *gsum,
+                         const int64_t *nsum);
+int su2hip_muca_get_sums(su2hip_batch_t *b, int slot, int64_t *gsum, int64_t *nsum);
 /* W(R) of the installed weight (get_weight, src/multicanonical.c:350-379). */
 int su2hip_muca_weight_of(su2hip_batch_t *b, int slot, double value,
                           double *weight);

@@ -77,6 +77,18 @@ int su2hip_labels_write(const char *path, const su2hip_params *p);
 int su2hip_measure_format(const su2hip_params *p, const double *sums, double weight,
                           double muca_param, char *buf, int buflen);
 
+/* gsum/nsum of the SLOW weight recursion, bins+2 entries, from a
+ * `weight_params` file (load_weight_params, src/multicanonical.c:291-333):
+ * a missing file leaves every entry at gsum 5, nsum 1 and sets *found = 0;
+ * a file for another bin count is an error.  Entries 0 and bins+1 are not
+ * in the file and stay 5 and 1. */
+int su2hip_weight_params_load(const char *path, int bins, int64_t *gsum, int64_t *nsum,
+                              int *found);
+/* Write `weight_params` as save_weight does for SLOW runs
+ * (src/multicanonical.c:99-109); nothing for other modes. */
+int su2hip_weight_params_save(const char *path, const su2hip_muca *m, const double *pos,
+                              const int64_t *gsum, const int64_t *nsum);
+
 /* Acceptance summary (print_acceptance, src/checkpoint.c:17-78) to `out`. */
 int su2hip_print_acceptance(FILE *out, const su2hip_run_config *cfg,
                             const su2hip_counters *c);

@@ -400,6 +400,14 @@ void ref_muca_get(su2hip_muca *m, double *pos, double *W, int32_t *hits) {
 }
 
 void ref_muca_set_mode(int mode) { w.mode = mode; }
+void ref_muca_set_interval(int n) { w.update_interval = n; }
+/* gsum/nsum exist only in SLOW mode (src/multicanonical.c:134-138); zeros otherwise */
+void ref_muca_sums(int64_t *gsum, int64_t *nsum) {
+  for (int i = 0; i < w.bins + 2; i++) {
+    gsum[i] = w.mode == SLOW && w.gsum ? w.gsum[i] : 0;
+    nsum[i] = w.mode == SLOW && w.nsum ? w.nsum[i] : 0;
+  }
+}
 double ref_muca_weight(double v) { return get_weight(&w, v); }
 int ref_muca_whichbin(double v) { return whichbin(&w, v); }
 double ref_muca_orderparam(int par) { return calc_orderparam(&l, &f, &p, &w, (char)par); }

@@ -1212,6 +1212,20 @@ void orc_muca_get(const orc_t *o, su2hip_muca *m, double *pos, double *W,
   if (hits && o->hits) memcpy(hits, o->hits, sizeof(int32_t) * (n + 2));
 }
 
+void orc_muca_set_sums(orc_t *o, const int64_t *gsum, const int64_t *nsum) {
+  for (int i = 0; i < o->m.bins + 2 && o->gsum; i++) {
+    if (gsum) o->gsum[i] = (long)gsum[i];
+    if (nsum) o->nsum[i] = (long)nsum[i];
+  }
+}
+
+void orc_muca_get_sums(const orc_t *o, int64_t *gsum, int64_t *nsum) {
+  for (int i = 0; i < o->m.bins + 2 && o->gsum; i++) {
+    if (gsum) gsum[i] = o->gsum[i];
+    if (nsum) nsum[i] = o->nsum[i];
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* passes and the sweep (src/update.c) */
 

@@ -86,6 +86,10 @@ void orc_muca_set(orc_t *o, const su2hip_muca *m, const double *pos,
                   const double *W);
 void orc_muca_get(const orc_t *o, su2hip_muca *m, double *pos, double *W,
                   int32_t *hits);
+/* gsum/nsum of the SLOW recursion (bins+2 entries; load_weight_params,
+ * src/multicanonical.c:291-333, initialises them to 5 and 1). */
+void orc_muca_set_sums(orc_t *o, const int64_t *gsum, const int64_t *nsum);
+void orc_muca_get_sums(const orc_t *o, int64_t *gsum, int64_t *nsum);
 double orc_muca_weight(const orc_t *o, double val);
 int orc_muca_whichbin(const orc_t *o, double val);
 double orc_muca_orderparam(orc_t *o, int parity);

@@ -254,6 +254,8 @@ def load():
         "su2hip_muca_set": (C.c_int, [h, C.c_int, P(Muca), dp, dp]),
         "su2hip_muca_get": (C.c_int, [h, C.c_int, P(Muca), dp, dp, P(C.c_int32)]),
         "su2hip_muca_weight_of": (C.c_int, [h, C.c_int, C.c_double, dp]),
+        "su2hip_muca_set_sums": (C.c_int, [h, C.c_int, P(C.c_int64), P(C.c_int64)]),
+        "su2hip_muca_get_sums": (C.c_int, [h, C.c_int, P(C.c_int64), P(C.c_int64)]),
         "su2hip_flow": (C.c_int, [h, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, P(C.c_int)]),
         "su2hip_flow_force": (C.c_int, [h, C.c_int, dp]),
         "su2hip_set_timing": (C.c_int, [h, C.c_int]),
@@ -273,6 +275,10 @@ def load():
                                           dp, dp, dp, dp, dp]),
         "su2hip_weight_load": (C.c_int, [C.c_char_p, P(Muca), dp, dp, C.c_int]),
         "su2hip_weight_save": (C.c_int, [C.c_char_p, P(Muca), dp, dp]),
+        "su2hip_weight_params_load": (C.c_int, [C.c_char_p, C.c_int, P(C.c_int64), P(C.c_int64),
+                                                P(C.c_int)]),
+        "su2hip_weight_params_save": (C.c_int, [C.c_char_p, P(Muca), dp, P(C.c_int64),
+                                                P(C.c_int64)]),
         "su2hip_labels_write": (C.c_int, [C.c_char_p, P(Params)]),
         "su2hip_measure_format": (C.c_int, [P(Params), dp, C.c_double, C.c_double, C.c_char_p, C.c_int]),
         "su2hip_print_acceptance": (C.c_int, [C.c_void_p, P(RunConfig), P(Counters)]),

@@ -1288,6 +1288,34 @@ int su2hip_muca_get(su2hip_batch_t *b, int slot, su2hip_muca *m, double *pos, do
   return SU2HIP_OK;
 }
 
+int su2hip_muca_set_sums(su2hip_batch_t *b, int slot, const int64_t *gsum, const int64_t *nsum) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  const MucaDev &h = b->slots[slot].hm;
+  if (!h.enabled) return fail(SU2HIP_ESTATE, "slot %d has no multicanonical weight", slot);
+  int n = h.bins + 2;
+  static_assert(sizeof(long long) == sizeof(int64_t), "gsum/nsum element size");
+  if (gsum) HIPTRY(hipMemcpy(h.gsum, gsum, sizeof(long long) * n, hipMemcpyHostToDevice));
+  if (nsum) HIPTRY(hipMemcpy(h.nsum, nsum, sizeof(long long) * n, hipMemcpyHostToDevice));
+  return SU2HIP_OK;
+}
+
+int su2hip_muca_get_sums(su2hip_batch_t *b, int slot, int64_t *gsum, int64_t *nsum) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  HIPTRY(hipStreamSynchronize(b->stream));
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  const MucaDev &h = b->slots[slot].hm;
+  if (!h.enabled) return fail(SU2HIP_ESTATE, "slot %d has no multicanonical weight", slot);
+  int n = h.bins + 2;
+  if (gsum) HIPTRY(hipMemcpy(gsum, h.gsum, sizeof(long long) * n, hipMemcpyDeviceToHost));
+  if (nsum) HIPTRY(hipMemcpy(nsum, h.nsum, sizeof(long long) * n, hipMemcpyDeviceToHost));
+  return SU2HIP_OK;
+}
+
 int su2hip_muca_weight_of(su2hip_batch_t *b, int slot, double v, double *weight) {
   int rc = check_slot(b, slot);
   if (rc) return rc;

@@ -536,6 +536,50 @@ int su2hip_weight_load(const char *path, su2hip_muca *m, double *pos, double *W,
   return su2hip_weight_save(path, m, pos, W);
 }
 
+/* load_weight_params (src/multicanonical.c:291-333) */
+int su2hip_weight_params_load(const char *path, int bins, int64_t *gsum, int64_t *nsum, int *found) {
+  if (!path || bins <= 0 || !gsum || !nsum) return ioerr(SU2HIP_EINVAL, "weight_params_load arguments");
+  for (int i = 0; i < bins + 2; i++) {
+    gsum[i] = 5; /* cannot be 0 */
+    nsum[i] = 1;
+  }
+  FILE *fp = NULL;
+  if (access(path, R_OK) == 0) fp = fopen(path, "r");
+  if (found) *found = fp != NULL;
+  if (!fp) return SU2HIP_OK; /* "Unable to access weight parameter file. Initializing new" */
+  int bins_read = 0;
+  if (fscanf(fp, "%d", &bins_read) != 1 || bins_read != bins) {
+    fclose(fp);
+    return ioerr(SU2HIP_EIO, "Error reading parameter file!! Expected %d bins, got %d", bins, bins_read);
+  }
+  for (int i = 1; i < bins + 1; i++) {
+    double temp;
+    long g, n;
+    int r = fscanf(fp, "%lf %ld %ld", &temp, &g, &n);
+    if (r != 3) {
+      fclose(fp);
+      return ioerr(SU2HIP_EIO, "Error reading 'weight_params'!! Got %d values at line %d", r, i + 1);
+    }
+    gsum[i] = g;
+    nsum[i] = n;
+  }
+  fclose(fp);
+  return SU2HIP_OK;
+}
+
+/* the SLOW part of save_weight (src/multicanonical.c:99-109) */
+int su2hip_weight_params_save(const char *path, const su2hip_muca *m, const double *pos,
+                              const int64_t *gsum, const int64_t *nsum) {
+  if (m->mode != SU2HIP_SLOW) return SU2HIP_OK;
+  FILE *fp = fopen(path, "w");
+  if (!fp) return ioerr(SU2HIP_EIO, "cannot write %s", path);
+  fprintf(fp, "%d \n", m->bins);
+  for (long i = 1; i < m->bins + 1; i++)
+    fprintf(fp, "%lf %ld %ld\n", pos[i], (long)gsum[i], (long)nsum[i]);
+  fclose(fp);
+  return SU2HIP_OK;
+}
+
 /* ------------------------------------------------------------------------
  * labels and measure lines (src/measure.c) */
 

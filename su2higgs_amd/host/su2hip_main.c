@@ -71,11 +71,24 @@ static void checkpoint(su2hip_batch_t *b, const su2hip_run_config *cfg, hostfiel
   printf("Wrote fields to %s.\n", cfg->latticefile);
 }
 
+/* gsum/nsum of the SLOW recursion, kept on the host between muca_set calls */
+static int64_t *g_gsum, *g_nsum;
+
 static void save_weight(su2hip_batch_t *b, const su2hip_run_config *cfg, double *pos, double *W) {
   if (!cfg->multicanonical) return;
   su2hip_muca m;
   TRY("muca_get", su2hip_muca_get(b, 0, &m, pos, W, NULL));
   TRY("weight_save", su2hip_weight_save(cfg->weightfile, &m, pos, W));
+  if (m.mode == SU2HIP_SLOW) {
+    TRY("muca_get_sums", su2hip_muca_get_sums(b, 0, g_gsum, g_nsum));
+    TRY("weight_params_save", su2hip_weight_params_save("weight_params", &m, pos, g_gsum, g_nsum));
+  }
+}
+
+/* install the weight; SLOW runs keep their gsum/nsum across re-installs */
+static void muca_install(su2hip_batch_t *b, const su2hip_muca *m, const double *pos, const double *W) {
+  TRY("muca_set", su2hip_muca_set(b, 0, m, pos, W));
+  if (g_gsum) TRY("muca_set_sums", su2hip_muca_set_sums(b, 0, g_gsum, g_nsum));
 }
 
 /* weight and order parameter columns of a measure line (src/measure.c:180-188) */
@@ -215,7 +228,18 @@ int main(int argc, char **argv) {
     printf("\nLoading weightfile: %s\n", cfg.weightfile);
     TRY("weight_load", su2hip_weight_load(cfg.weightfile, &cfg.muca, pos, W, MAXBINS));
     mode = cfg.muca.mode;
-    TRY("muca_set", su2hip_muca_set(b, 0, &cfg.muca, pos, W));
+    if (mode == SU2HIP_SLOW) {
+      /* load_weight reads weight_params for SLOW runs and save_weight writes
+       * it back at once (src/multicanonical.c:272-286) */
+      g_gsum = calloc(MAXBINS + 2, sizeof(int64_t));
+      g_nsum = calloc(MAXBINS + 2, sizeof(int64_t));
+      int found = 0;
+      printf("Using slow but safe weight updating: parameter file 'weight_params'\n");
+      TRY("weight_params_load", su2hip_weight_params_load("weight_params", cfg.muca.bins, g_gsum, g_nsum, &found));
+      if (!found) printf("!!! Unable to access weight parameter file. Initializing new\n");
+      TRY("weight_params_save", su2hip_weight_params_save("weight_params", &cfg.muca, pos, g_gsum, g_nsum));
+    }
+    muca_install(b, &cfg.muca, pos, W);
     printf("Using weight function with %d bins in range %lf, %lf\n", cfg.muca.bins, cfg.muca.min, cfg.muca.max);
   }
 
@@ -225,7 +249,7 @@ int main(int argc, char **argv) {
     if (cfg.multicanonical && mode != SU2HIP_READONLY) {
       TRY("muca_get", su2hip_muca_get(b, 0, &m, pos, W, NULL));
       m.mode = SU2HIP_READONLY;
-      TRY("muca_set", su2hip_muca_set(b, 0, &m, pos, W));
+      muca_install(b, &m, pos, W);
     }
     printf("\nThermalizing %ld iterations\n", (long)cfg.n_thermalize);
     double t0 = now();
@@ -240,7 +264,7 @@ int main(int argc, char **argv) {
       double val = m.param_value[0] + m.param_value[1];
       m.last_max = fabs(val - m.wrk_min) >= fabs(val - m.wrk_max);
       m.muca_count = 0;
-      TRY("muca_set", su2hip_muca_set(b, 0, &m, pos, W));
+      muca_install(b, &m, pos, W);
     }
     iter = 1;
     printf("\nStarting new simulation!\n");

@@ -171,6 +171,19 @@ class Batch:
         return {abi.KIND_NAMES[k]: tuple(t[3 * k:3 * k + 3]) for k in range(abi.NKINDS)
                 if not np.isnan(t[3 * k])}
 
+    def muca_sums(self, slot):
+        n = self.muca_get(slot)[0].bins + 2
+        g, s = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
+        check(self.lib.su2hip_muca_get_sums(self.h, slot, g.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            s.ctypes.data_as(C.POINTER(C.c_int64))))
+        return g, s
+
+    def muca_set_sums(self, slot, g, s):
+        g = np.ascontiguousarray(g, dtype=np.int64)
+        s = np.ascontiguousarray(s, dtype=np.int64)
+        check(self.lib.su2hip_muca_set_sums(self.h, slot, g.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            s.ctypes.data_as(C.POINTER(C.c_int64))))
+
     def time_pass(self, kind, repeats=100):
         """(average kernel ms, average ms of the back-to-back span) of
         `repeats` launches of one pass kind."""

@@ -8,7 +8,7 @@ the fields and counters after update_lattice sweeps with the Philox stream,
 the gradient-flow force and an Euler step, multicanonical sweeps, and the
 bytes of a saved lattice file.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture names, default all]
 """
 import ctypes as C
 import os
@@ -133,7 +133,18 @@ def write_weight(path, bins, lo, hi, seed=3, slope=5.0):
             f.write("%f %f\n" % (lo + i * dbin, W[i]))
 
 
-def muca_fixture(tag, op, mode, cps, slope):
+def write_params(path, bins, lo, hi, seed=4):
+    """A weight_params file (src/multicanonical.c:99-109 format) with
+    seeded gsum/nsum."""
+    rng = np.random.default_rng(seed)
+    dbin = (hi - lo) / bins
+    with open(path, "w") as f:
+        f.write("%d \n" % bins)
+        for i in range(1, bins + 1):
+            f.write("%f %d %d\n" % (lo + (i - 1) * dbin, rng.integers(5, 60), rng.integers(1, 40)))
+
+
+def muca_fixture(tag, op, mode, cps, slope, interval=None, params=False, sweeps=3, rng=None):
     p = make_params(L=L, **MODELS[tag], seed=2027)
     o = orc.Oracle(p)
     o.hot_start()
@@ -141,7 +152,9 @@ def muca_fixture(tag, op, mode, cps, slope):
     r = orc.Ref(p)
     r.set_fields(f)
     lo, hi = (-1.0, 1.0) if op == abi.PHI2MINUSSIGMA2 else (0.0, 2.0)
-    out = {"L": np.array(L)}
+    if rng:
+        lo, hi = rng
+    out = {"L": np.array(L), "nsweeps": np.array([sweeps]), "range": np.array([lo, hi])}
     for k in orc.FIELDS:
         out["in_" + k] = f[k]
     with tempfile.TemporaryDirectory() as d:
@@ -150,18 +163,34 @@ def muca_fixture(tag, op, mode, cps, slope):
         try:
             write_weight("weight", 40, lo, hi, slope=slope)
             out["weight_file"] = np.frombuffer(open("weight", "rb").read(), dtype=np.uint8)
+            if params:
+                write_params("weight_params", 40, lo, hi)
+                out["params_file"] = np.frombuffer(open("weight_params", "rb").read(), dtype=np.uint8)
             m = Muca()
             m.enabled, m.orderparam, m.mode, m.checks_per_sweep = 1, op, mode, cps
             m.bins, m.min, m.max, m.delta = 40, lo, hi, 0.3
             r.lib.ref_muca_load(b"weight", C.byref(m))
+            if interval:
+                r.lib.ref_muca_set_interval(interval)
+            n = 42
+            g0, n0 = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
+            r.lib.ref_muca_sums(g0.ctypes.data_as(C.POINTER(C.c_int64)), n0.ctypes.data_as(C.POINTER(C.c_int64)))
+            out["gsum0"], out["nsum0"] = g0, n0
             mr = Muca()
             pos, W, hits = np.zeros(42), np.zeros(42), np.zeros(42, dtype=np.int32)
             r.lib.ref_muca_get(C.byref(mr), ptr(pos), ptr(W), hits.ctypes.data_as(C.POINTER(C.c_int32)))
             out["pos"], out["W0"] = pos.copy(), W.copy()
             out["muca0"] = np.frombuffer(bytes(mr), dtype=np.uint8)
             r.rng_set(99, 0)
-            r.update(3)
+            r.update(sweeps)
             r.lib.ref_muca_get(C.byref(mr), ptr(pos), ptr(W), hits.ctypes.data_as(C.POINTER(C.c_int32)))
+            g3, n3 = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
+            r.lib.ref_muca_sums(g3.ctypes.data_as(C.POINTER(C.c_int64)), n3.ctypes.data_as(C.POINTER(C.c_int64)))
+            out["gsum3"], out["nsum3"] = g3, n3
+            # the files the reference's save_weight wrote after its last update
+            out["weight3_file"] = np.frombuffer(open("weight", "rb").read(), dtype=np.uint8)
+            if mode == abi.SLOW:
+                out["params3_file"] = np.frombuffer(open("weight_params", "rb").read(), dtype=np.uint8)
         finally:
             os.chdir(cwd)
     g = r.get_fields()
@@ -180,18 +209,28 @@ MUCA_CASES = {
     "muca_h_phisq_fast3": ("h", abi.PHISQ, abi.FAST, 3, 400.0),
     "muca_t_sigmasq_ro": ("t", abi.SIGMASQ, abi.READONLY, 1, 400.0),
     "muca_ht_phi2minus_fast2": ("ht", abi.PHI2MINUSSIGMA2, abi.FAST, 2, 5.0),
+    # weight recursions that run within the 3 sweeps (update interval set
+    # after load_weight); SLOW from a fresh and from a loaded weight_params
+    "muca_h_phisq_fast_i7": ("h", abi.PHISQ, abi.FAST, 1, 5.0, 7),
+    "muca_h_phisq_slow_i30": ("h", abi.PHISQ, abi.SLOW, 2, 5.0, 30, False, 6, (1.0, 2.0)),
+    "muca_t_sigmasq_slow_params": ("t", abi.SIGMASQ, abi.SLOW, 2, 5.0, 30, True, 6, (0.5, 1.5)),
 }
 
 
 def main():
+    only = sys.argv[1:]
     for tag in MODELS:
         if not orc.ref_available(tag):
             sys.exit("reference build absent: make -C oracle ref")
-        np.savez_compressed(os.path.join(HERE, "model_%s.npz" % tag), **model_fixture(tag))
-    np.savez_compressed(os.path.join(HERE, "flow_h.npz"), **flow_fixture())
-    np.savez_compressed(os.path.join(HERE, "flow_t.npz"), **triplet_flow_fixture())
+        if not only or "model_" + tag in only:
+            np.savez_compressed(os.path.join(HERE, "model_%s.npz" % tag), **model_fixture(tag))
+    if not only or "flow_h" in only:
+        np.savez_compressed(os.path.join(HERE, "flow_h.npz"), **flow_fixture())
+    if not only or "flow_t" in only:
+        np.savez_compressed(os.path.join(HERE, "flow_t.npz"), **triplet_flow_fixture())
     for name, args in MUCA_CASES.items():
-        np.savez_compressed(os.path.join(HERE, name + ".npz"), **muca_fixture(*args))
+        if not only or name in only:
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **muca_fixture(*args))
     print("fixtures written to", HERE)
 
 

@@ -58,6 +58,8 @@ def oracle_lib():
             "orc_measure": (None, [h, dp]),
             "orc_muca_set": (None, [h, P(Muca), dp, dp]),
             "orc_muca_get": (None, [h, P(Muca), dp, dp, P(C.c_int32)]),
+            "orc_muca_set_sums": (None, [h, P(C.c_int64), P(C.c_int64)]),
+            "orc_muca_get_sums": (None, [h, P(C.c_int64), P(C.c_int64)]),
             "orc_muca_weight": (C.c_double, [h, C.c_double]),
             "orc_muca_whichbin": (C.c_int, [h, C.c_double]),
             "orc_muca_orderparam": (C.c_double, [h, C.c_int]),
@@ -176,6 +178,17 @@ class Oracle:
                               hits.ctypes.data_as(P(C.c_int32)))
         return m, pos, W, hits
 
+    def muca_sums(self):
+        n = self.muca_get()[0].bins + 2
+        g, s = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
+        self.lib.orc_muca_get_sums(self.h, g.ctypes.data_as(P(C.c_int64)), s.ctypes.data_as(P(C.c_int64)))
+        return g, s
+
+    def muca_set_sums(self, g, s):
+        g = np.ascontiguousarray(g, dtype=np.int64)
+        s = np.ascontiguousarray(s, dtype=np.int64)
+        self.lib.orc_muca_set_sums(self.h, g.ctypes.data_as(P(C.c_int64)), s.ctypes.data_as(P(C.c_int64)))
+
     def flow_force(self):
         F = np.zeros((self.V, 3, 4))
         T = np.zeros((self.V, 3))
@@ -232,6 +245,8 @@ def ref_lib(tag):
         "ref_muca_load": (C.c_int, [C.c_char_p, P(Muca)]),
         "ref_muca_get": (None, [P(Muca), dp, dp, P(C.c_int32)]),
         "ref_muca_set_mode": (None, [C.c_int]),
+        "ref_muca_set_interval": (None, [C.c_int]),
+        "ref_muca_sums": (None, [P(C.c_int64), P(C.c_int64)]),
         "ref_muca_weight": (C.c_double, [C.c_double]),
         "ref_muca_whichbin": (C.c_int, [C.c_double]),
         "ref_muca_orderparam": (C.c_double, [C.c_int]),

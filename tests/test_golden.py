@@ -132,8 +132,25 @@ def test_muca_fixture(path):
     o.muca_set(m, g["pos"], g["W0"])
     m0 = o.muca_get()[0]
     assert abs(m0.param_value[0] - m.param_value[0]) < 1e-14
+    assert m0.update_interval == m.update_interval
+    lib = abi.load()
+    n = m.bins + 2
+    P64 = C.POINTER(C.c_int64)
+    if m.mode == abi.SLOW:
+        # gsum/nsum through our weight_params reader (or its missing-file default)
+        gs, ns, found = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64), C.c_int()
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "weight_params")
+            if "params_file" in g:
+                open(path, "wb").write(g["params_file"].tobytes())
+            assert lib.su2hip_weight_params_load(path.encode(), m.bins, gs.ctypes.data_as(P64),
+                                                 ns.ctypes.data_as(P64), C.byref(found)) == 0
+        assert found.value == ("params_file" in g)
+        assert (gs[1:-1] == g["gsum0"][1:-1]).all() and (ns[1:-1] == g["nsum0"][1:-1]).all()
+        o.muca_set_sums(gs, ns)
+    nsw = int(g["nsweeps"][0]) if "nsweeps" in g else 3
     o.rng_set(99, 0)
-    o.update(3)
+    o.update(nsw)
     h = o.get_fields()
     for k in orc.FIELDS:
         assert np.abs(h[k] - g["sweep3_" + k]).max() < 1e-11, k
@@ -146,3 +163,19 @@ def test_muca_fixture(path):
     assert (hits == g["hits3"]).all()
     assert abs(m3.param_value[1] - ref3.param_value[1]) < 1e-13
     assert m3.delta == ref3.delta and m3.last_max == ref3.last_max
+    assert m3.muca_count == ref3.muca_count
+    if m.mode == abi.SLOW:
+        g3, n3 = o.muca_sums()
+        assert (g3[1:-1] == g["gsum3"][1:-1]).all() and (n3[1:-1] == g["nsum3"][1:-1]).all()
+    if "weight3_file" in g and m.mode != abi.READONLY:
+        # the files save_weight wrote after the last update, from our writers
+        with tempfile.TemporaryDirectory() as d:
+            wp = os.path.join(d, "weight")
+            assert lib.su2hip_weight_save(wp.encode(), C.byref(m3), ptr(pos), ptr(W)) == 0
+            assert open(wp, "rb").read() == g["weight3_file"].tobytes()
+            if m.mode == abi.SLOW:
+                pp = os.path.join(d, "weight_params")
+                g3c, n3c = np.ascontiguousarray(g3), np.ascontiguousarray(n3)
+                assert lib.su2hip_weight_params_save(pp.encode(), C.byref(m3), ptr(pos),
+                                                     g3c.ctypes.data_as(P64), n3c.ctypes.data_as(P64)) == 0
+                assert open(pp, "rb").read() == g["params3_file"].tobytes()

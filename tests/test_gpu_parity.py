@@ -219,7 +219,7 @@ def muca_weight(bins=40, lo=0.0, hi=2.0, seed=3, slope=5.0):
     return pos, W
 
 
-@pytest.mark.parametrize("mode", [abi.READONLY, abi.FAST])
+@pytest.mark.parametrize("mode", [abi.READONLY, abi.FAST, abi.SLOW])
 @pytest.mark.parametrize("tag,op,cps,slope", [("h", abi.PHISQ, 1, 5.0), ("t", abi.SIGMASQ, 1, 5.0),
                                                ("h", abi.PHISQ, 3, 5.0),
                                                ("ht", abi.PHI2MINUSSIGMA2, 2, 5.0),
@@ -259,3 +259,73 @@ def test_muca_sweeps(tag, op, cps, slope, mode):
     assert np.abs(Wo - Wg).max() < 1e-12
     assert (ho == hg).all()
     assert abs(mo.param_value[1] - mg.param_value[1]) < 1e-13
+    if mode == abi.SLOW:
+        go, no = o.muca_sums()
+        gg, ng = b.muca_sums(0)
+        assert (go[1:-1] == gg[1:-1]).all() and (no[1:-1] == ng[1:-1]).all()
+
+
+def test_muca_slow_recursion_from_loaded_sums():
+    """SLOW weight updates (src/multicanonical.c:561-666) that change W, from
+    gsum/nsum loaded as a weight_params file would give them."""
+    p, o, b = pair("t", L=(4, 4, 6), seed=2027)
+    o.rng_set(99, 0)
+    b.rng_set(0, 99, 0)
+    m = abi.Muca()
+    m.enabled, m.orderparam, m.mode, m.checks_per_sweep = 1, abi.SIGMASQ, abi.SLOW, 2
+    m.bins, m.delta, m.do_acceptance, m.update_interval = 40, 0.3, 1, 30
+    m.min, m.max, m.wrk_min, m.wrk_max = 0.5, 1.5, 0.5, 1.5
+    pos, W = muca_weight(lo=0.5, hi=1.5, slope=5.0)
+    rng = np.random.default_rng(4)
+    gs = rng.integers(5, 60, 42).astype(np.int64)
+    ns = rng.integers(1, 40, 42).astype(np.int64)
+    o.muca_set(m, pos, W)
+    o.muca_set_sums(gs, ns)
+    b.muca_set(0, m, pos, W)
+    b.muca_set_sums(0, gs, ns)
+    o.update(6)
+    b.update(6)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-10
+    mo, _, Wo, _ = o.muca_get()
+    mg, _, Wg, _ = b.muca_get(0)
+    assert np.abs(Wo - Wg).max() < 1e-12 and np.abs(Wg - W).max() > 0  # W was updated
+    go, no = o.muca_sums()
+    gg, ng = b.muca_sums(0)
+    assert (go[1:-1] == gg[1:-1]).all() and (no[1:-1] == ng[1:-1]).all()
+    assert (gg != gs).any()
+
+
+def test_plain_16cubed_trajectory():
+    """The plain 16^3 case: SU(2)+Higgs, betaG 8, msq -0.25, lambda 0.02, hot
+    start from seed 1234, each sweep 1 link heatbath + 4 link overrelaxation
+    + 4 Higgs overrelaxation + 1 Higgs Metropolis, plaquette and <phi^+phi>
+    measured every sweep: the GPU's FP64 series follows the oracle's with
+    the same accept/reject decisions (exact counters).  The two differ by
+    rounding (fused multiply-adds, libm ulps), which the overrelaxation
+    dynamics amplify about 4x per sweep, so the series is compared over the
+    first 5 sweeps (1e-10 relative); the measure of the same fields agrees
+    to 1e-12."""
+    p = make_params(L=(16, 16, 16), nhiggs=1, betasu2=8.0, msq_phi=-0.25, lambda_phi=0.02,
+                    link_overrelax=4, scalar_sweeps=4, seed=1234)
+    o = orc.Oracle(p)
+    o.hot_start()
+    b = Batch(p)
+    b.hot_start(0)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-14
+    b.set_fields(0, o.get_fields())  # the same bits (the hot starts differ in libm ulps)
+    b.use_graph(True)
+    V = abi.volume(p)
+    for it in range(5):
+        so, sg = o.measure(), b.measure(0)
+        for name in ("wilson", "phi2"):
+            k = abi.SUM[name]
+            assert abs(so[k] - sg[k]) <= 1e-10 * abs(so[k]), (it, name)
+        o.update(1)
+        b.update(1)
+    assert o.counters() == b.counters(0)
+    c = Batch(p)
+    c.set_fields(0, o.get_fields())
+    so, sc = o.measure(), c.measure(0)
+    assert np.abs(so - sc).max() <= 1e-12 * np.abs(so).max()
+    plaq = 1 - sg[abi.SUM["wilson"]] / (V * p.betasu2) / 3
+    assert 0.5 < plaq < 1.0
