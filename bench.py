@@ -96,6 +96,9 @@ def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100):
     npass, upd = passes_and_updates(p)
     b = Batch(p)
     b.cold_start(0, phi0=1.0)
+    # workgroup chunk per pass kind picked by the autotuner (state restored)
+    tune = b.autotune(repeats=50)
+    picks = {name: b.chunk(abi.KIND_NAMES.index(name)) for name in tune}
     b.use_graph(True)
     b.update(warmup)
     b.sync()
@@ -104,6 +107,7 @@ def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100):
     b.sync()
     wall = time.perf_counter() - t0
     res = {"precision": precision, "V": V, "passes_per_sweep": npass, "updates_per_sweep_over_V": upd,
+           "autotune_ms_64_128_256": tune, "chunk": picks,
            "wall_s": wall, "ms_per_sweep": 1e3 * wall / steps,
            "site_updates_per_s": upd * V * steps / wall}
     b.use_graph(False)
@@ -220,6 +224,7 @@ def batched_scan(steps):
     b = Batch(ps)
     for i in range(len(ps)):
         b.hot_start(i)
+    b.autotune(repeats=20)
     b.use_graph(True)
     b.update(3)
     b.sync()
@@ -406,6 +411,9 @@ def main():
         "roofline_fp64": roof64,
         "kernel_avg_ms_fp32": r32["kernel_avg_ms"],
         "kernel_avg_ms_fp64": r64["kernel_avg_ms"],
+        "chunk_fp32": r32["chunk"],
+        "chunk_fp64": r64["chunk"],
+        "autotune_fp32": r32["autotune_ms_64_128_256"],
         "kernel_span_ms_fp32": r32["kernel_span_ms"],
         "kernel_span_ms_fp64": r64["kernel_span_ms"],
         "kernel_ms_fp32": r32["kernel_ms"],

@@ -297,6 +297,17 @@ int su2hip_muca_weight_of(su2hip_batch_t *b, int slot, double value,
 int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt,
                 double t_max, int meas_interval, double *rows, int max_rows,
                 int *nrows);
+/* The same flow, asynchronous: enqueued on the handle's second stream after
+ * the work already queued on its main stream, so the other lattices keep
+ * sweeping meanwhile.  The lattice must have sweep budget 0 (it must not
+ * change while its copy flows); its budget, fields and starts are refused
+ * (SU2HIP_ESTATE) until su2hip_flow_finish.  One flow per slot at a time. */
+int su2hip_flow_start(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max,
+                      int meas_interval);
+/* *done = 1 once the flow of `slot` has finished on the device. */
+int su2hip_flow_poll(su2hip_batch_t *b, int slot, int *done);
+/* Wait for the flow of `slot`; rows as su2hip_flow returns them. */
+int su2hip_flow_finish(su2hip_batch_t *b, int slot, double *rows, int max_rows, int *nrows);
 /* Gradient force on every link, double[V][3][4] with component 0 = 0
  * (calc_gradient's link part, src/gradflow.c:335-354). */
 int su2hip_flow_force(su2hip_batch_t *b, int slot, double *force);

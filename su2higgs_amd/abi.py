@@ -258,6 +258,9 @@ def load():
         "su2hip_muca_get_sums": (C.c_int, [h, C.c_int, P(C.c_int64), P(C.c_int64)]),
         "su2hip_flow": (C.c_int, [h, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, P(C.c_int)]),
         "su2hip_flow_force": (C.c_int, [h, C.c_int, dp]),
+        "su2hip_flow_start": (C.c_int, [h, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]),
+        "su2hip_flow_poll": (C.c_int, [h, C.c_int, P(C.c_int)]),
+        "su2hip_flow_finish": (C.c_int, [h, C.c_int, dp, C.c_int, P(C.c_int)]),
         "su2hip_set_timing": (C.c_int, [h, C.c_int]),
         "su2hip_kernel_times": (C.c_int, [h, dp, P(C.c_int64)]),
         "su2hip_reset_timing": (C.c_int, [h]),

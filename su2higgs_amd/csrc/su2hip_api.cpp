@@ -37,6 +37,17 @@ static int fail(int code, const char *fmt, ...) {
 
 namespace {
 
+// A gradient flow in flight: device descriptors, private measurement
+// buffers and the rows it measures, on the stream it was enqueued on.
+struct FlowJob {
+  bool active = false;
+  LatDesc *ddesc = nullptr;
+  BlockMap *dblk = nullptr;
+  double *d_partial = nullptr, *d_sums = nullptr, *d_rows = nullptr;
+  std::vector<double> t;  // flow time of each row
+  hipEvent_t done = nullptr;
+};
+
 struct Slot {
   bool used = false;
   su2hip_params p{};
@@ -51,6 +62,7 @@ struct Slot {
   MucaDev hm{};
   void *flow_mem = nullptr;
   size_t flow_bytes = 0;
+  FlowJob fj;
 };
 
 struct Step {
@@ -109,6 +121,7 @@ struct su2hip_batch {
   double kms[SU2HIP_NKINDS] = {};
   long long kcount[SU2HIP_NKINDS] = {};
   AltMap alt[2];                  // chunks 64 and 128; 256 is the main table
+  hipStream_t flow_stream = nullptr;  // asynchronous flows (su2hip_flow_start)
   int tune[SU2HIP_NKINDS] = {};   // chunk per pass kind, 0 = chosen by size
 };
 
@@ -660,8 +673,22 @@ static int alloc_slot(su2hip_batch *b, int i, const su2hip_params &p) {
   return SU2HIP_OK;
 }
 
+static void flow_release(FlowJob &j) {
+  if (j.ddesc) (void)hipFree(j.ddesc);
+  if (j.dblk) (void)hipFree(j.dblk);
+  if (j.d_partial) (void)hipFree(j.d_partial);
+  if (j.d_sums) (void)hipFree(j.d_sums);
+  if (j.d_rows) (void)hipFree(j.d_rows);
+  if (j.done) (void)hipEventDestroy(j.done);
+  j = FlowJob();
+}
+
 static void free_slot(su2hip_batch *b, int i) {
   Slot &s = b->slots[i];
+  if (s.fj.active) {
+    (void)hipEventSynchronize(s.fj.done);
+    flow_release(s.fj);
+  }
   if (s.mem) (void)hipFree(s.mem);
   if (s.bu_mem) (void)hipFree(s.bu_mem);
   if (s.muca_mem) (void)hipFree(s.muca_mem);
@@ -812,6 +839,7 @@ int su2hip_destroy(su2hip_batch_t *b) {
     if (m.d_cnt) (void)hipFree(m.d_cnt);
   }
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->flow_stream) (void)hipStreamDestroy(b->flow_stream);
   delete b;
   return SU2HIP_OK;
 }
@@ -873,6 +901,8 @@ int su2hip_set_fields(su2hip_batch_t *b, int slot, const double *su2link, const 
                       const double *doublet, const double *triplet, const double *singlet) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
+  if (b->slots[slot].fj.active)
+    return fail(SU2HIP_ESTATE, "slot %d has a flow in flight", slot);
   rc = prepare(b);
   if (rc) return rc;
   const su2hip_params &p = b->slots[slot].p;
@@ -922,6 +952,8 @@ int su2hip_get_fields(su2hip_batch_t *b, int slot, double *su2link, double *u1li
 int su2hip_cold_start(su2hip_batch_t *b, int slot, double phi0, double sigma0, double singlet0) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
+  if (b->slots[slot].fj.active)
+    return fail(SU2HIP_ESTATE, "slot %d has a flow in flight", slot);
   rc = prepare(b);
   if (rc) return rc;
   HIPTRY(launch_cold(ctx_of(b), slot, phi0, sigma0, singlet0));
@@ -932,6 +964,8 @@ int su2hip_cold_start(su2hip_batch_t *b, int slot, double phi0, double sigma0, d
 int su2hip_hot_start(su2hip_batch_t *b, int slot) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
+  if (b->slots[slot].fj.active)
+    return fail(SU2HIP_ESTATE, "slot %d has a flow in flight", slot);
   rc = prepare(b);
   if (rc) return rc;
   HIPTRY(launch_hot(ctx_of(b), slot));
@@ -980,6 +1014,8 @@ int su2hip_set_budget(su2hip_batch_t *b, int slot, int64_t sweeps) {
   if (rc) return rc;
   rc = pull_desc(b);
   if (rc) return rc;
+  if (b->slots[slot].fj.active && sweeps != 0)
+    return fail(SU2HIP_ESTATE, "slot %d has a flow in flight; finish it before sweeping", slot);
   b->slots[slot].hd.budget = sweeps < 0 ? -1 : sweeps;
   return push_desc(b);
 }
@@ -1408,17 +1444,12 @@ int su2hip_flow_force(su2hip_batch_t *b, int slot, double *force) {
   return SU2HIP_OK;
 }
 
-int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max,
-                int meas_interval, double *rows, int max_rows, int *nrows) {
-  int rc = check_slot(b, slot);
-  if (rc) return rc;
-  if (scheme != SU2HIP_FLOW_EULER && scheme != SU2HIP_FLOW_RK3)
-    return fail(SU2HIP_EINVAL, "flow scheme %d", scheme);
-  if (!(dt > 0) || meas_interval < 1) return fail(SU2HIP_EINVAL, "dt/meas_interval");
-  rc = prepare(b);
-  if (rc) return rc;
-  rc = pull_desc(b);
-  if (rc) return rc;
+// Enqueue a whole flow of `slot` on stream st: copy the links (+ triplet)
+// into the flow buffers, run the integrator stages, and at every
+// measurement point reduce the flowed fields into job-private buffers and
+// copy the sums into the job's row array.  Nothing waits on the host.
+static int flow_enqueue(su2hip_batch *b, int slot, int scheme, double dt, double t_max,
+                        int meas_interval, hipStream_t st, bool use_timing, FlowJob &j) {
   Slot &s = b->slots[slot];
   const su2hip_params &p = s.p;
   size_t Vh = (size_t)vol_of(p) / 2, w = wsize(p);
@@ -1444,10 +1475,27 @@ int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max
     Gb[1] = take(6 * Vh);
     ZG = take(6 * Vh);
   }
-  HIPTRY(hipMemcpyAsync(Ub[0], s.hd.U, 24 * Vh * w, hipMemcpyDeviceToDevice, b->stream));
-  if (p.triplet) HIPTRY(hipMemcpyAsync(Gb[0], s.hd.G, 6 * Vh * w, hipMemcpyDeviceToDevice, b->stream));
   double msq_t = p.triplet ? triplet_flow_msq(p) : p.msq_triplet;
-  // two descriptor arrays (links in buffer 0 / 1), only `slot` active
+  // the time of every row: grad_flow's loop (src/gradflow.c:91-141)
+  j.t.clear();
+  j.t.push_back(0.0);
+  std::vector<std::pair<double, int>> steps;  // (dt, measure after)
+  {
+    double t = 0.0, h = dt;
+    int iter = 1;
+    while (t < t_max) {
+      if (t + h > t_max) h = t_max - t;
+      t += h;
+      bool m = iter % meas_interval == 0;
+      steps.push_back({h, m ? 1 : 0});
+      if (m) j.t.push_back(t);
+      iter++;
+    }
+    if ((iter - 1) % meas_interval != 0) {
+      if (!steps.empty()) steps.back().second = 1;
+      j.t.push_back(t);
+    }
+  }
   int nb = (int)((Vh + BLK - 1) / BLK);
   std::vector<LatDesc> hd(2 * (size_t)b->max_slots);
   for (int k = 0; k < 2; k++)
@@ -1461,70 +1509,159 @@ int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max
         L.blk_base = 0;
         L.nblk = nb;
         L.active = 1;
+        L.budget = -1;
       } else {
         memset(&L, 0, sizeof(L));
       }
     }
   std::vector<BlockMap> bm(nb);
   for (int k = 0; k < nb; k++) bm[k] = BlockMap{slot, k * BLK};
-  LatDesc *ddesc = nullptr;
-  BlockMap *dblk = nullptr;
-  if (hipMalloc(&ddesc, sizeof(LatDesc) * hd.size()) != hipSuccess ||
-      hipMalloc(&dblk, sizeof(BlockMap) * nb) != hipSuccess) {
-    if (ddesc) (void)hipFree(ddesc);
-    return fail(SU2HIP_ENOMEM, "flow descriptors");
+  size_t nrows = j.t.size();
+  if (hipMalloc(&j.ddesc, sizeof(LatDesc) * hd.size()) != hipSuccess ||
+      hipMalloc(&j.dblk, sizeof(BlockMap) * nb) != hipSuccess ||
+      hipMalloc(&j.d_partial, sizeof(double) * SU2HIP_NSUMS * (size_t)nb) != hipSuccess ||
+      hipMalloc(&j.d_sums, sizeof(double) * SU2HIP_NSUMS * (size_t)b->max_slots) != hipSuccess ||
+      hipMalloc(&j.d_rows, sizeof(double) * SU2HIP_NSUMS * nrows) != hipSuccess ||
+      hipEventCreateWithFlags(&j.done, hipEventDisableTiming) != hipSuccess) {
+    flow_release(j);
+    return fail(SU2HIP_ENOMEM, "flow descriptors and measurement buffers");
   }
-  hipError_t e = hipMemcpyAsync(ddesc, hd.data(), sizeof(LatDesc) * hd.size(), hipMemcpyHostToDevice, b->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(dblk, bm.data(), sizeof(BlockMap) * nb, hipMemcpyHostToDevice, b->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-  int cur = 0, nr = 0;
-  double t = 0.0;
-  std::vector<double> sums((size_t)SU2HIP_NSUMS * b->max_slots);
+  j.active = true;
+  hipError_t e = hipMemcpyAsync(j.ddesc, hd.data(), sizeof(LatDesc) * hd.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(j.dblk, bm.data(), sizeof(BlockMap) * nb, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(Ub[0], s.hd.U, 24 * Vh * w, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess && p.triplet) e = hipMemcpyAsync(Gb[0], s.hd.G, 6 * Vh * w, hipMemcpyDeviceToDevice, st);
+  int cur = 0;
+  size_t r = 0;
+  auto ctx = [&]() {
+    LaunchCtx c = ctx_of(b);
+    c.stream = st;
+    c.desc = j.ddesc + (size_t)cur * b->max_slots;
+    c.blk = j.dblk;
+    c.nblk = nb;
+    c.partial = j.d_partial;
+    c.sums = j.d_sums;
+    return c;
+  };
   auto row = [&]() -> hipError_t {
-    LaunchCtx c;
-    single_ctx(b, slot, ddesc + (size_t)cur * b->max_slots, dblk, nb, c);
-    hipError_t r = timed(b, SU2HIP_K_MEASURE, [&] { return launch_measure(c); });
-    if (r != hipSuccess) return r;
-    r = hipMemcpyAsync(sums.data(), b->d_sums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost,
-                       b->stream);
-    if (r != hipSuccess) return r;
-    r = hipStreamSynchronize(b->stream);
-    if (r != hipSuccess) return r;
-    if (nr < max_rows && rows) {
-      double *o = rows + (size_t)nr * (1 + SU2HIP_NSUMS);
-      o[0] = t;
-      memcpy(o + 1, sums.data() + (size_t)slot * SU2HIP_NSUMS, sizeof(double) * SU2HIP_NSUMS);
-    }
-    nr++;
-    return hipSuccess;
+    LaunchCtx c = ctx();
+    hipError_t x = use_timing ? timed(b, SU2HIP_K_MEASURE, [&] { return launch_measure(c); })
+                              : launch_measure(c);
+    if (x == hipSuccess)
+      x = hipMemcpyAsync(j.d_rows + r * SU2HIP_NSUMS, j.d_sums + (size_t)slot * SU2HIP_NSUMS,
+                         sizeof(double) * SU2HIP_NSUMS, hipMemcpyDeviceToDevice, st);
+    r++;
+    return x;
   };
   if (e == hipSuccess) e = row();
-  int iter = 1;
-  while (e == hipSuccess && t < t_max) {
-    // grad_flow's time loop (src/gradflow.c:91-141)
-    if (t + dt > t_max) dt = t_max - t;
-    int stages = scheme == SU2HIP_FLOW_EULER ? 1 : 3;
-    for (int st = 0; st < stages && e == hipSuccess; st++) {
-      LaunchCtx c;
-      single_ctx(b, slot, ddesc + (size_t)cur * b->max_slots, dblk, nb, c);
-      int stage = scheme == SU2HIP_FLOW_EULER ? 0 : st + 1;
-      e = timed(b, SU2HIP_K_FLOW, [&] {
-        return launch_flow_stage(c, slot, Ub[cur], Ub[1 - cur], Gb[cur], Gb[1 - cur], Z, ZG, dt,
+  int stages = scheme == SU2HIP_FLOW_EULER ? 1 : 3;
+  for (size_t k = 0; k < steps.size() && e == hipSuccess; k++) {
+    double h = steps[k].first;
+    for (int sg = 0; sg < stages && e == hipSuccess; sg++) {
+      LaunchCtx c = ctx();
+      int stage = scheme == SU2HIP_FLOW_EULER ? 0 : sg + 1;
+      auto go = [&] {
+        return launch_flow_stage(c, slot, Ub[cur], Ub[1 - cur], Gb[cur], Gb[1 - cur], Z, ZG, h,
                                  msq_t, stage);
-      });
+      };
+      e = use_timing ? timed(b, SU2HIP_K_FLOW, go) : go();
       cur = 1 - cur;
     }
-    t += dt;
-    if (e == hipSuccess && iter % meas_interval == 0) e = row();
-    iter++;
+    if (e == hipSuccess && steps[k].second) e = row();
   }
-  if (e == hipSuccess && (iter - 1) % meas_interval != 0) e = row();
-  (void)hipStreamSynchronize(b->stream);
-  (void)hipFree(ddesc);
-  (void)hipFree(dblk);
-  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "flow: %s", hipGetErrorString(e));
-  if (nrows) *nrows = nr;
+  if (e == hipSuccess) e = hipEventRecord(j.done, st);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(st);
+    flow_release(j);
+    return fail(SU2HIP_EDEVICE, "flow: %s", hipGetErrorString(e));
+  }
   return SU2HIP_OK;
+}
+
+// rows of a finished flow job, then release it
+static int flow_collect(FlowJob &j, double *rows, int max_rows, int *nrows) {
+  hipError_t e = hipEventSynchronize(j.done);
+  size_t n = j.t.size();
+  std::vector<double> h(n * SU2HIP_NSUMS);
+  if (e == hipSuccess)
+    e = hipMemcpy(h.data(), j.d_rows, sizeof(double) * h.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && rows)
+    for (size_t r = 0; r < n && (int)r < max_rows; r++) {
+      double *o = rows + r * (1 + SU2HIP_NSUMS);
+      o[0] = j.t[r];
+      memcpy(o + 1, h.data() + r * SU2HIP_NSUMS, sizeof(double) * SU2HIP_NSUMS);
+    }
+  flow_release(j);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "flow: %s", hipGetErrorString(e));
+  if (nrows) *nrows = (int)n;
+  return SU2HIP_OK;
+}
+
+static int flow_args(su2hip_batch *b, int slot, int scheme, double dt, int meas_interval) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (scheme != SU2HIP_FLOW_EULER && scheme != SU2HIP_FLOW_RK3)
+    return fail(SU2HIP_EINVAL, "flow scheme %d", scheme);
+  if (!(dt > 0) || meas_interval < 1) return fail(SU2HIP_EINVAL, "dt/meas_interval");
+  if (b->slots[slot].fj.active) return fail(SU2HIP_ESTATE, "slot %d already has a flow in flight", slot);
+  return SU2HIP_OK;
+}
+
+int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max,
+                int meas_interval, double *rows, int max_rows, int *nrows) {
+  int rc = flow_args(b, slot, scheme, dt, meas_interval);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  FlowJob &j = b->slots[slot].fj;
+  rc = flow_enqueue(b, slot, scheme, dt, t_max, meas_interval, b->stream, true, j);
+  if (rc) return rc;
+  return flow_collect(j, rows, max_rows, nrows);
+}
+
+int su2hip_flow_start(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max,
+                      int meas_interval) {
+  int rc = flow_args(b, slot, scheme, dt, meas_interval);
+  if (rc) return rc;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  if (b->slots[slot].hd.budget != 0)
+    return fail(SU2HIP_ESTATE, "slot %d must have sweep budget 0 while its flow runs", slot);
+  if (!b->flow_stream && hipStreamCreateWithFlags(&b->flow_stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(SU2HIP_EDEVICE, "flow stream");
+  // the flow starts after the work already queued on the handle's stream
+  hipEvent_t ev;
+  HIPTRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ev, b->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->flow_stream, ev, 0);
+  (void)hipEventDestroy(ev);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "flow start: %s", hipGetErrorString(e));
+  return flow_enqueue(b, slot, scheme, dt, t_max, meas_interval, b->flow_stream, false,
+                      b->slots[slot].fj);
+}
+
+int su2hip_flow_poll(su2hip_batch_t *b, int slot, int *done) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!done) return fail(SU2HIP_EINVAL, "NULL done");
+  const FlowJob &j = b->slots[slot].fj;
+  if (!j.active) return fail(SU2HIP_ESTATE, "slot %d has no flow in flight", slot);
+  hipError_t e = hipEventQuery(j.done);
+  if (e != hipSuccess && e != hipErrorNotReady) return fail(SU2HIP_EDEVICE, "flow: %s", hipGetErrorString(e));
+  *done = e == hipSuccess;
+  return SU2HIP_OK;
+}
+
+int su2hip_flow_finish(su2hip_batch_t *b, int slot, double *rows, int max_rows, int *nrows) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  FlowJob &j = b->slots[slot].fj;
+  if (!j.active) return fail(SU2HIP_ESTATE, "slot %d has no flow in flight", slot);
+  return flow_collect(j, rows, max_rows, nrows);
 }
 
 int su2hip_set_timing(su2hip_batch_t *b, int enable) {

@@ -102,9 +102,28 @@ __device__ __forceinline__ Q<T> link_staple(const LatDesc &L, const T *U, const 
 
 // staple_doublet (src/staples.c:243-302)
 template <typename T, int M>
+__device__ __forceinline__ void doublet_hops(const LatDesc &L, const T *U, const T *A, const T *F,
+                                             const Site &x, int d, Q<T> &s) {
+  Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);
+  Q<T> t = qmul(U_at(L, U, x, d), F_at(L, F, xp));
+  if constexpr ((M & M_U1) != 0) t = qmul(t, u1twist<T>(A_at(L, A, x, d)));
+  s = qadd(s, qscale(t, (T)-1));
+  Q<T> t2 = qmul(qconj(U_at(L, U, xm, d)), F_at(L, F, xm));
+  if constexpr ((M & M_U1) != 0) t2 = qmul(t2, qconj(u1twist<T>(A_at(L, A, xm, d))));
+  s = qadd(s, qscale(t2, (T)-1));
+}
+
+template <typename T, int M>
 __device__ __forceinline__ Q<T> doublet_staple(const LatDesc &L, const T *U, const T *A, const T *F,
                                                const Site &x) {
   Q<T> s{(T)0, (T)0, (T)0, (T)0};
+  if constexpr ((M & M_U1) != 0 && sizeof(T) == 8) {
+    // one direction at a time: six inlined FP64 sincos side by side would
+    // hold 170 VGPRs (2 waves per SIMD)
+#pragma unroll 1
+    for (int d = 0; d < 3; d++) doublet_hops<T, M>(L, U, A, F, x, d, s);
+    return s;
+  }
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);

@@ -152,6 +152,7 @@ typedef struct {
   int with_flow;   /* footprint includes the flow scratch */
   int64_t lru;     /* tick of the last use */
   int32_t L;
+  int flowing;     /* its request's gradient flow is in flight */
   double vol;
 } chain_slot;
 
@@ -163,7 +164,7 @@ typedef struct {
   su2hip_batch_t *b;
   chain_slot *cs;
   uint64_t reserved;
-  int running, resident;
+  int running, resident, flowing;
   double t0;
   int64_t tick;
   double upd;
@@ -304,17 +305,34 @@ static int admit(server *S, int r) {
   return 1;
 }
 
-/* measure, flow and retire the chains whose budget is spent */
+/* a request is done: record it and retire (or park) its chain */
+static void complete(server *S, int slot) {
+  chain_slot *c = &S->cs[slot];
+  int r = c->req;
+  const su2hip_request *q = &S->req[r];
+  su2hip_result *o = &S->res[r];
+  o->status = 0;
+  o->t_done = elapsed(S);
+  S->st.completed++;
+  S->st.site_updates += (double)q->sweeps * c->vol * S->upd;
+  c->req = -1;
+  c->lru = S->tick;
+  if (!S->cfg->keep_chains) retire_slot(S, slot);
+}
+
+/* measure the chains whose budget is spent; start the gradient flow of those
+ * that asked for one (on the second stream, while the others keep
+ * sweeping) and complete the rest */
 static int finish(server *S, const int64_t *budgets, double *sums) {
   const su2hip_serve_config *cfg = S->cfg;
   int any = 0, rc;
   for (int i = 0; i < cfg->max_slots; i++)
-    if (S->cs[i].used && S->cs[i].req >= 0 && budgets[i] == 0) any = 1;
+    if (S->cs[i].used && S->cs[i].req >= 0 && !S->cs[i].flowing && budgets[i] == 0) any = 1;
   if (!any) return SU2HIP_OK;
   if ((rc = su2hip_measure_all(S->b, sums))) return rc;
   for (int i = 0; i < cfg->max_slots; i++) {
     chain_slot *c = &S->cs[i];
-    if (!c->used || c->req < 0 || budgets[i] != 0) continue;
+    if (!c->used || c->req < 0 || c->flowing || budgets[i] != 0) continue;
     int r = c->req;
     const su2hip_request *q = &S->req[r];
     su2hip_result *o = &S->res[r];
@@ -322,26 +340,41 @@ static int finish(server *S, const int64_t *budgets, double *sums) {
     uint64_t seed = 0;
     if ((rc = su2hip_rng_get(S->b, i, &seed, &o->pass))) return rc;
     o->sweeps_done = q->sweeps;
+    S->running--;
     if (q->flow) {
       /* rows at t = 0 and t = t_max only (one measurement interval longer
        * than the flow) */
-      double rows[2 * (1 + SU2HIP_NSUMS)];
-      int nr = 0;
-      rc = su2hip_flow(S->b, i, cfg->flow_scheme, cfg->flow_dt, cfg->flow_t_max, 1 << 30, rows, 2, &nr);
-      if (rc) return rc;
-      const double *last = rows + (size_t)(nr > 1 ? 1 : 0) * (1 + SU2HIP_NSUMS);
-      o->flow_t = last[0];
-      memcpy(o->flow_sums, last + 1, sizeof(o->flow_sums));
-      S->st.flows++;
+      if ((rc = su2hip_flow_start(S->b, i, cfg->flow_scheme, cfg->flow_dt, cfg->flow_t_max, 1 << 30)))
+        return rc;
+      c->flowing = 1;
+      S->flowing++;
+      continue;
     }
-    o->status = 0;
-    o->t_done = elapsed(S);
-    S->st.completed++;
-    S->st.site_updates += (double)q->sweeps * c->vol * S->upd;
-    c->req = -1;
-    c->lru = S->tick;
-    S->running--;
-    if (!cfg->keep_chains) retire_slot(S, i);
+    complete(S, i);
+  }
+  return SU2HIP_OK;
+}
+
+/* complete the requests whose flow has finished (wait: block on them) */
+static int collect_flows(server *S, int wait) {
+  int rc;
+  for (int i = 0; i < S->cfg->max_slots && S->flowing; i++) {
+    chain_slot *c = &S->cs[i];
+    if (!c->used || !c->flowing) continue;
+    int done = 1;
+    if (!wait && (rc = su2hip_flow_poll(S->b, i, &done))) return rc;
+    if (!done) continue;
+    double rows[2 * (1 + SU2HIP_NSUMS)];
+    int nr = 0;
+    if ((rc = su2hip_flow_finish(S->b, i, rows, 2, &nr))) return rc;
+    su2hip_result *o = &S->res[c->req];
+    const double *last = rows + (size_t)(nr > 1 ? 1 : 0) * (1 + SU2HIP_NSUMS);
+    o->flow_t = last[0];
+    memcpy(o->flow_sums, last + 1, sizeof(o->flow_sums));
+    S->st.flows++;
+    c->flowing = 0;
+    S->flowing--;
+    complete(S, i);
   }
   return SU2HIP_OK;
 }
@@ -396,6 +429,7 @@ int su2hip_serve(const su2hip_serve_config *cfg, const su2hip_request *req, int 
   int qh = 0, qt = 0, next = 0;
   S.t0 = now_s();
   while (!rc) {
+    if (S.flowing && (rc = collect_flows(&S, 0))) break;
     double t = elapsed(&S);
     while (next < n && (!cfg->realtime || req[next].arrival <= t)) {
       res[next].t_arrival = cfg->realtime ? req[next].arrival : 0.0;
@@ -411,6 +445,15 @@ int su2hip_serve(const su2hip_serve_config *cfg, const su2hip_request *req, int 
       qh++;
     }
     if (rc) break;
+    if (S.running == 0 && S.flowing) {
+      /* only flows in flight: wait for the next one, or for an arrival */
+      if (next >= n && qh == qt) {
+        if ((rc = collect_flows(&S, 1))) break;
+      } else {
+        sleep_s(0.0005);
+      }
+      continue;
+    }
     if (S.running == 0) {
       if (qh < qt) {
         /* nothing runs, nothing more can be evicted, and the head still

@@ -137,6 +137,20 @@ class Batch:
         check(self.lib.su2hip_flow(self.h, slot, scheme, dt, tmax, meas, ptr(rows), max_rows, C.byref(n)))
         return rows[:min(n.value, max_rows)]
 
+    def flow_start(self, slot, scheme, dt, tmax, meas):
+        check(self.lib.su2hip_flow_start(self.h, slot, scheme, dt, tmax, meas))
+
+    def flow_poll(self, slot):
+        d = C.c_int()
+        check(self.lib.su2hip_flow_poll(self.h, slot, C.byref(d)))
+        return bool(d.value)
+
+    def flow_finish(self, slot, max_rows=100000):
+        rows = np.zeros((max_rows, 1 + NSUMS))
+        n = C.c_int()
+        check(self.lib.su2hip_flow_finish(self.h, slot, ptr(rows), max_rows, C.byref(n)))
+        return rows[:min(n.value, max_rows)]
+
     def flow_force(self, slot):
         F = np.zeros((volume(self.params[slot]), 3, 4))
         check(self.lib.su2hip_flow_force(self.h, slot, ptr(F)))

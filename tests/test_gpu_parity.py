@@ -329,3 +329,34 @@ def test_plain_16cubed_trajectory():
     assert np.abs(so - sc).max() <= 1e-12 * np.abs(so).max()
     plaq = 1 - sg[abi.SUM["wilson"]] / (V * p.betasu2) / 3
     assert 0.5 < plaq < 1.0
+
+
+def test_async_flow_equals_sync_flow_while_others_sweep():
+    """su2hip_flow_start/poll/finish on the second stream gives the rows of
+    su2hip_flow bit for bit, while another lattice of the handle keeps
+    sweeping, and that lattice ends exactly as it would alone."""
+    ps = [make_params(L=(8, 8, 8), nhiggs=1, seed=7), make_params(L=(12, 8, 8), nhiggs=1, seed=8)]
+    sync = Batch(ps)
+    for i in range(2):
+        sync.hot_start(i)
+    rows_sync = sync.flow(0, abi.FLOW_RK3, 0.05, 1.0, 4)
+    b = Batch(ps)
+    for i in range(2):
+        b.hot_start(i)
+    b.set_budget(0, 0)
+    b.flow_start(0, abi.FLOW_RK3, 0.05, 1.0, 4)
+    with pytest.raises(abi.Su2hipError):
+        b.set_budget(0, 3)          # the flowing lattice may not sweep
+    with pytest.raises(abi.Su2hipError):
+        b.flow_start(0, abi.FLOW_RK3, 0.05, 1.0, 4)
+    b.update(5)                      # lattice 1 sweeps meanwhile
+    rows = b.flow_finish(0)
+    assert rows.shape == rows_sync.shape and np.array_equal(rows, rows_sync)
+    alone = Batch(ps[1])
+    alone.hot_start(0)
+    alone.update(5)
+    assert np.array_equal(alone.get_fields(0)["su2link"], b.get_fields(1)["su2link"])
+    assert not np.array_equal(sync.get_fields(1)["su2link"], b.get_fields(1)["su2link"])
+    # a lattice with a budget cannot start an asynchronous flow
+    with pytest.raises(abi.Su2hipError):
+        b.flow_start(1, abi.FLOW_RK3, 0.05, 1.0, 4)
