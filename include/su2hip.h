@@ -267,6 +267,17 @@ int su2hip_measure_columns(const su2hip_params *p, const double *sums,
                            double weight, double muca_param, double *cols,
                            int *ncols);
 
+/* Correlators of the reference's CORRELATORS build (measure_correlators,
+ * src/correlation.c:339-391) along direction dir (-1: the longest, as the
+ * reference picks it, src/layout.c:771-777).  For d = 0 .. L[dir]/2-1
+ * (at most max_dist rows), out[d][10] holds H(d) = 4/V sum_z h(z) h(z+d)
+ * with h the plane sums of Tr Phi^+ Phi / 2, the Sigma^2 correlator, and
+ * the photon correlators of photon_correl as re, im pairs: mode 0 at
+ * transverse momentum 1 and 2, mode 1 at 1 and 2.  Entries of absent fields
+ * are 0.  Plane sums in FP64.  *ndist = L[dir]/2. */
+int su2hip_correlators(su2hip_batch_t *b, int slot, int dir, double *out, int max_dist,
+                       int *ndist);
+
 /* ---- multicanonical --------------------------------------------------------
  * Install the weight of a lattice (load_weight's result, src/su2.h:483,
  * src/multicanonical.c:118-288) and compute both parity contributions of

@@ -35,6 +35,8 @@ typedef struct su2hip_run_config {
   char resultsfile[SU2HIP_PATHLEN];
   char latticefile[SU2HIP_PATHLEN];
   char weightfile[SU2HIP_PATHLEN];
+  int32_t correlators;       /* field content includes the correlator keys */
+  int32_t do_correlators, correlator_interval;
 } su2hip_run_config;
 
 /* Parse `path` the way get_parameters + get_weight_parameters do
@@ -42,7 +44,8 @@ typedef struct su2hip_run_config {
  * previous match and wrapping once, strtol/strtod conversion, unknown
  * algorithm names -> metropolis).  `model` overrides the config's `model` key
  * when non-NULL: '+'-separated tokens among higgs, u1, triplet, singlet,
- * gradflow; the default is the reference Makefile's build, higgs+singlet+u1. */
+ * gradflow, correlators; the default is the reference Makefile's build,
+ * higgs+singlet+u1. */
 int su2hip_config_read(const char *path, const char *model, su2hip_run_config *out);
 /* Re-read iterations, interval and checkpoint (read_updated_parameters,
  * src/parameters.c:460-494). */
@@ -76,6 +79,14 @@ int su2hip_labels_write(const char *path, const su2hip_params *p);
 /* One `measure` line (src/measure.c:252-306): "%g " per column, newline. */
 int su2hip_measure_format(const su2hip_params *p, const double *sums, double weight,
                           double muca_param, char *buf, int buflen);
+
+/* `labels_correl` (print_labels_correlators, src/correlation.c:313-337). */
+int su2hip_correl_labels_write(const char *path, const su2hip_params *p);
+/* Append one measurement to `path` as measure_correlators writes it
+ * (src/correlation.c:339-391): a "Measurement id" header, then per distance
+ * "%d " and "%g " for the model's columns of su2hip_correlators' rows. */
+int su2hip_correl_append(const char *path, const su2hip_params *p, int meas_id,
+                         const double *corr, int ndist);
 
 /* gsum/nsum of the SLOW weight recursion, bins+2 entries, from a
  * `weight_params` file (load_weight_params, src/multicanonical.c:291-333):

@@ -80,7 +80,7 @@ int __wrap_multicanonical_acceptance(lattice const *ll, weight *ww, double o, do
 }
 
 /* which model flags this build has: bit0 NHIGGS, bit1 U1, bit2 TRIPLET,
- * bit3 SINGLET, bit4 GRADFLOW */
+ * bit3 SINGLET, bit4 GRADFLOW, bit5 CORRELATORS */
 int ref_model(void) {
   int m = 0;
 #if (NHIGGS > 0)
@@ -97,6 +97,9 @@ int ref_model(void) {
 #endif
 #ifdef GRADFLOW
   m |= 16;
+#endif
+#ifdef CORRELATORS
+  m |= 32;
 #endif
   return m;
 }
@@ -529,3 +532,53 @@ double ref_phi0(const char *path) {
   (void)path;
   return p.phi0;
 }
+
+#ifdef CORRELATORS
+/* defined in src/correlation.c without a prototype in su2.h */
+complex photon_correl(lattice *l, fields const *f, params const *p, int d, int dir, int transmom,
+                      int mode);
+#if (NHIGGS > 0)
+double higgs_correlator(lattice *l, fields const *f, int d, int dir, int higgs_id);
+#endif
+#ifdef TRIPLET
+double triplet_correlator(lattice *l, fields const *f, int d, int dir);
+#endif
+
+/* measure_correlators' columns (src/correlation.c:339-391) for d = 0 ..
+ * L[dir]/2-1, 9 per d: H(d), Sigma^2 correlator, photon_correl(mode 0,
+ * p = 1, 2) and (mode 1, p = 1, 2) as re, im pairs; absent fields 0 */
+int ref_correlators(int dir, double *out) {
+  int maxd = l.L[dir] / 2;
+  for (int d = 0; d < maxd; d++) {
+    double *o = out + (size_t)d * 10;
+    for (int k = 0; k < 10; k++) o[k] = 0.0;
+#if (NHIGGS > 0)
+    o[0] = higgs_correlator(&l, &f, d, dir, 0);
+#endif
+#ifdef TRIPLET
+    o[1] = triplet_correlator(&l, &f, d, dir);
+    complex c;
+    c = photon_correl(&l, &f, &p, d, dir, 1, 0);
+    o[2] = c.re;
+    o[3] = c.im;
+    c = photon_correl(&l, &f, &p, d, dir, 2, 0);
+    o[4] = c.re;
+    o[5] = c.im;
+    c = photon_correl(&l, &f, &p, d, dir, 1, 1);
+    o[6] = c.re;
+    o[7] = c.im;
+    c = photon_correl(&l, &f, &p, d, dir, 2, 1);
+    o[8] = c.re;
+    o[9] = c.im;
+#endif
+  }
+  return maxd;
+}
+
+void ref_measure_correlators(const char *fname, int dir, int id) {
+  measure_correlators((char *)fname, &l, &f, &p, dir, id);
+}
+void ref_print_labels_correlators(void) { print_labels_correlators(); }
+#endif
+
+int ref_longest_dir(void) { return l.longest_dir; }

@@ -1477,6 +1477,116 @@ static double magfield(const orc_t *o, long i, int dir) {
   return res;
 }
 
+/* ------------------------------------------------------------------ */
+/* correlators (the reference's CORRELATORS build, src/correlation.c) */
+
+/* a.(b.(c.d)) */
+static void qmul4(const double *a, const double *b, const double *c, const double *d, double *r) {
+  double t[4], u[4];
+  qmul(c, d, t);
+  qmul(b, t, u);
+  qmul(a, u, r);
+}
+
+/* clover_su2 (src/su2u1.c:207-275): the four plaquettes of the (d1, d2)
+ * plane with a corner at site i, added */
+static void clover_su2(const orc_t *o, long i, int d1, int d2, double *cl) {
+  double a[4], b[4], c[4], d[4], t[4];
+  long s;
+  /* su2plaquette (src/su2u1.c:119-136) */
+  qconj(LNK(o, NX(o, i, d2), d1), c);
+  qconj(LNK(o, i, d2), d);
+  qmul4(LNK(o, i, d1), LNK(o, NX(o, i, d1), d2), c, d, cl);
+  /* U_d2(x) U_d1(x+d2-d1)^+ U_d2(x-d1)^+ U_d1(x-d1) */
+  s = PV(o, NX(o, i, d2), d1);
+  qconj(LNK(o, s, d1), b);
+  s = PV(o, i, d1);
+  qconj(LNK(o, s, d2), c);
+  qmul4(LNK(o, i, d2), b, c, LNK(o, s, d1), t);
+  for (int k = 0; k < 4; k++) cl[k] += t[k];
+  /* U_d1(x-d1)^+ U_d2(x-d1-d2)^+ U_d1(x-d1-d2) U_d2(x-d2) */
+  s = PV(o, i, d1);
+  qconj(LNK(o, s, d1), a);
+  s = PV(o, s, d2);
+  qconj(LNK(o, s, d2), b);
+  memcpy(c, LNK(o, s, d1), sizeof(c));
+  s = PV(o, i, d2);
+  qmul4(a, b, c, LNK(o, s, d2), t);
+  for (int k = 0; k < 4; k++) cl[k] += t[k];
+  /* U_d2(x-d2)^+ U_d1(x-d2) U_d2(x+d1-d2) U_d1(x)^+ */
+  s = PV(o, i, d2);
+  qconj(LNK(o, s, d2), a);
+  memcpy(b, LNK(o, s, d1), sizeof(b));
+  s = NX(o, s, d1);
+  qconj(LNK(o, i, d1), d);
+  qmul4(a, b, LNK(o, s, d2), d, t);
+  for (int k = 0; k < 4; k++) cl[k] += t[k];
+}
+
+static double alpha_proj(const orc_t *o, long i, int d1, int d2);
+
+/* measure_correlators (src/correlation.c:339-391) without the file:
+ * plane sums along `dir` (plane_sum, :16-34) of doubletsq, tripletsq and
+ * the photon operators of projected_photon_operator (:91-156, mode 0) and
+ * photon_projection_full (:207-255, mode 1) at transverse momenta 1 and 2
+ * (photon_correl, :261-305, whose momentum loop stops after dir0 = 0), then
+ * the correlators at distances 0 .. L[dir]/2-1, 10 values each. */
+int orc_correlators(const orc_t *o, int dir, double *out) {
+  int Ld = o->L[dir];
+  int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+  double *pl = (double *)calloc((size_t)Ld * 10, sizeof(double));
+  for (long i = 0; i < o->vol; i++) {
+    long nat = o->nat[i];
+    long c[D] = {nat % o->L[0], (nat / o->L[0]) % o->L[1], nat / ((long)o->L[0] * o->L[1])};
+    double *q = pl + (size_t)c[dir] * 10;
+    if (o->p.nhiggs) q[0] += doubletsq(PHI(o, i));
+    if (o->p.triplet) {
+      q[1] += tripletsq(SIG(o, i));
+      double cl[4];
+      clover_su2(o, i, d1, d2, cl);
+      const double *a = SIG(o, i);
+      double tr = a[0] * (cl[1] / 4.0) + a[1] * (cl[2] / 4.0) + a[2] * (cl[3] / 4.0);
+      double al = 2.0 * alpha_proj(o, i, d1, d2);
+      for (int m = 1; m <= 2; m++) {
+        double px = dir != 0 ? (double)(m * c[0]) / (double)o->L[0] : 0.0;
+        double cs = cos(2.0 * M_PI * px), sn = sin(2.0 * M_PI * px);
+        q[2 * m] += tr * cs;
+        q[2 * m + 1] += tr * sn;
+        q[4 + 2 * m] += al * cs;
+        q[4 + 2 * m + 1] += al * sn;
+      }
+    }
+  }
+  int maxd = Ld / 2;
+  for (int d = 0; d < maxd; d++) {
+    double *r = out + (size_t)d * 10;
+    for (int k = 0; k < 10; k++) r[k] = 0.0;
+    for (int z = 0; z < Ld; z++) {
+      const double *a = pl + (size_t)z * 10, *b = pl + (size_t)((z + d) % Ld) * 10;
+      r[0] += a[0] * b[0];
+      r[1] += a[1] * b[1];
+      for (int k = 2; k < 10; k += 2) {
+        /* cmult(op1, conj(op2)) */
+        double br = b[k], bi = -b[k + 1];
+        r[k] += a[k] * br - a[k + 1] * bi;
+        r[k + 1] += br * a[k + 1] + a[k] * bi;
+      }
+    }
+    r[0] *= 4.0 / (double)o->vol;
+    for (int k = 1; k < 10; k++) r[k] /= (double)o->vol;
+  }
+  free(pl);
+  return maxd;
+}
+
+/* longest direction; of equal ones the last (src/layout.c:771-777) */
+int orc_longest_dir(const orc_t *o) {
+  int l = D - 1;
+  for (int d = D - 1; d >= 0; d--)
+    if (o->L[d] > o->L[l]) l = d;
+  return l;
+}
+
 /* magcharge_cube (src/magfield.c:262-280) */
 static double magcharge_cube(const orc_t *o, long i) {
   double res = 0.0;

@@ -86,6 +86,14 @@ void orc_muca_set(orc_t *o, const su2hip_muca *m, const double *pos,
                   const double *W);
 void orc_muca_get(const orc_t *o, su2hip_muca *m, double *pos, double *W,
                   int32_t *hits);
+/* Correlators along dir of the reference's CORRELATORS build
+ * (measure_correlators, src/correlation.c:339-391): for d = 0..L[dir]/2-1,
+ * 10 values: H(d), Sigma^2 correlator, photon (mode 0, p=1) re im, (mode 0,
+ * p=2) re im, (mode 1, p=1) re im, (mode 1, p=2) re im; absent fields 0.
+ * Returns the number of distances. */
+int orc_correlators(const orc_t *o, int dir, double *out);
+int orc_longest_dir(const orc_t *o);
+
 /* gsum/nsum of the SLOW recursion (bins+2 entries; load_weight_params,
  * src/multicanonical.c:291-333, initialises them to 5 and 1). */
 void orc_muca_set_sums(orc_t *o, const int64_t *gsum, const int64_t *nsum);

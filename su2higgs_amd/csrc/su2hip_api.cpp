@@ -1324,6 +1324,56 @@ int su2hip_muca_get(su2hip_batch_t *b, int slot, su2hip_muca *m, double *pos, do
   return SU2HIP_OK;
 }
 
+int su2hip_correlators(su2hip_batch_t *b, int slot, int dir, double *out, int max_dist,
+                       int *ndist) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  const su2hip_params &p = b->slots[slot].p;
+  if (dir < 0) {
+    // the reference's longest direction; of equal ones the last
+    // (make_misc_tables, src/layout.c:771-777)
+    dir = 2;
+    for (int d = 2; d >= 0; d--)
+      if (p.L[d] > p.L[dir]) dir = d;
+  }
+  if (dir > 2) return fail(SU2HIP_EINVAL, "direction %d", dir);
+  rc = prepare(b);
+  if (rc) return rc;
+  int Ld = p.L[dir];
+  size_t V = (size_t)vol_of(p), P = V / Ld;
+  rc = ensure_stage(b, 10 * V + 10 * (size_t)Ld);
+  if (rc) return rc;
+  double *buf = b->d_stage, *planes = b->d_stage + 10 * V;
+  HIPTRY(launch_corr(ctx_of(b), slot, dir, (int)(V / 2), Ld, P, buf, planes));
+  std::vector<double> pl(10 * (size_t)Ld);
+  HIPTRY(hipMemcpyAsync(pl.data(), planes, sizeof(double) * pl.size(), hipMemcpyDeviceToHost, b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  // higgs_correlator, triplet_correlator and photon_correl
+  // (src/correlation.c:40-305) from the plane sums
+  int maxd = Ld / 2;
+  for (int d = 0; d < maxd && d < max_dist && out; d++) {
+    double r[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int z = 0; z < Ld; z++) {
+      int zd = (z + d) % Ld;
+      auto a = [&](int k) { return pl[(size_t)k * Ld + z]; };
+      auto c = [&](int k) { return pl[(size_t)k * Ld + zd]; };
+      r[0] += a(0) * c(0);
+      r[1] += a(1) * c(1);
+      for (int k = 2; k < 10; k += 2) {
+        // cmult(op1, conj(op2)) (src/generic/stddefs.h:93-98)
+        double br = c(k), bi = -c(k + 1);
+        r[k] += a(k) * br - a(k + 1) * bi;
+        r[k + 1] += br * a(k + 1) + a(k) * bi;
+      }
+    }
+    double *o = out + (size_t)d * 10;
+    o[0] = 4.0 * r[0] / (double)V;
+    for (int k = 1; k < 10; k++) o[k] = r[k] / (double)V;
+  }
+  if (ndist) *ndist = maxd;
+  return SU2HIP_OK;
+}
+
 int su2hip_muca_set_sums(su2hip_batch_t *b, int slot, const int64_t *gsum, const int64_t *nsum) {
   int rc = check_slot(b, slot);
   if (rc) return rc;

@@ -142,6 +142,10 @@ hipError_t launch_copy_probe(const double *src, double *dst, size_t n,
                              hipStream_t s);
 // gradient flow on one lattice (slot) whose descriptor in c.desc points at
 // the flow buffers: stage 0 = Euler, 1..3 = RK3 stages
+// correlator plane sums of lattice `slot` along dir: buf holds 10 V
+// doubles, planes 10 Ld
+hipError_t launch_corr(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, size_t P,
+                       double *buf, double *planes);
 hipError_t launch_flow_stage(const LaunchCtx &c, int slot, const void *Uin,
                              void *Uout, const void *Gin, void *Gout, void *Z,
                              void *ZG, double dt, double msq_t, int stage);

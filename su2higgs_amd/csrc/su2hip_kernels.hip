@@ -1032,7 +1032,20 @@ __device__ Cm project_u1(const LatDesc &L, const T *U, const T *G, const Site &x
   return cm_mul(cm_mul(pl, u, false), pr, false);
 }
 
-// alpha_proj + magfield + magcharge_cube (src/magfield.c:189-280)
+// alpha_proj (src/magfield.c:189-216)
+template <typename T>
+__device__ double alpha_proj(const LatDesc &L, const T *U, const T *G, const Site &x, int d1, int d2) {
+  Cm u1 = project_u1(L, U, G, x, d1);
+  Cm u2 = project_u1(L, U, G, shift(L, x, d1, +1), d2);
+  Cm u3 = project_u1(L, U, G, shift(L, x, d2, +1), d1);
+  Cm u4 = project_u1(L, U, G, x, d2);
+  u1 = cm_mul(u1, u2, false);
+  u1 = cm_mul(u1, u3, true);
+  u1 = cm_mul(u1, u4, true);
+  return ::atan2(u1.im[0] + u1.im[3], u1.re[0] + u1.re[3]) * ::sqrt(L.beta);
+}
+
+// magfield + magcharge_cube (src/magfield.c:221-280)
 template <typename T>
 __device__ double magfield(const LatDesc &L, const T *U, const T *G, const Site &x, int dir) {
   double res = 0.0;
@@ -1040,14 +1053,7 @@ __device__ double magfield(const LatDesc &L, const T *U, const T *G, const Site 
     if (d1 == dir) continue;
     for (int d2 = d1 + 1; d2 < 3; d2++) {
       if (d2 == dir) continue;
-      Cm u1 = project_u1(L, U, G, x, d1);
-      Cm u2 = project_u1(L, U, G, shift(L, x, d1, +1), d2);
-      Cm u3 = project_u1(L, U, G, shift(L, x, d2, +1), d1);
-      Cm u4 = project_u1(L, U, G, x, d2);
-      u1 = cm_mul(u1, u2, false);
-      u1 = cm_mul(u1, u3, true);
-      u1 = cm_mul(u1, u4, true);
-      double al = ::atan2(u1.im[0] + u1.im[3], u1.re[0] + u1.re[3]) * ::sqrt(L.beta);
+      double al = alpha_proj(L, U, G, x, d1, d2);
       if (d1 > dir)
         res += al;
       else if (d2 > dir)
@@ -1215,6 +1221,90 @@ __global__ __launch_bounds__(BLK) void k_measure_final(const LatDesc *__restrict
     double t = block_sum(v, lds);
     if (threadIdx.x == 0) out[j] = t;
   }
+}
+
+// ------------------------------------------------------------------------
+// correlators (the reference's CORRELATORS build, src/correlation.c):
+// per-site operators stored by (value, plane along dir, position in the
+// plane), then one fixed-order reduction per (value, plane)
+
+// a.(b.(c.d)) in FP64
+__device__ __forceinline__ Q<double> qmul4(const Q<double> &a, const Q<double> &b,
+                                           const Q<double> &c, const Q<double> &d) {
+  return qmul(a, qmul(b, qmul(c, d)));
+}
+
+// clover_su2 (src/su2u1.c:207-275): the four plaquettes of the (d1, d2)
+// plane with a corner at x
+template <typename T>
+__device__ Q<double> clover(const LatDesc &L, const T *U, const Site &x, int d1, int d2) {
+  auto u = [&](const Site &c, int d) { return qd(U_at(L, U, c, d)); };
+  Site x1 = shift(L, x, d1, +1), x2 = shift(L, x, d2, +1);
+  Site m1 = shift(L, x, d1, -1), m2 = shift(L, x, d2, -1);
+  Site x2m1 = shift(L, x2, d1, -1), m1m2 = shift(L, m1, d2, -1), m2p1 = shift(L, m2, d1, +1);
+  Q<double> c = qmul4(u(x, d1), u(x1, d2), qconj(u(x2, d1)), qconj(u(x, d2)));
+  c = qadd(c, qmul4(u(x, d2), qconj(u(x2m1, d1)), qconj(u(m1, d2)), u(m1, d1)));
+  c = qadd(c, qmul4(qconj(u(m1, d1)), qconj(u(m1m2, d2)), u(m1m2, d1), u(m2, d2)));
+  c = qadd(c, qmul4(qconj(u(m2, d2)), u(m2, d1), u(m2p1, d2), qconj(u(x, d1))));
+  return c;
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_corr_sites(const LatDesc *__restrict__ lats, int slot,
+                                                    int dir, double *__restrict__ buf) {
+  const LatDesc &L = lats[slot];
+  int s = blockIdx.x * BLK + (int)threadIdx.x;
+  if (s >= L.Vh) return;
+  const T *U = (const T *)L.U, *F = (const T *)L.F, *G = (const T *)L.G;
+  int Ld = dir == 0 ? L.L0 : (dir == 1 ? L.L1 : L.L2);
+  size_t P = (size_t)L.vol / Ld, V = (size_t)L.vol;
+  // transverse directions of projected_photon_operator (src/correlation.c:101-108)
+  int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+  for (int par = 0; par < 2; par++) {
+    Site x = site_of(L, par, s);
+    int cd = dir == 0 ? x.x : (dir == 1 ? x.y : x.z);
+    size_t pos = dir == 0 ? (size_t)x.y + (size_t)L.L1 * x.z
+                          : (dir == 1 ? (size_t)x.x + (size_t)L.L0 * x.z
+                                      : (size_t)x.x + (size_t)L.L0 * x.y);
+    double *o = buf + (size_t)cd * P + pos;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if constexpr ((M & M_NH) != 0) {
+      Q<double> f = qd(F_at(L, F, x));
+      v[0] = 0.5 * qdot(f, f);
+    }
+    if constexpr ((M & M_TR) != 0) {
+      V3<T> gt = G_at(L, G, x);
+      double a0 = gt.x, a1 = gt.y, a2 = gt.z;
+      v[1] = 0.5 * (a0 * a0 + a1 * a1 + a2 * a2);
+      Q<double> cl = clover(L, U, x, d1, d2);
+      double tr = a0 * (cl.b / 4.0) + a1 * (cl.c / 4.0) + a2 * (cl.d / 4.0);
+      double al = 2.0 * alpha_proj(L, U, G, x, d1, d2);
+#pragma unroll
+      for (int m = 1; m <= 2; m++) {
+        // photon_correl's momentum: transverse momentum m along x only when
+        // dir is not 0 (src/correlation.c:270-275)
+        double px = dir != 0 ? (double)(m * x.x) / (double)L.L0 : 0.0;
+        double cs = ::cos(2.0 * PI * px), sn = ::sin(2.0 * PI * px);
+        v[2 * m] = tr * cs;
+        v[2 * m + 1] = tr * sn;
+        v[4 + 2 * m] = al * cs;
+        v[4 + 2 * m + 1] = al * sn;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 10; k++) o[(size_t)k * V] = v[k];
+  }
+}
+
+// planes[b] = sum of the P values of (value, plane) b, in a fixed order
+__global__ __launch_bounds__(BLK) void k_corr_reduce(const double *__restrict__ buf, size_t P,
+                                                     double *__restrict__ planes) {
+  __shared__ double lds[BLK / 64];
+  const double *src = buf + (size_t)blockIdx.x * P;
+  double v = 0.0;
+  for (size_t i = threadIdx.x; i < P; i += BLK) v += src[i];
+  double t = block_sum(v, lds);
+  if (threadIdx.x == 0) planes[blockIdx.x] = t;
 }
 
 // ------------------------------------------------------------------------
@@ -1567,6 +1657,24 @@ hipError_t launch_measure(const LaunchCtx &c) {
   }
   hipLaunchKernelGGL(k_measure_final, dim3(c.max_slots), dim3(BLK), 0, c.stream, c.desc,
                      c.partial, c.sums);
+  return hipGetLastError();
+}
+
+hipError_t launch_corr(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, size_t P,
+                       double *buf, double *planes) {
+  int nb = (Vh + BLK - 1) / BLK;
+  hipError_t e = with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_corr_sites<T, M>), dim3(nb), dim3(BLK), 0, c.stream, c.desc, slot, dir,
+                         buf);
+      return hipGetLastError();
+    });
+  });
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_corr_reduce, dim3(10 * Ld), dim3(BLK), 0, c.stream, (const double *)buf, P,
+                     planes);
   return hipGetLastError();
 }
 

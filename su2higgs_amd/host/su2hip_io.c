@@ -170,6 +170,7 @@ static int algorithm_of(const char *s, const char *label) {
 static int parse_model(const char *s, su2hip_run_config *c) {
   c->p.nhiggs = c->p.u1 = c->p.triplet = c->p.singlet = 0;
   c->gradflow = 0;
+  c->correlators = 0;
   char buf[MAXLINE];
   snprintf(buf, sizeof(buf), "%s", s);
   for (char *tok = strtok(buf, "+,"); tok; tok = strtok(NULL, "+,")) {
@@ -183,6 +184,8 @@ static int parse_model(const char *s, su2hip_run_config *c) {
       c->p.singlet = 1;
     else if (!strcasecmp(tok, "gradflow"))
       c->gradflow = 1;
+    else if (!strcasecmp(tok, "correlators"))
+      c->correlators = 1;
     else if (!strcasecmp(tok, "su2") || !strcasecmp(tok, "gauge"))
       ;
     else
@@ -286,6 +289,10 @@ int su2hip_config_read(const char *path, const char *model, su2hip_run_config *o
       GETD(&f, "a1_s", p->a1_s);
       GETD(&f, "a2_s", p->a2_s);
     }
+  }
+  if (c.correlators) {
+    GETI(&f, "do_correlators", c.do_correlators);
+    GETI(&f, "correlator_interval", c.correlator_interval);
   }
   c.flow_scheme = SU2HIP_FLOW_EULER;
   if (c.gradflow) {
@@ -534,6 +541,50 @@ int su2hip_weight_load(const char *path, su2hip_muca *m, double *pos, double *W,
   m->enabled = 1;
   m->do_acceptance = 1;
   return su2hip_weight_save(path, m, pos, W);
+}
+
+/* print_labels_correlators (src/correlation.c:313-337) */
+int su2hip_correl_labels_write(const char *path, const su2hip_params *p) {
+  FILE *f = fopen(path, "w");
+  if (!f) return ioerr(SU2HIP_EIO, "cannot write %s", path);
+  int k = 1;
+  fprintf(f, "%d distance\n", k++);
+  if (p->nhiggs) fprintf(f, "%d H(l) = sum_z h(z) h(z+l) / V\n", k++);
+  if (p->triplet) {
+    fprintf(f, "%d Sigma^2 correlator\n", k++);
+    fprintf(f, "%d projected photon correlator RE\n", k++);
+    fprintf(f, "%d projected photon correlator IM\n", k++);
+    fprintf(f, "%d projected photon correlator RE (p=2)\n", k++);
+    fprintf(f, "%d projected photon correlator IM (p=2)\n", k++);
+    fprintf(f, "%d alpha_munu RE (p=1)\n", k++);
+    fprintf(f, "%d alpha_munu IM (p=1)\n", k++);
+    fprintf(f, "%d alpha_munu RE (p=2)\n", k++);
+    fprintf(f, "%d alpha_munu IM (p=2)\n", k++);
+  }
+  fclose(f);
+  return SU2HIP_OK;
+}
+
+/* measure_correlators' file output (src/correlation.c:339-391) */
+int su2hip_correl_append(const char *path, const su2hip_params *p, int meas_id,
+                         const double *corr, int ndist) {
+  if (!path || !p || (ndist > 0 && !corr)) return ioerr(SU2HIP_EINVAL, "correl_append arguments");
+  FILE *f = fopen(path, "a");
+  if (!f) return ioerr(SU2HIP_EIO, "cannot append to %s", path);
+  fprintf(f, "\n =========== Measurement id: %d ===========\n", meas_id);
+  for (int d = 0; d < ndist; d++) {
+    const double *r = corr + (size_t)d * 10;
+    fprintf(f, "%d ", d);
+    if (p->nhiggs) fprintf(f, "%g ", r[0]);
+    if (p->triplet) {
+      fprintf(f, "%g ", r[1]);
+      fprintf(f, "%g %g %g %g ", r[2], r[3], r[4], r[5]);
+      fprintf(f, "%g %g %g %g ", r[6], r[7], r[8], r[9]);
+    }
+    fprintf(f, "\n");
+  }
+  fclose(f);
+  return SU2HIP_OK;
 }
 
 /* load_weight_params (src/multicanonical.c:291-333) */

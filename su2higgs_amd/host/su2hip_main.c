@@ -280,6 +280,18 @@ int main(int argc, char **argv) {
   }
   double start = now();
   int flow_id = 1;
+  int correlator_id = 1;
+  double *corr = NULL;
+  if (cfg.correlators && cfg.do_correlators) {
+    /* main.c:98-104: labels, the longest direction */
+    TRY("correl_labels", su2hip_correl_labels_write("labels_correl", p));
+    int ld = 2;
+    for (int d = 2; d >= 0; d--)
+      if (p->L[d] > p->L[ld]) ld = d;
+    printf("\nMeasuring correlation functions along direction %d every %d iterations\n", ld + 1,
+           cfg.correlator_interval);
+    corr = calloc((size_t)(p->L[ld] / 2 + 1) * 10, sizeof(double));
+  }
   char line[4096];
   while (iter <= cfg.iterations) {
     if (iter % cfg.interval == 0) {
@@ -291,6 +303,12 @@ int main(int argc, char **argv) {
       fflush(res);
     }
     if (cfg.gradflow && cfg.do_flow && iter % cfg.flow_interval == 0) flow(b, &cfg, flow_id++);
+    if (corr && iter % cfg.correlator_interval == 0) {
+      /* measure_correlators("correl0", ...) (src/main.c:330-349) */
+      int nd = 0;
+      TRY("correlators", su2hip_correlators(b, 0, -1, corr, 1 << 20, &nd));
+      TRY("correl_append", su2hip_correl_append("correl0", p, correlator_id++, corr, nd));
+    }
     TRY("update", su2hip_update(b, 1));
     if (iter % cfg.checkpoint == 0) {
       TRY("sync", su2hip_sync(b));

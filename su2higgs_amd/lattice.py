@@ -185,6 +185,13 @@ class Batch:
         return {abi.KIND_NAMES[k]: tuple(t[3 * k:3 * k + 3]) for k in range(abi.NKINDS)
                 if not np.isnan(t[3 * k])}
 
+    def correlators(self, slot, d=-1):
+        L = max(self.params[slot].L)
+        out = np.zeros((L // 2, 10))
+        n = C.c_int()
+        check(self.lib.su2hip_correlators(self.h, slot, d, ptr(out), L // 2, C.byref(n)))
+        return out[:n.value]
+
     def muca_sums(self, slot):
         n = self.muca_get(slot)[0].bins + 2
         g, s = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)

@@ -5,8 +5,8 @@ Each fixture is data only: an input configuration (a Philox hot start of a
 4x4x6 lattice) and what the reference's own functions compute from it:
 per-link staples and local actions, the measure sums and the measure line,
 the fields and counters after update_lattice sweeps with the Philox stream,
-the gradient-flow force and an Euler step, multicanonical sweeps, and the
-bytes of a saved lattice file.
+the gradient-flow force and an Euler step, multicanonical sweeps, the
+bytes of a saved lattice file, and the correlators of the CORRELATORS build.
 
     python tests/golden/make_golden.py [fixture names, default all]
 """
@@ -204,6 +204,27 @@ def muca_fixture(tag, op, mode, cps, slope, interval=None, params=False, sweeps=
     return out
 
 
+def correl_fixture(tag, kw):
+    """Correlators of the reference's CORRELATORS build along every
+    direction of a hot start (src/correlation.c:339-391)."""
+    p = make_params(L=L, **kw, seed=31)
+    o = orc.Oracle(p)
+    o.hot_start()
+    f = o.get_fields()
+    r = orc.Ref(p, correlators=True)
+    r.set_fields(f)
+    out = {"L": np.array(L), "longest": np.array([r.lib.ref_longest_dir()])}
+    for k in orc.FIELDS:
+        out["in_" + k] = f[k]
+    for d in range(3):
+        out["correl_%d" % d] = r.correlators(d)
+    return out
+
+
+CORREL_CASES = {"correl_hc": ("hc", dict(nhiggs=1)),
+                "correl_htc": ("htc", dict(nhiggs=1, triplet=1))}
+
+
 MUCA_CASES = {
     "muca_h_phisq_ro": ("h", abi.PHISQ, abi.READONLY, 1, 5.0),
     "muca_h_phisq_fast3": ("h", abi.PHISQ, abi.FAST, 3, 400.0),
@@ -231,6 +252,9 @@ def main():
     for name, args in MUCA_CASES.items():
         if not only or name in only:
             np.savez_compressed(os.path.join(HERE, name + ".npz"), **muca_fixture(*args))
+    for name, (tag, kw) in CORREL_CASES.items():
+        if not only or name in only:
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **correl_fixture(tag, kw))
     print("fixtures written to", HERE)
 
 

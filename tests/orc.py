@@ -59,6 +59,8 @@ def oracle_lib():
             "orc_muca_set": (None, [h, P(Muca), dp, dp]),
             "orc_muca_get": (None, [h, P(Muca), dp, dp, P(C.c_int32)]),
             "orc_muca_set_sums": (None, [h, P(C.c_int64), P(C.c_int64)]),
+            "orc_correlators": (C.c_int, [h, C.c_int, dp]),
+            "orc_longest_dir": (C.c_int, [h]),
             "orc_muca_get_sums": (None, [h, P(C.c_int64), P(C.c_int64)]),
             "orc_muca_weight": (C.c_double, [h, C.c_double]),
             "orc_muca_whichbin": (C.c_int, [h, C.c_double]),
@@ -178,6 +180,11 @@ class Oracle:
                               hits.ctypes.data_as(P(C.c_int32)))
         return m, pos, W, hits
 
+    def correlators(self, d):
+        out = np.zeros((self.p.L[d] // 2, 10))
+        n = self.lib.orc_correlators(self.h, d, ptr(out))
+        return out[:n]
+
     def muca_sums(self):
         n = self.muca_get()[0].bins + 2
         g, s = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
@@ -204,12 +211,13 @@ class Oracle:
 # --------------------------------------------------------------------------
 _refs = {}
 
-REF_FLAGS = {1: "h", 3: "hu", 4: "t", 5: "ht", 17: "hg", 20: "tg", 11: "hsu"}
+REF_FLAGS = {1: "h", 3: "hu", 4: "t", 5: "ht", 17: "hg", 20: "tg", 11: "hsu",
+             33: "hc", 36: "tc", 37: "htc"}
 
 
-def ref_tag(p, gradflow=False):
+def ref_tag(p, gradflow=False, correlators=False):
     m = (1 if p.nhiggs else 0) | (2 if p.u1 else 0) | (4 if p.triplet else 0) \
-        | (8 if p.singlet else 0) | (16 if gradflow else 0)
+        | (8 if p.singlet else 0) | (16 if gradflow else 0) | (32 if correlators else 0)
     return REF_FLAGS.get(m)
 
 
@@ -264,6 +272,10 @@ def ref_lib(tag):
         "ref_flow_force": (None, [dp, dp]),
         "ref_flow_euler_step": (None, [C.c_double]),
         "ref_grad_flow": (None, [C.c_double, C.c_double, C.c_int, C.c_int]),
+        "ref_correlators": (C.c_int, [C.c_int, dp]),
+        "ref_measure_correlators": (None, [C.c_char_p, C.c_int, C.c_int]),
+        "ref_print_labels_correlators": (None, []),
+        "ref_longest_dir": (C.c_int, []),
     }
     for n, (r, a) in sig.items():
         fn = getattr(lib, n)
@@ -281,8 +293,8 @@ def ref_lib(tag):
 class Ref:
     """The reference compiled from source, one global lattice per tag."""
 
-    def __init__(self, p, gradflow=False):
-        self.tag = ref_tag(p, gradflow)
+    def __init__(self, p, gradflow=False, correlators=False):
+        self.tag = ref_tag(p, gradflow, correlators)
         self.lib = ref_lib(self.tag)
         self.p = p
         self.V = volume(p)
@@ -320,6 +332,11 @@ class Ref:
         s = np.zeros(NSUMS)
         self.lib.ref_measure_sums(ptr(s))
         return s
+
+    def correlators(self, d):
+        out = np.zeros((self.p.L[d] // 2, 10))
+        n = self.lib.ref_correlators(d, ptr(out))
+        return out[:n]
 
     def measure_text(self):
         buf = C.create_string_buffer(8192)

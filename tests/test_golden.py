@@ -179,3 +179,16 @@ def test_muca_fixture(path):
                 assert lib.su2hip_weight_params_save(pp.encode(), C.byref(m3), ptr(pos),
                                                      g3c.ctypes.data_as(P64), n3c.ctypes.data_as(P64)) == 0
                 assert open(pp, "rb").read() == g["params3_file"].tobytes()
+
+
+@pytest.mark.parametrize("name,kw", [("correl_hc", dict(nhiggs=1)),
+                                     ("correl_htc", dict(nhiggs=1, triplet=1))])
+def test_correlator_fixture(name, kw):
+    g = load(name)
+    p = make_params(L=tuple(g["L"]), **kw, seed=31)
+    o = orc.Oracle(p)
+    o.set_fields(fields(g, "in_"))
+    assert o.lib.orc_longest_dir(o.h) == int(g["longest"][0])
+    for d in range(3):
+        a, b = o.correlators(d), g["correl_%d" % d]
+        assert (np.abs(a - b) <= 1e-12 * (np.abs(b).max(0) + 1e-300)).all(), d

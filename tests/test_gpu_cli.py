@@ -62,9 +62,10 @@ def run_cli(d, cfg):
     return r.stdout
 
 
-def ref_trajectory(tag, p, n, flow_every=0, d=None):
-    """measure lines of the reference (and measure_flow written into d)."""
-    r = orc.Ref(p, gradflow=bool(flow_every))
+def ref_trajectory(tag, p, n, flow_every=0, d=None, corr_every=0):
+    """measure lines of the reference (and measure_flow / correl0 written
+    into d), in the order of its main loop (src/main.c:302-353)."""
+    r = orc.Ref(p, gradflow=bool(flow_every), correlators=bool(corr_every))
     o = orc.Oracle(p)
     o.cold_start(phi0=1.0)
     r.set_fields(o.get_fields())
@@ -78,6 +79,8 @@ def ref_trajectory(tag, p, n, flow_every=0, d=None):
             lines.append(r.measure_text())
             if flow_every and it % flow_every == 0:
                 r.lib.ref_grad_flow(0.5, 0.05, 3, it // flow_every)
+            if corr_every and it % corr_every == 0:
+                r.lib.ref_measure_correlators(b"correl0", r.lib.ref_longest_dir(), it // corr_every)
             r.update(1)
     finally:
         os.chdir(cwd)
@@ -148,3 +151,55 @@ def test_cli_gradient_flow_matches_reference(tmp_path):
     assert open(os.path.join(d_ours, "measure")).read().splitlines(keepends=True) == lines
     assert open(os.path.join(d_ours, "measure_flow")).read() == \
         open(os.path.join(d_ref, "measure_flow")).read()
+
+
+@pytest.mark.parametrize("model,tag,kw", [
+    ("higgs+correlators", "hc", dict(nhiggs=1)),
+    ("triplet+correlators", "tc", dict(nhiggs=0, triplet=1)),
+])
+def test_cli_correlators_match_reference(tmp_path, model, tag, kw):
+    """`correl0` and `labels_correl` of a run with correlators every 2nd
+    iteration, byte for byte against the reference's CORRELATORS build on
+    the same trajectory."""
+    if not orc.ref_available(tag):
+        pytest.skip("reference build absent")
+    d_ours, d_ref = str(tmp_path / "ours"), str(tmp_path / "ref")
+    os.makedirs(d_ours)
+    os.makedirs(d_ref)
+    cfg = CONFIG.format(model=model, it=6, do_flow=0)
+    cfg += "do_correlators 1\ncorrelator_interval 2\nmsq_triplet -0.3\nb4 0.1\nsigma0 0.4\n"
+    cfg += "algorithm_su2triplet overrelax\nupdate_triplet 1\n"
+    run_cli(d_ours, cfg)
+    p = make_params(L=(4, 6, 8), **kw, betasu2=8.0, msq_phi=-0.25, lambda_phi=0.02, scalar_sweeps=3,
+                    msq_triplet=-0.3, b4=0.1, seed=4242)
+    r = orc.Ref(p, correlators=True)
+    o = orc.Oracle(p)
+    o.cold_start(phi0=1.0, sigma0=0.4)
+    r.set_fields(o.get_fields())
+    r.rng_set(p.seed, 0)
+    cwd = os.getcwd()
+    os.chdir(d_ref)
+    try:
+        lines = []
+        for it in range(1, 7):
+            lines.append(r.measure_text())
+            if it % 2 == 0:
+                r.lib.ref_measure_correlators(b"correl0", r.lib.ref_longest_dir(), it // 2)
+            r.update(1)
+        r.lib.ref_print_labels_correlators()
+    finally:
+        os.chdir(cwd)
+    # byte for byte, except the total magnetic charge, an integer sum that
+    # is 0 up to rounding noise whose digits depend on the summation order
+    ours_m = open(os.path.join(d_ours, "measure")).read().splitlines(keepends=True)
+    assert len(ours_m) == len(lines)
+    for a, b in zip(ours_m, lines):
+        ta, tb = a.split(), b.split()
+        assert len(ta) == len(tb)
+        for x, y in zip(ta, tb):
+            assert x == y or (abs(float(x)) < 1e-10 and abs(float(y)) < 1e-10), (a, b)
+    assert open(os.path.join(d_ours, "labels_correl")).read() == \
+        open(os.path.join(d_ref, "labels_correl")).read()
+    ours = open(os.path.join(d_ours, "correl0")).read()
+    assert ours == open(os.path.join(d_ref, "correl0")).read()
+    assert ours.count("Measurement id") == 3

@@ -360,3 +360,28 @@ def test_async_flow_equals_sync_flow_while_others_sweep():
     # a lattice with a budget cannot start an asynchronous flow
     with pytest.raises(abi.Su2hipError):
         b.flow_start(1, abi.FLOW_RK3, 0.05, 1.0, 4)
+
+
+@pytest.mark.parametrize("precision", [FP64, abi.FP32])
+@pytest.mark.parametrize("tag", ["h", "t", "ht"])
+@pytest.mark.parametrize("L", [(4, 6, 8), (6, 6, 6)])
+def test_correlators(tag, L, precision):
+    """Plane-sum correlators (the CORRELATORS build's measure_correlators)
+    against the oracle on the same fields: FP64 to 1e-12 of each column's
+    scale, FP32 to float rounding; every direction and the longest one."""
+    p = make_params(L=L, **MODELS[tag], precision=precision, seed=5)
+    o = orc.Oracle(make_params(L=L, **MODELS[tag], seed=5))
+    o.hot_start()
+    f = o.get_fields()
+    if precision == abi.FP32:
+        f = {k: (v.astype(np.float32).astype(np.float64) if v is not None else None) for k, v in f.items()}
+        o.set_fields(f)
+    b = Batch(p)
+    b.set_fields(0, f)
+    tol = 1e-12 if precision == FP64 else 2e-5
+    for d in (0, 1, 2, -1):
+        dd = o.lib.orc_longest_dir(o.h) if d < 0 else d
+        a, g = o.correlators(dd), b.correlators(0, d)
+        assert g.shape == a.shape
+        scale = np.abs(a).max(0) + 1e-300
+        assert (np.abs(a - g) <= tol * scale).all(), (d, np.abs(a - g).max(0) / scale)

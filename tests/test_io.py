@@ -319,3 +319,44 @@ def test_weight_file_matches_reference(lib, tmp_path, mode, exists):
         assert getattr(mo, k) == getattr(mr, k), k
     if mode != abi.READONLY:
         assert open("w_our").read() == open("w_ref").read()
+
+
+CORR = {"hc": dict(nhiggs=1), "tc": dict(nhiggs=0, triplet=1), "htc": dict(nhiggs=1, triplet=1)}
+
+
+@pytest.mark.parametrize("tag", list(CORR))
+def test_correlator_files_match_reference(lib, tmp_path, tag):
+    """labels_correl and the correl0 block of the CORRELATORS build, our
+    writers fed with the reference's own correlator values."""
+    need_ref(tag)
+    p = make_params(L=(4, 6, 8), **CORR[tag], seed=9)
+    o = orc.Oracle(p)
+    o.hot_start()
+    r = orc.Ref(p, correlators=True)
+    r.set_fields(o.get_fields())
+    os.chdir(tmp_path)
+    r.lib.ref_print_labels_correlators()
+    assert lib.su2hip_correl_labels_write(b"labels_ours", C.byref(p)) == 0
+    assert open("labels_ours").read() == open("labels_correl").read()
+    d = r.lib.ref_longest_dir()
+    r.lib.ref_measure_correlators(b"correl0", d, 3)
+    r.lib.ref_measure_correlators(b"correl0", d, 4)
+    vals = np.ascontiguousarray(r.correlators(d))
+    for mid in (3, 4):
+        assert lib.su2hip_correl_append(b"correl_ours", C.byref(p), mid, ptr(vals), len(vals)) == 0
+    assert open("correl_ours").read() == open("correl0").read()
+
+
+def test_config_reads_correlator_keys(lib, tmp_path):
+    """With the correlators token the reader takes do_correlators and
+    correlator_interval, in the reference CORRELATORS build's lookup order."""
+    need_ref("hc")
+    path = str(tmp_path / "config")
+    with open(path, "w") as f:
+        f.write(CONFIG.format(L1=4, muca=0, op="phisq") + "do_correlators 1\ncorrelator_interval 7\n")
+    c = RunConfig()
+    assert lib.su2hip_config_read(path.encode(), b"higgs+correlators", C.byref(c)) == 0
+    assert c.correlators == 1 and c.do_correlators == 1 and c.correlator_interval == 7
+    c2 = RunConfig()
+    assert lib.su2hip_config_read(path.encode(), b"higgs", C.byref(c2)) == 0
+    assert c2.correlators == 0 and c2.do_correlators == 0

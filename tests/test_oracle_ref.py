@@ -184,3 +184,30 @@ def test_polysolve3_against_reference():
         assert x == y or abs(x - y) < 1e-12 * max(1.0, abs(y))
         # and it is a root
         assert abs(((a * x + b) * x + c) * x + d) < 1e-8 * (abs(d) + c * abs(x) + 1)
+
+
+CORR_MODELS = {"hc": dict(nhiggs=1), "tc": dict(nhiggs=0, triplet=1),
+               "htc": dict(nhiggs=1, triplet=1)}
+
+
+@pytest.mark.parametrize("tag", list(CORR_MODELS))
+@pytest.mark.parametrize("L", [(4, 6, 8), (6, 6, 6), (8, 4, 6)])
+def test_correlators(tag, L):
+    """measure_correlators of the CORRELATORS build against the oracle on a
+    hot start, along every direction (the photon operators carry momentum
+    only when the direction is not 0)."""
+    if not orc.ref_available(tag):
+        pytest.skip("reference build absent")
+    p = make_params(L=L, **CORR_MODELS[tag], seed=31)
+    o = orc.Oracle(p)
+    o.hot_start()
+    r = orc.Ref(p, correlators=True)
+    r.set_fields(o.get_fields())
+    assert o.lib.orc_longest_dir(o.h) == r.lib.ref_longest_dir()
+    for d in range(3):
+        a, b = o.correlators(d), r.correlators(d)
+        assert a.shape == b.shape == (L[d] // 2, 10)
+        scale = np.abs(b).max(0) + 1e-300
+        assert (np.abs(a - b) <= 1e-12 * scale).all(), (d, np.abs(a - b).max(0) / scale)
+        if p.triplet:
+            assert np.abs(b[:, 2:]).max() > 0
