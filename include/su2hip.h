@@ -225,6 +225,13 @@ int su2hip_cold_start(su2hip_batch_t *b, int slot, double phi0, double sigma0,
  * No reference counterpart (the reference only cold-starts). */
 int su2hip_hot_start(su2hip_batch_t *b, int slot);
 
+/* Two-phase start with an interface across direction dir (-1: the
+ * reference's z direction, the longest, of equal ones the last): the
+ * structure and value ranges of prepare_wall (src/z_coord.c:304-376, the
+ * MEASURE_Z build's setup_wall) with the small deviations drawn from the
+ * lattice's Philox key. */
+int su2hip_wall_start(su2hip_batch_t *b, int slot, int dir);
+
 /* ---- sweep ---------------------------------------------------------------
  * n_sweeps compound sweeps of every active lattice whose sweep budget is not
  * exhausted; one sweep is the reference's update_lattice() (src/su2.h:426,
@@ -277,6 +284,13 @@ int su2hip_measure_columns(const su2hip_params *p, const double *sums,
  * are 0.  Plane sums in FP64.  *ndist = L[dir]/2. */
 int su2hip_correlators(su2hip_batch_t *b, int slot, int dir, double *out, int max_dist,
                        int *ndist);
+
+/* Plane averages along direction dir (-1: the longest, of equal ones the
+ * last) of measure_along_z (src/z_coord.c:206-298, the MEASURE_Z build):
+ * out[z][4] = Tr Phi^+Phi/2, Tr Sigma^2/2, S, S^2 averaged over the plane
+ * (0 for absent fields).  *nrows = L[dir]. */
+int su2hip_zprofile(su2hip_batch_t *b, int slot, int dir, double *out, int max_rows,
+                    int *nrows);
 
 /* ---- multicanonical --------------------------------------------------------
  * Install the weight of a lattice (load_weight's result, src/su2.h:483,

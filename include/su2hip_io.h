@@ -37,6 +37,9 @@ typedef struct su2hip_run_config {
   char weightfile[SU2HIP_PATHLEN];
   int32_t correlators;       /* field content includes the correlator keys */
   int32_t do_correlators, correlator_interval;
+  int32_t measure_z;         /* field content includes the z-profile keys */
+  int32_t do_z_meas, setup_wall;
+  int64_t interval_z;
 } su2hip_run_config;
 
 /* Parse `path` the way get_parameters + get_weight_parameters do
@@ -44,7 +47,7 @@ typedef struct su2hip_run_config {
  * previous match and wrapping once, strtol/strtod conversion, unknown
  * algorithm names -> metropolis).  `model` overrides the config's `model` key
  * when non-NULL: '+'-separated tokens among higgs, u1, triplet, singlet,
- * gradflow, correlators; the default is the reference Makefile's build,
+ * gradflow, correlators, measure_z; the default is the reference Makefile's build,
  * higgs+singlet+u1. */
 int su2hip_config_read(const char *path, const char *model, su2hip_run_config *out);
 /* Re-read iterations, interval and checkpoint (read_updated_parameters,
@@ -87,6 +90,13 @@ int su2hip_correl_labels_write(const char *path, const su2hip_params *p);
  * "%d " and "%g " for the model's columns of su2hip_correlators' rows. */
 int su2hip_correl_append(const char *path, const su2hip_params *p, int meas_id,
                          const double *corr, int ndist);
+
+/* `labels_z` (print_z_labels, src/z_coord.c:78-106). */
+int su2hip_zprof_labels_write(const char *path, const su2hip_params *p);
+/* Append one profile to `path` as measure_along_z writes it
+ * (src/z_coord.c:265-292): per plane "%ld " and "%g " for the model's
+ * columns of su2hip_zprofile's rows. */
+int su2hip_zprof_append(const char *path, const su2hip_params *p, const double *prof, int nz);
 
 /* gsum/nsum of the SLOW weight recursion, bins+2 entries, from a
  * `weight_params` file (load_weight_params, src/multicanonical.c:291-333):

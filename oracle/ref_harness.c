@@ -80,7 +80,7 @@ int __wrap_multicanonical_acceptance(lattice const *ll, weight *ww, double o, do
 }
 
 /* which model flags this build has: bit0 NHIGGS, bit1 U1, bit2 TRIPLET,
- * bit3 SINGLET, bit4 GRADFLOW, bit5 CORRELATORS */
+ * bit3 SINGLET, bit4 GRADFLOW, bit5 CORRELATORS, bit6 MEASURE_Z */
 int ref_model(void) {
   int m = 0;
 #if (NHIGGS > 0)
@@ -100,6 +100,9 @@ int ref_model(void) {
 #endif
 #ifdef CORRELATORS
   m |= 32;
+#endif
+#ifdef MEASURE_Z
+  m |= 64;
 #endif
   return m;
 }
@@ -582,3 +585,14 @@ void ref_print_labels_correlators(void) { print_labels_correlators(); }
 #endif
 
 int ref_longest_dir(void) { return l.longest_dir; }
+
+#ifdef MEASURE_Z
+/* print_z_labels (sets p.n_meas_z) and measure_along_z into ./measure_z
+ * (src/z_coord.c:78-298); the z direction (init_z_coord, :18-24) */
+void ref_print_z_labels(void) { print_z_labels(&l, &p); }
+void ref_measure_along_z(long id) { measure_along_z(&l, &f, &p, id); }
+int ref_z_dir(void) { return l.z_dir; }
+/* prepare_wall (src/z_coord.c:304-376), its draws from the interposed
+ * uniform stream */
+void ref_prepare_wall(void) { prepare_wall(&l, &f, &p); }
+#endif

@@ -1796,3 +1796,63 @@ int orc_flow(orc_t *o, int scheme, double dt, double t_max, int meas_interval,
   free(S0);
   return nr;
 }
+
+/* prepare_wall (src/z_coord.c:304-376) with the deviations drawn from the
+ * Philox stream (pass ORC_WALL_PASS) in the reference's per-site order:
+ * doublet, triplet, singlet, then the links of every direction */
+void orc_wall_start(orc_t *o, int zdir) {
+  if (zdir < 0) zdir = orc_longest_dir(o);
+  for (long i = 0; i < o->vol; i++) {
+    long nat = o->nat[i];
+    long c[D] = {nat % o->L[0], (nat / o->L[0]) % o->L[1], nat / ((long)o->L[0] * o->L[1])};
+    int lo = c[zdir] < 0.5 * o->L[zdir];
+    ctx(o, ORC_WALL_PASS, (uint32_t)i);
+    if (o->p.nhiggs) {
+      double a = lo ? 0.2 : 1.2, e = lo ? 0.01 : 0.05;
+      double *f = PHI(o, i);
+      f[0] = a + e * dran(o);
+      f[1] = e * dran(o);
+      f[2] = e * dran(o);
+      f[3] = e * dran(o);
+    }
+    if (o->p.triplet) {
+      double a = lo ? 1.5 : 0.2, e = lo ? 0.05 : 0.01;
+      double *g = SIG(o, i);
+      g[0] = a + e * dran(o);
+      g[1] = e * dran(o);
+      g[2] = e * dran(o);
+    }
+    if (o->p.singlet) o->S[i] = (lo ? 0.8 : 0.1) + 0.01 * dran(o);
+    for (int d = 0; d < D; d++) {
+      double u = 1.0 - (lo ? 0.03 : 0.4) * dran(o);
+      double *l = LNK(o, i, d);
+      l[0] = u;
+      l[1] = sqrt((double)(1.0 - u * u));
+      l[2] = 0.0;
+      l[3] = 0.0;
+    }
+  }
+}
+
+/* measure_along_z (src/z_coord.c:206-298): plane averages along dir of
+ * doubletsq, tripletsq, S and S^2, out[z][4]; returns L[dir] */
+int orc_zprofile(const orc_t *o, int dir, double *out) {
+  if (dir < 0) dir = orc_longest_dir(o);
+  int Ld = o->L[dir];
+  double area = (double)(o->vol / Ld);
+  for (int k = 0; k < Ld * 4; k++) out[k] = 0.0;
+  for (long i = 0; i < o->vol; i++) {
+    long nat = o->nat[i];
+    long c[D] = {nat % o->L[0], (nat / o->L[0]) % o->L[1], nat / ((long)o->L[0] * o->L[1])};
+    double *r = out + (size_t)c[dir] * 4;
+    if (o->p.nhiggs) r[0] += doubletsq(PHI(o, i));
+    if (o->p.triplet) r[1] += tripletsq(SIG(o, i));
+    if (o->p.singlet) {
+      r[2] += o->S[i];
+      r[3] += o->S[i] * o->S[i];
+    }
+  }
+  for (int k = 0; k < Ld * 4; k++) out[k] /= area;
+  return Ld;
+}
+

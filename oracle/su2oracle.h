@@ -27,6 +27,7 @@ extern "C" {
 
 #define ORC_MUCA_SITE 0xFFFFFFFFu
 #define ORC_HOTSTART_PASS 0xFFFFFFFFFFFFFFFFull
+#define ORC_WALL_PASS 0xFFFFFFFFFFFFFFFEull
 
 typedef struct orc orc_t;
 
@@ -93,6 +94,10 @@ void orc_muca_get(const orc_t *o, su2hip_muca *m, double *pos, double *W,
  * Returns the number of distances. */
 int orc_correlators(const orc_t *o, int dir, double *out);
 int orc_longest_dir(const orc_t *o);
+/* Two-phase start of the MEASURE_Z build (prepare_wall) and its plane
+ * averages (measure_along_z); dir -1 = the longest direction. */
+void orc_wall_start(orc_t *o, int zdir);
+int orc_zprofile(const orc_t *o, int dir, double *out);
 
 /* gsum/nsum of the SLOW recursion (bins+2 entries; load_weight_params,
  * src/multicanonical.c:291-333, initialises them to 5 and 1). */

@@ -103,6 +103,8 @@ class RunConfig(C.Structure):
         ("resultsfile", C.c_char * 200), ("latticefile", C.c_char * 200),
         ("weightfile", C.c_char * 200), ("correlators", C.c_int32),
         ("do_correlators", C.c_int32), ("correlator_interval", C.c_int32),
+        ("measure_z", C.c_int32), ("do_z_meas", C.c_int32), ("setup_wall", C.c_int32),
+        ("interval_z", C.c_int64),
     ]
 
 
@@ -256,6 +258,8 @@ def load():
         "su2hip_muca_get": (C.c_int, [h, C.c_int, P(Muca), dp, dp, P(C.c_int32)]),
         "su2hip_muca_weight_of": (C.c_int, [h, C.c_int, C.c_double, dp]),
         "su2hip_correlators": (C.c_int, [h, C.c_int, C.c_int, dp, C.c_int, P(C.c_int)]),
+        "su2hip_zprofile": (C.c_int, [h, C.c_int, C.c_int, dp, C.c_int, P(C.c_int)]),
+        "su2hip_wall_start": (C.c_int, [h, C.c_int, C.c_int]),
         "su2hip_muca_set_sums": (C.c_int, [h, C.c_int, P(C.c_int64), P(C.c_int64)]),
         "su2hip_muca_get_sums": (C.c_int, [h, C.c_int, P(C.c_int64), P(C.c_int64)]),
         "su2hip_flow": (C.c_int, [h, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, P(C.c_int)]),
@@ -287,6 +291,8 @@ def load():
         "su2hip_labels_write": (C.c_int, [C.c_char_p, P(Params)]),
         "su2hip_correl_labels_write": (C.c_int, [C.c_char_p, P(Params)]),
         "su2hip_correl_append": (C.c_int, [C.c_char_p, P(Params), C.c_int, dp, C.c_int]),
+        "su2hip_zprof_labels_write": (C.c_int, [C.c_char_p, P(Params)]),
+        "su2hip_zprof_append": (C.c_int, [C.c_char_p, P(Params), dp, C.c_int]),
         "su2hip_measure_format": (C.c_int, [P(Params), dp, C.c_double, C.c_double, C.c_char_p, C.c_int]),
         "su2hip_print_acceptance": (C.c_int, [C.c_void_p, P(RunConfig), P(Counters)]),
         # request loop (include/su2hip_serve.h)

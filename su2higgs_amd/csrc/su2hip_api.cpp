@@ -973,6 +973,53 @@ int su2hip_hot_start(su2hip_batch_t *b, int slot) {
   return orderparam_refresh(b);
 }
 
+// the reference's z direction: the longest, of equal ones the last
+// (init_z_coord, src/z_coord.c:18-24; make_misc_tables, src/layout.c:771-777)
+static int longest_dir(const su2hip_params &p) {
+  int l = 2;
+  for (int d = 2; d >= 0; d--)
+    if (p.L[d] > p.L[l]) l = d;
+  return l;
+}
+
+int su2hip_wall_start(su2hip_batch_t *b, int slot, int dir) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (b->slots[slot].fj.active)
+    return fail(SU2HIP_ESTATE, "slot %d has a flow in flight", slot);
+  if (dir > 2) return fail(SU2HIP_EINVAL, "direction %d", dir);
+  if (dir < 0) dir = longest_dir(b->slots[slot].p);
+  rc = prepare(b);
+  if (rc) return rc;
+  HIPTRY(launch_wall(ctx_of(b), slot, dir));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return orderparam_refresh(b);
+}
+
+int su2hip_zprofile(su2hip_batch_t *b, int slot, int dir, double *out, int max_rows, int *nrows) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  const su2hip_params &p = b->slots[slot].p;
+  if (dir > 2) return fail(SU2HIP_EINVAL, "direction %d", dir);
+  if (dir < 0) dir = longest_dir(p);
+  rc = prepare(b);
+  if (rc) return rc;
+  int Ld = p.L[dir];
+  size_t V = (size_t)vol_of(p), P = V / Ld;
+  rc = ensure_stage(b, 4 * V + 4 * (size_t)Ld);
+  if (rc) return rc;
+  double *buf = b->d_stage, *planes = b->d_stage + 4 * V;
+  HIPTRY(launch_zprof(ctx_of(b), slot, dir, (int)(V / 2), Ld, P, buf, planes));
+  std::vector<double> pl(4 * (size_t)Ld);
+  HIPTRY(hipMemcpyAsync(pl.data(), planes, sizeof(double) * pl.size(), hipMemcpyDeviceToHost, b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  // plane averages (src/z_coord.c:282-292)
+  for (int z = 0; z < Ld && z < max_rows && out; z++)
+    for (int k = 0; k < 4; k++) out[(size_t)z * 4 + k] = pl[(size_t)k * Ld + z] / (double)P;
+  if (nrows) *nrows = Ld;
+  return SU2HIP_OK;
+}
+
 int su2hip_update(su2hip_batch_t *b, int n_sweeps) {
   if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
   if (n_sweeps < 0) return fail(SU2HIP_EINVAL, "n_sweeps < 0");
@@ -1329,13 +1376,7 @@ int su2hip_correlators(su2hip_batch_t *b, int slot, int dir, double *out, int ma
   int rc = check_slot(b, slot);
   if (rc) return rc;
   const su2hip_params &p = b->slots[slot].p;
-  if (dir < 0) {
-    // the reference's longest direction; of equal ones the last
-    // (make_misc_tables, src/layout.c:771-777)
-    dir = 2;
-    for (int d = 2; d >= 0; d--)
-      if (p.L[d] > p.L[dir]) dir = d;
-  }
+  if (dir < 0) dir = longest_dir(p);
   if (dir > 2) return fail(SU2HIP_EINVAL, "direction %d", dir);
   rc = prepare(b);
   if (rc) return rc;

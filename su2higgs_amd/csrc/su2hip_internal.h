@@ -27,6 +27,7 @@ constexpr int BLK = 256;          // threads per workgroup in every site kernel
 constexpr int NCOUNT = 10;        // counted pass kinds (SU2HIP_K_LINK_HEATBATH .. SINGLET_METROPOLIS)
 constexpr uint32_t MUCA_SITE = 0xFFFFFFFFu;
 constexpr uint64_t HOTSTART_PASS = 0xFFFFFFFFFFFFFFFFull;
+constexpr uint64_t WALL_PASS = 0xFFFFFFFFFFFFFFFEull;
 
 // model bits
 enum : int { M_NH = 1, M_U1 = 2, M_TR = 4, M_SG = 8 };
@@ -146,6 +147,11 @@ hipError_t launch_copy_probe(const double *src, double *dst, size_t n,
 // doubles, planes 10 Ld
 hipError_t launch_corr(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, size_t P,
                        double *buf, double *planes);
+// z-profile plane sums (4 values) of lattice `slot` along dir
+hipError_t launch_zprof(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, size_t P,
+                        double *buf, double *planes);
+// two-phase wall start of lattice `slot` with the interface across zdir
+hipError_t launch_wall(const LaunchCtx &c, int slot, int zdir);
 hipError_t launch_flow_stage(const LaunchCtx &c, int slot, const void *Uin,
                              void *Uout, const void *Gin, void *Gout, void *Z,
                              void *ZG, double dt, double msq_t, int stage);

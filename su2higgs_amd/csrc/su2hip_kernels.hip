@@ -1296,6 +1296,78 @@ __global__ __launch_bounds__(BLK) void k_corr_sites(const LatDesc *__restrict__ 
   }
 }
 
+// measure_along_z's site values (src/z_coord.c:206-263): Tr Phi^+Phi/2,
+// Tr Sigma^2/2, S, S^2, stored like k_corr_sites' values
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_zprof_sites(const LatDesc *__restrict__ lats, int slot,
+                                                     int dir, double *__restrict__ buf) {
+  const LatDesc &L = lats[slot];
+  int s = blockIdx.x * BLK + (int)threadIdx.x;
+  if (s >= L.Vh) return;
+  int Ld = dir == 0 ? L.L0 : (dir == 1 ? L.L1 : L.L2);
+  size_t P = (size_t)L.vol / Ld, V = (size_t)L.vol;
+  for (int par = 0; par < 2; par++) {
+    Site x = site_of(L, par, s);
+    int cd = dir == 0 ? x.x : (dir == 1 ? x.y : x.z);
+    size_t pos = dir == 0 ? (size_t)x.y + (size_t)L.L1 * x.z
+                          : (dir == 1 ? (size_t)x.x + (size_t)L.L0 * x.z
+                                      : (size_t)x.x + (size_t)L.L0 * x.y);
+    double *o = buf + (size_t)cd * P + pos;
+    double v[4] = {0, 0, 0, 0};
+    if constexpr ((M & M_NH) != 0) {
+      Q<double> f = qd(F_at(L, (const T *)L.F, x));
+      v[0] = 0.5 * qdot(f, f);
+    }
+    if constexpr ((M & M_TR) != 0) {
+      V3<T> g = G_at(L, (const T *)L.G, x);
+      v[1] = 0.5 * ((double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z);
+    }
+    if constexpr ((M & M_SG) != 0) {
+      double sv = S_at(L, (const T *)L.S, x);
+      v[2] = sv;
+      v[3] = sv * sv;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[(size_t)k * V] = v[k];
+  }
+}
+
+// prepare_wall (src/z_coord.c:304-376): two phases with an interface at
+// z = L[zdir]/2 in the reference's value ranges; the small deviations are
+// uniform draws of the lattice's Philox stream (pass WALL_PASS) in the
+// order the reference takes them at a site; U(1) angles are left alone
+template <typename T, int M>
+__global__ void k_wall(LatDesc L, int zdir) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.Vh) return;
+  int Lz = zdir == 0 ? L.L0 : (zdir == 1 ? L.L1 : L.L2);
+  for (int par = 0; par < 2; par++) {
+    Site x = site_of(L, par, s);
+    int zc = zdir == 0 ? x.x : (zdir == 1 ? x.y : x.z);
+    bool lo = zc < 0.5 * Lz;
+    Rng r;
+    rng_init(r, L.key0, L.key1, WALL_PASS, (uint32_t)(par * L.Vh + s));
+    if constexpr ((M & M_NH) != 0) {
+      double a = lo ? 0.2 : 1.2, e = lo ? 0.01 : 0.05;
+      double f0 = a + e * rng_u53(r), f1 = e * rng_u53(r), f2 = e * rng_u53(r), f3 = e * rng_u53(r);
+      stF((T *)L.F, L.Vh, par, s, Q<T>{(T)f0, (T)f1, (T)f2, (T)f3});
+    }
+    if constexpr ((M & M_TR) != 0) {
+      double a = lo ? 1.5 : 0.2, e = lo ? 0.05 : 0.01;
+      double g0 = a + e * rng_u53(r), g1 = e * rng_u53(r), g2 = e * rng_u53(r);
+      stG((T *)L.G, L.Vh, par, s, V3<T>{(T)g0, (T)g1, (T)g2});
+    }
+    if constexpr ((M & M_SG) != 0) {
+      double sv = (lo ? 0.8 : 0.1) + 0.01 * rng_u53(r);
+      ((T *)L.S)[(size_t)par * L.Vh + s] = (T)sv;
+    }
+    for (int d = 0; d < 3; d++) {
+      double u = 1.0 - (lo ? 0.03 : 0.4) * rng_u53(r);
+      stU((T *)L.U, L.Vh, d, par, s, Q<T>{(T)u, (T)::sqrt(1.0 - u * u), (T)0, (T)0});
+    }
+  }
+}
+
 // planes[b] = sum of the P values of (value, plane) b, in a fixed order
 __global__ __launch_bounds__(BLK) void k_corr_reduce(const double *__restrict__ buf, size_t P,
                                                      double *__restrict__ planes) {
@@ -1678,6 +1750,24 @@ hipError_t launch_corr(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, si
   return hipGetLastError();
 }
 
+hipError_t launch_zprof(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, size_t P,
+                        double *buf, double *planes) {
+  int nb = (Vh + BLK - 1) / BLK;
+  hipError_t e = with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_zprof_sites<T, M>), dim3(nb), dim3(BLK), 0, c.stream, c.desc, slot, dir,
+                         buf);
+      return hipGetLastError();
+    });
+  });
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_corr_reduce, dim3(4 * Ld), dim3(BLK), 0, c.stream, (const double *)buf, P,
+                     planes);
+  return hipGetLastError();
+}
+
 static LatDesc host_desc_of(const LaunchCtx &c, int slot, hipError_t &e) {
   LatDesc L;
   e = hipMemcpyAsync(&L, c.desc + slot, sizeof(LatDesc), hipMemcpyDeviceToHost, c.stream);
@@ -1710,6 +1800,21 @@ hipError_t launch_hot(const LaunchCtx &c, int slot) {
     return with_model(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_hot<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L);
+      return hipGetLastError();
+    });
+  });
+}
+
+hipError_t launch_wall(const LaunchCtx &c, int slot, int zdir) {
+  hipError_t e;
+  LatDesc L = host_desc_of(c, slot, e);
+  if (e != hipSuccess) return e;
+  int nb = (L.Vh + BLK - 1) / BLK;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_wall<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L, zdir);
       return hipGetLastError();
     });
   });

@@ -171,6 +171,7 @@ static int parse_model(const char *s, su2hip_run_config *c) {
   c->p.nhiggs = c->p.u1 = c->p.triplet = c->p.singlet = 0;
   c->gradflow = 0;
   c->correlators = 0;
+  c->measure_z = 0;
   char buf[MAXLINE];
   snprintf(buf, sizeof(buf), "%s", s);
   for (char *tok = strtok(buf, "+,"); tok; tok = strtok(NULL, "+,")) {
@@ -186,6 +187,8 @@ static int parse_model(const char *s, su2hip_run_config *c) {
       c->gradflow = 1;
     else if (!strcasecmp(tok, "correlators"))
       c->correlators = 1;
+    else if (!strcasecmp(tok, "measure_z"))
+      c->measure_z = 1;
     else if (!strcasecmp(tok, "su2") || !strcasecmp(tok, "gauge"))
       ;
     else
@@ -258,6 +261,7 @@ int su2hip_config_read(const char *path, const char *model, su2hip_run_config *o
   GETI(&f, "reset", c.reset);
   GETI(&f, "multicanonical", c.multicanonical);
   GETI(&f, "measure_local", c.measure_local);
+  if (c.measure_z) GETL(&f, "interval_z", c.interval_z);
   GETI(&f, "update_links", p->update_links);
   GETI(&f, "update_doublet", p->update_su2doublet);
   GETI(&f, "scalar_sweeps", p->scalar_sweeps);
@@ -301,6 +305,10 @@ int su2hip_config_read(const char *path, const char *model, su2hip_run_config *o
     GETI(&f, "flow_meas_interval", c.flow_meas_interval);
     GETD(&f, "flow_dt", c.flow_dt);
     GETD(&f, "flow_t_max", c.flow_t_max);
+  }
+  if (c.measure_z) {
+    GETI(&f, "do_z_meas", c.do_z_meas);
+    GETI(&f, "setup_wall", c.setup_wall);
   }
   /* keys of this build only (the reference ignores them) */
   p->precision = SU2HIP_FP64;
@@ -581,6 +589,39 @@ int su2hip_correl_append(const char *path, const su2hip_params *p, int meas_id,
       fprintf(f, "%g %g %g %g ", r[2], r[3], r[4], r[5]);
       fprintf(f, "%g %g %g %g ", r[6], r[7], r[8], r[9]);
     }
+    fprintf(f, "\n");
+  }
+  fclose(f);
+  return SU2HIP_OK;
+}
+
+/* print_z_labels (src/z_coord.c:78-106) */
+int su2hip_zprof_labels_write(const char *path, const su2hip_params *p) {
+  FILE *f = fopen(path, "w+");
+  if (!f) return ioerr(SU2HIP_EIO, "cannot write %s", path);
+  int k = 1;
+  fprintf(f, "%d z\n", k++);
+  if (p->nhiggs) fprintf(f, "%d phi^2\n", k++);
+  if (p->triplet) fprintf(f, "%d Sigma^2\n", k++);
+  if (p->singlet) {
+    fprintf(f, "%d S\n", k++);
+    fprintf(f, "%d S^2\n", k++);
+  }
+  fclose(f);
+  return SU2HIP_OK;
+}
+
+/* measure_along_z's file output (src/z_coord.c:265-292) */
+int su2hip_zprof_append(const char *path, const su2hip_params *p, const double *prof, int nz) {
+  if (!path || !p || (nz > 0 && !prof)) return ioerr(SU2HIP_EINVAL, "zprof_append arguments");
+  FILE *f = fopen(path, "a");
+  if (!f) return ioerr(SU2HIP_EIO, "cannot append to %s", path);
+  for (long z = 0; z < nz; z++) {
+    const double *r = prof + (size_t)z * 4;
+    fprintf(f, "%ld ", z);
+    if (p->nhiggs) fprintf(f, "%g ", r[0]);
+    if (p->triplet) fprintf(f, "%g ", r[1]);
+    if (p->singlet) fprintf(f, "%g %g ", r[2], r[3]);
     fprintf(f, "\n");
   }
   fclose(f);

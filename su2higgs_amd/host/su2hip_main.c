@@ -218,6 +218,22 @@ int main(int argc, char **argv) {
     }
     cfg.reset = 1;
   }
+  double *zprof = NULL;
+  if (cfg.measure_z) {
+    /* src/main.c:188-199: a two-phase wall overrides the start; labels_z */
+    int zd = 2;
+    for (int d = 2; d >= 0; d--)
+      if (p->L[d] > p->L[zd]) zd = d;
+    if (cfg.reset && cfg.setup_wall) {
+      TRY("wall_start", su2hip_wall_start(b, 0, zd));
+      printf("Wall profile initialized.\n");
+    }
+    if (cfg.do_z_meas) {
+      TRY("zprof_labels", su2hip_zprof_labels_write("labels_z", p));
+      printf("Measuring profiles along direction %d every %ld iterations\n", zd + 1, (long)cfg.interval_z);
+      zprof = calloc((size_t)p->L[zd] * 4, sizeof(double));
+    }
+  }
   TRY("labels", su2hip_labels_write("labels", p));
 
   double *pos = NULL, *W = NULL;
@@ -301,6 +317,12 @@ int main(int argc, char **argv) {
       TRY("measure_format", su2hip_measure_format(p, sums, weight, param, line, sizeof(line)));
       fputs(line, res);
       fflush(res);
+    }
+    if (zprof && iter % cfg.interval_z == 0) {
+      /* measure_along_z (src/main.c:308-312) */
+      int nz = 0;
+      TRY("zprofile", su2hip_zprofile(b, 0, -1, zprof, 1 << 20, &nz));
+      TRY("zprof_append", su2hip_zprof_append("measure_z", p, zprof, nz));
     }
     if (cfg.gradflow && cfg.do_flow && iter % cfg.flow_interval == 0) flow(b, &cfg, flow_id++);
     if (corr && iter % cfg.correlator_interval == 0) {

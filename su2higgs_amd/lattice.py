@@ -192,6 +192,16 @@ class Batch:
         check(self.lib.su2hip_correlators(self.h, slot, d, ptr(out), L // 2, C.byref(n)))
         return out[:n.value]
 
+    def zprofile(self, slot, d=-1):
+        L = max(self.params[slot].L)
+        out = np.zeros((L, 4))
+        n = C.c_int()
+        check(self.lib.su2hip_zprofile(self.h, slot, d, ptr(out), L, C.byref(n)))
+        return out[:n.value]
+
+    def wall_start(self, slot, d=-1):
+        check(self.lib.su2hip_wall_start(self.h, slot, d))
+
     def muca_sums(self, slot):
         n = self.muca_get(slot)[0].bins + 2
         g, s = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)

@@ -53,6 +53,18 @@ flow_t_max 0.5
 """
 
 
+def same_lines(ours, ref):
+    """measure lines byte for byte, except columns that are 0 up to rounding
+    noise whose digits depend on the summation order (the total magnetic
+    charge, a cold start's covariant singlet term)."""
+    assert len(ours) == len(ref)
+    for a, b in zip(ours, ref):
+        ta, tb = a.split(), b.split()
+        assert len(ta) == len(tb)
+        for x, y in zip(ta, tb):
+            assert x == y or (abs(float(x)) < 1e-10 and abs(float(y)) < 1e-10), (a, b)
+
+
 def run_cli(d, cfg):
     with open(os.path.join(d, "config"), "w") as f:
         f.write(cfg)
@@ -189,17 +201,52 @@ def test_cli_correlators_match_reference(tmp_path, model, tag, kw):
         r.lib.ref_print_labels_correlators()
     finally:
         os.chdir(cwd)
-    # byte for byte, except the total magnetic charge, an integer sum that
-    # is 0 up to rounding noise whose digits depend on the summation order
-    ours_m = open(os.path.join(d_ours, "measure")).read().splitlines(keepends=True)
-    assert len(ours_m) == len(lines)
-    for a, b in zip(ours_m, lines):
-        ta, tb = a.split(), b.split()
-        assert len(ta) == len(tb)
-        for x, y in zip(ta, tb):
-            assert x == y or (abs(float(x)) < 1e-10 and abs(float(y)) < 1e-10), (a, b)
+    same_lines(open(os.path.join(d_ours, "measure")).read().splitlines(keepends=True), lines)
     assert open(os.path.join(d_ours, "labels_correl")).read() == \
         open(os.path.join(d_ref, "labels_correl")).read()
     ours = open(os.path.join(d_ours, "correl0")).read()
     assert ours == open(os.path.join(d_ref, "correl0")).read()
     assert ours.count("Measurement id") == 3
+
+
+def test_cli_zprofile_matches_reference(tmp_path):
+    """`measure_z` and `labels_z` of a run with z profiles every 2nd
+    iteration against the reference's MEASURE_Z build on the same
+    trajectory (cold start; setup_wall 0)."""
+    if not orc.ref_available("hsuz"):
+        pytest.skip("reference build absent")
+    d_ours, d_ref = str(tmp_path / "ours"), str(tmp_path / "ref")
+    os.makedirs(d_ours)
+    os.makedirs(d_ref)
+    cfg = CONFIG.format(model="higgs+u1+singlet+measure_z", it=6, do_flow=0)
+    cfg += ("interval_z 2\ndo_z_meas 1\nsetup_wall 0\nupdate_singlet 1\nsinglet0 0.3\n"
+            "algorithm_singlet overrelax\nmsq_s -0.1\nb1_s 0.2\nb3_s 0.1\nb4_s 0.3\n"
+            "a1_s 0.3\na2_s 0.25\n")
+    run_cli(d_ours, cfg)
+    p = make_params(L=(4, 6, 8), nhiggs=1, u1=1, singlet=1, betasu2=8.0, betau1=9.0,
+                    msq_phi=-0.25, lambda_phi=0.02, scalar_sweeps=3, msq_s=-0.1, b1_s=0.2,
+                    b3_s=0.1, b4_s=0.3, a1_s=0.3, a2_s=0.25, seed=4242)
+    r = orc.Ref(p, measure_z=True)
+    o = orc.Oracle(p)
+    o.cold_start(phi0=1.0, singlet0=0.3)
+    r.set_fields(o.get_fields())
+    r.rng_set(p.seed, 0)
+    cwd = os.getcwd()
+    os.chdir(d_ref)
+    try:
+        r.lib.ref_print_z_labels()
+        lines = []
+        for it in range(1, 7):
+            lines.append(r.measure_text())
+            if it % 2 == 0:
+                r.lib.ref_measure_along_z(it // 2)
+            r.update(1)
+    finally:
+        os.chdir(cwd)
+    same_lines(open(os.path.join(d_ours, "measure")).read().splitlines(keepends=True), lines)
+    assert open(os.path.join(d_ours, "labels_z")).read() == open(os.path.join(d_ref, "labels_z")).read()
+    ours = open(os.path.join(d_ours, "measure_z")).read().split()
+    ref = open(os.path.join(d_ref, "measure_z")).read().split()
+    assert len(ours) == len(ref) == 3 * 8 * 4  # z, phi^2, S, S^2 per plane
+    for x, y in zip(ours, ref):
+        assert x == y or abs(float(x) - float(y)) <= 1e-5 * abs(float(y))

@@ -385,3 +385,36 @@ def test_correlators(tag, L, precision):
         assert g.shape == a.shape
         scale = np.abs(a).max(0) + 1e-300
         assert (np.abs(a - g) <= tol * scale).all(), (d, np.abs(a - g).max(0) / scale)
+
+
+@pytest.mark.parametrize("tag", ["h", "t", "hsu"])
+@pytest.mark.parametrize("precision", [FP64, abi.FP32])
+def test_wall_start_and_zprofile(tag, precision):
+    """su2hip_wall_start equals the oracle's two-phase start (same Philox
+    draws; FP32 rounds the same values), and su2hip_zprofile its plane
+    averages, along the longest direction and a given one."""
+    L = (4, 8, 6)
+    p = make_params(L=L, **MODELS[tag], precision=precision, seed=21)
+    o = orc.Oracle(make_params(L=L, **MODELS[tag], seed=21))
+    o.cold_start()
+    o.wall_start()
+    b = Batch(p)
+    b.wall_start(0)
+    fo, fg = o.get_fields(), b.get_fields(0)
+    tol = 1e-13 if precision == FP64 else 1e-7  # sqrt(1 - u^2) with a fused multiply-add
+    for k in FIELDS:
+        if fg.get(k) is not None and k != "u1link":
+            assert np.abs(fo[k] - fg[k]).max() <= tol, k
+    if precision == abi.FP32:
+        o.set_fields({k: v for k, v in fg.items() if v is not None})
+    for d in (-1, 0, 2):
+        a = o.zprofile(d)
+        g = b.zprofile(0, d)
+        assert g.shape == a.shape == (L[1 if d < 0 else d], 4)
+        assert np.abs(a - g).max() <= 1e-12 * max(1.0, np.abs(a).max())
+    # the two phases: doublet/singlet low then high, triplet the other way
+    z = b.zprofile(0)
+    if p.nhiggs:
+        assert z[0, 0] < 0.1 and z[-1, 0] > 0.5
+    if p.triplet:
+        assert z[0, 1] > 1.0 and z[-1, 1] < 0.1

@@ -360,3 +360,62 @@ def test_config_reads_correlator_keys(lib, tmp_path):
     c2 = RunConfig()
     assert lib.su2hip_config_read(path.encode(), b"higgs", C.byref(c2)) == 0
     assert c2.correlators == 0 and c2.do_correlators == 0
+
+
+ZMODELS = {"hz": dict(nhiggs=1), "tz": dict(nhiggs=0, triplet=1),
+           "hsuz": dict(nhiggs=1, u1=1, singlet=1)}
+
+
+@pytest.mark.parametrize("tag", list(ZMODELS))
+def test_zprofile_files_match_reference(lib, tmp_path, tag):
+    """labels_z and measure_z of the MEASURE_Z build: the oracle's plane
+    averages written by our writer against the reference's own output."""
+    need_ref(tag)
+    p = make_params(L=(4, 8, 6), **ZMODELS[tag], seed=12)
+    o = orc.Oracle(p)
+    o.hot_start()
+    r = orc.Ref(p, measure_z=True)
+    r.set_fields(o.get_fields())
+    os.chdir(tmp_path)
+    r.lib.ref_print_z_labels()
+    assert lib.su2hip_zprof_labels_write(b"labels_ours", C.byref(p)) == 0
+    assert open("labels_ours").read() == open("labels_z").read()
+    assert o.lib.orc_longest_dir(o.h) == r.lib.ref_z_dir() == 1
+    r.lib.ref_measure_along_z(1)
+    prof = np.ascontiguousarray(o.zprofile())
+    assert lib.su2hip_zprof_append(b"ours", C.byref(p), ptr(prof), len(prof)) == 0
+    ours, ref = open("ours").read().split("\n"), open("measure_z").read().split("\n")
+    assert len(ours) == len(ref)
+    for a, b in zip(ours, ref):
+        ta, tb = a.split(), b.split()
+        assert len(ta) == len(tb)
+        for x, y in zip(ta, tb):  # %g text; equal unless a 6th digit rounds differently
+            assert x == y or abs(float(x) - float(y)) <= 1e-5 * abs(float(y)), (a, b)
+
+
+@pytest.mark.parametrize("tag", list(ZMODELS))
+def test_wall_start_structure_matches_reference(tag):
+    """prepare_wall's phases: each site of the oracle's wall start lies within
+    the deviation of the reference's value at that site (the draws differ:
+    the reference takes them from its global stream)."""
+    need_ref(tag)
+    p = make_params(L=(4, 8, 6), **ZMODELS[tag], seed=12)
+    o = orc.Oracle(p)
+    o.cold_start()
+    r = orc.Ref(p, measure_z=True)
+    r.set_fields(o.get_fields())
+    r.lib.ref_prepare_wall()
+    o.wall_start()
+    a, b = o.get_fields(), r.get_fields()
+    have = abi.field_arrays(p)
+    dev = {"doublet": 0.05, "triplet": 0.05, "singlet": 0.01}
+    for k, e in dev.items():
+        if have[k] is not None:
+            assert np.abs(a[k] - b[k]).max() <= e + 1e-12, k
+    # links: u0 = 1 - c u with c = 0.03 / 0.4 on the two sides
+    u0a, u0b = a["su2link"][..., 0], b["su2link"][..., 0]
+    assert np.abs(u0a - u0b).max() <= 0.4 + 1e-12
+    assert np.allclose(a["su2link"][..., 2:], 0) and np.allclose(b["su2link"][..., 2:], 0)
+    assert np.allclose((a["su2link"] ** 2).sum(-1), 1)
+    if have["doublet"] is not None:  # the two phases sit on the same sides
+        assert np.array_equal(a["doublet"][:, 0] > 0.6, b["doublet"][:, 0] > 0.6)
