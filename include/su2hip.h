@@ -285,6 +285,17 @@ int su2hip_measure_columns(const su2hip_params *p, const double *sums,
 int su2hip_correlators(su2hip_batch_t *b, int slot, int dir, double *out, int max_dist,
                        int *ndist);
 
+/* Correlators on blocked lattices (the BLOCKING build: main.c:106-152,
+ * measure_blocked_correlators, src/correlation.c:393-421): every direction
+ * but the longest is halved per level, the links of those directions and the
+ * triplet smeared (smear_link, smear_triplet, src/su2u1.c:752-857), up to
+ * `levels` levels (capped as max_block_level, src/blocking.c:19-38).
+ * out[level][L/2][10] in su2hip_correlators' column order with L the
+ * longest extent.  Triplet-only models; with a doublet the reference
+ * measures nothing and *nlevels = 0. */
+int su2hip_blocked_correlators(su2hip_batch_t *b, int slot, int levels, double *out,
+                               int *nlevels, int *ndist);
+
 /* Plane averages along direction dir (-1: the longest, of equal ones the
  * last) of measure_along_z (src/z_coord.c:206-298, the MEASURE_Z build):
  * out[z][4] = Tr Phi^+Phi/2, Tr Sigma^2/2, S, S^2 averaged over the plane

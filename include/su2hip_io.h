@@ -40,6 +40,8 @@ typedef struct su2hip_run_config {
   int32_t measure_z;         /* field content includes the z-profile keys */
   int32_t do_z_meas, setup_wall;
   int64_t interval_z;
+  int32_t blocking;          /* field content includes the blocking key */
+  int32_t blocks;            /* blocking levels of the blocked correlators */
 } su2hip_run_config;
 
 /* Parse `path` the way get_parameters + get_weight_parameters do
@@ -47,7 +49,8 @@ typedef struct su2hip_run_config {
  * previous match and wrapping once, strtol/strtod conversion, unknown
  * algorithm names -> metropolis).  `model` overrides the config's `model` key
  * when non-NULL: '+'-separated tokens among higgs, u1, triplet, singlet,
- * gradflow, correlators, measure_z; the default is the reference Makefile's build,
+ * gradflow, correlators, measure_z, blocking (implies correlators); the
+ * default is the reference Makefile's build,
  * higgs+singlet+u1. */
 int su2hip_config_read(const char *path, const char *model, su2hip_run_config *out);
 /* Re-read iterations, interval and checkpoint (read_updated_parameters,

@@ -80,7 +80,8 @@ int __wrap_multicanonical_acceptance(lattice const *ll, weight *ww, double o, do
 }
 
 /* which model flags this build has: bit0 NHIGGS, bit1 U1, bit2 TRIPLET,
- * bit3 SINGLET, bit4 GRADFLOW, bit5 CORRELATORS, bit6 MEASURE_Z */
+ * bit3 SINGLET, bit4 GRADFLOW, bit5 CORRELATORS, bit6 MEASURE_Z,
+ * bit7 BLOCKING */
 int ref_model(void) {
   int m = 0;
 #if (NHIGGS > 0)
@@ -103,6 +104,9 @@ int ref_model(void) {
 #endif
 #ifdef MEASURE_Z
   m |= 64;
+#endif
+#ifdef BLOCKING
+  m |= 128;
 #endif
   return m;
 }
@@ -595,4 +599,79 @@ int ref_z_dir(void) { return l.z_dir; }
 /* prepare_wall (src/z_coord.c:304-376), its draws from the interposed
  * uniform stream */
 void ref_prepare_wall(void) { prepare_wall(&l, &f, &p); }
+#endif
+
+#ifdef BLOCKING
+/* The blocked lattices of main() (src/main.c:106-152): blocking in every
+ * direction but the longest, up to `levels` (capped at max_block_level). */
+static lattice *g_bl;
+static fields *g_bf;
+static int g_nbl, g_bdir[3];
+
+int ref_blocked_setup(int levels) {
+  for (int d = 0; d < l.dim; d++) g_bdir[d] = d != l.longest_dir;
+  l.blocklist.sends = 0;
+  l.blocklist.recvs = 0;
+  realloc_comlist(&l.blocklist, RECV);
+  l.standby = 0;
+  l.blocking_level = 0;
+  int max = max_block_level(&l, g_bdir);
+  if (levels > max) levels = max;
+  g_bl = calloc(levels > 0 ? levels : 1, sizeof(lattice));
+  g_bf = calloc(levels > 0 ? levels : 1, sizeof(fields));
+  for (int k = 0; k < levels; k++) {
+    lattice *base = k ? &g_bl[k - 1] : &l;
+    g_bl[k].blocklist.sends = 0;
+    g_bl[k].blocklist.recvs = 0;
+    block_lattice(base, &g_bl[k], g_bdir);
+    alloc_fields(&g_bl[k], &g_bf[k]);
+    g_bl[k].blocking_level = k + 1;
+  }
+  g_nbl = levels;
+  return levels;
+}
+
+/* the numbers measure_blocked_correlators prints (src/correlation.c:393-421):
+ * smear_fields + make_blocked_fields into each level, then the correlator
+ * columns of ref_correlators on the blocked lattice, out[level][L/2][10] */
+int ref_blocked_values(double *out) {
+  int dir = l.longest_dir, maxd = l.L[dir] / 2;
+  for (int k = 0; k < g_nbl; k++) {
+    lattice *base = k ? &g_bl[k - 1] : &l;
+    fields *fb = k ? &g_bf[k - 1] : &f;
+    fields sm;
+    alloc_fields(base, &sm);
+    smear_fields(base, fb, &sm, g_bdir);
+    make_blocked_fields(base, &g_bl[k], &sm, &g_bf[k]);
+    free_fields(base, &sm);
+    lattice *b = &g_bl[k];
+    for (int d = 0; d < maxd; d++) {
+      double *o = out + ((size_t)k * maxd + d) * 10;
+      for (int q = 0; q < 10; q++) o[q] = 0.0;
+      o[1] = triplet_correlator(b, &g_bf[k], d, dir);
+      complex c;
+      c = photon_correl(b, &g_bf[k], &p, d, dir, 1, 0);
+      o[2] = c.re;
+      o[3] = c.im;
+      c = photon_correl(b, &g_bf[k], &p, d, dir, 2, 0);
+      o[4] = c.re;
+      o[5] = c.im;
+      c = photon_correl(b, &g_bf[k], &p, d, dir, 1, 1);
+      o[6] = c.re;
+      o[7] = c.im;
+      c = photon_correl(b, &g_bf[k], &p, d, dir, 2, 1);
+      o[8] = c.re;
+      o[9] = c.im;
+    }
+  }
+  return g_nbl;
+}
+
+/* measure_blocked_correlators for every level (src/main.c:334-347): files
+ * correl1, correl2, ... in the working directory */
+void ref_blocked_correlators(int id) {
+  for (int k = 0; k < g_nbl; k++)
+    measure_blocked_correlators(k ? &g_bl[k - 1] : &l, &g_bl[k], k ? &g_bf[k - 1] : &f, &g_bf[k],
+                                &p, g_bdir, l.longest_dir, id);
+}
 #endif

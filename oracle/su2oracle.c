@@ -1856,3 +1856,127 @@ int orc_zprofile(const orc_t *o, int dir, double *out) {
   return Ld;
 }
 
+
+/* ------------------------------------------------------------------ */
+/* blocking (the reference's BLOCKING build, src/blocking.c, su2u1.c:736-895) */
+
+/* max_block_level (src/blocking.c:19-38) */
+static int max_block_level(const int *L, const int *bdir) {
+  int count = 0, len[D];
+  for (int d = 0; d < D; d++) len[d] = L[d];
+  for (;;) {
+    for (int d = 0; d < D; d++) {
+      if (bdir[d] && (len[d] % 2 != 0 || len[d] <= 2)) return count;
+      if (bdir[d]) len[d] /= 2;
+    }
+    count++;
+  }
+}
+
+/* su2staple_wilson_onedir with dagger = 1 (src/staples.c:197-234): the
+ * conjugates of the upper and lower staples of U_mu(x) in the nu plane */
+static void staple_onedir_dag(const orc_t *o, long i, int mu, int nu, double *res) {
+  double a[4], b[4], t[4], up[4], lo[4];
+  /* U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ */
+  qconj(LNK(o, NX(o, i, nu), mu), a);
+  qconj(LNK(o, i, nu), b);
+  qmul(a, b, t);
+  qmul(LNK(o, NX(o, i, mu), nu), t, up);
+  /* U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu) */
+  long s = PV(o, NX(o, i, mu), nu);
+  qconj(LNK(o, s, nu), a);
+  qconj(LNK(o, PV(o, i, nu), mu), b);
+  qmul(b, LNK(o, PV(o, i, nu), nu), t);
+  qmul(a, t, lo);
+  res[0] = up[0] + lo[0];
+  for (int k = 1; k < 4; k++) res[k] = -up[k] + -lo[k];
+}
+
+/* smear_link (src/su2u1.c:752-797) */
+static void smear_link(const orc_t *o, const int *bdir, long i, int dir, double *res) {
+  double v2[4], st[4];
+  memcpy(res, LNK(o, i, dir), sizeof(double) * 4);
+  memcpy(v2, LNK(o, NX(o, i, dir), dir), sizeof(double) * 4);
+  int paths = 1;
+  for (int j = 0; j < D; j++) {
+    if (j == dir || !bdir[j]) continue;
+    staple_onedir_dag(o, i, dir, j, st);
+    for (int k = 0; k < 4; k++) res[k] += st[k];
+    staple_onedir_dag(o, NX(o, i, dir), dir, j, st);
+    for (int k = 0; k < 4; k++) v2[k] += st[k];
+    paths += 2;
+  }
+  for (int k = 0; k < 4; k++) {
+    res[k] /= (double)paths;
+    v2[k] /= (double)paths;
+  }
+  double r[4];
+  qmul(res, v2, r);
+  double det = sqrt(qsq(r));
+  for (int k = 0; k < 4; k++) res[k] = r[k] / det;
+}
+
+/* smear_triplet (src/su2u1.c:810-857): Sigma(x) + sum over the smearing
+ * directions of U_j(x) Sigma(x+j) U_j(x)^+ + U_j(x-j)^+ Sigma(x-j) U_j(x-j),
+ * over the number of sites */
+static void smear_triplet(const orc_t *o, const int *bdir, long i, double *res) {
+  int sites = 1;
+  double cov[3] = {0, 0, 0}, r[3], uc[4];
+  for (int d = 0; d < D; d++) {
+    if (!bdir[d]) continue;
+    rotate(LNK(o, i, d), SIG(o, NX(o, i, d)), r);
+    for (int k = 0; k < 3; k++) cov[k] += r[k];
+    long pv = PV(o, i, d);
+    qconj(LNK(o, pv, d), uc);
+    rotate(uc, SIG(o, pv), r);
+    for (int k = 0; k < 3; k++) cov[k] += r[k];
+    sites += 2;
+  }
+  for (int k = 0; k < 3; k++) res[k] = (SIG(o, i)[k] + cov[k]) / (double)sites;
+}
+
+/* Correlators on the blocked lattices (measure_blocked_correlators,
+ * src/correlation.c:393-421, with the blocking of main, src/main.c:106-152
+ * and smear_fields/make_blocked_fields): block every direction but the
+ * longest, level after level, smearing the triplet and the links of the
+ * blocked directions.  out[level][L[dir]/2][10]; returns the number of
+ * levels (at most `levels`, capped like max_block_level).  Only triplet
+ * models: with a doublet the reference measures nothing (returns 0). */
+int orc_blocked_correlators(const orc_t *o, int levels, double *out) {
+  if (o->p.nhiggs || !o->p.triplet) return 0;
+  int dir = orc_longest_dir(o);
+  int bdir[D];
+  for (int d = 0; d < D; d++) bdir[d] = d != dir;
+  int max = max_block_level(o->L, bdir);
+  if (levels > max) levels = max;
+  int maxd = o->L[dir] / 2;
+  const orc_t *base = o;
+  orc_t *prev = NULL;
+  for (int k = 0; k < levels; k++) {
+    su2hip_params pb = base->p;
+    for (int d = 0; d < D; d++) pb.L[d] = bdir[d] ? base->L[d] / 2 : base->L[d];
+    orc_t *b = orc_create(&pb);
+    for (long ib = 0; ib < b->vol; ib++) {
+      long nb = b->nat[ib];
+      long cb[D] = {nb % b->L[0], (nb / b->L[0]) % b->L[1], nb / ((long)b->L[0] * b->L[1])};
+      long c[D];
+      for (int d = 0; d < D; d++) c[d] = bdir[d] ? 2 * cb[d] : cb[d];
+      long nat = c[0] + base->L[0] * (c[1] + (long)base->L[1] * c[2]);
+      long i = 0;
+      while (base->nat[i] != nat) i++; /* the fine site (small lattices only) */
+      for (int d = 0; d < D; d++) {
+        if (bdir[d])
+          smear_link(base, bdir, i, d, LNK(b, ib, d));
+        else
+          memcpy(LNK(b, ib, d), LNK(base, i, d), sizeof(double) * 4);
+      }
+      smear_triplet(base, bdir, i, SIG(b, ib));
+    }
+    orc_correlators(b, dir, out + (size_t)k * maxd * 10);
+    if (prev) orc_destroy(prev);
+    prev = b;
+    base = b;
+  }
+  if (prev) orc_destroy(prev);
+  return levels;
+}

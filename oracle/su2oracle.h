@@ -97,6 +97,10 @@ int orc_longest_dir(const orc_t *o);
 /* Two-phase start of the MEASURE_Z build (prepare_wall) and its plane
  * averages (measure_along_z); dir -1 = the longest direction. */
 void orc_wall_start(orc_t *o, int zdir);
+/* Correlators on the blocked lattices of the BLOCKING build, out[level][
+ * L[longest]/2][10] as orc_correlators; returns the number of levels (0
+ * with a doublet, where the reference measures nothing). */
+int orc_blocked_correlators(const orc_t *o, int levels, double *out);
 int orc_zprofile(const orc_t *o, int dir, double *out);
 
 /* gsum/nsum of the SLOW recursion (bins+2 entries; load_weight_params,

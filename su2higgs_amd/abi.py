@@ -104,7 +104,7 @@ class RunConfig(C.Structure):
         ("weightfile", C.c_char * 200), ("correlators", C.c_int32),
         ("do_correlators", C.c_int32), ("correlator_interval", C.c_int32),
         ("measure_z", C.c_int32), ("do_z_meas", C.c_int32), ("setup_wall", C.c_int32),
-        ("interval_z", C.c_int64),
+        ("interval_z", C.c_int64), ("blocking", C.c_int32), ("blocks", C.c_int32),
     ]
 
 
@@ -259,6 +259,7 @@ def load():
         "su2hip_muca_weight_of": (C.c_int, [h, C.c_int, C.c_double, dp]),
         "su2hip_correlators": (C.c_int, [h, C.c_int, C.c_int, dp, C.c_int, P(C.c_int)]),
         "su2hip_zprofile": (C.c_int, [h, C.c_int, C.c_int, dp, C.c_int, P(C.c_int)]),
+        "su2hip_blocked_correlators": (C.c_int, [h, C.c_int, C.c_int, dp, P(C.c_int), P(C.c_int)]),
         "su2hip_wall_start": (C.c_int, [h, C.c_int, C.c_int]),
         "su2hip_muca_set_sums": (C.c_int, [h, C.c_int, P(C.c_int64), P(C.c_int64)]),
         "su2hip_muca_get_sums": (C.c_int, [h, C.c_int, P(C.c_int64), P(C.c_int64)]),

@@ -1415,6 +1415,116 @@ int su2hip_correlators(su2hip_batch_t *b, int slot, int dir, double *out, int ma
   return SU2HIP_OK;
 }
 
+// blocked lattice of `base` in the directions of bmask, fields at U/G
+static LatDesc blocked_desc(const LatDesc &base, int bmask, void *U, void *G) {
+  LatDesc B = base;
+  B.L0 = (bmask & 1) ? base.L0 / 2 : base.L0;
+  B.L1 = (bmask & 2) ? base.L1 / 2 : base.L1;
+  B.L2 = (bmask & 4) ? base.L2 / 2 : base.L2;
+  B.h0 = B.L0 / 2;
+  B.vol = B.L0 * B.L1 * B.L2;
+  B.Vh = B.vol / 2;
+  B.div_h0 = make_fastdiv((uint32_t)B.h0);
+  B.div_L1 = make_fastdiv((uint32_t)B.L1);
+  B.U = U;
+  B.G = G;
+  B.F = B.A = B.S = B.B = nullptr;
+  B.blk_base = 0;
+  B.nblk = (B.Vh + BLK - 1) / BLK;
+  B.active = 1;
+  return B;
+}
+
+int su2hip_blocked_correlators(su2hip_batch_t *b, int slot, int levels, double *out,
+                               int *nlevels, int *ndist) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  const su2hip_params &p = b->slots[slot].p;
+  int dir = longest_dir(p), maxd = p.L[dir] / 2;
+  if (nlevels) *nlevels = 0;
+  if (ndist) *ndist = maxd;
+  // measure_blocked_correlators smears the triplet only and measures
+  // nothing with a doublet (src/correlation.c:403-408)
+  if (p.nhiggs || !p.triplet || levels <= 0) return SU2HIP_OK;
+  int bmask = 7 & ~(1 << dir);
+  // max_block_level (src/blocking.c:19-38)
+  int len[3] = {p.L[0], p.L[1], p.L[2]}, maxl = 0;
+  for (bool ok = true; ok;) {
+    for (int d = 0; d < 3 && ok; d++)
+      if ((bmask & (1 << d)) && (len[d] % 2 != 0 || len[d] <= 2)) ok = false;
+    if (!ok) break;
+    for (int d = 0; d < 3; d++)
+      if (bmask & (1 << d)) len[d] /= 2;
+    maxl++;
+  }
+  if (levels > maxl) levels = maxl;
+  if (levels == 0) return SU2HIP_OK;
+  rc = prepare(b);
+  if (rc) return rc;
+  rc = pull_desc(b);
+  if (rc) return rc;
+  LatDesc fine = b->slots[slot].hd;
+  size_t w = wsize(p), Vh1 = (size_t)fine.Vh / 4;  // first level: a quarter of the sites
+  size_t words = align256(24 * Vh1 * w) + align256(6 * Vh1 * w);
+  char *mem = nullptr;
+  LatDesc *ddesc = nullptr;
+  if (hipMalloc(&mem, 2 * words) != hipSuccess) return fail(SU2HIP_ENOMEM, "blocked fields");
+  if (hipMalloc(&ddesc, sizeof(LatDesc)) != hipSuccess) {
+    (void)hipFree(mem);
+    return fail(SU2HIP_ENOMEM, "blocked descriptor");
+  }
+  hipError_t e = hipSuccess;
+  LatDesc base = fine;
+  for (int k = 0; k < levels && e == hipSuccess; k++) {
+    char *m = mem + (size_t)(k & 1) * words;
+    LatDesc B = blocked_desc(base, bmask, m, m + align256(24 * Vh1 * w));
+    e = launch_block(ctx_of(b), base, B, bmask);
+    if (e == hipSuccess) e = hipMemcpyAsync(ddesc, &B, sizeof(LatDesc), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) break;
+    // correlators on the blocked lattice (su2hip_correlators' plane sums)
+    size_t V = (size_t)B.vol, P = V / p.L[dir];
+    if (ensure_stage(b, 10 * V + 10 * (size_t)p.L[dir])) {
+      e = hipErrorOutOfMemory;
+      break;
+    }
+    LaunchCtx c = ctx_of(b);
+    c.desc = ddesc;
+    double *buf = b->d_stage, *planes = b->d_stage + 10 * V;
+    e = launch_corr(c, 0, dir, B.Vh, p.L[dir], P, buf, planes);
+    std::vector<double> pl(10 * (size_t)p.L[dir]);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(pl.data(), planes, sizeof(double) * pl.size(), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) break;
+    int Ld = p.L[dir];
+    for (int d = 0; d < maxd && out; d++) {
+      double r[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int z = 0; z < Ld; z++) {
+        int zd = (z + d) % Ld;
+        auto a = [&](int q) { return pl[(size_t)q * Ld + z]; };
+        auto cc = [&](int q) { return pl[(size_t)q * Ld + zd]; };
+        r[1] += a(1) * cc(1);
+        for (int q = 2; q < 10; q += 2) {
+          double br = cc(q), bi = -cc(q + 1);
+          r[q] += a(q) * br - a(q + 1) * bi;
+          r[q + 1] += br * a(q + 1) + a(q) * bi;
+        }
+      }
+      double *o = out + ((size_t)k * maxd + d) * 10;
+      o[0] = 0.0;
+      for (int q = 1; q < 10; q++) o[q] = r[q] / (double)V;
+    }
+    base = B;
+  }
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(ddesc);
+  (void)hipFree(mem);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "blocked correlators: %s", hipGetErrorString(e));
+  if (nlevels) *nlevels = levels;
+  return SU2HIP_OK;
+}
+
 int su2hip_muca_set_sums(su2hip_batch_t *b, int slot, const int64_t *gsum, const int64_t *nsum) {
   int rc = check_slot(b, slot);
   if (rc) return rc;

@@ -150,6 +150,9 @@ hipError_t launch_corr(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, si
 // z-profile plane sums (4 values) of lattice `slot` along dir
 hipError_t launch_zprof(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, size_t P,
                         double *buf, double *planes);
+// one blocking level: smeared fields of `fine` into `blocked` (bmask: the
+// blocked directions), both descriptors by value
+hipError_t launch_block(const LaunchCtx &c, const LatDesc &fine, const LatDesc &blocked, int bmask);
 // two-phase wall start of lattice `slot` with the interface across zdir
 hipError_t launch_wall(const LaunchCtx &c, int slot, int zdir);
 hipError_t launch_flow_stage(const LaunchCtx &c, int slot, const void *Uin,

@@ -1368,6 +1368,87 @@ __global__ void k_wall(LatDesc L, int zdir) {
   }
 }
 
+// ------------------------------------------------------------------------
+// blocking (the reference's BLOCKING build, src/blocking.c, su2u1.c:736-895)
+
+// the site at coordinates (x, y, z)
+__device__ __forceinline__ Site site_at(const LatDesc &L, int x, int y, int z) {
+  Site c;
+  c.x = x;
+  c.y = y;
+  c.z = z;
+  c.par = (x + y + z) & 1;
+  c.s = (x + L.L0 * (y + L.L1 * z)) >> 1;
+  return c;
+}
+
+// su2staple_wilson_onedir with dagger = 1 (src/staples.c:197-234)
+template <typename T>
+__device__ Q<double> staple_onedir_dag(const LatDesc &L, const T *U, const Site &x, int mu, int nu) {
+  auto u = [&](const Site &c, int d) { return qd(U_at(L, U, c, d)); };
+  Site xp = shift(L, x, mu, +1), xn = shift(L, x, nu, +1), xm = shift(L, x, nu, -1);
+  Site xpm = shift(L, xp, nu, -1);
+  Q<double> up = qmul(u(xp, nu), qmul(qconj(u(xn, mu)), qconj(u(x, nu))));
+  Q<double> lo = qmul(qconj(u(xpm, nu)), qmul(qconj(u(xm, mu)), u(xm, nu)));
+  return Q<double>{up.a + lo.a, -up.b + -lo.b, -up.c + -lo.c, -up.d + -lo.d};
+}
+
+// smear_fields + make_blocked_fields (src/su2u1.c:752-895, blocking.c:566-596)
+// for one level: each blocked site takes the smeared links (blocked
+// directions) or the plain links, and the smeared triplet, of the fine site
+// at twice its coordinates in the blocked directions
+template <typename T>
+__global__ void k_block(LatDesc F, LatDesc B, int bmask) {
+  int sb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sb >= B.Vh) return;
+  const T *U = (const T *)F.U, *G = (const T *)F.G;
+  for (int par = 0; par < 2; par++) {
+    Site cb = site_of(B, par, sb);
+    Site x = site_at(F, (bmask & 1) ? 2 * cb.x : cb.x, (bmask & 2) ? 2 * cb.y : cb.y,
+                     (bmask & 4) ? 2 * cb.z : cb.z);
+    int paths = 1 + 2 * (__popc(bmask) - 1);
+    for (int d = 0; d < 3; d++) {
+      Q<double> r = qd(U_at(F, U, x, d));
+      if (bmask & (1 << d)) {
+        // smear_link (src/su2u1.c:752-797)
+        Site xd = shift(F, x, d, +1);
+        Q<double> v2 = qd(U_at(F, U, xd, d));
+        for (int j = 0; j < 3; j++) {
+          if (j == d || !(bmask & (1 << j))) continue;
+          r = qadd(r, staple_onedir_dag(F, U, x, d, j));
+          v2 = qadd(v2, staple_onedir_dag(F, U, xd, d, j));
+        }
+        r = Q<double>{r.a / paths, r.b / paths, r.c / paths, r.d / paths};
+        v2 = Q<double>{v2.a / paths, v2.b / paths, v2.c / paths, v2.d / paths};
+        r = qmul(r, v2);
+        double det = ::sqrt(qdot(r, r));
+        r = Q<double>{r.a / det, r.b / det, r.c / det, r.d / det};
+      }
+      stU((T *)B.U, B.Vh, d, par, sb, Q<T>{(T)r.a, (T)r.b, (T)r.c, (T)r.d});
+    }
+    // smear_triplet (src/su2u1.c:810-857)
+    V3<double> g0 = [&] { V3<T> t = G_at(F, G, x); return V3<double>{t.x, t.y, t.z}; }();
+    V3<double> cov{0, 0, 0};
+    int sites = 1;
+    for (int d = 0; d < 3; d++) {
+      if (!(bmask & (1 << d))) continue;
+      Site xn = shift(F, x, d, +1), xm = shift(F, x, d, -1);
+      V3<T> a = G_at(F, G, xn), b = G_at(F, G, xm);
+      V3<double> r1 = rotate(qd(U_at(F, U, x, d)), V3<double>{a.x, a.y, a.z});
+      V3<double> r2 = rotate(qconj(qd(U_at(F, U, xm, d))), V3<double>{b.x, b.y, b.z});
+      cov.x += r1.x;
+      cov.y += r1.y;
+      cov.z += r1.z;
+      cov.x += r2.x;
+      cov.y += r2.y;
+      cov.z += r2.z;
+      sites += 2;
+    }
+    stG((T *)B.G, B.Vh, par, sb,
+        V3<T>{(T)((g0.x + cov.x) / sites), (T)((g0.y + cov.y) / sites), (T)((g0.z + cov.z) / sites)});
+  }
+}
+
 // planes[b] = sum of the P values of (value, plane) b, in a fixed order
 __global__ __launch_bounds__(BLK) void k_corr_reduce(const double *__restrict__ buf, size_t P,
                                                      double *__restrict__ planes) {
@@ -1748,6 +1829,15 @@ hipError_t launch_corr(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, si
   hipLaunchKernelGGL(k_corr_reduce, dim3(10 * Ld), dim3(BLK), 0, c.stream, (const double *)buf, P,
                      planes);
   return hipGetLastError();
+}
+
+hipError_t launch_block(const LaunchCtx &c, const LatDesc &fine, const LatDesc &blocked, int bmask) {
+  int nb = (blocked.Vh + BLK - 1) / BLK;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    hipLaunchKernelGGL((k_block<T>), dim3(nb), dim3(BLK), 0, c.stream, fine, blocked, bmask);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_zprof(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, size_t P,

@@ -172,6 +172,7 @@ static int parse_model(const char *s, su2hip_run_config *c) {
   c->gradflow = 0;
   c->correlators = 0;
   c->measure_z = 0;
+  c->blocking = 0;
   char buf[MAXLINE];
   snprintf(buf, sizeof(buf), "%s", s);
   for (char *tok = strtok(buf, "+,"); tok; tok = strtok(NULL, "+,")) {
@@ -189,6 +190,8 @@ static int parse_model(const char *s, su2hip_run_config *c) {
       c->correlators = 1;
     else if (!strcasecmp(tok, "measure_z"))
       c->measure_z = 1;
+    else if (!strcasecmp(tok, "blocking"))
+      c->blocking = c->correlators = 1;
     else if (!strcasecmp(tok, "su2") || !strcasecmp(tok, "gauge"))
       ;
     else
@@ -298,6 +301,7 @@ int su2hip_config_read(const char *path, const char *model, su2hip_run_config *o
     GETI(&f, "do_correlators", c.do_correlators);
     GETI(&f, "correlator_interval", c.correlator_interval);
   }
+  if (c.blocking) GETI(&f, "blocks", c.blocks);
   c.flow_scheme = SU2HIP_FLOW_EULER;
   if (c.gradflow) {
     GETI(&f, "do_flow", c.do_flow);

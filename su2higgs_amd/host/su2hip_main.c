@@ -306,7 +306,8 @@ int main(int argc, char **argv) {
       if (p->L[d] > p->L[ld]) ld = d;
     printf("\nMeasuring correlation functions along direction %d every %d iterations\n", ld + 1,
            cfg.correlator_interval);
-    corr = calloc((size_t)(p->L[ld] / 2 + 1) * 10, sizeof(double));
+    corr = calloc((size_t)(p->L[ld] / 2 + 1) * 10 * (size_t)(cfg.blocks > 1 ? cfg.blocks : 1),
+                  sizeof(double));
   }
   char line[4096];
   while (iter <= cfg.iterations) {
@@ -329,7 +330,23 @@ int main(int argc, char **argv) {
       /* measure_correlators("correl0", ...) (src/main.c:330-349) */
       int nd = 0;
       TRY("correlators", su2hip_correlators(b, 0, -1, corr, 1 << 20, &nd));
-      TRY("correl_append", su2hip_correl_append("correl0", p, correlator_id++, corr, nd));
+      TRY("correl_append", su2hip_correl_append("correl0", p, correlator_id, corr, nd));
+      if (cfg.blocking && cfg.blocks > 0) {
+        /* the blocked lattices (src/main.c:334-347), files correl1, correl2, ... */
+        if (p->nhiggs) {
+          for (int k = 0; k < cfg.blocks; k++)
+            printf("Error in correlation.c: Higgs smearing not yet implemented!!!\n");
+        } else {
+          int nl = 0;
+          TRY("blocked_correlators", su2hip_blocked_correlators(b, 0, cfg.blocks, corr, &nl, &nd));
+          for (int k = 0; k < nl; k++) {
+            char fn[32];
+            snprintf(fn, sizeof(fn), "correl%d", k + 1);
+            TRY("correl_append", su2hip_correl_append(fn, p, correlator_id, corr + (size_t)k * nd * 10, nd));
+          }
+        }
+      }
+      correlator_id++;
     }
     TRY("update", su2hip_update(b, 1));
     if (iter % cfg.checkpoint == 0) {

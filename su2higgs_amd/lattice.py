@@ -192,6 +192,13 @@ class Batch:
         check(self.lib.su2hip_correlators(self.h, slot, d, ptr(out), L // 2, C.byref(n)))
         return out[:n.value]
 
+    def blocked_correlators(self, slot, levels):
+        L = max(self.params[slot].L)
+        nl, nd = C.c_int(), C.c_int()
+        buf = np.zeros(max(levels, 1) * (L // 2 + 1) * 10)
+        check(self.lib.su2hip_blocked_correlators(self.h, slot, levels, ptr(buf), C.byref(nl), C.byref(nd)))
+        return buf[:nl.value * nd.value * 10].reshape(nl.value, nd.value, 10)
+
     def zprofile(self, slot, d=-1):
         L = max(self.params[slot].L)
         out = np.zeros((L, 4))

@@ -63,6 +63,7 @@ def oracle_lib():
             "orc_longest_dir": (C.c_int, [h]),
             "orc_zprofile": (C.c_int, [h, C.c_int, dp]),
             "orc_wall_start": (None, [h, C.c_int]),
+            "orc_blocked_correlators": (C.c_int, [h, C.c_int, dp]),
             "orc_muca_get_sums": (None, [h, P(C.c_int64), P(C.c_int64)]),
             "orc_muca_weight": (C.c_double, [h, C.c_double]),
             "orc_muca_whichbin": (C.c_int, [h, C.c_double]),
@@ -195,6 +196,12 @@ class Oracle:
     def wall_start(self, d=-1):
         self.lib.orc_wall_start(self.h, d)
 
+    def blocked_correlators(self, levels):
+        L = self.p.L[self.lib.orc_longest_dir(self.h)]
+        out = np.zeros((max(levels, 1), L // 2, 10))
+        n = self.lib.orc_blocked_correlators(self.h, levels, ptr(out))
+        return out[:n]
+
     def muca_sums(self):
         n = self.muca_get()[0].bins + 2
         g, s = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
@@ -222,13 +229,13 @@ class Oracle:
 _refs = {}
 
 REF_FLAGS = {1: "h", 3: "hu", 4: "t", 5: "ht", 17: "hg", 20: "tg", 11: "hsu",
-             33: "hc", 36: "tc", 37: "htc", 65: "hz", 68: "tz", 75: "hsuz"}
+             33: "hc", 36: "tc", 37: "htc", 65: "hz", 68: "tz", 75: "hsuz", 164: "tcb"}
 
 
-def ref_tag(p, gradflow=False, correlators=False, measure_z=False):
+def ref_tag(p, gradflow=False, correlators=False, measure_z=False, blocking=False):
     m = (1 if p.nhiggs else 0) | (2 if p.u1 else 0) | (4 if p.triplet else 0) \
         | (8 if p.singlet else 0) | (16 if gradflow else 0) | (32 if correlators else 0) \
-        | (64 if measure_z else 0)
+        | (64 if measure_z else 0) | (128 if blocking else 0)
     return REF_FLAGS.get(m)
 
 
@@ -291,6 +298,9 @@ def ref_lib(tag):
         "ref_measure_along_z": (None, [C.c_long]),
         "ref_z_dir": (C.c_int, []),
         "ref_prepare_wall": (None, []),
+        "ref_blocked_setup": (C.c_int, [C.c_int]),
+        "ref_blocked_correlators": (None, [C.c_int]),
+        "ref_blocked_values": (C.c_int, [dp]),
     }
     for n, (r, a) in sig.items():
         fn = getattr(lib, n)
@@ -308,8 +318,8 @@ def ref_lib(tag):
 class Ref:
     """The reference compiled from source, one global lattice per tag."""
 
-    def __init__(self, p, gradflow=False, correlators=False, measure_z=False):
-        self.tag = ref_tag(p, gradflow, correlators, measure_z)
+    def __init__(self, p, gradflow=False, correlators=False, measure_z=False, blocking=False):
+        self.tag = ref_tag(p, gradflow, correlators or blocking, measure_z, blocking)
         self.lib = ref_lib(self.tag)
         self.p = p
         self.V = volume(p)

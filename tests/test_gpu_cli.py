@@ -250,3 +250,47 @@ def test_cli_zprofile_matches_reference(tmp_path):
     assert len(ours) == len(ref) == 3 * 8 * 4  # z, phi^2, S, S^2 per plane
     for x, y in zip(ours, ref):
         assert x == y or abs(float(x) - float(y)) <= 1e-5 * abs(float(y))
+
+
+def test_cli_blocked_correlators_match_reference(tmp_path):
+    """`correl0`, `correl1`, `correl2` of a triplet run with blocking (blocks
+    3, capped at 2 levels for 8 x 8 x 12) against the reference's BLOCKING
+    build on the same trajectory."""
+    if not orc.ref_available("tcb"):
+        pytest.skip("reference build absent")
+    d_ours, d_ref = str(tmp_path / "ours"), str(tmp_path / "ref")
+    os.makedirs(d_ours)
+    os.makedirs(d_ref)
+    cfg = CONFIG.format(model="triplet+blocking", it=4, do_flow=0).replace("L1 4\n", "L1 8\n")
+    cfg = cfg.replace("L2 6\n", "L2 8\n").replace("L3 8\n", "L3 12\n")
+    cfg += ("do_correlators 1\ncorrelator_interval 2\nblocks 3\nmsq_triplet -0.3\nb4 0.1\n"
+            "sigma0 0.4\nalgorithm_su2triplet overrelax\nupdate_triplet 1\n")
+    run_cli(d_ours, cfg)
+    p = make_params(L=(8, 8, 12), nhiggs=0, triplet=1, betasu2=8.0, scalar_sweeps=3, msq_triplet=-0.3,
+                    b4=0.1, seed=4242)
+    r = orc.Ref(p, blocking=True)
+    o = orc.Oracle(p)
+    o.cold_start(sigma0=0.4)
+    r.set_fields(o.get_fields())
+    r.rng_set(p.seed, 0)
+    cwd = os.getcwd()
+    os.chdir(d_ref)
+    try:
+        assert r.lib.ref_blocked_setup(3) == 2
+        for it in range(1, 5):
+            if it % 2 == 0:
+                r.lib.ref_measure_correlators(b"correl0", r.lib.ref_longest_dir(), it // 2)
+                r.lib.ref_blocked_correlators(it // 2)
+            r.update(1)
+    finally:
+        os.chdir(cwd)
+    for fn in ("correl0", "correl1", "correl2"):
+        ours = open(os.path.join(d_ours, fn)).read().split()
+        ref = open(os.path.join(d_ref, fn)).read().split()
+        assert len(ours) == len(ref) > 0, fn
+        for x, y in zip(ours, ref):
+            if x == y:
+                continue
+            fx, fy = float(x), float(y)
+            assert abs(fx - fy) <= 1e-5 * abs(fy) or (abs(fx) < 1e-12 and abs(fy) < 1e-12), (fn, x, y)
+    assert not os.path.exists(os.path.join(d_ours, "correl3"))

@@ -418,3 +418,26 @@ def test_wall_start_and_zprofile(tag, precision):
         assert z[0, 0] < 0.1 and z[-1, 0] > 0.5
     if p.triplet:
         assert z[0, 1] > 1.0 and z[-1, 1] < 0.1
+
+
+@pytest.mark.parametrize("precision", [FP64, abi.FP32])
+@pytest.mark.parametrize("L", [(8, 8, 12), (16, 8, 8)])
+def test_blocked_correlators(L, precision):
+    """Smeared, blocked lattices and their correlators against the oracle;
+    models with a doublet measure nothing, as in the reference."""
+    p = make_params(L=L, nhiggs=0, triplet=1, precision=precision, seed=33)
+    o = orc.Oracle(make_params(L=L, nhiggs=0, triplet=1, seed=33))
+    o.hot_start()
+    f = o.get_fields()
+    if precision == abi.FP32:
+        f = {k: (v.astype(np.float32).astype(np.float64) if v is not None else None) for k, v in f.items()}
+        o.set_fields(f)
+    b = Batch(p)
+    b.set_fields(0, f)
+    a, g = o.blocked_correlators(5), b.blocked_correlators(0, 5)
+    assert a.shape == g.shape and a.shape[0] == 2
+    tol = 1e-12 if precision == FP64 else 1e-4
+    scale = np.abs(a).max(1, keepdims=True) + 1e-300
+    assert (np.abs(a - g) <= tol * scale).all(), np.abs(a - g).max((0, 1))
+    h = Batch(make_params(L=L, nhiggs=1, triplet=1, precision=precision))
+    assert h.blocked_correlators(0, 3).shape[0] == 0

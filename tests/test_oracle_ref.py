@@ -211,3 +211,25 @@ def test_correlators(tag, L):
         assert (np.abs(a - b) <= 1e-12 * scale).all(), (d, np.abs(a - b).max(0) / scale)
         if p.triplet:
             assert np.abs(b[:, 2:]).max() > 0
+
+
+@pytest.mark.parametrize("L", [(8, 8, 12), (8, 12, 8), (16, 8, 8)])
+def test_blocked_correlators(L):
+    """Correlators on the blocked lattices of the BLOCKING build (smeared
+    links and triplet, every direction but the longest halved per level)
+    against the oracle."""
+    if not orc.ref_available("tcb"):
+        pytest.skip("reference build absent")
+    p = make_params(L=L, nhiggs=0, triplet=1, seed=33)
+    o = orc.Oracle(p)
+    o.hot_start()
+    r = orc.Ref(p, blocking=True)
+    r.set_fields(o.get_fields())
+    levels = r.lib.ref_blocked_setup(5)
+    d = r.lib.ref_longest_dir()
+    ref = np.zeros((levels, L[d] // 2, 10))
+    assert r.lib.ref_blocked_values(orc.ptr(ref)) == levels >= 1
+    ours = o.blocked_correlators(5)
+    assert ours.shape == ref.shape
+    scale = np.abs(ref).max(1, keepdims=True) + 1e-300
+    assert (np.abs(ours - ref) <= 1e-12 * scale).all(), np.abs(ours - ref).max((0, 1))
