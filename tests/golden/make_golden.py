@@ -221,6 +221,49 @@ def correl_fixture(tag, kw):
     return out
 
 
+def blocked_fixture():
+    """Correlators on the blocked lattices of the BLOCKING build."""
+    Lb = (8, 8, 12)
+    p = make_params(L=Lb, nhiggs=0, triplet=1, seed=33)
+    o = orc.Oracle(p)
+    o.hot_start()
+    f = o.get_fields()
+    r = orc.Ref(p, blocking=True)
+    r.set_fields(f)
+    levels = r.lib.ref_blocked_setup(5)
+    vals = np.zeros((levels, Lb[r.lib.ref_longest_dir()] // 2, 10))
+    r.lib.ref_blocked_values(ptr(vals))
+    out = {"L": np.array(Lb), "blocked": vals}
+    for k in orc.FIELDS:
+        out["in_" + k] = f[k]
+    return out
+
+
+def zprofile_fixture():
+    """measure_along_z text and labels_z of the MEASURE_Z build on a hot start."""
+    Lz = (4, 8, 6)
+    p = make_params(L=Lz, nhiggs=1, u1=1, singlet=1, seed=12)
+    o = orc.Oracle(p)
+    o.hot_start()
+    f = o.get_fields()
+    r = orc.Ref(p, measure_z=True)
+    r.set_fields(f)
+    out = {"L": np.array(Lz)}
+    for k in orc.FIELDS:
+        out["in_" + k] = f[k]
+    with tempfile.TemporaryDirectory() as d:
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            r.lib.ref_print_z_labels()
+            r.lib.ref_measure_along_z(1)
+            out["labels_z"] = np.frombuffer(open("labels_z", "rb").read(), dtype=np.uint8)
+            out["measure_z"] = np.frombuffer(open("measure_z", "rb").read(), dtype=np.uint8)
+        finally:
+            os.chdir(cwd)
+    return out
+
+
 CORREL_CASES = {"correl_hc": ("hc", dict(nhiggs=1)),
                 "correl_htc": ("htc", dict(nhiggs=1, triplet=1))}
 
@@ -255,6 +298,10 @@ def main():
     for name, (tag, kw) in CORREL_CASES.items():
         if not only or name in only:
             np.savez_compressed(os.path.join(HERE, name + ".npz"), **correl_fixture(tag, kw))
+    if not only or "blocked_tcb" in only:
+        np.savez_compressed(os.path.join(HERE, "blocked_tcb.npz"), **blocked_fixture())
+    if not only or "zprofile_hsuz" in only:
+        np.savez_compressed(os.path.join(HERE, "zprofile_hsuz.npz"), **zprofile_fixture())
     print("fixtures written to", HERE)
 
 

@@ -192,3 +192,31 @@ def test_correlator_fixture(name, kw):
     for d in range(3):
         a, b = o.correlators(d), g["correl_%d" % d]
         assert (np.abs(a - b) <= 1e-12 * (np.abs(b).max(0) + 1e-300)).all(), d
+
+
+def test_blocked_correlator_fixture():
+    g = load("blocked_tcb")
+    p = make_params(L=tuple(g["L"]), nhiggs=0, triplet=1, seed=33)
+    o = orc.Oracle(p)
+    o.set_fields(fields(g, "in_"))
+    ours, ref = o.blocked_correlators(5), g["blocked"]
+    assert ours.shape == ref.shape
+    assert (np.abs(ours - ref) <= 1e-12 * (np.abs(ref).max(1, keepdims=True) + 1e-300)).all()
+
+
+def test_zprofile_fixture():
+    g = load("zprofile_hsuz")
+    p = make_params(L=tuple(g["L"]), nhiggs=1, u1=1, singlet=1, seed=12)
+    o = orc.Oracle(p)
+    o.set_fields(fields(g, "in_"))
+    prof = np.ascontiguousarray(o.zprofile())
+    lib = abi.load()
+    with tempfile.TemporaryDirectory() as d:
+        lp, mp = os.path.join(d, "labels_z"), os.path.join(d, "measure_z")
+        assert lib.su2hip_zprof_labels_write(lp.encode(), C.byref(p)) == 0
+        assert open(lp, "rb").read() == g["labels_z"].tobytes()
+        assert lib.su2hip_zprof_append(mp.encode(), C.byref(p), ptr(prof), len(prof)) == 0
+        ours, ref = open(mp).read().split(), g["measure_z"].tobytes().decode().split()
+    assert len(ours) == len(ref)
+    for x, y in zip(ours, ref):
+        assert x == y or abs(float(x) - float(y)) <= 1e-5 * abs(float(y))
