@@ -10,7 +10,7 @@
  *
  * Fields cross this boundary in the reference's host layout and site order
  * (src/alloc.c:30-75, src/layout.c:826-851): su2link double[V][3][4],
- * u1link double[V][3], doublet double[V][4], triplet double[V][3],
+ * u1link double[V][3], doublet double[nhiggs][V][4], triplet double[V][3],
  * singlet double[V], with the V sites ordered "even parity first, odd
  * second, natural x-fastest order inside each parity".  All functions return
  * 0 on success and a negative SU2HIP_E* code otherwise; su2hip_error_string()
@@ -39,6 +39,7 @@ extern "C" {
 #define SU2HIP_PHISQ 1
 #define SU2HIP_SIGMASQ 2
 #define SU2HIP_PHI2MINUSSIGMA2 3
+#define SU2HIP_PHI2SQ 4
 
 #define SU2HIP_READONLY 0
 #define SU2HIP_FAST 1
@@ -65,11 +66,13 @@ extern "C" {
 
 /* Run parameters of one lattice.  The scalar couplings keep the reference's
  * names and meaning (src/su2.h:76-181, parsed at src/parameters.c:157-338).
- * Field content is a runtime choice here: nhiggs (0/1), u1, triplet and
- * singlet replace the reference's -DNHIGGS/-DU1/-DTRIPLET/-DSINGLET flags. */
+ * Field content is a runtime choice here: nhiggs (0/1/2), u1, triplet and
+ * singlet replace the reference's -DNHIGGS/-DU1/-DTRIPLET/-DSINGLET flags.
+ * nhiggs = 2 is the two-Higgs-doublet model (-DNHIGGS=2), with or without
+ * U(1); its couplings are the last fields (src/su2.h:110-114). */
 typedef struct su2hip_params {
   int32_t L[3];              /* extents; all even */
-  int32_t nhiggs;            /* fundamental doublets: 0 or 1 */
+  int32_t nhiggs;            /* fundamental doublets: 0, 1 or 2 */
   int32_t u1;                /* compact U(1) hypercharge links */
   int32_t triplet;           /* adjoint scalar */
   int32_t singlet;           /* real singlet scalar */
@@ -93,10 +96,17 @@ typedef struct su2hip_params {
   double msq_triplet, a2, b4;
   double b1_s, msq_s, b3_s, b4_s, a1_s, a2_s;
   uint64_t seed;             /* Philox key of this lattice */
+  /* two-doublet potential (nhiggs = 2, src/su2u1.c:537-554): msq_phi and
+   * lambda_phi above are m1^2 and lam1 */
+  double msq_phi2;
+  double m12sq_re, m12sq_im;
+  double lam2, lam3, lam4;
+  double lam5_re, lam5_im, lam6_re, lam6_im, lam7_re, lam7_im;
 } su2hip_params;
 
 /* Accept/total counters, the reference's `counters` (src/su2.h:208-231) for
- * one lattice and one doublet, plus the link overrelaxation counts. */
+ * one lattice, plus the link overrelaxation counts.  The *_doublet members
+ * count the first doublet, the *_doublet2 members the second. */
 typedef struct su2hip_counters {
   int64_t total_su2link, accepted_su2link;
   int64_t total_u1link, accepted_u1link;
@@ -108,6 +118,8 @@ typedef struct su2hip_counters {
   int64_t total_overrelax_singlet, acc_overrelax_singlet;
   int64_t total_muca, accepted_muca;
   int64_t total_link_or, accepted_link_or;
+  int64_t total_doublet2, accepted_doublet2;
+  int64_t total_overrelax_doublet2, acc_overrelax_doublet2;
 } su2hip_counters;
 
 /* Multicanonical state of one lattice (the scalar part of the reference's
@@ -115,7 +127,7 @@ typedef struct su2hip_counters {
  * separately; index 0 and bins+1 are the reference's virtual end bins. */
 typedef struct su2hip_muca {
   int32_t enabled;           /* multicanonical != 0 */
-  int32_t orderparam;        /* SU2HIP_PHISQ / SIGMASQ / PHI2MINUSSIGMA2 */
+  int32_t orderparam;        /* SU2HIP_PHISQ / SIGMASQ / PHI2MINUSSIGMA2 / PHI2SQ */
   int32_t mode;              /* SU2HIP_READONLY / FAST / SLOW */
   int32_t checks_per_sweep;
   int32_t bins;
@@ -152,6 +164,11 @@ enum {
   SU2HIP_SUM_S4,
   SU2HIP_SUM_SPHISQ,
   SU2HIP_SUM_S2PHISQ,
+  SU2HIP_SUM_COV_PHI_2,     /* second doublet (nhiggs = 2): covariant, phi^2, phi^4 */
+  SU2HIP_SUM_PHI2_2,
+  SU2HIP_SUM_PHI4_2,
+  SU2HIP_SUM_R12,           /* Re phi1^+ phi2 */
+  SU2HIP_SUM_I12,           /* Im phi1^+ phi2 */
   SU2HIP_NSUMS
 };
 
@@ -245,7 +262,9 @@ int su2hip_set_budget(su2hip_batch_t *b, int slot, int64_t sweeps);
 int su2hip_get_budgets(su2hip_batch_t *b, int64_t *budgets);
 /* One checkerboard pass of one kind over every active lattice (the
  * reference's checkerboard_sweep_* for one parity and direction,
- * src/update.c:35-259).  dir is ignored for scalar kinds. */
+ * src/update.c:35-259).  For the doublet kinds of a two-doublet model dir
+ * is the doublet (0 or 1, the reference's higgs_id, > 1 taken as 1); other
+ * scalar passes ignore it. */
 int su2hip_pass(su2hip_batch_t *b, int kind, int parity, int dir);
 /* Capture each compound sweep into a hipGraph that is replayed (1) or launch
  * kernels one by one (0, default). */

@@ -48,10 +48,10 @@ typedef struct su2hip_run_config {
  * (case-insensitive labels, `#` comments, first match searching on from the
  * previous match and wrapping once, strtol/strtod conversion, unknown
  * algorithm names -> metropolis).  `model` overrides the config's `model` key
- * when non-NULL: '+'-separated tokens among higgs, u1, triplet, singlet,
- * gradflow, correlators, measure_z, blocking (implies correlators); the
- * default is the reference Makefile's build,
- * higgs+singlet+u1. */
+ * when non-NULL: '+'-separated tokens among higgs, 2hdm (two doublets, the
+ * -DNHIGGS=2 build), u1, triplet, singlet, gradflow, correlators, measure_z,
+ * blocking (implies correlators); the default is the reference Makefile's
+ * build, higgs+singlet+u1. */
 int su2hip_config_read(const char *path, const char *model, su2hip_run_config *out);
 /* Re-read iterations, interval and checkpoint (read_updated_parameters,
  * src/parameters.c:460-494). */
@@ -59,8 +59,9 @@ int su2hip_config_reread(const char *path, su2hip_run_config *cfg);
 
 /* Binary checkpoint (save_lattice/load_lattice, src/checkpoint.c:84-214):
  * int size(=1), int dim(=3), int L[3], long iter, double total_time,
- * double comms_time, then su2link, u1link, doublet, triplet, singlet for the
- * fields of p, each in the reference host layout. */
+ * double comms_time, then su2link, u1link, doublet(s), triplet, singlet for
+ * the fields of p, each in the reference host layout (with two doublets
+ * `doublet` holds both, 2 V 4 doubles, the first one first). */
 int su2hip_lattice_write(const char *path, const su2hip_params *p, int64_t iter,
                          double total_time, double comms_time, const double *su2link,
                          const double *u1link, const double *doublet,

@@ -63,6 +63,9 @@ WRAP(int, metro_u1link, (lattice const *ll, fields *ff, params const *pp, long i
 WRAP(int, metro_doublet, (lattice const *ll, fields *ff, params const *pp, long i, int h), (ll, ff, pp, i, h))
 WRAP(int, overrelax_doublet, (lattice const *ll, fields *ff, params const *pp, long i), (ll, ff, pp, i))
 #endif
+#if (NHIGGS == 2)
+WRAP(int, overrelax_higgs2, (lattice const *ll, fields *ff, params const *pp, long i, int h), (ll, ff, pp, i, h))
+#endif
 #ifdef TRIPLET
 WRAP(int, metro_triplet, (lattice const *ll, fields *ff, params const *pp, long i), (ll, ff, pp, i))
 WRAP(int, overrelax_triplet, (lattice const *ll, fields *ff, params const *pp, long i), (ll, ff, pp, i))
@@ -81,7 +84,7 @@ int __wrap_multicanonical_acceptance(lattice const *ll, weight *ww, double o, do
 
 /* which model flags this build has: bit0 NHIGGS, bit1 U1, bit2 TRIPLET,
  * bit3 SINGLET, bit4 GRADFLOW, bit5 CORRELATORS, bit6 MEASURE_Z,
- * bit7 BLOCKING */
+ * bit7 BLOCKING, bit8 NHIGGS == 2 */
 int ref_model(void) {
   int m = 0;
 #if (NHIGGS > 0)
@@ -107,6 +110,9 @@ int ref_model(void) {
 #endif
 #ifdef BLOCKING
   m |= 128;
+#endif
+#if (NHIGGS == 2)
+  m |= 256;
 #endif
   return m;
 }
@@ -160,6 +166,20 @@ int ref_setup(const su2hip_params *P) {
   p.a2_s = P->a2_s;
 #endif
 #endif
+#if (NHIGGS == 2)
+  p.msq_phi2 = P->msq_phi2;
+  p.m12sq.re = P->m12sq_re;
+  p.m12sq.im = P->m12sq_im;
+  p.lam2 = P->lam2;
+  p.lam3 = P->lam3;
+  p.lam4 = P->lam4;
+  p.lam5.re = P->lam5_re;
+  p.lam5.im = P->lam5_im;
+  p.lam6.re = P->lam6_re;
+  p.lam6.im = P->lam6_im;
+  p.lam7.re = P->lam7_re;
+  p.lam7.im = P->lam7_im;
+#endif
   w.do_acceptance = 0;
   w.orderparam = -1;
   init_counters(&c);
@@ -186,7 +206,8 @@ void ref_put_fields(const double *link, const double *u1, const double *db,
   if (u1) memcpy(&f.u1link[0][0], u1, sizeof(double) * V * 3);
 #endif
 #if (NHIGGS > 0)
-  if (db) memcpy(&f.su2doublet[0][0][0], db, sizeof(double) * V * 4);
+  for (int h = 0; h < NHIGGS; h++)
+    if (db) memcpy(&f.su2doublet[h][0][0], db + h * V * 4, sizeof(double) * V * 4);
 #endif
 #ifdef TRIPLET
   if (tr) memcpy(&f.su2triplet[0][0], tr, sizeof(double) * V * 3);
@@ -207,7 +228,8 @@ void ref_get_fields(double *link, double *u1, double *db, double *tr, double *sg
   if (u1) memcpy(u1, &f.u1link[0][0], sizeof(double) * V * 3);
 #endif
 #if (NHIGGS > 0)
-  if (db) memcpy(db, &f.su2doublet[0][0][0], sizeof(double) * V * 4);
+  for (int h = 0; h < NHIGGS; h++)
+    if (db) memcpy(db + h * V * 4, &f.su2doublet[h][0][0], sizeof(double) * V * 4);
 #endif
 #ifdef TRIPLET
   if (tr) memcpy(tr, &f.su2triplet[0][0], sizeof(double) * V * 3);
@@ -239,6 +261,12 @@ void ref_counters(su2hip_counters *o) {
   o->total_overrelax_doublet = c.total_overrelax_doublet[0];
   o->acc_overrelax_doublet = c.acc_overrelax_doublet[0];
 #endif
+#if (NHIGGS == 2)
+  o->total_doublet2 = c.total_doublet[1];
+  o->accepted_doublet2 = c.accepted_doublet[1];
+  o->total_overrelax_doublet2 = c.total_overrelax_doublet[1];
+  o->acc_overrelax_doublet2 = c.acc_overrelax_doublet[1];
+#endif
   o->total_triplet = c.total_triplet;
   o->accepted_triplet = c.accepted_triplet;
   o->total_overrelax_triplet = c.total_overrelax_triplet;
@@ -258,7 +286,8 @@ void ref_update(int n) {
   for (int k = 0; k < n; k++) update_lattice(&l, &f, &p, &c, &w);
 }
 
-/* one per-site update with an explicit pass number */
+/* one per-site update with an explicit pass number; for the doublet kinds
+ * dir is the doublet (higgs_id) */
 int ref_site(int kind, long i, int dir, uint64_t pass) {
   h_cur_pass = pass;
   h_site = (uint32_t)i;
@@ -269,7 +298,10 @@ int ref_site(int kind, long i, int dir, uint64_t pass) {
 #ifdef U1
     case SU2HIP_K_U1_METROPOLIS: return __real_metro_u1link(&l, &f, &p, i, dir);
 #endif
-#if (NHIGGS > 0)
+#if (NHIGGS == 2)
+    case SU2HIP_K_DOUBLET_OVERRELAX: return __real_overrelax_higgs2(&l, &f, &p, i, dir);
+    case SU2HIP_K_DOUBLET_METROPOLIS: return __real_metro_doublet(&l, &f, &p, i, dir);
+#elif (NHIGGS > 0)
     case SU2HIP_K_DOUBLET_OVERRELAX: return __real_overrelax_doublet(&l, &f, &p, i);
     case SU2HIP_K_DOUBLET_METROPOLIS: return __real_metro_doublet(&l, &f, &p, i, 0);
 #endif
@@ -293,6 +325,8 @@ double ref_polysolve3(double a, double b, double cc, double d) { return polysolv
 #if (NHIGGS > 0)
 void ref_doublet_staple(long i, double *s) { staple_doublet(s, &l, &f, &p, i, 0); }
 double ref_localact_doublet(long i) { return localact_doublet(&l, &f, &p, i, 0); }
+void ref_doublet_staple_h(long i, int h, double *s) { staple_doublet(s, &l, &f, &p, i, h); }
+double ref_localact_doublet_h(long i, int h) { return localact_doublet(&l, &f, &p, i, h); }
 #endif
 #ifdef U1
 double ref_localact_u1link(long i, int dir) { return localact_u1link(&l, &f, &p, i, dir); }
@@ -323,6 +357,15 @@ void ref_measure_sums(double *s) {
 #ifdef TRIPLET
     s[SU2HIP_SUM_PHI2SIGMA2] += mod * tripletsq(f.su2triplet[i]);
 #endif
+#endif
+#if (NHIGGS == 2)
+    double mod2 = doubletsq(f.su2doublet[1][i]);
+    s[SU2HIP_SUM_COV_PHI_2] += covariant_doublet(&l, &f, i, 1);
+    s[SU2HIP_SUM_PHI2_2] += mod2;
+    s[SU2HIP_SUM_PHI4_2] += mod2 * mod2;
+    complex f12 = get_phi12(f.su2doublet[0][i], f.su2doublet[1][i]);
+    s[SU2HIP_SUM_R12] += f12.re;
+    s[SU2HIP_SUM_I12] += f12.im;
 #endif
 #ifdef TRIPLET
     double tm = tripletsq(f.su2triplet[i]);
@@ -507,6 +550,20 @@ int ref_get_parameters(const char *path, su2hip_params *out, long *longs,
   out->a1_s = p2.a1_s;
   out->a2_s = p2.a2_s;
 #endif
+#endif
+#if (NHIGGS == 2)
+  out->msq_phi2 = p2.msq_phi2;
+  out->m12sq_re = p2.m12sq.re;
+  out->m12sq_im = p2.m12sq.im;
+  out->lam2 = p2.lam2;
+  out->lam3 = p2.lam3;
+  out->lam4 = p2.lam4;
+  out->lam5_re = p2.lam5.re;
+  out->lam5_im = p2.lam5.im;
+  out->lam6_re = p2.lam6.re;
+  out->lam6_im = p2.lam6.im;
+  out->lam7_re = p2.lam7.re;
+  out->lam7_im = p2.lam7.im;
 #endif
   longs[0] = p2.iterations;
   longs[1] = p2.interval;

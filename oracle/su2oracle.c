@@ -10,7 +10,8 @@
  * Where the reference computes in long double (polysolve3, local_su2wilson)
  * this file does too.
  *
- * Parity status: every per-site function, the sweep, measure sums, the
+ * Parity status: every per-site function (the two-doublet model's
+ * included), the sweep, measure sums, the
  * multicanonical steps and the Euler flow are pinned against the reference
  * compiled from source (oracle/_ref, driven by oracle/ref_harness.c) and the
  * committed golden fixtures.  Functions with no reference counterpart are
@@ -34,6 +35,7 @@ struct orc {
   long *next, *prev, *nat;
   double *U, *A, *phi, *sig, *S;
   double *bu_phi, *bu_sig;
+  double *phi2, *bu_phi2; /* second doublet (nhiggs = 2) */
   uint64_t seed, pass;
   su2hip_counters c;
   su2hip_muca m;
@@ -146,6 +148,8 @@ static inline void rotate(const double *u, const double *v, double *r) {
 
 #define LNK(o, i, d) ((o)->U + ((size_t)(i) * D + (d)) * 4)
 #define PHI(o, i) ((o)->phi + (size_t)(i) * 4)
+/* doublet h (the reference's higgs_id) at site i */
+#define PHIH(o, h, i) (((h) ? (o)->phi2 : (o)->phi) + (size_t)(i) * 4)
 #define SIG(o, i) ((o)->sig + (size_t)(i) * 3)
 #define ALP(o, i, d) ((o)->A[(size_t)(i) * D + (d)])
 #define NX(o, i, d) ((o)->next[(size_t)(i) * D + (d)])
@@ -207,6 +211,10 @@ orc_t *orc_create(const su2hip_params *p) {
   o->S = (double *)calloc((size_t)V, sizeof(double));
   o->bu_phi = (double *)calloc((size_t)V * 4, sizeof(double));
   o->bu_sig = (double *)calloc((size_t)V * 3, sizeof(double));
+  if (p->nhiggs == 2) {
+    o->phi2 = (double *)calloc((size_t)V * 4, sizeof(double));
+    o->bu_phi2 = (double *)calloc((size_t)V * 4, sizeof(double));
+  }
   o->seed = p->seed;
   o->pass = 0;
   return o;
@@ -238,6 +246,8 @@ void orc_destroy(orc_t *o) {
   free(o->S);
   free(o->bu_phi);
   free(o->bu_sig);
+  free(o->phi2);
+  free(o->bu_phi2);
   free_muca(o);
   free(o);
 }
@@ -258,6 +268,7 @@ void orc_set_fields(orc_t *o, const double *su2link, const double *u1link,
   if (su2link) memcpy(o->U, su2link, sizeof(double) * V * D * 4);
   if (u1link) memcpy(o->A, u1link, sizeof(double) * V * D);
   if (doublet) memcpy(o->phi, doublet, sizeof(double) * V * 4);
+  if (doublet && o->phi2) memcpy(o->phi2, doublet + V * 4, sizeof(double) * V * 4);
   if (triplet) memcpy(o->sig, triplet, sizeof(double) * V * 3);
   if (singlet) memcpy(o->S, singlet, sizeof(double) * V);
 }
@@ -268,6 +279,7 @@ void orc_get_fields(const orc_t *o, double *su2link, double *u1link,
   if (su2link) memcpy(su2link, o->U, sizeof(double) * V * D * 4);
   if (u1link) memcpy(u1link, o->A, sizeof(double) * V * D);
   if (doublet) memcpy(doublet, o->phi, sizeof(double) * V * 4);
+  if (doublet && o->phi2) memcpy(doublet + V * 4, o->phi2, sizeof(double) * V * 4);
   if (triplet) memcpy(triplet, o->sig, sizeof(double) * V * 3);
   if (singlet) memcpy(singlet, o->S, sizeof(double) * V);
 }
@@ -281,9 +293,11 @@ void orc_cold_start(orc_t *o, double phi0, double sigma0, double singlet0) {
       u[1] = u[2] = u[3] = 0.0;
       ALP(o, i, d) = 0.0;
     }
-    double *f = PHI(o, i);
-    f[0] = phi0;
-    f[1] = f[2] = f[3] = 0.0;
+    for (int h = 0; h < (o->p.nhiggs == 2 ? 2 : 1); h++) {
+      double *f = PHIH(o, h, i);
+      f[0] = phi0;
+      f[1] = f[2] = f[3] = 0.0;
+    }
     double *s = SIG(o, i);
     s[0] = sigma0;
     s[1] = s[2] = 0.0;
@@ -300,7 +314,7 @@ static inline void gauss2(double u1, double u2, double *z0, double *z1) {
 }
 
 /* Hot start (no reference counterpart).  Draw order at each site: links
- * dir 0..2 (4 Gaussians each), U(1) angles, doublet (4 Gaussians), triplet
+ * dir 0..2 (4 Gaussians each), U(1) angles, doublets (4 Gaussians each), triplet
  * (2 pairs, last Gaussian unused), singlet (1 pair, second unused). */
 void orc_hot_start(orc_t *o) {
   for (long i = 0; i < o->vol; i++) {
@@ -319,7 +333,7 @@ void orc_hot_start(orc_t *o) {
     }
     if (o->p.u1)
       for (int d = 0; d < D; d++) ALP(o, i, d) = M_PI * (2.0 * dran(o) - 1.0);
-    if (o->p.nhiggs) {
+    for (int h = 0; h < o->p.nhiggs; h++) {
       double z[4], u0, u1;
       u0 = dran(o);
       u1 = dran(o);
@@ -327,7 +341,7 @@ void orc_hot_start(orc_t *o) {
       u0 = dran(o);
       u1 = dran(o);
       gauss2(u0, u1, &z[2], &z[3]);
-      for (int k = 0; k < 4; k++) PHI(o, i)[k] = z[k];
+      for (int k = 0; k < 4; k++) PHIH(o, h, i)[k] = z[k];
     }
     if (o->p.triplet) {
       double z[4], u0, u1;
@@ -402,12 +416,12 @@ static inline double tripletsq(const double *a) {
   return 0.5 * (a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
 }
 
-/* hopping_doublet_forward (src/su2u1.c:428-445):
+/* hopping_doublet_forward (src/su2u1.c:428-445) of doublet h:
  * -Re Tr Phi(x)^+ U_j(x) Phi(x+j) exp(-i alpha_j(x) sigma_3) */
-static double hop_doublet_fwd(const orc_t *o, long i, int d) {
+static double hop_doublet_fwd(const orc_t *o, long i, int d, int h) {
   double t[4], s[4];
-  qmul_cn(PHI(o, i), LNK(o, i, d), t);
-  qmul(t, PHI(o, NX(o, i, d)), s);
+  qmul_cn(PHIH(o, h, i), LNK(o, i, d), t);
+  qmul(t, PHIH(o, h, NX(o, i, d)), s);
   if (o->p.u1) {
     double e[4];
     u1twist(ALP(o, i, d), e);
@@ -418,11 +432,11 @@ static double hop_doublet_fwd(const orc_t *o, long i, int d) {
 }
 
 /* hopping_doublet_backward (src/su2u1.c:451-472) */
-static double hop_doublet_bwd(const orc_t *o, long i, int d) {
+static double hop_doublet_bwd(const orc_t *o, long i, int d, int h) {
   long pv = PV(o, i, d);
   double t[4], s[4];
-  qmul_cn(PHI(o, pv), LNK(o, pv, d), t);
-  qmul(t, PHI(o, i), s);
+  qmul_cn(PHIH(o, h, pv), LNK(o, pv, d), t);
+  qmul(t, PHIH(o, h, i), s);
   if (o->p.u1) {
     double e[4];
     u1twist(ALP(o, pv, d), e);
@@ -433,10 +447,16 @@ static double hop_doublet_bwd(const orc_t *o, long i, int d) {
 }
 
 /* covariant_doublet (src/su2u1.c:477-488) */
-static double covariant_doublet(const orc_t *o, long i) {
-  double tot = 0.0, mod = doubletsq(PHI(o, i));
-  for (int d = 0; d < D; d++) tot += 2.0 * mod + hop_doublet_fwd(o, i, d);
+static double covariant_doublet(const orc_t *o, long i, int h) {
+  double tot = 0.0, mod = doubletsq(PHIH(o, h, i));
+  for (int d = 0; d < D; d++) tot += 2.0 * mod + hop_doublet_fwd(o, i, d, h);
   return tot;
+}
+
+/* get_phi12 (src/su2u1.c:513-520): R + i I = phi1^+ phi2 */
+static inline void phi12(const double *h1, const double *h2, double *R, double *I) {
+  *R = 0.5 * (h1[0] * h2[0] + h1[1] * h2[1] + h1[2] * h2[2] + h1[3] * h2[3]);
+  *I = 0.5 * (h1[3] * h2[0] + h1[2] * h2[1] - h1[1] * h2[2] - h1[0] * h2[3]);
 }
 
 /* hopping_triplet_forward (src/su2u1.c:605-617): -2 Tr A(x) U A(x+j) U^+ */
@@ -484,6 +504,15 @@ static double higgspotential(const orc_t *o, long i) {
     mod = doubletsq(PHI(o, i));
     pot += p->msq_phi * mod + p->lambda_phi * mod * mod;
   }
+  if (p->nhiggs == 2) {
+    /* two-doublet terms (src/su2u1.c:537-554) */
+    double f11 = mod, f22 = doubletsq(PHIH(o, 1, i)), R, I;
+    phi12(PHI(o, i), PHIH(o, 1, i), &R, &I);
+    pot += p->msq_phi2 * f22 + p->m12sq_re * R - p->m12sq_im * I + p->lam2 * f22 * f22 +
+           p->lam3 * f11 * f22 + p->lam4 * (R * R + I * I) + p->lam5_re * (R * R - I * I) -
+           2.0 * p->lam5_im * R * I + f11 * (p->lam6_re * R - p->lam6_im * I) +
+           f22 * (p->lam7_re * R + p->lam7_im * I);
+  }
   if (p->triplet) {
     double mt = tripletsq(SIG(o, i));
     pot += p->msq_triplet * mt + p->b4 * mt * mt;
@@ -502,7 +531,7 @@ double orc_localact_su2link(const orc_t *o, long i, int dir) {
     tot += (1.0 - 0.5 * ptrace(o, PV(o, i, d2), dir, d2));
   }
   tot *= o->p.betasu2;
-  if (o->p.nhiggs) tot += hop_doublet_fwd(o, i, dir);
+  for (int h = 0; h < o->p.nhiggs; h++) tot += hop_doublet_fwd(o, i, dir, h);
   if (o->p.triplet) tot += hop_triplet_fwd(o, i, dir);
   return tot;
 }
@@ -516,17 +545,18 @@ double orc_localact_u1link(const orc_t *o, long i, int dir) {
     tot += (1.0 - cos(r * u1ptrace(o, PV(o, i, d2), dir, d2)));
   }
   tot *= o->p.betau1;
-  if (o->p.nhiggs) tot += hop_doublet_fwd(o, i, dir);
+  for (int h = 0; h < o->p.nhiggs; h++) tot += hop_doublet_fwd(o, i, dir, h);
   return tot;
 }
 
-/* localact_doublet (src/su2u1.c:495-510) */
-double orc_localact_doublet(const orc_t *o, long i) {
-  double tot = covariant_doublet(o, i);
-  for (int d = 0; d < D; d++) tot += hop_doublet_bwd(o, i, d);
+/* localact_doublet (src/su2u1.c:495-510) of doublet h */
+double orc_localact_doublet_h(const orc_t *o, long i, int h) {
+  double tot = covariant_doublet(o, i, h);
+  for (int d = 0; d < D; d++) tot += hop_doublet_bwd(o, i, d, h);
   tot += higgspotential(o, i);
   return tot;
 }
+double orc_localact_doublet(const orc_t *o, long i) { return orc_localact_doublet_h(o, i, 0); }
 
 /* localact_triplet (src/su2u1.c:657-679) */
 double orc_localact_triplet(const orc_t *o, long i) {
@@ -554,7 +584,7 @@ double orc_action_local(const orc_t *o, long i) {
   tot += local_su2wilson(o, i);
   if (o->p.u1) tot += local_u1wilson(o, i);
   tot += higgspotential(o, i);
-  if (o->p.nhiggs) tot += covariant_doublet(o, i);
+  for (int h = 0; h < o->p.nhiggs; h++) tot += covariant_doublet(o, i, h);
   if (o->p.triplet) tot += covariant_triplet(o, i);
   if (o->p.singlet) {
     double S = o->S[i];
@@ -594,9 +624,9 @@ void orc_wilson_staple(const orc_t *o, long i, int mu, double V[4]) {
 void orc_link_staple(const orc_t *o, long i, int dir, double V[4]) {
   orc_wilson_staple(o, i, dir, V);
   for (int k = 0; k < 4; k++) V[k] *= -0.5 * o->p.betasu2;
-  if (o->p.nhiggs) {
+  for (int h = 0; h < o->p.nhiggs; h++) {
     double b[4], t[4];
-    const double *nb = PHI(o, NX(o, i, dir));
+    const double *nb = PHIH(o, h, NX(o, i, dir));
     if (o->p.u1) {
       double e[4];
       u1twist(ALP(o, i, dir), e);
@@ -604,18 +634,19 @@ void orc_link_staple(const orc_t *o, long i, int dir, double V[4]) {
     } else {
       memcpy(b, nb, sizeof(b));
     }
-    qmul_nc(b, PHI(o, i), t);
+    qmul_nc(b, PHIH(o, h, i), t);
     for (int k = 0; k < 4; k++) V[k] -= 0.5 * t[k];
   }
 }
 
-/* staple_doublet (src/staples.c:243-302): s_a with S = phi_a s_a + ... */
-void orc_doublet_staple(const orc_t *o, long i, double s[4]) {
+/* staple_doublet (src/staples.c:243-302) of doublet h: s_a with
+ * S = phi_a s_a + ... */
+void orc_doublet_staple_h(const orc_t *o, long i, int h, double s[4]) {
   for (int k = 0; k < 4; k++) s[k] = 0.0;
   for (int d = 0; d < D; d++) {
     double t[4], q[4];
     /* forward: -(U_d(x) Phi(x+d) E_d(x)) */
-    qmul(LNK(o, i, d), PHI(o, NX(o, i, d)), t);
+    qmul(LNK(o, i, d), PHIH(o, h, NX(o, i, d)), t);
     if (o->p.u1) {
       double e[4];
       u1twist(ALP(o, i, d), e);
@@ -626,7 +657,7 @@ void orc_doublet_staple(const orc_t *o, long i, double s[4]) {
     for (int k = 0; k < 4; k++) s[k] -= q[k];
     /* backward: -(U_d(x-d)^+ Phi(x-d) E_d(x-d)^+) */
     long pv = PV(o, i, d);
-    qmul_cn(LNK(o, pv, d), PHI(o, pv), t);
+    qmul_cn(LNK(o, pv, d), PHIH(o, h, pv), t);
     if (o->p.u1) {
       double e[4], ec[4];
       u1twist(ALP(o, pv, d), e);
@@ -638,6 +669,7 @@ void orc_doublet_staple(const orc_t *o, long i, double s[4]) {
     for (int k = 0; k < 4; k++) s[k] -= q[k];
   }
 }
+void orc_doublet_staple(const orc_t *o, long i, double s[4]) { orc_doublet_staple_h(o, i, 0, s); }
 
 /* hopping staple of the triplet (src/overrelax.c:143-180):
  * s = -sum_d [ R(U_d(x)) Sigma(x+d) + R(U_d(x-d))^T Sigma(x-d) ] */
@@ -839,13 +871,86 @@ static int site_or_doublet(orc_t *o, long i) {
   return 0;
 }
 
-/* metro_doublet (src/metropolis.c:77-106) */
-static int site_metro_doublet(orc_t *o, long i) {
-  double *f = PHI(o, i), old[4];
+/* overrelax_higgs2 (src/overrelax.c:295-417): the (X,Y) overrelaxation of
+ * doublet h of two, Y kept fixed, the local potential a quartic in X whose
+ * coefficients come from the other doublet H and G = -+ i sigma_2 H */
+static int site_or_higgs2(orc_t *o, long i, int h) {
+  const su2hip_params *p = &o->p;
+  double *f = PHIH(o, h, i);
+  const double *H = PHIH(o, 1 - h, i);
+  double s[4];
+  orc_doublet_staple_h(o, i, h, s);
+  double sg = h ? 1.0 : -1.0;
+  double G[4] = {sg * H[3], sg * H[2], -sg * H[1], -sg * H[0]};
+  double lam12, msq, l67re, l67im, o67re, o67im;
+  if (h == 0) {
+    lam12 = p->lambda_phi;
+    msq = p->msq_phi;
+    l67re = p->lam7_re;
+    l67im = p->lam7_im;
+    o67re = p->lam6_re;
+    o67im = -p->lam6_im;
+  } else {
+    lam12 = p->lam2;
+    msq = p->msq_phi2;
+    l67re = p->lam6_re;
+    l67im = -p->lam6_im;
+    o67re = p->lam7_re;
+    o67im = p->lam7_im;
+  }
+  double Hsq = doubletsq(H);
+  for (int k = 0; k < 4; k++) {
+    s[k] += 0.5 * (p->m12sq_re + Hsq * l67re) * H[k];
+    s[k] += 0.5 * (-1.0 * p->m12sq_im + Hsq * l67im) * G[k];
+    s[k] = -1.0 * s[k];
+  }
+  double F = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + s[3] * s[3]);
+  double X = 0.0;
+  for (int k = 0; k < 4; k++) {
+    s[k] /= F;
+    X += f[k] * s[k];
+  }
+  double Y[4], Ysq = 0.0;
+  for (int k = 0; k < 4; k++) {
+    Y[k] = f[k] - X * s[k];
+    Ysq += Y[k] * Y[k];
+  }
+  double Hf = 0.0, Gf = 0.0, HY = 0.0, GY = 0.0;
+  for (int k = 0; k < 4; k++) {
+    Hf += H[k] * s[k];
+    Gf += G[k] * s[k];
+    HY += H[k] * Y[k];
+    GY += G[k] * Y[k];
+  }
+  double b4 = 0.25 * lam12;
+  double b3 = 0.25 * (Hf * o67re + Gf * o67im);
+  double b2 = 0.5 * msq + 1.0 * D + 0.5 * Ysq * lam12 + 0.25 * (HY * o67re + GY * o67im);
+  double b1 = -1.0 * F + 0.25 * Ysq * (Hf * o67re + Gf * o67im);
+  b2 += 0.5 * Hsq * p->lam3 + 0.25 * p->lam4 * (Hf * Hf + Gf * Gf) +
+        0.25 * p->lam5_re * (Hf * Hf - Gf * Gf) - 0.5 * p->lam5_im * Gf * Hf;
+  b1 += 0.5 * Hf * (HY * p->lam4 + HY * p->lam5_re - GY * p->lam5_im) +
+        0.5 * Gf * (GY * p->lam4 - GY * p->lam5_re - HY * p->lam5_im);
+  long double aa = b4;
+  long double bb = b3 + b4 * X;
+  long double cc = b2 + b3 * X + b4 * X * X;
+  long double dd = b1 + b2 * X + b3 * X * X + b4 * X * X * X;
+  double nX = polysolve3_ld(aa, bb, cc, dd);
+  double dV = 4.0 * b4 * X * X * X + 3.0 * b3 * X * X + 2.0 * b2 * X + b1;
+  double dVn = 4.0 * b4 * nX * nX * nX + 3.0 * b3 * nX * nX + 2.0 * b2 * nX + b1;
+  if (fabs(dV / dVn) >= dran(o)) {
+    for (int k = 0; k < 4; k++) f[k] = f[k] + s[k] * (nX - X);
+    return 1;
+  }
+  return 0;
+}
+
+/* metro_doublet (src/metropolis.c:77-106) of doublet h */
+static int site_metro_doublet(orc_t *o, long i, int h) {
+  double *f = PHIH(o, h, i), old[4];
   memcpy(old, f, sizeof(old));
-  double act_old = orc_localact_doublet(o, i);
+  double act_old = orc_localact_doublet_h(o, i, h);
   for (int k = 0; k < 4; k++) f[k] += 1.0 * (dran(o) - 0.5);
-  double diff = orc_localact_doublet(o, i) - act_old;
+  double diff = orc_localact_doublet_h(o, i, h) - act_old;
   if (metro_accept(o, diff)) return 1;
   memcpy(f, old, sizeof(old));
   return 0;
@@ -945,8 +1050,9 @@ static int site_dispatch(orc_t *o, int kind, long i, int dir) {
     case SU2HIP_K_LINK_METROPOLIS: return site_metro_link(o, i, dir);
     case SU2HIP_K_LINK_OVERRELAX: return site_link_or(o, i, dir);
     case SU2HIP_K_U1_METROPOLIS: return site_metro_u1(o, i, dir);
-    case SU2HIP_K_DOUBLET_OVERRELAX: return site_or_doublet(o, i);
-    case SU2HIP_K_DOUBLET_METROPOLIS: return site_metro_doublet(o, i);
+    case SU2HIP_K_DOUBLET_OVERRELAX:
+      return o->p.nhiggs == 2 ? site_or_higgs2(o, i, dir != 0) : site_or_doublet(o, i);
+    case SU2HIP_K_DOUBLET_METROPOLIS: return site_metro_doublet(o, i, o->p.nhiggs == 2 && dir);
     case SU2HIP_K_TRIPLET_OVERRELAX: return site_or_triplet(o, i);
     case SU2HIP_K_TRIPLET_METROPOLIS: return site_metro_triplet(o, i);
     case SU2HIP_K_SINGLET_OVERRELAX: return site_or_singlet(o, i);
@@ -1134,6 +1240,9 @@ double orc_muca_orderparam(orc_t *o, int par) {
     case SU2HIP_PHISQ:
       for (long i = off; i < max; i++) tot += doubletsq(PHI(o, i));
       break;
+    case SU2HIP_PHI2SQ:
+      for (long i = off; i < max; i++) tot += doubletsq(PHIH(o, 1, i));
+      break;
   }
   tot = tot / o->vol;
   o->m.param_value[par] = tot;
@@ -1229,8 +1338,18 @@ void orc_muca_get_sums(const orc_t *o, int64_t *gsum, int64_t *nsum) {
 /* ------------------------------------------------------------------ */
 /* passes and the sweep (src/update.c) */
 
-static void count(orc_t *o, int kind, int acc) {
+static void count(orc_t *o, int kind, int acc, int h) {
   su2hip_counters *c = &o->c;
+  if (h && kind == SU2HIP_K_DOUBLET_OVERRELAX) {
+    c->acc_overrelax_doublet2 += acc;
+    c->total_overrelax_doublet2++;
+    return;
+  }
+  if (h && kind == SU2HIP_K_DOUBLET_METROPOLIS) {
+    c->accepted_doublet2 += acc;
+    c->total_doublet2++;
+    return;
+  }
   switch (kind) {
     case SU2HIP_K_LINK_HEATBATH:
     case SU2HIP_K_LINK_METROPOLIS:
@@ -1280,8 +1399,9 @@ static void cp_parity(const orc_t *o, const double *src, double *dst, int dofs,
   memcpy(dst + off * dofs, src + off * dofs, sizeof(double) * dofs * (max - off));
 }
 
-/* checkerboard_sweep_* (src/update.c:35-259) for one kind/parity/dir,
- * including the multicanonical checks of src/update.c:95-153 */
+/* checkerboard_sweep_* (src/update.c:35-259) for one kind/parity/dir (for
+ * the doublet kinds dir is the doublet), including the multicanonical
+ * checks of src/update.c:95-153 */
 void orc_pass(orc_t *o, int kind, int par, int dir) {
   uint64_t pass = o->pass++;
   long off = par == SU2HIP_EVEN ? 0 : o->even;
@@ -1289,13 +1409,16 @@ void orc_pass(orc_t *o, int kind, int par, int dir) {
   int field = 0; /* 1 doublet, 2 triplet */
   if (kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS) field = 1;
   if (kind == SU2HIP_K_TRIPLET_OVERRELAX || kind == SU2HIP_K_TRIPLET_METROPOLIS) field = 2;
+  int h = field == 1 && o->p.nhiggs == 2 && dir;
   int do_muca = 0;
   if (field && o->m.enabled && o->m.do_acceptance) {
     int op = o->m.orderparam;
-    if (field == 1 && (op == SU2HIP_PHISQ || op == SU2HIP_PHI2MINUSSIGMA2)) do_muca = 1;
+    if (field == 1 && h == 0 && (op == SU2HIP_PHISQ || op == SU2HIP_PHI2MINUSSIGMA2)) do_muca = 1;
+    if (field == 1 && h == 1 && op == SU2HIP_PHI2SQ) do_muca = 1;
     if (field == 2 && (op == SU2HIP_SIGMASQ || op == SU2HIP_PHI2MINUSSIGMA2)) do_muca = 1;
   }
-  double *fld = field == 1 ? o->phi : o->sig, *bu = field == 1 ? o->bu_phi : o->bu_sig;
+  double *fld = field == 1 ? (h ? o->phi2 : o->phi) : o->sig;
+  double *bu = field == 1 ? (h ? o->bu_phi2 : o->bu_phi) : o->bu_sig;
   int dofs = field == 1 ? 4 : 3;
   long interval = o->vol, cnt = 0;
   uint32_t check = 0;
@@ -1307,7 +1430,7 @@ void orc_pass(orc_t *o, int kind, int par, int dir) {
   for (long i = off; i < max; i++) {
     ctx(o, pass, (uint32_t)i);
     int acc = site_dispatch(o, kind, i, dir);
-    count(o, kind, acc);
+    count(o, kind, acc, h);
     if (do_muca) {
       cnt++;
       if (cnt % interval == 0) {
@@ -1357,7 +1480,8 @@ void orc_update(orc_t *o, int n) {
                               SU2HIP_K_DOUBLET_METROPOLIS);
         if (kd >= 0)
           for (int k = 0; k < p->update_su2doublet; k++)
-            for (int par = 0; par < 2; par++) orc_pass(o, kd, par, 0);
+            for (int h = 0; h < p->nhiggs; h++)
+              for (int par = 0; par < 2; par++) orc_pass(o, kd, par, h);
       }
       if (p->triplet) {
         int kt = scalar_round(p->algorithm_su2triplet, last, SU2HIP_K_TRIPLET_OVERRELAX,
@@ -1607,10 +1731,19 @@ void orc_measure(const orc_t *o, double *sums) {
     if (p->u1) s[SU2HIP_SUM_U1WILSON] += local_u1wilson(o, i);
     if (p->nhiggs) {
       double mod = doubletsq(PHI(o, i));
-      s[SU2HIP_SUM_COV_PHI] += covariant_doublet(o, i);
+      s[SU2HIP_SUM_COV_PHI] += covariant_doublet(o, i, 0);
       s[SU2HIP_SUM_PHI2] += mod;
       s[SU2HIP_SUM_PHI4] += mod * mod;
       if (p->triplet) s[SU2HIP_SUM_PHI2SIGMA2] += mod * tripletsq(SIG(o, i));
+    }
+    if (p->nhiggs == 2) {
+      double mod = doubletsq(PHIH(o, 1, i)), R, I;
+      s[SU2HIP_SUM_COV_PHI_2] += covariant_doublet(o, i, 1);
+      s[SU2HIP_SUM_PHI2_2] += mod;
+      s[SU2HIP_SUM_PHI4_2] += mod * mod;
+      phi12(PHI(o, i), PHIH(o, 1, i), &R, &I);
+      s[SU2HIP_SUM_R12] += R;
+      s[SU2HIP_SUM_I12] += I;
     }
     if (p->triplet) {
       double tm = tripletsq(SIG(o, i));
@@ -1799,7 +1932,7 @@ int orc_flow(orc_t *o, int scheme, double dt, double t_max, int meas_interval,
 
 /* prepare_wall (src/z_coord.c:304-376) with the deviations drawn from the
  * Philox stream (pass ORC_WALL_PASS) in the reference's per-site order:
- * doublet, triplet, singlet, then the links of every direction */
+ * doublets, triplet, singlet, then the links of every direction */
 void orc_wall_start(orc_t *o, int zdir) {
   if (zdir < 0) zdir = orc_longest_dir(o);
   for (long i = 0; i < o->vol; i++) {
@@ -1807,9 +1940,9 @@ void orc_wall_start(orc_t *o, int zdir) {
     long c[D] = {nat % o->L[0], (nat / o->L[0]) % o->L[1], nat / ((long)o->L[0] * o->L[1])};
     int lo = c[zdir] < 0.5 * o->L[zdir];
     ctx(o, ORC_WALL_PASS, (uint32_t)i);
-    if (o->p.nhiggs) {
+    for (int h = 0; h < o->p.nhiggs; h++) {
       double a = lo ? 0.2 : 1.2, e = lo ? 0.01 : 0.05;
-      double *f = PHI(o, i);
+      double *f = PHIH(o, h, i);
       f[0] = a + e * dran(o);
       f[1] = e * dran(o);
       f[2] = e * dran(o);

@@ -60,6 +60,9 @@ void orc_reset_counters(orc_t *o);
 void orc_link_staple(const orc_t *o, long i, int dir, double V[4]);
 void orc_wilson_staple(const orc_t *o, long i, int dir, double V[4]);
 void orc_doublet_staple(const orc_t *o, long i, double s[4]);
+/* doublet h (0 or 1) of the two-doublet model; h = 0 is the one above */
+void orc_doublet_staple_h(const orc_t *o, long i, int h, double s[4]);
+double orc_localact_doublet_h(const orc_t *o, long i, int h);
 void orc_triplet_staple(const orc_t *o, long i, double s[3]);
 double orc_localact_su2link(const orc_t *o, long i, int dir);
 double orc_localact_u1link(const orc_t *o, long i, int dir);
@@ -70,7 +73,8 @@ double orc_action_local(const orc_t *o, long i);
 double orc_polysolve3(double a, double b, double c, double d);
 
 /* One per-site update of `kind` (SU2HIP_K_*) at site i with draws from
- * (seed, pass, i, 0..).  Returns 1 if accepted. */
+ * (seed, pass, i, 0..); for the doublet kinds dir is the doublet.  Returns
+ * 1 if accepted. */
 int orc_site_update(orc_t *o, int kind, long i, int dir, uint64_t pass);
 /* One checkerboard pass over all sites of a parity (consumes one pass
  * number; multicanonical checks run inside scalar passes). */

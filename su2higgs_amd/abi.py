@@ -15,7 +15,7 @@ LIBPATH = os.path.join(LIBDIR, "libsu2hip.so")
 
 METROPOLIS, HEATBATH, OVERRELAX = 1, 2, 3
 EVEN, ODD = 0, 1
-PHISQ, SIGMASQ, PHI2MINUSSIGMA2 = 1, 2, 3
+PHISQ, SIGMASQ, PHI2MINUSSIGMA2, PHI2SQ = 1, 2, 3, 4
 READONLY, FAST, SLOW = 0, 1, 2
 FP64, FP32 = 64, 32
 OK, EINVAL, ENOMEM, EDEVICE, ERANGE, EUNSUPPORTED, ESTATE, EIO = 0, -1, -2, -3, -4, -5, -6, -7
@@ -34,7 +34,7 @@ KIND_NAMES = ["link_heatbath", "link_metropolis", "link_overrelax",
 SUM_NAMES = ["action", "wilson", "u1wilson", "cov_phi", "phi2", "phi4",
              "hop_sigma", "sigma2", "sigma4", "phi2sigma2", "magcharge",
              "magcharge_abs", "s_cov", "s", "s2", "s3", "s4", "sphisq",
-             "s2phisq"]
+             "s2phisq", "cov_phi_2", "phi2_2", "phi4_2", "r12", "i12"]
 NSUMS = len(SUM_NAMES)
 SUM = {n: i for i, n in enumerate(SUM_NAMES)}
 
@@ -57,6 +57,10 @@ class Params(C.Structure):
         ("b1_s", C.c_double), ("msq_s", C.c_double), ("b3_s", C.c_double),
         ("b4_s", C.c_double), ("a1_s", C.c_double), ("a2_s", C.c_double),
         ("seed", C.c_uint64),
+        ("msq_phi2", C.c_double), ("m12sq_re", C.c_double), ("m12sq_im", C.c_double),
+        ("lam2", C.c_double), ("lam3", C.c_double), ("lam4", C.c_double),
+        ("lam5_re", C.c_double), ("lam5_im", C.c_double), ("lam6_re", C.c_double),
+        ("lam6_im", C.c_double), ("lam7_re", C.c_double), ("lam7_im", C.c_double),
     ]
 
 
@@ -68,7 +72,8 @@ class Counters(C.Structure):
         "accepted_triplet", "total_overrelax_triplet", "acc_overrelax_triplet",
         "total_singlet", "accepted_singlet", "total_overrelax_singlet",
         "acc_overrelax_singlet", "total_muca", "accepted_muca",
-        "total_link_or", "accepted_link_or")]
+        "total_link_or", "accepted_link_or", "total_doublet2",
+        "accepted_doublet2", "total_overrelax_doublet2", "acc_overrelax_doublet2")]
 
     def asdict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -180,6 +185,12 @@ def make_params(L=(8, 8, 8), nhiggs=1, u1=0, triplet=0, singlet=0,
     p.b4 = 0.12
     p.b1_s, p.msq_s, p.b3_s, p.b4_s, p.a1_s, p.a2_s = 0.3, -0.1, 0.2, 0.3, 0.3, 0.25
     p.seed = 1234
+    # two-doublet couplings (used when nhiggs == 2): every term of the
+    # potential switched on
+    p.msq_phi2, p.m12sq_re, p.m12sq_im = -0.15, 0.02, 0.01
+    p.lam2, p.lam3, p.lam4 = 0.03, 0.015, -0.01
+    p.lam5_re, p.lam5_im, p.lam6_re, p.lam6_im, p.lam7_re, p.lam7_im = \
+        0.008, 0.004, 0.005, -0.003, 0.004, 0.002
     for k, v in kw.items():
         if k == "L":
             continue
@@ -199,7 +210,8 @@ def field_arrays(p):
     return {
         "su2link": np.zeros((V, 3, 4)),
         "u1link": np.zeros((V, 3)) if p.u1 else None,
-        "doublet": np.zeros((V, 4)) if p.nhiggs else None,
+        "doublet": (np.zeros((2, V, 4)) if p.nhiggs == 2 else np.zeros((V, 4)))
+        if p.nhiggs else None,
         "triplet": np.zeros((V, 3)) if p.triplet else None,
         "singlet": np.zeros(V) if p.singlet else None,
     }

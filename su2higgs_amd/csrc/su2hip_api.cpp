@@ -135,7 +135,7 @@ static size_t field_words(const su2hip_params &p) {
   size_t Vh = (size_t)vol_of(p) / 2;
   size_t w = 24 * Vh;
   if (p.u1) w += 6 * Vh;
-  if (p.nhiggs) w += 8 * Vh;
+  if (p.nhiggs) w += 8 * Vh * p.nhiggs;
   if (p.triplet) w += 6 * Vh;
   if (p.singlet) w += 2 * Vh;
   return w;
@@ -152,7 +152,8 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static bool supported_model(int m) {
   return m == 0 || m == M_NH || m == (M_NH | M_U1) || m == (M_NH | M_SG) ||
-         m == (M_NH | M_U1 | M_SG) || m == M_TR || m == (M_NH | M_TR);
+         m == (M_NH | M_U1 | M_SG) || m == M_TR || m == (M_NH | M_TR) || m == (M_NH | M_H2) ||
+         m == (M_NH | M_H2 | M_U1);
 }
 
 static int validate(const su2hip_params *p) {
@@ -162,8 +163,8 @@ static int validate(const su2hip_params *p) {
       return fail(SU2HIP_EINVAL, "extent L%d = %d must be even and >= 2", d + 1, p->L[d]);
   long long V = (long long)p->L[0] * p->L[1] * p->L[2];
   if (V > (1ll << 28)) return fail(SU2HIP_ERANGE, "volume %lld too large", V);
-  if (p->nhiggs < 0 || p->nhiggs > 1)
-    return fail(SU2HIP_EUNSUPPORTED, "nhiggs = %d (0 or 1 supported)", p->nhiggs);
+  if (p->nhiggs < 0 || p->nhiggs > 2)
+    return fail(SU2HIP_EUNSUPPORTED, "nhiggs = %d (0, 1 or 2 supported)", p->nhiggs);
   if (!supported_model(model_bits(*p)))
     return fail(SU2HIP_EUNSUPPORTED,
                 "field content nhiggs=%d u1=%d triplet=%d singlet=%d is not supported",
@@ -252,6 +253,7 @@ static void fill_desc(Slot &s) {
   L.U = take(24 * Vh);
   L.A = p.u1 ? take(6 * Vh) : nullptr;
   L.F = p.nhiggs ? take(8 * Vh) : nullptr;
+  L.F2 = p.nhiggs == 2 ? take(8 * Vh) : nullptr;
   L.G = p.triplet ? take(6 * Vh) : nullptr;
   L.S = p.singlet ? take(2 * Vh) : nullptr;
   L.B = s.bu_mem;
@@ -280,6 +282,18 @@ static void fill_desc(Slot &s) {
   L.b4_s = p.b4_s;
   L.a1_s = p.a1_s;
   L.a2_s = p.a2_s;
+  L.msq2 = p.msq_phi2;
+  L.m12re = p.m12sq_re;
+  L.m12im = p.m12sq_im;
+  L.lam2 = p.lam2;
+  L.lam3 = p.lam3;
+  L.lam4 = p.lam4;
+  L.lam5re = p.lam5_re;
+  L.lam5im = p.lam5_im;
+  L.lam6re = p.lam6_re;
+  L.lam6im = p.lam6_im;
+  L.lam7re = p.lam7_re;
+  L.lam7im = p.lam7_im;
 }
 
 static size_t slot_bytes(const su2hip_params &p) {
@@ -287,6 +301,7 @@ static size_t slot_bytes(const su2hip_params &p) {
   size_t b = align256(24 * Vh * w);
   if (p.u1) b += align256(6 * Vh * w);
   if (p.nhiggs) b += align256(8 * Vh * w);
+  if (p.nhiggs == 2) b += align256(8 * Vh * w);
   if (p.triplet) b += align256(6 * Vh * w);
   if (p.singlet) b += align256(2 * Vh * w);
   return b;
@@ -360,6 +375,14 @@ static void add_counts(su2hip_counters &c, int kind, long long acc, long long to
     case SU2HIP_K_SINGLET_METROPOLIS:
       c.accepted_singlet += acc;
       c.total_singlet += tot;
+      break;
+    case CNT_DOUBLET2_OR:
+      c.acc_overrelax_doublet2 += acc;
+      c.total_overrelax_doublet2 += tot;
+      break;
+    case CNT_DOUBLET2_METRO:
+      c.accepted_doublet2 += acc;
+      c.total_doublet2 += tot;
       break;
   }
 }
@@ -488,10 +511,12 @@ static int rebuild_map(su2hip_batch *b) {
   return SU2HIP_OK;
 }
 
+// field 1 the first doublet, 2 the triplet, 3 the second doublet
 static bool muca_on_field(const MucaDev &m, int field) {
   if (!m.enabled || !m.do_acceptance) return false;
   if (field == 1) return m.orderparam == SU2HIP_PHISQ || m.orderparam == SU2HIP_PHI2MINUSSIGMA2;
   if (field == 2) return m.orderparam == SU2HIP_SIGMASQ || m.orderparam == SU2HIP_PHI2MINUSSIGMA2;
+  if (field == 3) return m.orderparam == SU2HIP_PHI2SQ;
   return false;
 }
 
@@ -512,11 +537,11 @@ static int scalar_round(int alg, bool last, int kor, int kme) {
 static void build_sched(su2hip_batch *b) {
   if (!b->sched_dirty) return;
   const su2hip_params &p = b->tmpl;
-  bool muca_f[3] = {false, false, false};
+  bool muca_f[4] = {false, false, false, false};
   int steps = 1;
   for (auto &s : b->slots) {
     if (!s.used) continue;
-    for (int f = 1; f <= 2; f++)
+    for (int f = 1; f <= 3; f++)
       if (muca_on_field(s.hm, f)) {
         muca_f[f] = true;
         steps = std::max(steps, muca_nsteps(s.hd.Vh, std::max(1, s.hm.cps)));
@@ -546,9 +571,11 @@ static void build_sched(su2hip_batch *b) {
     if (p.nhiggs) {
       int kd = scalar_round(p.algorithm_su2doublet, last, SU2HIP_K_DOUBLET_OVERRELAX,
                             SU2HIP_K_DOUBLET_METROPOLIS);
+      // first EVEN and ODD of doublet 1, then doublet 2 (src/update.c:361-373)
       if (kd >= 0)
         for (int k = 0; k < p.update_su2doublet; k++)
-          for (int par = 0; par < 2; par++) add(kd, par, 0, muca_f[1]);
+          for (int h = 0; h < p.nhiggs; h++)
+            for (int par = 0; par < 2; par++) add(kd, par, h, muca_f[h ? 3 : 1]);
     }
     if (p.triplet) {
       int kt = scalar_round(p.algorithm_su2triplet, last, SU2HIP_K_TRIPLET_OVERRELAX,
@@ -611,7 +638,7 @@ static hipError_t enqueue_step(su2hip_batch *b, const Step &st) {
     });
     if (e != hipSuccess) return e;
     e = timed(b, SU2HIP_K_MUCA,
-              [&] { return launch_muca_accept(c, st.kind, st.parity, st.pass_off, k); });
+              [&] { return launch_muca_accept(c, st.kind, st.dir, st.parity, st.pass_off, k); });
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -909,11 +936,13 @@ int su2hip_set_fields(su2hip_batch_t *b, int slot, const double *su2link, const 
   size_t V = vol_of(p);
   rc = ensure_stage(b, V * 12);
   if (rc) return rc;
-  const double *src[5] = {su2link, u1link, doublet, triplet, singlet};
-  int have[5] = {1, p.u1, p.nhiggs, p.triplet, p.singlet};
-  int dofs[5] = {12, 3, 4, 3, 1};
+  // the second doublet follows the first in `doublet` (field 5 on the device)
+  const double *src[6] = {su2link, u1link, doublet, triplet, singlet,
+                          doublet && p.nhiggs == 2 ? doublet + V * 4 : nullptr};
+  int have[6] = {1, p.u1, p.nhiggs, p.triplet, p.singlet, p.nhiggs == 2};
+  int dofs[6] = {12, 3, 4, 3, 1, 4};
   LaunchCtx c = ctx_of(b);
-  for (int f = 0; f < 5; f++) {
+  for (int f = 0; f < 6; f++) {
     if (!src[f]) continue;
     if (!have[f]) return fail(SU2HIP_EINVAL, "field %d is not part of this model", f);
     HIPTRY(hipMemcpyAsync(b->d_stage, src[f], V * dofs[f] * sizeof(double), hipMemcpyHostToDevice,
@@ -934,11 +963,12 @@ int su2hip_get_fields(su2hip_batch_t *b, int slot, double *su2link, double *u1li
   size_t V = vol_of(p);
   rc = ensure_stage(b, V * 12);
   if (rc) return rc;
-  double *dst[5] = {su2link, u1link, doublet, triplet, singlet};
-  int have[5] = {1, p.u1, p.nhiggs, p.triplet, p.singlet};
-  int dofs[5] = {12, 3, 4, 3, 1};
+  double *dst[6] = {su2link, u1link, doublet, triplet, singlet,
+                    doublet && p.nhiggs == 2 ? doublet + V * 4 : nullptr};
+  int have[6] = {1, p.u1, p.nhiggs, p.triplet, p.singlet, p.nhiggs == 2};
+  int dofs[6] = {12, 3, 4, 3, 1, 4};
   LaunchCtx c = ctx_of(b);
-  for (int f = 0; f < 5; f++) {
+  for (int f = 0; f < 6; f++) {
     if (!dst[f]) continue;
     if (!have[f]) return fail(SU2HIP_EINVAL, "field %d is not part of this model", f);
     HIPTRY(launch_to_host_layout(c, slot, f, b->d_stage));
@@ -1079,11 +1109,14 @@ int su2hip_get_budgets(su2hip_batch_t *b, int64_t *budgets) {
 
 int su2hip_pass(su2hip_batch_t *b, int kind, int parity, int dir) {
   if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
-  if (kind < 0 || kind >= NCOUNT) return fail(SU2HIP_EINVAL, "kind %d", kind);
+  if (kind < 0 || kind > SU2HIP_K_SINGLET_METROPOLIS) return fail(SU2HIP_EINVAL, "kind %d", kind);
   if (parity < 0 || parity > 1 || dir < 0 || dir > 2) return fail(SU2HIP_EINVAL, "parity/dir");
+  bool dbl = kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS;
+  if (dbl && b->tmpl.nhiggs < 2) dir = 0;  // one doublet: dir is ignored as for other scalars
+  if (dbl && dir > 1) dir = 1;
   int rc = prepare(b);
   if (rc) return rc;
-  int field = (kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS)   ? 1
+  int field = dbl ? (dir ? 3 : 1)
               : (kind == SU2HIP_K_TRIPLET_OVERRELAX || kind == SU2HIP_K_TRIPLET_METROPOLIS) ? 2
                                                                                              : 0;
   bool muca = false;
@@ -1212,6 +1245,13 @@ int su2hip_measure_columns(const su2hip_params *p, const double *s, double weigh
     cols[n++] = s[SU2HIP_SUM_PHI2] / vol;
     cols[n++] = s[SU2HIP_SUM_PHI4] / vol;
   }
+  if (p->nhiggs == 2) {
+    cols[n++] = s[SU2HIP_SUM_COV_PHI_2] / (vol * 3);
+    cols[n++] = s[SU2HIP_SUM_PHI2_2] / vol;
+    cols[n++] = s[SU2HIP_SUM_PHI4_2] / vol;
+    cols[n++] = s[SU2HIP_SUM_R12] / vol;
+    cols[n++] = s[SU2HIP_SUM_I12] / vol;
+  }
   if (p->triplet) {
     cols[n++] = s[SU2HIP_SUM_HOP_SIGMA] / (vol * 3);
     cols[n++] = s[SU2HIP_SUM_SIGMA2] / vol;
@@ -1248,7 +1288,8 @@ int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m, const dou
     if (m->mode < SU2HIP_READONLY || m->mode > SU2HIP_SLOW) return fail(SU2HIP_EINVAL, "mode %d", m->mode);
     int op = m->orderparam;
     bool ok = (op == SU2HIP_PHISQ && s.p.nhiggs) || (op == SU2HIP_SIGMASQ && s.p.triplet) ||
-              (op == SU2HIP_PHI2MINUSSIGMA2 && s.p.nhiggs && s.p.triplet);
+              (op == SU2HIP_PHI2MINUSSIGMA2 && s.p.nhiggs && s.p.triplet) ||
+              (op == SU2HIP_PHI2SQ && s.p.nhiggs == 2);
     if (!ok) return fail(SU2HIP_EINVAL, "order parameter %d needs a field this model lacks", op);
   }
   HIPTRY(hipStreamSynchronize(b->stream));
@@ -1914,7 +1955,10 @@ static int time_pass_impl(su2hip_batch *b, int kind, int repeats, double *ms_ker
   if (kind < 0 || kind >= NCOUNT || repeats < 1) return fail(SU2HIP_EINVAL, "kind/repeats");
   int rc = prepare(b);
   if (rc) return rc;
+  // directions of link kinds, doublets of doublet kinds
   int ndir = (kind <= SU2HIP_K_U1_METROPOLIS) ? 3 : 1;
+  if (kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS)
+    ndir = std::max(1, (int)b->tmpl.nhiggs);
   // one start/stop pair per launch, stamped by the kernel dispatch itself
   std::vector<hipEvent_t> ev(2 * (size_t)repeats + 2, nullptr);
   hipError_t e = hipSuccess;

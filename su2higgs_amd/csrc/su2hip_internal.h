@@ -7,6 +7,7 @@
 // preserved" site order, src/layout.c:826-851):
 //   links    U[d][p][s][c]   d<3, c<4    (24 Vh words; a 4-vector per link)
 //   doublet  F[p][s][c]      c<4         (8 Vh; a 4-vector per site)
+//   doublet2 F2[p][s][c]     c<4         (8 Vh; the second doublet, nhiggs = 2)
 //   U(1)     A[d][p][s]                  (6 Vh)
 //   triplet  G[c][p][s]      c<3         (6 Vh)
 //   singlet  S[p][s]                     (2 Vh)
@@ -24,17 +25,20 @@
 namespace su2 {
 
 constexpr int BLK = 256;          // threads per workgroup in every site kernel
-constexpr int NCOUNT = 10;        // counted pass kinds (SU2HIP_K_LINK_HEATBATH .. SINGLET_METROPOLIS)
+constexpr int NCOUNT = 12;        // counter rows: pass kinds SU2HIP_K_LINK_HEATBATH .. SINGLET_METROPOLIS,
+                                  // then the second doublet's two kinds
+constexpr int CNT_DOUBLET2_OR = 10;
+constexpr int CNT_DOUBLET2_METRO = 11;
 constexpr uint32_t MUCA_SITE = 0xFFFFFFFFu;
 constexpr uint64_t HOTSTART_PASS = 0xFFFFFFFFFFFFFFFFull;
 constexpr uint64_t WALL_PASS = 0xFFFFFFFFFFFFFFFEull;
 
-// model bits
-enum : int { M_NH = 1, M_U1 = 2, M_TR = 4, M_SG = 8 };
+// model bits; M_H2 (two doublets) always comes with M_NH
+enum : int { M_NH = 1, M_U1 = 2, M_TR = 4, M_SG = 8, M_H2 = 16 };
 
 inline int model_bits(const su2hip_params &p) {
-  return (p.nhiggs ? M_NH : 0) | (p.u1 ? M_U1 : 0) | (p.triplet ? M_TR : 0) |
-         (p.singlet ? M_SG : 0);
+  return (p.nhiggs ? M_NH : 0) | (p.nhiggs == 2 ? M_H2 : 0) | (p.u1 ? M_U1 : 0) |
+         (p.triplet ? M_TR : 0) | (p.singlet ? M_SG : 0);
 }
 
 // fast division by a runtime constant (Granlund-Montgomery, round-up
@@ -65,6 +69,7 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 // Per-lattice descriptor, resident in device memory.
 struct LatDesc {
   void *U, *A, *F, *G, *S, *B;  // field arrays (element type = precision)
+  void *F2;                     // second doublet (nhiggs = 2), else null
   int L0, L1, L2, h0;           // extents, h0 = L0/2
   int Vh, vol;                  // sites per parity, volume
   FastDiv div_h0, div_L1;
@@ -77,6 +82,9 @@ struct LatDesc {
   int nblk;                     // logical blocks of this lattice
   double beta, betau1, r_u1, msq, lambda, msq_t, a2, b4;
   double b1_s, msq_s, b3_s, b4_s, a1_s, a2_s;
+  // two-doublet potential (src/su2u1.c:537-554)
+  double msq2, m12re, m12im, lam2, lam3, lam4;
+  double lam5re, lam5im, lam6re, lam6im, lam7re, lam7im;
 };
 
 // Multicanonical state of one lattice on the device.
@@ -121,12 +129,14 @@ struct LaunchCtx {
   int block = BLK;
 };
 
-// One checkerboard pass.  muca_step < 0: plain pass; otherwise the pass is
-// step `muca_step` of a multicanonical parity sweep.
+// One checkerboard pass (dir: the direction of a link kind, the doublet of
+// a doublet kind).  muca_step < 0: plain pass; otherwise the pass is step
+// `muca_step` of a multicanonical parity sweep.
 hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir,
                        int pass_off, int muca_step, int reunit);
-// multicanonical global accept/reject after a scalar pass step
-hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int parity,
+// multicanonical global accept/reject after a scalar pass step (hid: the
+// doublet of a doublet kind)
+hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int hid, int parity,
                               int pass_off, int muca_step);
 // sweep epilogue: pass counter += passes, budget -= 1
 hipError_t launch_end_sweep(const LaunchCtx &c, int passes);

@@ -97,6 +97,15 @@ __device__ __forceinline__ Q<T> link_staple(const LatDesc &L, const T *U, const 
     Q<T> t = qmul(b, qconj(F_at(L, F, x)));
     V = qadd(V, qscale(t, (T)-0.5));
   }
+  if constexpr ((M & M_H2) != 0) {
+    // the second doublet's hopping term (the doublet loop of
+    // src/staples.c:150-190)
+    const T *F2 = (const T *)L.F2;
+    Q<T> b = F_at(L, F2, shift(L, x, mu, +1));
+    if constexpr ((M & M_U1) != 0) b = qmul(b, u1twist<T>(A_at(L, A, x, mu)));
+    Q<T> t = qmul(b, qconj(F_at(L, F2, x)));
+    V = qadd(V, qscale(t, (T)-0.5));
+  }
   return V;
 }
 
@@ -236,6 +245,75 @@ __device__ __forceinline__ Q<T> normalise(const Q<T> &q) {
   if constexpr (sizeof(T) == 8) n = 1.0 / ::sqrt(qdot(q, q));
   else n = frsqrt(qdot(q, q));
   return qscale(q, n);
+}
+
+template <typename T>
+__device__ __forceinline__ Q<double> qd(const Q<T> &q) {
+  return Q<double>{(double)q.a, (double)q.b, (double)q.c, (double)q.d};
+}
+
+// higgspotential of two doublets (src/su2u1.c:528-554), f1 = phi_1 and
+// f2 = phi_2 at one site
+__device__ __forceinline__ double pot2(const LatDesc &L, const Q<double> &h1, const Q<double> &h2) {
+  double f11 = 0.5 * qdot(h1, h1), f22 = 0.5 * qdot(h2, h2);
+  // get_phi12 (src/su2u1.c:513-520)
+  double R = 0.5 * (h1.a * h2.a + h1.b * h2.b + h1.c * h2.c + h1.d * h2.d);
+  double I = 0.5 * (h1.d * h2.a + h1.c * h2.b - h1.b * h2.c - h1.a * h2.d);
+  return L.msq * f11 + L.lambda * f11 * f11 + L.msq2 * f22 + L.m12re * R - L.m12im * I +
+         L.lam2 * f22 * f22 + L.lam3 * f11 * f22 + L.lam4 * (R * R + I * I) +
+         L.lam5re * (R * R - I * I) - 2.0 * L.lam5im * R * I + f11 * (L.lam6re * R - L.lam6im * I) +
+         f22 * (L.lam7re * R + L.lam7im * I);
+}
+
+// overrelax_higgs2's move (src/overrelax.c:295-417) for doublet h of two:
+// f the updated doublet, H the other one, st its hopping staple; returns the
+// unit direction e, X = f.e, the new X (nX) and the accept probability p.
+// The potential along e with Y = f - X e fixed is the quartic
+// b4 X^4 + b3 X^3 + b2 X^2 + b1 X, whose other root of V(X') = V(X) is the
+// real root of a cubic (polysolve3).
+__device__ __forceinline__ void higgs2_or(const LatDesc &L, int h, const Q<double> &f,
+                                          const Q<double> &H, const Q<double> &st, Q<double> &e,
+                                          double &X, double &nX, double &p) {
+  double sg = h ? 1.0 : -1.0;
+  Q<double> G{sg * H.d, sg * H.c, -sg * H.b, -sg * H.a};
+  double lam12, msq, l67re, l67im, o67re, o67im;
+  if (h == 0) {
+    lam12 = L.lambda;
+    msq = L.msq;
+    l67re = L.lam7re;
+    l67im = L.lam7im;
+    o67re = L.lam6re;
+    o67im = -L.lam6im;
+  } else {
+    lam12 = L.lam2;
+    msq = L.msq2;
+    l67re = L.lam6re;
+    l67im = -L.lam6im;
+    o67re = L.lam7re;
+    o67im = L.lam7im;
+  }
+  double Hsq = 0.5 * qdot(H, H);
+  double cH = 0.5 * (L.m12re + Hsq * l67re), cG = 0.5 * (-1.0 * L.m12im + Hsq * l67im);
+  Q<double> s{-(st.a + cH * H.a + cG * G.a), -(st.b + cH * H.b + cG * G.b),
+              -(st.c + cH * H.c + cG * G.c), -(st.d + cH * H.d + cG * G.d)};
+  double F = ::sqrt(qdot(s, s));
+  e = Q<double>{s.a / F, s.b / F, s.c / F, s.d / F};
+  X = qdot(f, e);
+  Q<double> Y{f.a - X * e.a, f.b - X * e.b, f.c - X * e.c, f.d - X * e.d};
+  double Ysq = qdot(Y, Y);
+  double Hf = qdot(H, e), Gf = qdot(G, e), HY = qdot(H, Y), GY = qdot(G, Y);
+  double b4 = 0.25 * lam12;
+  double b3 = 0.25 * (Hf * o67re + Gf * o67im);
+  double b2 = 0.5 * msq + 1.0 * 3 + 0.5 * Ysq * lam12 + 0.25 * (HY * o67re + GY * o67im);
+  double b1 = -1.0 * F + 0.25 * Ysq * (Hf * o67re + Gf * o67im);
+  b2 += 0.5 * Hsq * L.lam3 + 0.25 * L.lam4 * (Hf * Hf + Gf * Gf) +
+        0.25 * L.lam5re * (Hf * Hf - Gf * Gf) - 0.5 * L.lam5im * Gf * Hf;
+  b1 += 0.5 * Hf * (HY * L.lam4 + HY * L.lam5re - GY * L.lam5im) +
+        0.5 * Gf * (GY * L.lam4 - GY * L.lam5re - HY * L.lam5im);
+  nX = polysolve3(b4, b3 + b4 * X, b2 + b3 * X + b4 * X * X, b1 + b2 * X + b3 * X * X + b4 * X * X * X);
+  double dV = 4.0 * b4 * X * X * X + 3.0 * b3 * X * X + 2.0 * b2 * X + b1;
+  double dVn = 4.0 * b4 * nX * nX * nX + 3.0 * b3 * nX * nX + 2.0 * b2 * nX + b1;
+  p = ::fabs(dV / dVn);
 }
 
 // ------------------------------------------------------------------------
@@ -411,6 +489,10 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       if constexpr ((M & M_NH) != 0) {
         // hop = -(q0 cos a + q3 sin a), q = Phi(x)^+ U_mu(x) Phi(x+mu)
         Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F, xp));
+        if constexpr ((M & M_H2) != 0) {
+          const T *F2 = (const T *)L.F2;
+          q = qadd(q, qmul(qmul(qconj(ldF(F2, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F2, xp)));
+        }
         dS -= q.a * (::cos(na) - ::cos(al)) + q.d * (::sin(na) - ::sin(al));
       }
     } else {
@@ -431,12 +513,50 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       dS = (T)2 * (T)L.betau1 * fsin(rr * h) * sum;
       if constexpr ((M & M_NH) != 0) {
         Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F, xp));
+        if constexpr ((M & M_H2) != 0) {
+          const T *F2 = (const T *)L.F2;
+          q = qadd(q, qmul(qmul(qconj(ldF(F2, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F2, xp)));
+        }
         // -(q0 (cos na - cos al) + q3 (sin na - sin al))
         dS -= (T)2 * fsin(h) * (q.d * fcos(m) - q.a * fsin(m));
       }
     }
     if (!metro_accept(dS, r)) return 0;
     A[(size_t)(mu * 2 + par) * Vh + s] = na;
+    return 1;
+  } else if constexpr ((KIND == SU2HIP_K_DOUBLET_OVERRELAX ||
+                        KIND == SU2HIP_K_DOUBLET_METROPOLIS) && (M & M_H2) != 0) {
+    // two doublets; mu is the one updated (the reference's higgs_id)
+    T *Fh = mu ? (T *)L.F2 : F;
+    const T *Fo = mu ? (const T *)F : (const T *)L.F2;
+    Q<T> ft = ldF(Fh, Vh, par, s);
+    Q<double> f = qd(ft), H = qd(ldF(Fo, Vh, par, s));
+    Q<double> st = qd(doublet_staple<T, M>(L, U, A, Fh, x));
+    Q<T> nf;
+    if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX) {
+      // overrelax_higgs2 (src/overrelax.c:295-417) in FP64 for both
+      // precisions: Y kept, X moved to the other root of the quartic
+      double nX, X, p;
+      Q<double> e;
+      higgs2_or(L, mu, f, H, st, e, X, nX, p);
+      if (!(p >= (double)uni<T>(r))) return 0;
+      double k = nX - X;
+      nf = Q<T>{(T)(f.a + e.a * k), (T)(f.b + e.b * k), (T)(f.c + e.c * k), (T)(f.d + e.d * k)};
+    } else {
+      // metro_doublet (src/metropolis.c:77-106) with localact_doublet
+      // (src/su2u1.c:495-510) as a difference: hopping, kinetic and the
+      // two-doublet potential
+      nf = Q<T>{ft.a + (T)1 * (uni<T>(r) - (T)0.5), ft.b + (T)1 * (uni<T>(r) - (T)0.5),
+                ft.c + (T)1 * (uni<T>(r) - (T)0.5), ft.d + (T)1 * (uni<T>(r) - (T)0.5)};
+      Q<double> n = qd(nf);
+      double o2 = qdot(f, f), n2 = qdot(n, n);
+      double pn = mu ? pot2(L, H, n) : pot2(L, n, H);
+      double po = mu ? pot2(L, H, f) : pot2(L, f, H);
+      double diff = (n.a - f.a) * st.a + (n.b - f.b) * st.b + (n.c - f.c) * st.c +
+                    (n.d - f.d) * st.d + 3.0 * (n2 - o2) + (pn - po);
+      if (!metro_accept((T)diff, r)) return 0;
+    }
+    stF(Fh, Vh, par, s, nf);
     return 1;
   } else if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX ||
                        KIND == SU2HIP_K_DOUBLET_METROPOLIS) {
@@ -599,10 +719,13 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
 // ------------------------------------------------------------------------
 // multicanonical helpers (src/multicanonical.c, src/update.c:82-156)
 
-__device__ __forceinline__ bool muca_field_on(const MucaDev &m, int kind) {
+// hid: the doublet of a doublet kind (the first carries PHISQ and
+// PHI2MINUSSIGMA2, the second PHI2SQ, src/update.c:95-101)
+__device__ __forceinline__ bool muca_field_on(const MucaDev &m, int kind, int hid) {
   if (!m.enabled || !m.do_acceptance) return false;
   bool dbl = kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS;
   bool trp = kind == SU2HIP_K_TRIPLET_OVERRELAX || kind == SU2HIP_K_TRIPLET_METROPOLIS;
+  if (dbl && hid) return m.orderparam == SU2HIP_PHI2SQ;
   if (dbl) return m.orderparam == SU2HIP_PHISQ || m.orderparam == SU2HIP_PHI2MINUSSIGMA2;
   if (trp) return m.orderparam == SU2HIP_SIGMASQ || m.orderparam == SU2HIP_PHI2MINUSSIGMA2;
   return false;
@@ -628,6 +751,12 @@ __device__ __forceinline__ double orderparam_at(const LatDesc &L, int op, int pa
   if constexpr ((M & M_NH) != 0) {
     if (op == SU2HIP_PHISQ || op == SU2HIP_PHI2MINUSSIGMA2) {
       Q<T> f = ldF((const T *)L.F, L.Vh, par, s);
+      v += 0.5 * ((double)f.a * f.a + (double)f.b * f.b + (double)f.c * f.c + (double)f.d * f.d);
+    }
+  }
+  if constexpr ((M & M_H2) != 0) {
+    if (op == SU2HIP_PHI2SQ) {
+      Q<T> f = ldF((const T *)L.F2, L.Vh, par, s);
       v += 0.5 * ((double)f.a * f.a + (double)f.b * f.b + (double)f.c * f.c + (double)f.d * f.d);
     }
   }
@@ -662,7 +791,7 @@ __global__ __launch_bounds__(BLK) void k_pass(const LatDesc *__restrict__ lats,
   int op = 0;
   if (step >= 0) {
     const MucaDev &m = mucas[bm.lat];
-    muca = muca_field_on(m, KIND);
+    muca = muca_field_on(m, KIND, (M & M_H2) ? mu : 0);
     if (muca) {
       if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
       op = m.orderparam;
@@ -677,7 +806,7 @@ __global__ __launch_bounds__(BLK) void k_pass(const LatDesc *__restrict__ lats,
       // backup of every site of the step, accepted locally or not: a
       // rejected global check restores the whole step from it
       if constexpr (KIND == SU2HIP_K_DOUBLET_OVERRELAX || KIND == SU2HIP_K_DOUBLET_METROPOLIS)
-        stF((T *)L.B, L.Vh, par, s, ldF((const T *)L.F, L.Vh, par, s));
+        stF((T *)L.B, L.Vh, par, s, ldF((const T *)((M & M_H2) && mu ? L.F2 : L.F), L.Vh, par, s));
       if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX || KIND == SU2HIP_K_TRIPLET_METROPOLIS)
         stG((T *)L.B, L.Vh, par, s, ldG((const T *)L.G, L.Vh, par, s));
     }
@@ -689,8 +818,13 @@ __global__ __launch_bounds__(BLK) void k_pass(const LatDesc *__restrict__ lats,
   int nacc = __syncthreads_count(acc);
   int ntot = __syncthreads_count(upd);
   if (threadIdx.x == 0) {
-    cnt[(size_t)(KIND * 2 + 0) * nblk + blockIdx.x] += nacc;
-    cnt[(size_t)(KIND * 2 + 1) * nblk + blockIdx.x] += ntot;
+    int row = KIND;
+    if constexpr ((M & M_H2) != 0 && KIND == SU2HIP_K_DOUBLET_OVERRELAX)
+      row = mu ? CNT_DOUBLET2_OR : KIND;
+    if constexpr ((M & M_H2) != 0 && KIND == SU2HIP_K_DOUBLET_METROPOLIS)
+      row = mu ? CNT_DOUBLET2_METRO : KIND;
+    cnt[(size_t)(row * 2 + 0) * nblk + blockIdx.x] += nacc;
+    cnt[(size_t)(row * 2 + 1) * nblk + blockIdx.x] += ntot;
   }
   if (muca) {
     double v = valid ? orderparam_at<T, M>(L, op, par, s) : 0.0;
@@ -854,11 +988,11 @@ __device__ void muca_update_slow(MucaDev &m) {
 __global__ __launch_bounds__(BLK) void k_muca_accept(const LatDesc *__restrict__ lats,
                                                      MucaDev *__restrict__ mucas,
                                                      const double *__restrict__ partial, int kind,
-                                                     int par, int pass_off, int step) {
+                                                     int hid, int par, int pass_off, int step) {
   __shared__ double lds[BLK / 64];
   const LatDesc &L = lats[blockIdx.x];
   MucaDev &m = mucas[blockIdx.x];
-  if (!L.active || L.budget == 0 || !muca_field_on(m, kind)) return;
+  if (!L.active || L.budget == 0 || !muca_field_on(m, kind, hid)) return;
   int lo, hi;
   if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
   // fixed-order sum of this lattice's block partials
@@ -899,18 +1033,18 @@ template <typename T>
 __global__ __launch_bounds__(BLK) void k_muca_restore(const LatDesc *__restrict__ lats,
                                                       const MucaDev *__restrict__ mucas,
                                                       const BlockMap *__restrict__ blk, int kind,
-                                                      int par, int step) {
+                                                      int hid, int par, int step) {
   const BlockMap bm = blk[blockIdx.x];
   const LatDesc &L = lats[bm.lat];
   const MucaDev &m = mucas[bm.lat];
-  if (!L.active || L.budget == 0 || !muca_field_on(m, kind) || !m.rejected) return;
+  if (!L.active || L.budget == 0 || !muca_field_on(m, kind, hid) || !m.rejected) return;
   int lo, hi;
   if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
   int s = bm.s0 + (int)threadIdx.x;
   if (s >= L.Vh || s < lo || s >= hi) return;
   bool dbl = kind == SU2HIP_K_DOUBLET_OVERRELAX || kind == SU2HIP_K_DOUBLET_METROPOLIS;
   if (dbl)
-    stF((T *)L.F, L.Vh, par, s, ldF((const T *)L.B, L.Vh, par, s));
+    stF((T *)(hid ? L.F2 : L.F), L.Vh, par, s, ldF((const T *)L.B, L.Vh, par, s));
   else
     stG((T *)L.G, L.Vh, par, s, ldG((const T *)L.B, L.Vh, par, s));
 }
@@ -1065,11 +1199,6 @@ __device__ double magfield(const LatDesc &L, const T *U, const T *G, const Site 
   return res;
 }
 
-template <typename T>
-__device__ __forceinline__ Q<double> qd(const Q<T> &q) {
-  return Q<double>{(double)q.a, (double)q.b, (double)q.c, (double)q.d};
-}
-
 template <typename T, int M>
 __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
   const T *U = (const T *)L.U, *A = (const T *)L.A, *F = (const T *)L.F, *G = (const T *)L.G,
@@ -1125,7 +1254,33 @@ __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
     acc[SU2HIP_SUM_COV_PHI] += cov;
     acc[SU2HIP_SUM_PHI2] += mod;
     acc[SU2HIP_SUM_PHI4] += mod * mod;
-    action += cov + L.msq * mod + L.lambda * mod * mod;
+    if constexpr ((M & M_H2) != 0) {
+      // the second doublet's covariant derivative, moments and phi1^+ phi2;
+      // pot2 holds the whole two-doublet potential
+      const T *F2 = (const T *)L.F2;
+      Q<double> f2 = qd(F_at(L, F2, x));
+      double mod2 = 0.5 * qdot(f2, f2), cov2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        Q<double> t = qmul(qmul(qconj(f2), qd(U_at(L, U, x, d))), qd(F_at(L, F2, shift(L, x, d, +1))));
+        double hop;
+        if constexpr ((M & M_U1) != 0) {
+          double al = A_at(L, A, x, d);
+          hop = -(t.a * ::cos(al) + t.d * ::sin(al));
+        } else {
+          hop = -t.a;
+        }
+        cov2 += 2.0 * mod2 + hop;
+      }
+      acc[SU2HIP_SUM_COV_PHI_2] += cov2;
+      acc[SU2HIP_SUM_PHI2_2] += mod2;
+      acc[SU2HIP_SUM_PHI4_2] += mod2 * mod2;
+      acc[SU2HIP_SUM_R12] += 0.5 * (f.a * f2.a + f.b * f2.b + f.c * f2.c + f.d * f2.d);
+      acc[SU2HIP_SUM_I12] += 0.5 * (f.d * f2.a + f.c * f2.b - f.b * f2.c - f.a * f2.d);
+      action += cov + cov2 + pot2(L, f, f2);
+    } else {
+      action += cov + L.msq * mod + L.lambda * mod * mod;
+    }
   }
   if constexpr ((M & M_TR) != 0) {
     V3<T> gt = G_at(L, G, x);
@@ -1348,9 +1503,11 @@ __global__ void k_wall(LatDesc L, int zdir) {
     Rng r;
     rng_init(r, L.key0, L.key1, WALL_PASS, (uint32_t)(par * L.Vh + s));
     if constexpr ((M & M_NH) != 0) {
-      double a = lo ? 0.2 : 1.2, e = lo ? 0.01 : 0.05;
-      double f0 = a + e * rng_u53(r), f1 = e * rng_u53(r), f2 = e * rng_u53(r), f3 = e * rng_u53(r);
-      stF((T *)L.F, L.Vh, par, s, Q<T>{(T)f0, (T)f1, (T)f2, (T)f3});
+      for (int h = 0; h < ((M & M_H2) ? 2 : 1); h++) {
+        double a = lo ? 0.2 : 1.2, e = lo ? 0.01 : 0.05;
+        double f0 = a + e * rng_u53(r), f1 = e * rng_u53(r), f2 = e * rng_u53(r), f3 = e * rng_u53(r);
+        stF((T *)(h ? L.F2 : L.F), L.Vh, par, s, Q<T>{(T)f0, (T)f1, (T)f2, (T)f3});
+      }
     }
     if constexpr ((M & M_TR) != 0) {
       double a = lo ? 1.5 : 0.2, e = lo ? 0.05 : 0.01;
@@ -1474,6 +1631,7 @@ __global__ void k_cold(LatDesc L, double phi0, double sigma0, double singlet0) {
       if constexpr ((M & M_U1) != 0) ((T *)L.A)[(size_t)(d * 2 + par) * L.Vh + s] = 0;
     }
     if constexpr ((M & M_NH) != 0) stF((T *)L.F, L.Vh, par, s, Q<T>{(T)phi0, (T)0, (T)0, (T)0});
+    if constexpr ((M & M_H2) != 0) stF((T *)L.F2, L.Vh, par, s, Q<T>{(T)phi0, (T)0, (T)0, (T)0});
     if constexpr ((M & M_TR) != 0) stG((T *)L.G, L.Vh, par, s, V3<T>{(T)sigma0, (T)0, (T)0});
     if constexpr ((M & M_SG) != 0) ((T *)L.S)[(size_t)par * L.Vh + s] = (T)singlet0;
   }
@@ -1506,10 +1664,12 @@ __global__ void k_hot(LatDesc L) {
       for (int d = 0; d < 3; d++)
         ((T *)L.A)[(size_t)(d * 2 + par) * L.Vh + s] = (T)(PI * (2.0 * rng_u53(r) - 1.0));
     if constexpr ((M & M_NH) != 0) {
-      double z[4];
-      gauss2(r, z[0], z[1]);
-      gauss2(r, z[2], z[3]);
-      stF((T *)L.F, L.Vh, par, s, Q<T>{(T)z[0], (T)z[1], (T)z[2], (T)z[3]});
+      for (int h = 0; h < ((M & M_H2) ? 2 : 1); h++) {
+        double z[4];
+        gauss2(r, z[0], z[1]);
+        gauss2(r, z[2], z[3]);
+        stF((T *)(h ? L.F2 : L.F), L.Vh, par, s, Q<T>{(T)z[0], (T)z[1], (T)z[2], (T)z[3]});
+      }
     }
     if constexpr ((M & M_TR) != 0) {
       double z[4];
@@ -1526,9 +1686,22 @@ __global__ void k_hot(LatDesc L) {
 }
 
 // reference host layout (site i = par*Vh + s, components innermost)
-// <-> device SoA; field 0 links, 1 U(1), 2 doublet, 3 triplet, 4 singlet
+// <-> device SoA; field 0 links, 1 U(1), 2 doublet, 3 triplet, 4 singlet,
+// 5 second doublet
 __host__ __device__ inline int field_dofs(int field) {
+  if (field == 5) return 4;
   return field == 0 ? 12 : (field == 1 ? 3 : (field == 2 ? 4 : (field == 3 ? 3 : 1)));
+}
+
+__device__ __forceinline__ void *field_ptr(const LatDesc &L, int field) {
+  switch (field) {
+    case 0: return L.U;
+    case 1: return L.A;
+    case 2: return L.F;
+    case 3: return L.G;
+    case 4: return L.S;
+    default: return L.F2;
+  }
 }
 
 __device__ __forceinline__ size_t soa_index(int field, int Vh, int par, int s, int k) {
@@ -1536,7 +1709,7 @@ __device__ __forceinline__ size_t soa_index(int field, int Vh, int par, int s, i
     int d = k >> 2, c = k & 3;
     return ((size_t)(d * 2 + par) * Vh + s) * 4 + c;
   }
-  if (field == 2) return ((size_t)par * Vh + s) * 4 + k;
+  if (field == 2 || field == 5) return ((size_t)par * Vh + s) * 4 + k;
   return (size_t)(k * 2 + par) * Vh + s;
 }
 
@@ -1544,7 +1717,7 @@ template <typename T>
 __global__ void k_to_device(LatDesc L, int field, const double *__restrict__ h) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.Vh) return;
-  void *dst = field == 0 ? L.U : (field == 1 ? L.A : (field == 2 ? L.F : (field == 3 ? L.G : L.S)));
+  void *dst = field_ptr(L, field);
   int n = field_dofs(field);
   for (int par = 0; par < 2; par++) {
     size_t i = (size_t)par * L.Vh + s;
@@ -1556,7 +1729,7 @@ template <typename T>
 __global__ void k_to_host(LatDesc L, int field, double *__restrict__ h) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.Vh) return;
-  const void *src = field == 0 ? L.U : (field == 1 ? L.A : (field == 2 ? L.F : (field == 3 ? L.G : L.S)));
+  const void *src = field_ptr(L, field);
   int n = field_dofs(field);
   for (int par = 0; par < 2; par++) {
     size_t i = (size_t)par * L.Vh + s;
@@ -1703,6 +1876,8 @@ static hipError_t with_model(int model, Fn &&fn) {
     case M_NH | M_U1 | M_SG: return fn(std::integral_constant<int, M_NH | M_U1 | M_SG>{});
     case M_TR: return fn(std::integral_constant<int, M_TR>{});
     case M_NH | M_TR: return fn(std::integral_constant<int, M_NH | M_TR>{});
+    case M_NH | M_H2: return fn(std::integral_constant<int, M_NH | M_H2>{});
+    case M_NH | M_H2 | M_U1: return fn(std::integral_constant<int, M_NH | M_H2 | M_U1>{});
   }
   return hipErrorInvalidValue;
 }
@@ -1758,16 +1933,16 @@ hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir, int pa
   });
 }
 
-hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int parity, int pass_off,
+hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int hid, int parity, int pass_off,
                               int muca_step) {
   hipLaunchKernelGGL(k_muca_accept, dim3(c.max_slots), dim3(BLK), 0, c.stream, c.desc, c.muca,
-                     c.partial, kind, parity, pass_off, muca_step);
+                     c.partial, kind, hid, parity, pass_off, muca_step);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || c.nblk == 0) return e;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
     hipLaunchKernelGGL(k_muca_restore<T>, dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.muca,
-                       c.blk, kind, parity, muca_step);
+                       c.blk, kind, hid, parity, muca_step);
     return hipGetLastError();
   });
 }

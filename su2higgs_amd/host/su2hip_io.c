@@ -177,7 +177,9 @@ static int parse_model(const char *s, su2hip_run_config *c) {
   snprintf(buf, sizeof(buf), "%s", s);
   for (char *tok = strtok(buf, "+,"); tok; tok = strtok(NULL, "+,")) {
     if (!strcasecmp(tok, "higgs"))
-      c->p.nhiggs = 1;
+      c->p.nhiggs = c->p.nhiggs == 2 ? 2 : 1;
+    else if (!strcasecmp(tok, "2hdm"))
+      c->p.nhiggs = 2;
     else if (!strcasecmp(tok, "u1"))
       c->p.u1 = 1;
     else if (!strcasecmp(tok, "triplet"))
@@ -297,6 +299,20 @@ int su2hip_config_read(const char *path, const char *model, su2hip_run_config *o
       GETD(&f, "a2_s", p->a2_s);
     }
   }
+  if (p->nhiggs == 2) {
+    GETD(&f, "msq_phi2", p->msq_phi2);
+    GETD(&f, "m12sq_re", p->m12sq_re);
+    GETD(&f, "m12sq_im", p->m12sq_im);
+    GETD(&f, "lam2", p->lam2);
+    GETD(&f, "lam3", p->lam3);
+    GETD(&f, "lam4", p->lam4);
+    GETD(&f, "lam5_re", p->lam5_re);
+    GETD(&f, "lam5_im", p->lam5_im);
+    GETD(&f, "lam6_re", p->lam6_re);
+    GETD(&f, "lam6_im", p->lam6_im);
+    GETD(&f, "lam7_re", p->lam7_re);
+    GETD(&f, "lam7_im", p->lam7_im);
+  }
   if (c.correlators) {
     GETI(&f, "do_correlators", c.do_correlators);
     GETI(&f, "correlator_interval", c.correlator_interval);
@@ -357,7 +373,7 @@ int su2hip_config_read(const char *path, const char *model, su2hip_run_config *o
     else if (!strcasecmp(v, "phi2minusSigma2"))
       c.muca.orderparam = SU2HIP_PHI2MINUSSIGMA2;
     else if (!strcasecmp(v, "phi2sq"))
-      return ioerr(SU2HIP_EUNSUPPORTED, "orderparam phi2sq needs two Higgs doublets");
+      c.muca.orderparam = SU2HIP_PHI2SQ;
     else {
       fprintf(stderr, "WARNING: Unknown multicanonical order parameter!! using phisq");
       c.muca.orderparam = SU2HIP_PHISQ;
@@ -414,7 +430,9 @@ int su2hip_lattice_write(const char *path, const su2hip_params *p, int64_t iter,
            fwrite(&comms_time, sizeof(double), 1, fp) == 1;
   ok = ok && fwrite(su2link, sizeof(double), V * 12, fp) == (size_t)(V * 12);
   if (ok && p->u1) ok = u1link && fwrite(u1link, sizeof(double), V * 3, fp) == (size_t)(V * 3);
-  if (ok && p->nhiggs) ok = doublet && fwrite(doublet, sizeof(double), V * 4, fp) == (size_t)(V * 4);
+  /* doublets one after the other (src/checkpoint.c:105-107) */
+  long nd = V * 4 * p->nhiggs;
+  if (ok && p->nhiggs) ok = doublet && fwrite(doublet, sizeof(double), nd, fp) == (size_t)nd;
   if (ok && p->triplet) ok = triplet && fwrite(triplet, sizeof(double), V * 3, fp) == (size_t)(V * 3);
   if (ok && p->singlet) ok = singlet && fwrite(singlet, sizeof(double), V, fp) == (size_t)V;
   if (fclose(fp) != 0) ok = 0;
@@ -453,7 +471,7 @@ int su2hip_lattice_read(const char *path, const su2hip_params *p, int64_t *iter,
   long V = vol_of(p);
   double *dst[5] = {su2link, u1link, doublet, triplet, singlet};
   int have[5] = {1, p->u1, p->nhiggs, p->triplet, p->singlet};
-  long n[5] = {V * 12, V * 3, V * 4, V * 3, V};
+  long n[5] = {V * 12, V * 3, V * 4 * p->nhiggs, V * 3, V};
   for (int k = 0; k < 5; k++) {
     if (!have[k]) continue;
     if (dst[k]) {
@@ -693,6 +711,13 @@ int su2hip_labels_write(const char *path, const su2hip_params *p) {
     fprintf(f, "%d phi^2\n", k++);
     fprintf(f, "%d phi^4\n", k++);
   }
+  if (p->nhiggs == 2) {
+    fprintf(f, "%d 2*(phi^2 - phi^+(x) U_i(x) phi(x+i)) for phi_2 (avg over directions)\n", k++);
+    fprintf(f, "%d phi2^2\n", k++);
+    fprintf(f, "%d phi2^4\n", k++);
+    fprintf(f, "%d R = Re phi1^+ phi2\n", k++);
+    fprintf(f, "%d I = Im phi1^+ phi2\n", k++);
+  }
   if (p->triplet) {
     fprintf(f, "%d hopping_Sigma (avg over directions)\n", k++);
     fprintf(f, "%d Sigma^2\n", k++);
@@ -747,13 +772,16 @@ int su2hip_print_acceptance(FILE *out, const su2hip_run_config *cfg, const su2hi
     fprintf(out, "SU(2) link %.2lf%%, ", 100.0 * c->accepted_su2link / c->total_su2link);
   if (p->u1 && p->algorithm_u1link == SU2HIP_METROPOLIS)
     fprintf(out, "U(1) link %.2lf%%, ", 100.0 * c->accepted_u1link / c->total_u1link);
-  if (p->nhiggs) {
+  for (int db = 0; db < p->nhiggs; db++) {
+    double acc = db ? c->accepted_doublet2 : c->accepted_doublet;
+    double tot = db ? c->total_doublet2 : c->total_doublet;
+    double oacc = db ? c->acc_overrelax_doublet2 : c->acc_overrelax_doublet;
+    double otot = db ? c->total_overrelax_doublet2 : c->total_overrelax_doublet;
     if (p->algorithm_su2doublet == SU2HIP_METROPOLIS)
-      fprintf(out, "Higgs#%d %.2lf%%, ", 1, 100.0 * c->accepted_doublet / c->total_doublet);
+      fprintf(out, "Higgs#%d %.2lf%%, ", db + 1, 100.0 * acc / tot);
     else if (p->algorithm_su2doublet == SU2HIP_OVERRELAX)
-      fprintf(out, "Higgs#%d overrelax %.2lf%%, Higgs#%d Metropolis %.2lf%%, ", 1,
-              100.0 * c->acc_overrelax_doublet / c->total_overrelax_doublet, 1,
-              100.0 * c->accepted_doublet / c->total_doublet);
+      fprintf(out, "Higgs#%d overrelax %.2lf%%, Higgs#%d Metropolis %.2lf%%, ", db + 1,
+              100.0 * oacc / otot, db + 1, 100.0 * acc / tot);
   }
   if (p->triplet) {
     if (p->algorithm_su2triplet == SU2HIP_METROPOLIS)

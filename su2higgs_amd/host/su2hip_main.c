@@ -43,7 +43,7 @@ static void alloc_fields(const su2hip_params *p, hostfields *h) {
   long V = (long)p->L[0] * p->L[1] * p->L[2];
   h->link = calloc(V * 12, sizeof(double));
   h->u1 = p->u1 ? calloc(V * 3, sizeof(double)) : NULL;
-  h->db = p->nhiggs ? calloc(V * 4, sizeof(double)) : NULL;
+  h->db = p->nhiggs ? calloc(V * 4 * p->nhiggs, sizeof(double)) : NULL;
   h->tr = p->triplet ? calloc(V * 3, sizeof(double)) : NULL;
   h->sg = p->singlet ? calloc(V, sizeof(double)) : NULL;
   if (!h->link) {
@@ -161,7 +161,8 @@ int main(int argc, char **argv) {
       path = argv[i];
   }
   if (!path) {
-    printf("Usage: su2hip [--model higgs+u1+triplet+singlet+gradflow] [--fp32] [--no-graph] <config file>\n");
+    printf("Usage: su2hip [--model higgs|2hdm+u1+triplet+singlet+gradflow+correlators+blocking+measure_z]"
+           " [--fp32] [--no-graph] <config file>\n");
     return 0;
   }
   su2hip_run_config cfg;

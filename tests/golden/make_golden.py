@@ -32,6 +32,8 @@ MODELS = {
     "t": dict(nhiggs=0, triplet=1),
     "ht": dict(nhiggs=1, triplet=1),
     "hsu": dict(nhiggs=1, u1=1, singlet=1),
+    "d": dict(nhiggs=2),
+    "du": dict(nhiggs=2, u1=1),
 }
 L = (4, 4, 6)
 
@@ -56,6 +58,9 @@ def model_fixture(tag):
     out["action_local"] = np.array([r.lib.ref_action_local(i) for i in range(V)])
     if p.nhiggs:
         out["doublet_staple"] = np.array([r.vec("ref_doublet_staple", 4, i) for i in range(V)])
+    if p.nhiggs == 2:
+        out["doublet2_staple"] = np.array([r.vec("ref_doublet_staple_h", 4, i, 1) for i in range(V)])
+        out["localact_doublet2"] = np.array([r.lib.ref_localact_doublet_h(i, 1) for i in range(V)])
     out["measure_sums"] = r.measure()
     out["measure_text"] = np.frombuffer(r.measure_text().encode(), dtype=np.uint8)
     # two compound sweeps of the reference's update_lattice, Philox seed 99
@@ -278,6 +283,10 @@ MUCA_CASES = {
     "muca_h_phisq_fast_i7": ("h", abi.PHISQ, abi.FAST, 1, 5.0, 7),
     "muca_h_phisq_slow_i30": ("h", abi.PHISQ, abi.SLOW, 2, 5.0, 30, False, 6, (1.0, 2.0)),
     "muca_t_sigmasq_slow_params": ("t", abi.SIGMASQ, abi.SLOW, 2, 5.0, 30, True, 6, (0.5, 1.5)),
+    # two doublets, the second one's phi^2 as order parameter (2 sweeps:
+    # the two-doublet sweep amplifies rounding)
+    "muca_d_phi2sq_fast2": ("d", abi.PHI2SQ, abi.FAST, 2, 5.0, None, False, 2),
+    "muca_du_phi2sq_ro": ("du", abi.PHI2SQ, abi.READONLY, 3, 400.0, None, False, 2),
 }
 
 

@@ -48,6 +48,8 @@ def oracle_lib():
             "orc_localact_su2link": (C.c_double, [h, C.c_long, C.c_int]),
             "orc_localact_u1link": (C.c_double, [h, C.c_long, C.c_int]),
             "orc_localact_doublet": (C.c_double, [h, C.c_long]),
+            "orc_doublet_staple_h": (None, [h, C.c_long, C.c_int, dp]),
+            "orc_localact_doublet_h": (C.c_double, [h, C.c_long, C.c_int]),
             "orc_localact_triplet": (C.c_double, [h, C.c_long]),
             "orc_localact_singlet": (C.c_double, [h, C.c_long]),
             "orc_action_local": (C.c_double, [h, C.c_long]),
@@ -99,7 +101,8 @@ FIELDS = ("su2link", "u1link", "doublet", "triplet", "singlet")
 def _arrs(p):
     V = volume(p)
     return {"su2link": np.zeros((V, 3, 4)), "u1link": np.zeros((V, 3)),
-            "doublet": np.zeros((V, 4)), "triplet": np.zeros((V, 3)),
+            "doublet": np.zeros((2, V, 4)) if p.nhiggs == 2 else np.zeros((V, 4)),
+            "triplet": np.zeros((V, 3)),
             "singlet": np.zeros(V)}
 
 
@@ -229,13 +232,14 @@ class Oracle:
 _refs = {}
 
 REF_FLAGS = {1: "h", 3: "hu", 4: "t", 5: "ht", 17: "hg", 20: "tg", 11: "hsu",
-             33: "hc", 36: "tc", 37: "htc", 65: "hz", 68: "tz", 75: "hsuz", 164: "tcb"}
+             33: "hc", 36: "tc", 37: "htc", 65: "hz", 68: "tz", 75: "hsuz", 164: "tcb",
+             257: "d", 259: "du", 275: "dug"}
 
 
 def ref_tag(p, gradflow=False, correlators=False, measure_z=False, blocking=False):
     m = (1 if p.nhiggs else 0) | (2 if p.u1 else 0) | (4 if p.triplet else 0) \
         | (8 if p.singlet else 0) | (16 if gradflow else 0) | (32 if correlators else 0) \
-        | (64 if measure_z else 0) | (128 if blocking else 0)
+        | (64 if measure_z else 0) | (128 if blocking else 0) | (256 if p.nhiggs == 2 else 0)
     return REF_FLAGS.get(m)
 
 
@@ -283,6 +287,8 @@ def ref_lib(tag):
     opt = {
         "ref_doublet_staple": (None, [C.c_long, dp]),
         "ref_localact_doublet": (C.c_double, [C.c_long]),
+        "ref_doublet_staple_h": (None, [C.c_long, C.c_int, dp]),
+        "ref_localact_doublet_h": (C.c_double, [C.c_long, C.c_int]),
         "ref_localact_u1link": (C.c_double, [C.c_long, C.c_int]),
         "ref_localact_triplet": (C.c_double, [C.c_long]),
         "ref_magcharge_cube": (C.c_double, [C.c_long]),

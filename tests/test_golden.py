@@ -19,6 +19,8 @@ MODELS = {
     "t": dict(nhiggs=0, triplet=1),
     "ht": dict(nhiggs=1, triplet=1),
     "hsu": dict(nhiggs=1, u1=1, singlet=1),
+    "d": dict(nhiggs=2),
+    "du": dict(nhiggs=2, u1=1),
 }
 
 
@@ -57,19 +59,30 @@ def test_model_fixture(tag):
     if p.nhiggs:
         ds = np.array([o.vec("orc_doublet_staple", 4, i) for i in range(V)])
         assert rel(ds, g["doublet_staple"]) < 1e-12
+    if p.nhiggs == 2:
+        ds = np.array([o.vec("orc_doublet_staple_h", 4, i, 1) for i in range(V)])
+        assert rel(ds, g["doublet2_staple"]) < 1e-12
+        la = np.array([o.lib.orc_localact_doublet_h(o.h, i, 1) for i in range(V)])
+        assert rel(la, g["localact_doublet2"]) < 1e-12
     s = o.measure()
-    assert (np.abs(s - g["measure_sums"]) / np.maximum(np.abs(g["measure_sums"]), 1)).max() < 1e-12
+    # fixtures of one-doublet models hold the sums before the second
+    # doublet's slots were appended; those slots are 0 there
+    sums = np.zeros(abi.NSUMS)
+    sums[:len(g["measure_sums"])] = g["measure_sums"]
+    assert (np.abs(s - sums) / np.maximum(np.abs(sums), 1)).max() < 1e-12
     # the measure line of the reference from its own sums, through our formatter
     lib = abi.load()
     buf = C.create_string_buffer(4096)
-    sums = np.ascontiguousarray(g["measure_sums"])
     assert lib.su2hip_measure_format(C.byref(p), ptr(sums), -0.0, 0.0, buf, 4096) == 0
     assert buf.value == g["measure_text"].tobytes()
     o.rng_set(99, 0)
     o.update(2)
     h = o.get_fields()
+    # two doublets: rounding grows 10-30x per sweep under the two-doublet
+    # overrelaxation (test_oracle_ref.test_higgs2_rounding_growth)
+    tol = 1e-11 if p.nhiggs < 2 else 1e-9
     for k in orc.FIELDS:
-        assert np.abs(h[k] - g["sweep2_" + k]).max() < 1e-11, k
+        assert np.abs(h[k] - g["sweep2_" + k]).max() < tol, k
     c = o.counters()
     for name, v in zip(g["counter_names"], g["sweep2_counters"]):
         assert c[str(name)] == v, name
@@ -152,8 +165,9 @@ def test_muca_fixture(path):
     o.rng_set(99, 0)
     o.update(nsw)
     h = o.get_fields()
+    tol = 1e-11 if p.nhiggs < 2 else 1e-9
     for k in orc.FIELDS:
-        assert np.abs(h[k] - g["sweep3_" + k]).max() < 1e-11, k
+        assert np.abs(h[k] - g["sweep3_" + k]).max() < tol, k
     c = o.counters()
     for nm, v in zip(g["counter_names"], g["sweep3_counters"]):
         assert c[str(nm)] == v, nm
@@ -161,7 +175,7 @@ def test_muca_fixture(path):
     ref3 = Muca.from_buffer_copy(g["muca3"].tobytes())
     assert np.abs(W - g["W3"]).max() < 1e-12
     assert (hits == g["hits3"]).all()
-    assert abs(m3.param_value[1] - ref3.param_value[1]) < 1e-13
+    assert abs(m3.param_value[1] - ref3.param_value[1]) < (1e-13 if p.nhiggs < 2 else 1e-10)
     assert m3.delta == ref3.delta and m3.last_max == ref3.last_max
     assert m3.muca_count == ref3.muca_count
     if m.mode == abi.SLOW:

@@ -128,6 +128,38 @@ def test_cli_measure_and_lattice_match_reference(tmp_path):
             assert np.abs(g[k] - f[k]).max() < 1e-10
 
 
+TWO_HIGGS = dict(msq_phi2=-0.15, m12sq_re=0.02, m12sq_im=0.01, lam2=0.03, lam3=0.015, lam4=-0.01,
+                 lam5_re=0.008, lam5_im=0.004, lam6_re=0.005, lam6_im=-0.003, lam7_re=0.004,
+                 lam7_im=0.002)
+
+
+def test_cli_two_doublets_match_reference(tmp_path):
+    """The two-doublet model (the reference's -DNHIGGS=2 -DU1 build) from a
+    cold start: measure lines, labels and the final lattice."""
+    if not orc.ref_available("du"):
+        pytest.skip("reference build absent")
+    d = str(tmp_path)
+    extra = "".join("%s %r\n" % kv for kv in TWO_HIGGS.items())
+    run_cli(d, CONFIG.format(model="2hdm+u1", it=6, do_flow=0) + extra)
+    p = make_params(L=(4, 6, 8), nhiggs=2, u1=1, betasu2=8.0, betau1=9.0, msq_phi=-0.25,
+                    lambda_phi=0.02, scalar_sweeps=3, seed=4242, **TWO_HIGGS)
+    r, lines = ref_trajectory("du", p, 6)
+    ours = open(os.path.join(d, "measure")).read().splitlines(keepends=True)
+    same_lines(ours, lines)
+    lib = abi.load()
+    os.chdir(d)
+    r.lib.ref_print_labels()
+    assert lib.su2hip_labels_write(b"labels_ours", C.byref(p)) == 0
+    assert open("labels_ours").read() == open("labels").read()
+    g = abi.field_arrays(p)
+    it = C.c_int64()
+    assert lib.su2hip_lattice_read(b"lattice", C.byref(p), C.byref(it), None, None,
+                                   *[ptr(g[k]) for k in orc.FIELDS]) == 0
+    f = r.get_fields()
+    for k in ("su2link", "u1link", "doublet"):
+        assert np.abs(g[k] - f[k]).max() < 1e-10, k
+
+
 def test_cli_resume_is_exact(tmp_path):
     """6 sweeps in one run = 3 + 3 with a restart from the final checkpoint
     (the Philox position travels in lattice.rng).  As in the reference, the

@@ -26,6 +26,8 @@ MODELS = {
     "hsu": dict(nhiggs=1, u1=1, singlet=1),
     "t": dict(nhiggs=0, triplet=1),
     "ht": dict(nhiggs=1, triplet=1),
+    "d": dict(nhiggs=2),
+    "du": dict(nhiggs=2, u1=1),
 }
 FIELDS = ("su2link", "u1link", "doublet", "triplet", "singlet")
 
@@ -74,7 +76,7 @@ def kinds(p):
     if p.u1:
         ks.append((K_U1_METROPOLIS, 3))
     if p.nhiggs:
-        ks += [(K_DOUBLET_OVERRELAX, 1), (K_DOUBLET_METROPOLIS, 1)]
+        ks += [(K_DOUBLET_OVERRELAX, p.nhiggs), (K_DOUBLET_METROPOLIS, p.nhiggs)]
     if p.triplet:
         ks += [(K_TRIPLET_OVERRELAX, 1), (K_TRIPLET_METROPOLIS, 1)]
     if p.singlet:
@@ -106,7 +108,9 @@ def test_compound_sweeps(tag):
     p, o, b = pair(tag, link_overrelax=2)
     o.update(2)
     b.update(2)
-    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-10
+    # two doublets: the two-doublet overrelaxation amplifies rounding 10-30x
+    # per sweep (test_oracle_ref.test_higgs2_rounding_growth)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < (1e-10 if p.nhiggs < 2 else 1e-8)
     co, cg = o.counters(), b.counters(0)
     for k in co:
         assert co[k] == cg[k], k
@@ -224,7 +228,9 @@ def muca_weight(bins=40, lo=0.0, hi=2.0, seed=3, slope=5.0):
                                                ("h", abi.PHISQ, 3, 5.0),
                                                ("ht", abi.PHI2MINUSSIGMA2, 2, 5.0),
                                                ("h", abi.PHISQ, 3, 400.0),
-                                               ("t", abi.SIGMASQ, 2, 400.0)])
+                                               ("t", abi.SIGMASQ, 2, 400.0),
+                                               ("d", abi.PHI2SQ, 2, 5.0),
+                                               ("du", abi.PHI2SQ, 3, 400.0)])
 def test_muca_sweeps(tag, op, cps, slope, mode):
     p, o, b = pair(tag, L=(4, 4, 6))
     m = abi.Muca()
@@ -246,9 +252,11 @@ def test_muca_sweeps(tag, op, cps, slope, mode):
     mo, _, _, _ = o.muca_get()
     mg, _, _, _ = b.muca_get(0)
     assert abs(mo.param_value[0] - mg.param_value[0]) < 1e-13
-    o.update(3)
-    b.update(3)
-    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-10
+    nsw = 3 if p.nhiggs < 2 else 2
+    o.update(nsw)
+    b.update(nsw)
+    tol = 1e-10 if p.nhiggs < 2 else 1e-8
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < tol
     co, cg = o.counters(), b.counters(0)
     for k in co:
         assert co[k] == cg[k], k
@@ -256,9 +264,9 @@ def test_muca_sweeps(tag, op, cps, slope, mode):
         assert 0 < co["accepted_muca"] < co["total_muca"]  # the restore path ran
     mo, po, Wo, ho = o.muca_get()
     mg, pg, Wg, hg = b.muca_get(0)
-    assert np.abs(Wo - Wg).max() < 1e-12
+    assert np.abs(Wo - Wg).max() < (1e-12 if p.nhiggs < 2 else 1e-9)
     assert (ho == hg).all()
-    assert abs(mo.param_value[1] - mg.param_value[1]) < 1e-13
+    assert abs(mo.param_value[1] - mg.param_value[1]) < (1e-13 if p.nhiggs < 2 else 1e-9)
     if mode == abi.SLOW:
         go, no = o.muca_sums()
         gg, ng = b.muca_sums(0)

@@ -17,9 +17,13 @@ MODELS = {
     "t": dict(nhiggs=0, triplet=1),
     "ht": dict(nhiggs=1, triplet=1),
     "hsu": dict(nhiggs=1, u1=1, singlet=1),
+    "d": dict(nhiggs=2),
+    "du": dict(nhiggs=2, u1=1),
 }
 MODEL_KEY = {"h": "higgs", "hu": "higgs+u1", "t": "triplet", "ht": "higgs+triplet",
-             "hsu": "higgs+u1+singlet"}
+             "hsu": "higgs+u1+singlet", "d": "2hdm", "du": "2hdm+u1"}
+TWO_HIGGS = ("msq_phi2", "m12sq_re", "m12sq_im", "lam2", "lam3", "lam4", "lam5_re", "lam5_im",
+             "lam6_re", "lam6_im", "lam7_re", "lam7_im")
 
 CONFIG = """### test config for the su2higgs config reader
 dim 3
@@ -51,6 +55,18 @@ b3_s 2
 b4_s 0.3
 a1_s 3
 a2_s 0.25
+msq_phi2 -0.131
+m12sq_re 0.021
+m12sq_im -0.0113
+lam2 0.031
+lam3 0.0152
+lam4 -0.0107
+lam5_re 0.0081
+lam5_im 0.0043
+lam6_re 0.0052
+lam6_im -0.0031
+lam7_re 0.0041
+lam7_im 0.0022
 phi0 0.2
 sigma0 0.3
 singlet0 0.4
@@ -113,7 +129,7 @@ def ref_config(tag, path):
 @pytest.mark.parametrize("muca", [0, 1])
 def test_config_matches_reference(lib, tmp_path, tag, muca):
     need_ref(tag)
-    op = "Sigmasq" if tag == "t" else "phisq"
+    op = "Sigmasq" if tag == "t" else ("phi2sq" if MODELS[tag]["nhiggs"] == 2 else "phisq")
     path = str(tmp_path / "config")
     with open(path, "w") as f:
         f.write(CONFIG.format(L1=4, muca=muca, op=op))
@@ -143,6 +159,9 @@ def test_config_matches_reference(lib, tmp_path, tag, muca):
     if q.singlet:
         for k in ("update_singlet", "algorithm_singlet", "b1_s", "msq_s", "b3_s", "b4_s", "a1_s", "a2_s"):
             assert getattr(q, k) == getattr(p, k), k
+    if q.nhiggs == 2:
+        for k in TWO_HIGGS:
+            assert getattr(q, k) == getattr(p, k) != 0, k
     assert [rc.iterations, rc.interval, rc.checkpoint, rc.n_thermalize, rc.reset, rc.multicanonical,
             rc.run_checks, rc.measure_local] == longs[:8]
     assert rc.latticefile.decode() == lat

@@ -16,6 +16,8 @@ MODELS = {
     "t": dict(nhiggs=0, triplet=1),
     "ht": dict(nhiggs=1, triplet=1),
     "hsu": dict(nhiggs=1, u1=1, singlet=1),
+    "d": dict(nhiggs=2),
+    "du": dict(nhiggs=2, u1=1),
 }
 
 
@@ -95,6 +97,12 @@ def test_deterministic_pieces(tag):
             b = r.vec("ref_doublet_staple", 4, i)
             assert rel(a, b) < 1e-12
             assert rel(o.lib.orc_localact_doublet(o.h, i), r.lib.ref_localact_doublet(i)) < 1e-12
+        for hh in range(1, p.nhiggs):
+            a = o.vec("orc_doublet_staple_h", 4, i, hh)
+            b = r.vec("ref_doublet_staple_h", 4, i, hh)
+            assert rel(a, b) < 1e-12
+            assert rel(o.lib.orc_localact_doublet_h(o.h, i, hh),
+                       r.lib.ref_localact_doublet_h(i, hh)) < 1e-12
         if p.triplet:
             assert rel(o.lib.orc_localact_triplet(o.h, i), r.lib.ref_localact_triplet(i)) < 1e-12
         if p.singlet:
@@ -102,15 +110,17 @@ def test_deterministic_pieces(tag):
 
 
 def kinds_for(p):
-    ks = [(K_LINK_HEATBATH, True), (K_LINK_METROPOLIS, True)]
+    """(kind, number of values of its dir argument): directions for links,
+    doublets for the doublet kinds"""
+    ks = [(K_LINK_HEATBATH, 3), (K_LINK_METROPOLIS, 3)]
     if p.u1:
-        ks.append((K_U1_METROPOLIS, True))
+        ks.append((K_U1_METROPOLIS, 3))
     if p.nhiggs:
-        ks += [(K_DOUBLET_OVERRELAX, False), (K_DOUBLET_METROPOLIS, False)]
+        ks += [(K_DOUBLET_OVERRELAX, p.nhiggs), (K_DOUBLET_METROPOLIS, p.nhiggs)]
     if p.triplet:
-        ks += [(K_TRIPLET_OVERRELAX, False), (K_TRIPLET_METROPOLIS, False)]
+        ks += [(K_TRIPLET_OVERRELAX, 1), (K_TRIPLET_METROPOLIS, 1)]
     if p.singlet:
-        ks += [(K_SINGLET_OVERRELAX, False), (K_SINGLET_METROPOLIS, False)]
+        ks += [(K_SINGLET_OVERRELAX, 1), (K_SINGLET_METROPOLIS, 1)]
     return ks
 
 
@@ -120,9 +130,9 @@ def test_site_updates(tag):
     need_ref(tag)
     p, o, r, f = setup(tag)
     pas = 11
-    for kind, has_dir in kinds_for(p):
+    for kind, ndir in kinds_for(p):
         for i in range(0, o.V, 5):
-            d = (i // 5) % 3 if has_dir else 0
+            d = (i // 5) % ndir
             a = o.site(kind, i, d, pas)
             b = r.site(kind, i, d, pas)
             assert a == b, (kind, i)
@@ -150,14 +160,48 @@ def test_full_sweeps(tag, variant):
     o.update(3)
     r.update(3)
     fo, fr = o.get_fields(), r.get_fields()
+    # the two-doublet overrelaxation keeps Y and moves X to the other root
+    # (src/overrelax.c:295-417); that map amplifies a rounding difference
+    # 10-30x per sweep (test_higgs2_rounding_growth), so 3 sweeps from 1e-15
+    # per-site agreement end near 1e-10
+    tol = 1e-11 if p.nhiggs < 2 else 1e-9
     for k in orc.FIELDS:
-        assert np.abs(fo[k] - fr[k]).max() < 1e-11, k
+        assert np.abs(fo[k] - fr[k]).max() < tol, k
     co, cr = o.counters(), r.counters()
     for k in cr:
         if k in ("total_link_or", "accepted_link_or"):
             continue
         assert co[k] == cr[k], k
     assert o.rng_get()[1] == r.lib.ref_rng_pass()
+
+
+def test_higgs2_rounding_growth():
+    """The tolerance of the two-doublet sweep comparison: a 1e-14 relative
+    change of the doublets grows by more than 10x per sweep under the
+    oracle's own two-doublet dynamics, but stays near 1e-13 with one
+    doublet."""
+    def growth(**kw):
+        p = make_params(L=(4, 4, 6), **kw)
+        a = orc.Oracle(p)
+        a.hot_start()
+        f = a.get_fields()
+        b = orc.Oracle(p)
+        g = dict(f)
+        g["doublet"] = f["doublet"] * (1 + 1e-14)
+        b.set_fields(g)
+        a.rng_set(99, 0)
+        b.rng_set(99, 0)
+        out = []
+        for _ in range(4):
+            a.update(1)
+            b.update(1)
+            fa, fb = a.get_fields(), b.get_fields()
+            out.append(max(np.abs(fa[k] - fb[k]).max() for k in orc.FIELDS if fa[k] is not None))
+        return out
+    one = growth(nhiggs=1)
+    two = growth(nhiggs=2)
+    assert max(one) < 1e-12
+    assert two[3] > 100 * two[1] > 0
 
 
 @pytest.mark.parametrize("tag", list(MODELS))
