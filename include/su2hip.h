@@ -223,6 +223,8 @@ int su2hip_remove(su2hip_batch_t *b, int slot);
 /* Number of slots / whether a slot is in use. */
 int su2hip_slots(su2hip_batch_t *b, int *max_slots, int *active);
 int su2hip_slot_active(su2hip_batch_t *b, int slot, int *active);
+/* The parameters a slot was created with. */
+int su2hip_slot_params(su2hip_batch_t *b, int slot, su2hip_params *p);
 
 /* ---- fields ------------------------------------------------------------
  * Host <-> device in the reference layout (src/checkpoint.c:101-115 order of
@@ -271,6 +273,17 @@ int su2hip_pass(su2hip_batch_t *b, int kind, int parity, int dir);
 int su2hip_use_graph(su2hip_batch_t *b, int enable);
 /* Wait for all queued work. */
 int su2hip_sync(su2hip_batch_t *b);
+/* The reference's random_sweeps (src/update.c:292-300, :341-345): every
+ * sweep of every lattice takes its link passes in a shuffled (parity,
+ * direction) order (parities and directions shuffled separately, as
+ * shuffle() does, src/update.c:461-473) and each scalar round's parities in
+ * a random order.  The draws come from the lattice's Philox stream at (pass
+ * counter at the sweep start, site 0xFFFFFFFE), on the device, so lattices
+ * of one batch take their own orders in the same launches.  Off by default,
+ * as the reference's main sets it (src/main.c:187); its heatbath
+ * trajectories switch it on. */
+int su2hip_set_random_order(su2hip_batch_t *b, int enable);
+int su2hip_get_random_order(su2hip_batch_t *b, int *enabled);
 
 /* Counters of a lattice since the last reset (src/init.c:167-199). */
 int su2hip_get_counters(su2hip_batch_t *b, int slot, su2hip_counters *c);
@@ -339,6 +352,20 @@ int su2hip_muca_get(su2hip_batch_t *b, int slot, su2hip_muca *m, double *pos,
 int su2hip_muca_set_sums(su2hip_batch_t *b, int slot, const int64_t *gsum,
                          const int64_t *nsum);
 int su2hip_muca_get_sums(su2hip_batch_t *b, int slot, int64_t *gsum, int64_t *nsum);
+/* Multicanonical accept/reject of a lattice on (the default once a weight
+ * is installed) or off (w->do_acceptance, src/su2.h:244): off, sweeps run
+ * without checks or backups and the weight is not updated, as in the
+ * reference's heatbath trajectories (src/hb_trajectory.c:15-19). */
+int su2hip_muca_acceptance(su2hip_batch_t *b, int slot, int enable);
+/* The order parameter's parity contributions (param_value[EVEN], [ODD]) and
+ * their sum; with recompute = 1 both are first recomputed from the fields
+ * (calc_orderparam of EVEN and ODD, src/multicanonical.c:709-751).  pv or
+ * value may be NULL. */
+int su2hip_muca_orderparam(su2hip_batch_t *b, int slot, int recompute, double *pv,
+                           double *value);
+/* Overwrite the parity contributions, e.g. with the values the reference
+ * leaves after its trajectories (computed on their copy of the fields). */
+int su2hip_muca_set_orderparam(su2hip_batch_t *b, int slot, const double *pv);
 /* W(R) of the installed weight (get_weight, src/multicanonical.c:350-379). */
 int su2hip_muca_weight_of(su2hip_batch_t *b, int slot, double value,
                           double *weight);

@@ -42,6 +42,8 @@ typedef struct su2hip_run_config {
   int64_t interval_z;
   int32_t blocking;          /* field content includes the blocking key */
   int32_t blocks;            /* blocking levels of the blocked correlators */
+  int32_t trajectory;        /* field content includes the heatbath-trajectory key */
+  int32_t do_trajectory;     /* heatbath trajectories every mode_interval iterations */
 } su2hip_run_config;
 
 /* Parse `path` the way get_parameters + get_weight_parameters do
@@ -50,8 +52,8 @@ typedef struct su2hip_run_config {
  * algorithm names -> metropolis).  `model` overrides the config's `model` key
  * when non-NULL: '+'-separated tokens among higgs, 2hdm (two doublets, the
  * -DNHIGGS=2 build), u1, triplet, singlet, gradflow, correlators, measure_z,
- * blocking (implies correlators); the default is the reference Makefile's
- * build, higgs+singlet+u1. */
+ * blocking (implies correlators), trajectory (-DHB_TRAJECTORY); the default
+ * is the reference Makefile's build, higgs+singlet+u1. */
 int su2hip_config_read(const char *path, const char *model, su2hip_run_config *out);
 /* Re-read iterations, interval and checkpoint (read_updated_parameters,
  * src/parameters.c:460-494). */

@@ -10,6 +10,11 @@
  * every call the reference's own update_lattice (src/update.c:270-430)
  * makes: a call at the first site of a parity opens a new pass, and each
  * call sets the site context before running the reference function.
+ * update_lattice itself is wrapped too: draws it takes outside a per-site
+ * function (the random pass order, src/update.c:292-300, :341-345) come
+ * from (pass at the start of the sweep, ORC_SCHED_SITE, k).  iran() is the
+ * 64-bit Philox word and dran() its top 53 bits, as genrand64_real2 is
+ * built from genrand64_int64 (src/generic/mersenne.c:141-150).
  */
 #include "su2.h"
 
@@ -26,15 +31,16 @@ static fields f;
 static counters c;
 static weight w;
 
-static uint64_t h_seed, h_next_pass, h_cur_pass;
-static uint32_t h_site, h_k, h_check;
+static uint64_t h_seed, h_next_pass, h_cur_pass, h_sched_pass;
+static uint32_t h_site, h_k, h_check, h_sched_k;
+static int h_in_site;
 
-double genrand64_real2(void) {
-  return orc_uniform(h_seed, h_cur_pass, h_site, h_k++);
-}
 unsigned long long genrand64_int64(void) {
-  double u = genrand64_real2();
-  return (unsigned long long)(u * 18446744073709551616.0);
+  if (h_in_site) return orc_word(h_seed, h_cur_pass, h_site, h_k++);
+  return orc_word(h_seed, h_sched_pass, ORC_SCHED_SITE, h_sched_k++);
+}
+double genrand64_real2(void) {
+  return (double)(genrand64_int64() >> 11) * (1.0 / 9007199254740992.0);
 }
 void init_genrand64(unsigned long long s) { (void)s; }
 
@@ -51,8 +57,18 @@ static void site_ctx(long i) {
   ret __real_##name decl;          \
   ret __wrap_##name decl {         \
     site_ctx(i);                   \
-    return __real_##name args;     \
+    h_in_site = 1;                 \
+    ret r_ = __real_##name args;   \
+    h_in_site = 0;                 \
+    return r_;                     \
   }
+
+void __real_update_lattice(lattice *ll, fields *ff, params const *pp, counters *cc, weight *ww);
+void __wrap_update_lattice(lattice *ll, fields *ff, params const *pp, counters *cc, weight *ww) {
+  h_sched_pass = h_next_pass;
+  h_sched_k = 0;
+  __real_update_lattice(ll, ff, pp, cc, ww);
+}
 
 WRAP(int, heatbath_su2link, (lattice const *ll, fields *ff, params const *pp, long i, int dir), (ll, ff, pp, i, dir))
 WRAP(int, metro_su2link, (lattice const *ll, fields *ff, params const *pp, long i, int dir), (ll, ff, pp, i, dir))
@@ -79,12 +95,15 @@ int __real_multicanonical_acceptance(lattice const *ll, weight *ww, double o, do
 int __wrap_multicanonical_acceptance(lattice const *ll, weight *ww, double o, double n) {
   h_site = ORC_MUCA_SITE;
   h_k = h_check++;
-  return __real_multicanonical_acceptance(ll, ww, o, n);
+  h_in_site = 1;
+  int r = __real_multicanonical_acceptance(ll, ww, o, n);
+  h_in_site = 0;
+  return r;
 }
 
 /* which model flags this build has: bit0 NHIGGS, bit1 U1, bit2 TRIPLET,
  * bit3 SINGLET, bit4 GRADFLOW, bit5 CORRELATORS, bit6 MEASURE_Z,
- * bit7 BLOCKING, bit8 NHIGGS == 2 */
+ * bit7 BLOCKING, bit8 NHIGGS == 2, bit9 HB_TRAJECTORY */
 int ref_model(void) {
   int m = 0;
 #if (NHIGGS > 0)
@@ -113,6 +132,9 @@ int ref_model(void) {
 #endif
 #if (NHIGGS == 2)
   m |= 256;
+#endif
+#ifdef HB_TRAJECTORY
+  m |= 512;
 #endif
   return m;
 }
@@ -286,35 +308,41 @@ void ref_update(int n) {
   for (int k = 0; k < n; k++) update_lattice(&l, &f, &p, &c, &w);
 }
 
+/* the random pass order of update_lattice (p->random_sweeps) */
+void ref_set_random_sweeps(int on) { p.random_sweeps = on; }
+
 /* one per-site update with an explicit pass number; for the doublet kinds
  * dir is the doublet (higgs_id) */
 int ref_site(int kind, long i, int dir, uint64_t pass) {
   h_cur_pass = pass;
   h_site = (uint32_t)i;
   h_k = 0;
+  h_in_site = 1;
+  int r = -1;
   switch (kind) {
-    case SU2HIP_K_LINK_HEATBATH: return __real_heatbath_su2link(&l, &f, &p, i, dir);
-    case SU2HIP_K_LINK_METROPOLIS: return __real_metro_su2link(&l, &f, &p, i, dir);
+    case SU2HIP_K_LINK_HEATBATH: r = __real_heatbath_su2link(&l, &f, &p, i, dir); break;
+    case SU2HIP_K_LINK_METROPOLIS: r = __real_metro_su2link(&l, &f, &p, i, dir); break;
 #ifdef U1
-    case SU2HIP_K_U1_METROPOLIS: return __real_metro_u1link(&l, &f, &p, i, dir);
+    case SU2HIP_K_U1_METROPOLIS: r = __real_metro_u1link(&l, &f, &p, i, dir); break;
 #endif
 #if (NHIGGS == 2)
-    case SU2HIP_K_DOUBLET_OVERRELAX: return __real_overrelax_higgs2(&l, &f, &p, i, dir);
-    case SU2HIP_K_DOUBLET_METROPOLIS: return __real_metro_doublet(&l, &f, &p, i, dir);
+    case SU2HIP_K_DOUBLET_OVERRELAX: r = __real_overrelax_higgs2(&l, &f, &p, i, dir); break;
+    case SU2HIP_K_DOUBLET_METROPOLIS: r = __real_metro_doublet(&l, &f, &p, i, dir); break;
 #elif (NHIGGS > 0)
-    case SU2HIP_K_DOUBLET_OVERRELAX: return __real_overrelax_doublet(&l, &f, &p, i);
-    case SU2HIP_K_DOUBLET_METROPOLIS: return __real_metro_doublet(&l, &f, &p, i, 0);
+    case SU2HIP_K_DOUBLET_OVERRELAX: r = __real_overrelax_doublet(&l, &f, &p, i); break;
+    case SU2HIP_K_DOUBLET_METROPOLIS: r = __real_metro_doublet(&l, &f, &p, i, 0); break;
 #endif
 #ifdef TRIPLET
-    case SU2HIP_K_TRIPLET_OVERRELAX: return __real_overrelax_triplet(&l, &f, &p, i);
-    case SU2HIP_K_TRIPLET_METROPOLIS: return __real_metro_triplet(&l, &f, &p, i);
+    case SU2HIP_K_TRIPLET_OVERRELAX: r = __real_overrelax_triplet(&l, &f, &p, i); break;
+    case SU2HIP_K_TRIPLET_METROPOLIS: r = __real_metro_triplet(&l, &f, &p, i); break;
 #endif
 #ifdef SINGLET
-    case SU2HIP_K_SINGLET_OVERRELAX: return __real_overrelax_singlet(&l, &f, &p, i);
-    case SU2HIP_K_SINGLET_METROPOLIS: return __real_metro_singlet(&l, &f, &p, i);
+    case SU2HIP_K_SINGLET_OVERRELAX: r = __real_overrelax_singlet(&l, &f, &p, i); break;
+    case SU2HIP_K_SINGLET_METROPOLIS: r = __real_metro_singlet(&l, &f, &p, i); break;
 #endif
   }
-  return -1;
+  h_in_site = 0;
+  return r;
 }
 
 void ref_link_staple(long i, int dir, double *V) { su2link_staple(&l, &f, &p, i, dir, V); }
@@ -494,6 +522,27 @@ void ref_grad_flow(double t_max, double dt, int meas_interval, int id) {
   p.flow_meas_interval = meas_interval;
   grad_flow(&l, &f, &p, &w, t_max, dt, id);
 }
+#endif
+
+/* multicanonical accept/reject on or off (w->do_acceptance) */
+void ref_muca_set_acceptance(int on) { w.do_acceptance = on; }
+
+#ifdef HB_TRAJECTORY
+/* one set of heatbath trajectories (make_realtime_trajectories,
+ * src/hb_trajectory.c:11-95), appending to ./trajectory */
+void ref_trajectories(int n_traj, int interval, double min, double max, int id) {
+  trajectory tr;
+  memset(&tr, 0, sizeof(tr));
+  tr.n_traj = n_traj;
+  tr.interval = interval;
+  tr.mode_interval = 1;
+  tr.min = min;
+  tr.max = max;
+  make_realtime_trajectories(&l, &f, &p, &c, &w, &tr, id);
+}
+/* read_realtime_config (src/hb_trajectory.c:99-163) calls check_set,
+ * which the reference's sources do not define: this library is loaded with
+ * lazy binding and that function is never called. */
 #endif
 
 /* checkpoint I/O through the reference (src/checkpoint.c:84-214) */

@@ -43,6 +43,7 @@ struct orc {
   int32_t *hits;
   long *gsum, *nsum;
   int nb;
+  int random_order; /* the reference's random_sweeps */
   uint64_t r_pass;
   uint32_t r_site, r_k;
 };
@@ -76,14 +77,18 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
   out[3] = c3;
 }
 
-double orc_uniform(uint64_t seed, uint64_t pass, uint32_t site, uint32_t k) {
+uint64_t orc_word(uint64_t seed, uint64_t pass, uint32_t site, uint32_t k) {
   uint32_t ctr[4] = {k >> 1, site, (uint32_t)pass, (uint32_t)(pass >> 32)};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   uint32_t o[4];
   orc_philox4x32_10(ctr, key, o);
-  uint64_t x = (k & 1) ? (((uint64_t)o[3] << 32) | o[2])
-                       : (((uint64_t)o[1] << 32) | o[0]);
-  return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+  return (k & 1) ? (((uint64_t)o[3] << 32) | o[2]) : (((uint64_t)o[1] << 32) | o[0]);
+}
+
+double orc_uniform(uint64_t seed, uint64_t pass, uint32_t site, uint32_t k) {
+  /* dran = (iran >> 11) 2^-53, as genrand64_real2 from genrand64_int64
+   * (src/generic/mersenne.c:141-150) */
+  return (double)(orc_word(seed, pass, site, k) >> 11) * (1.0 / 9007199254740992.0);
 }
 
 static inline double dran(orc_t *o) {
@@ -1312,6 +1317,8 @@ void orc_muca_set(orc_t *o, const su2hip_muca *m, const double *pos,
   }
 }
 
+void orc_muca_set_acceptance(orc_t *o, int on) { o->m.do_acceptance = on != 0; }
+
 void orc_muca_get(const orc_t *o, su2hip_muca *m, double *pos, double *W,
                   int32_t *hits) {
   *m = o->m;
@@ -1455,47 +1462,76 @@ static int scalar_round(int alg, int last, int kor, int kme) {
   return alg == SU2HIP_OVERRELAX ? kor : kme;
 }
 
+void orc_set_random_order(orc_t *o, int on) { o->random_order = on != 0; }
+
+/* shuffle (src/update.c:461-473) with iran() = the 64-bit word of draw k */
+static void shuffle(int *a, int len, uint64_t seed, uint64_t pass, uint32_t *k) {
+  for (int i = len - 1; i > 0; i--) {
+    unsigned int j = orc_word(seed, pass, ORC_SCHED_SITE, (*k)++) % (unsigned long long)(i + 1);
+    int t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+  }
+}
+
 /* update_lattice (src/update.c:270-430) with link_overrelax rounds after
- * the heatbath/Metropolis link passes */
+ * the heatbath/Metropolis link passes (in the same order).  With the random
+ * order (random_sweeps, src/update.c:292-300, :341-345) the sweep's
+ * shuffles and parity draws come from pass = the first pass of the sweep,
+ * site ORC_SCHED_SITE, in the reference's order: per link round a shuffle
+ * of the parities then of the directions (5 draws each), per scalar round
+ * one uniform for the first parity. */
 void orc_update(orc_t *o, int n) {
   const su2hip_params *p = &o->p;
   for (int it = 0; it < n; it++) {
+    uint64_t sp = o->pass;
+    uint32_t sk = 0;
     for (int k = 0; k < p->update_links; k++) {
       int lk = p->algorithm_su2link == SU2HIP_HEATBATH ? SU2HIP_K_LINK_HEATBATH
                                                         : SU2HIP_K_LINK_METROPOLIS;
-      for (int d = 0; d < D; d++)
-        for (int par = 0; par < 2; par++) orc_pass(o, lk, par, d);
+      int par_a[2 * D], dir_a[2 * D];
+      for (int j = 0; j < 2 * D; j++) {
+        par_a[j] = j & 1;
+        dir_a[j] = j >> 1;
+      }
+      if (o->random_order) {
+        shuffle(par_a, 2 * D, o->seed, sp, &sk);
+        shuffle(dir_a, 2 * D, o->seed, sp, &sk);
+      }
+      for (int j = 0; j < 2 * D; j++) orc_pass(o, lk, par_a[j], dir_a[j]);
       for (int r = 0; r < p->link_overrelax; r++)
-        for (int d = 0; d < D; d++)
-          for (int par = 0; par < 2; par++) orc_pass(o, SU2HIP_K_LINK_OVERRELAX, par, d);
+        for (int j = 0; j < 2 * D; j++) orc_pass(o, SU2HIP_K_LINK_OVERRELAX, par_a[j], dir_a[j]);
       if (p->u1)
-        for (int d = 0; d < D; d++)
-          for (int par = 0; par < 2; par++) orc_pass(o, SU2HIP_K_U1_METROPOLIS, par, d);
+        for (int j = 0; j < 2 * D; j++) orc_pass(o, SU2HIP_K_U1_METROPOLIS, par_a[j], dir_a[j]);
     }
     int ns = p->scalar_sweeps + 1;
     for (int s = 0; s < ns; s++) {
       int last = (s >= ns - 1);
+      int p0 = 0;
+      if (o->random_order)
+        p0 = orc_uniform(o->seed, sp, ORC_SCHED_SITE, sk++) < 0.5 ? SU2HIP_EVEN : SU2HIP_ODD;
+      int pa[2] = {p0, 1 - p0};
       if (p->nhiggs) {
         int kd = scalar_round(p->algorithm_su2doublet, last, SU2HIP_K_DOUBLET_OVERRELAX,
                               SU2HIP_K_DOUBLET_METROPOLIS);
         if (kd >= 0)
           for (int k = 0; k < p->update_su2doublet; k++)
             for (int h = 0; h < p->nhiggs; h++)
-              for (int par = 0; par < 2; par++) orc_pass(o, kd, par, h);
+              for (int j = 0; j < 2; j++) orc_pass(o, kd, pa[j], h);
       }
       if (p->triplet) {
         int kt = scalar_round(p->algorithm_su2triplet, last, SU2HIP_K_TRIPLET_OVERRELAX,
                               SU2HIP_K_TRIPLET_METROPOLIS);
         if (kt >= 0)
           for (int k = 0; k < p->update_su2triplet; k++)
-            for (int par = 0; par < 2; par++) orc_pass(o, kt, par, 0);
+            for (int j = 0; j < 2; j++) orc_pass(o, kt, pa[j], 0);
       }
       if (p->singlet) {
         int ks = scalar_round(p->algorithm_singlet, last, SU2HIP_K_SINGLET_OVERRELAX,
                               SU2HIP_K_SINGLET_METROPOLIS);
         if (ks >= 0)
           for (int k = 0; k < p->update_singlet; k++)
-            for (int par = 0; par < 2; par++) orc_pass(o, ks, par, 0);
+            for (int j = 0; j < 2; j++) orc_pass(o, ks, pa[j], 0);
       }
     }
   }

@@ -28,12 +28,16 @@ extern "C" {
 #define ORC_MUCA_SITE 0xFFFFFFFFu
 #define ORC_HOTSTART_PASS 0xFFFFFFFFFFFFFFFFull
 #define ORC_WALL_PASS 0xFFFFFFFFFFFFFFFEull
+/* site of a sweep's pass-order draws (random order) */
+#define ORC_SCHED_SITE 0xFFFFFFFEu
 
 typedef struct orc orc_t;
 
 /* Philox4x32-10 block function (Salmon, Moraes, Dror, Shaw, SC'11). */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-/* 53-bit uniform in [0,1) for draw k at (seed, pass, site). */
+/* 64-bit word of draw k at (seed, pass, site): the reference's iran(). */
+uint64_t orc_word(uint64_t seed, uint64_t pass, uint32_t site, uint32_t k);
+/* 53-bit uniform in [0,1) for draw k at (seed, pass, site): (word >> 11) 2^-53. */
 double orc_uniform(uint64_t seed, uint64_t pass, uint32_t site, uint32_t k);
 
 orc_t *orc_create(const su2hip_params *p);
@@ -81,6 +85,12 @@ int orc_site_update(orc_t *o, int kind, long i, int dir, uint64_t pass);
 void orc_pass(orc_t *o, int kind, int parity, int dir);
 /* n compound sweeps (update_lattice). */
 void orc_update(orc_t *o, int n);
+/* The reference's random_sweeps: shuffled link pass order and a random
+ * first parity of every scalar round, drawn per sweep. */
+void orc_set_random_order(orc_t *o, int on);
+/* multicanonical global accept/reject on (the default after orc_muca_set)
+ * or off (w->do_acceptance, src/su2.h:244) */
+void orc_muca_set_acceptance(orc_t *o, int on);
 
 /* FP64 volume sums in the SU2HIP_SUM_* layout. */
 void orc_measure(const orc_t *o, double *sums);

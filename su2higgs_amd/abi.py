@@ -110,6 +110,15 @@ class RunConfig(C.Structure):
         ("do_correlators", C.c_int32), ("correlator_interval", C.c_int32),
         ("measure_z", C.c_int32), ("do_z_meas", C.c_int32), ("setup_wall", C.c_int32),
         ("interval_z", C.c_int64), ("blocking", C.c_int32), ("blocks", C.c_int32),
+        ("trajectory", C.c_int32), ("do_trajectory", C.c_int32),
+    ]
+
+
+class TrajectoryConfig(C.Structure):
+    """su2hip_trajectory_config (include/su2hip_traj.h)."""
+    _fields_ = [
+        ("n_traj", C.c_int32), ("interval", C.c_int32), ("mode_interval", C.c_int32),
+        ("reserved", C.c_int32), ("min", C.c_double), ("max", C.c_double),
     ]
 
 
@@ -258,6 +267,15 @@ def load():
         "su2hip_get_budgets": (C.c_int, [h, P(C.c_int64)]),
         "su2hip_pass": (C.c_int, [h, C.c_int, C.c_int, C.c_int]),
         "su2hip_use_graph": (C.c_int, [h, C.c_int]),
+        "su2hip_set_random_order": (C.c_int, [h, C.c_int]),
+        "su2hip_get_random_order": (C.c_int, [h, P(C.c_int)]),
+        "su2hip_slot_params": (C.c_int, [h, C.c_int, P(Params)]),
+        "su2hip_realtime_config_read": (C.c_int, [C.c_char_p, P(TrajectoryConfig)]),
+        "su2hip_trajectory_header": (C.c_int, [C.c_char_p, P(TrajectoryConfig)]),
+        "su2hip_trajectories": (C.c_int, [h, C.c_int, P(TrajectoryConfig), C.c_int, C.c_char_p]),
+        "su2hip_muca_acceptance": (C.c_int, [h, C.c_int, C.c_int]),
+        "su2hip_muca_orderparam": (C.c_int, [h, C.c_int, C.c_int, dp, dp]),
+        "su2hip_muca_set_orderparam": (C.c_int, [h, C.c_int, dp]),
         "su2hip_sync": (C.c_int, [h]),
         "su2hip_get_counters": (C.c_int, [h, C.c_int, P(Counters)]),
         "su2hip_reset_counters": (C.c_int, [h, C.c_int]),

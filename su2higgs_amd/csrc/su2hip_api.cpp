@@ -68,6 +68,7 @@ struct Slot {
 struct Step {
   int kind, parity, dir, pass_off;
   bool muca;
+  uint16_t info;  // place in the random pass order (step_info)
 };
 
 struct Ev {
@@ -123,6 +124,10 @@ struct su2hip_batch {
   AltMap alt[2];                  // chunks 64 and 128; 256 is the main table
   hipStream_t flow_stream = nullptr;  // asynchronous flows (su2hip_flow_start)
   int tune[SU2HIP_NKINDS] = {};   // chunk per pass kind, 0 = chosen by size
+  bool random_order = false;      // the reference's random_sweeps
+  uint8_t *d_order = nullptr;     // [max_slots][passes] parity | dir << 1
+  uint16_t *d_info = nullptr;     // [passes] step_info of each step
+  size_t order_cap = 0, info_cap = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -551,19 +556,23 @@ static void build_sched(su2hip_batch *b) {
   std::vector<Step> &v = b->sched;
   v.clear();
   int off = 0;
-  auto add = [&](int kind, int par, int dir, bool muca) {
-    v.push_back(Step{kind, par, dir, off++, muca});
+  auto add = [&](int kind, int par, int dir, bool muca, uint16_t info) {
+    v.push_back(Step{kind, par, dir, off++, muca, info});
   };
+  // link rounds: position j = 2 dir + parity of the default order, which the
+  // random order replaces by (par_a[j], dir_a[j]) of the round
   int lk = p.algorithm_su2link == SU2HIP_HEATBATH ? SU2HIP_K_LINK_HEATBATH : SU2HIP_K_LINK_METROPOLIS;
   for (int k = 0; k < p.update_links; k++) {
     for (int d = 0; d < 3; d++)
-      for (int par = 0; par < 2; par++) add(lk, par, d, false);
+      for (int par = 0; par < 2; par++) add(lk, par, d, false, step_info(false, k, 2 * d + par));
     for (int r = 0; r < p.link_overrelax; r++)
       for (int d = 0; d < 3; d++)
-        for (int par = 0; par < 2; par++) add(SU2HIP_K_LINK_OVERRELAX, par, d, false);
+        for (int par = 0; par < 2; par++)
+          add(SU2HIP_K_LINK_OVERRELAX, par, d, false, step_info(false, k, 2 * d + par));
     if (p.u1)
       for (int d = 0; d < 3; d++)
-        for (int par = 0; par < 2; par++) add(SU2HIP_K_U1_METROPOLIS, par, d, false);
+        for (int par = 0; par < 2; par++)
+          add(SU2HIP_K_U1_METROPOLIS, par, d, false, step_info(false, k, 2 * d + par));
   }
   int ns = p.scalar_sweeps + 1;
   for (int s = 0; s < ns; s++) {
@@ -575,21 +584,22 @@ static void build_sched(su2hip_batch *b) {
       if (kd >= 0)
         for (int k = 0; k < p.update_su2doublet; k++)
           for (int h = 0; h < p.nhiggs; h++)
-            for (int par = 0; par < 2; par++) add(kd, par, h, muca_f[h ? 3 : 1]);
+            for (int par = 0; par < 2; par++)
+              add(kd, par, h, muca_f[h ? 3 : 1], step_info(true, s, par));
     }
     if (p.triplet) {
       int kt = scalar_round(p.algorithm_su2triplet, last, SU2HIP_K_TRIPLET_OVERRELAX,
                             SU2HIP_K_TRIPLET_METROPOLIS);
       if (kt >= 0)
         for (int k = 0; k < p.update_su2triplet; k++)
-          for (int par = 0; par < 2; par++) add(kt, par, 0, muca_f[2]);
+          for (int par = 0; par < 2; par++) add(kt, par, 0, muca_f[2], step_info(true, s, par));
     }
     if (p.singlet) {
       int ks = scalar_round(p.algorithm_singlet, last, SU2HIP_K_SINGLET_OVERRELAX,
                             SU2HIP_K_SINGLET_METROPOLIS);
       if (ks >= 0)
         for (int k = 0; k < p.update_singlet; k++)
-          for (int par = 0; par < 2; par++) add(ks, par, 0, false);
+          for (int par = 0; par < 2; par++) add(ks, par, 0, false, step_info(true, s, par));
     }
   }
   b->passes = off;
@@ -623,11 +633,19 @@ static hipError_t timed(su2hip_batch *b, int kind, Fn &&fn) {
   return r;
 }
 
-static hipError_t enqueue_step(su2hip_batch *b, const Step &st) {
+// in_sweep: the step is part of a compound sweep, whose random pass order
+// (when on) replaces the step's parity and link direction
+static hipError_t enqueue_step(su2hip_batch *b, const Step &st, bool in_sweep = true) {
   LaunchCtx c = ctx_of(b);
+  if (in_sweep && b->random_order) {
+    c.order = b->d_order;
+    c.npass = b->passes;
+  }
   int reunit = b->tmpl.reunitarize && !b->fp64;
   if (!st.muca) {
     LaunchCtx ct = ctx_for(b, st.kind);
+    ct.order = c.order;
+    ct.npass = c.npass;
     return timed(b, st.kind, [&] {
       return launch_pass(ct, st.kind, st.parity, st.dir, st.pass_off, -1, reunit);
     });
@@ -645,6 +663,14 @@ static hipError_t enqueue_step(su2hip_batch *b, const Step &st) {
 }
 
 static hipError_t enqueue_sweep(su2hip_batch *b) {
+  if (b->random_order) {
+    const su2hip_params &p = b->tmpl;
+    LaunchCtx c = ctx_of(b);
+    hipError_t e = timed(b, SU2HIP_K_AUX, [&] {
+      return launch_order(c, b->d_order, b->d_info, b->passes, p.update_links, p.scalar_sweeps + 1);
+    });
+    if (e != hipSuccess) return e;
+  }
   for (const Step &st : b->sched) {
     hipError_t e = enqueue_step(b, st);
     if (e != hipSuccess) return e;
@@ -653,10 +679,38 @@ static hipError_t enqueue_sweep(su2hip_batch *b) {
   return timed(b, SU2HIP_K_AUX, [&] { return launch_end_sweep(c, b->passes); });
 }
 
+// device tables of the random pass order for the current schedule
+static int ensure_order(su2hip_batch *b) {
+  if (!b->random_order || b->passes == 0) return SU2HIP_OK;
+  size_t no = (size_t)b->max_slots * b->passes, ni = (size_t)b->passes;
+  if (no > b->order_cap) {
+    if (b->d_order) HIPTRY(hipFree(b->d_order));
+    b->d_order = nullptr;
+    b->order_cap = 0;
+    if (hipMalloc(&b->d_order, no) != hipSuccess) return fail(SU2HIP_ENOMEM, "order table");
+    b->order_cap = no;
+  }
+  if (ni > b->info_cap) {
+    if (b->d_info) HIPTRY(hipFree(b->d_info));
+    b->d_info = nullptr;
+    b->info_cap = 0;
+    if (hipMalloc(&b->d_info, ni * sizeof(uint16_t)) != hipSuccess)
+      return fail(SU2HIP_ENOMEM, "order table");
+    b->info_cap = ni;
+  }
+  std::vector<uint16_t> h(ni);
+  for (size_t j = 0; j < ni; j++) h[j] = b->sched[j].info;
+  HIPTRY(hipMemcpyAsync(b->d_info, h.data(), ni * sizeof(uint16_t), hipMemcpyHostToDevice, b->stream));
+  HIPTRY(hipStreamSynchronize(b->stream));
+  return SU2HIP_OK;
+}
+
 static int prepare(su2hip_batch *b) {
   int rc = rebuild_map(b);
   if (rc) return rc;
+  bool rebuilt = b->sched_dirty;
   build_sched(b);
+  if (rebuilt) return ensure_order(b);
   return SU2HIP_OK;
 }
 
@@ -739,14 +793,16 @@ static int pull_muca(su2hip_batch *b, int slot) {
   return SU2HIP_OK;
 }
 
-static int orderparam_refresh(su2hip_batch *b) {
+// both parity contributions of the order parameter from the fields
+// (calc_orderparam) of slot `only`, or of every slot with only < 0
+static int orderparam_refresh(su2hip_batch *b, int only = -1) {
   bool any = false;
-  for (auto &s : b->slots)
-    if (s.used && s.hm.enabled) any = true;
+  for (int i = 0; i < b->max_slots; i++)
+    if (b->slots[i].used && b->slots[i].hm.enabled && (only < 0 || only == i)) any = true;
   if (!any) return SU2HIP_OK;
   int rc = prepare(b);
   if (rc) return rc;
-  HIPTRY(launch_orderparam_init(ctx_of(b)));
+  HIPTRY(launch_orderparam_init(ctx_of(b), only));
   HIPTRY(hipStreamSynchronize(b->stream));
   return SU2HIP_OK;
 }
@@ -861,6 +917,8 @@ int su2hip_destroy(su2hip_batch_t *b) {
   if (b->d_partial) (void)hipFree(b->d_partial);
   if (b->d_sums) (void)hipFree(b->d_sums);
   if (b->d_stage) (void)hipFree(b->d_stage);
+  if (b->d_order) (void)hipFree(b->d_order);
+  if (b->d_info) (void)hipFree(b->d_info);
   for (auto &m : b->alt) {
     if (m.d_blk) (void)hipFree(m.d_blk);
     if (m.d_cnt) (void)hipFree(m.d_cnt);
@@ -950,7 +1008,7 @@ int su2hip_set_fields(su2hip_batch_t *b, int slot, const double *su2link, const 
     HIPTRY(launch_to_device_layout(c, slot, f, b->d_stage));
     HIPTRY(hipStreamSynchronize(b->stream));
   }
-  return orderparam_refresh(b);
+  return orderparam_refresh(b, slot);
 }
 
 int su2hip_get_fields(su2hip_batch_t *b, int slot, double *su2link, double *u1link,
@@ -988,7 +1046,7 @@ int su2hip_cold_start(su2hip_batch_t *b, int slot, double phi0, double sigma0, d
   if (rc) return rc;
   HIPTRY(launch_cold(ctx_of(b), slot, phi0, sigma0, singlet0));
   HIPTRY(hipStreamSynchronize(b->stream));
-  return orderparam_refresh(b);
+  return orderparam_refresh(b, slot);
 }
 
 int su2hip_hot_start(su2hip_batch_t *b, int slot) {
@@ -1000,7 +1058,7 @@ int su2hip_hot_start(su2hip_batch_t *b, int slot) {
   if (rc) return rc;
   HIPTRY(launch_hot(ctx_of(b), slot));
   HIPTRY(hipStreamSynchronize(b->stream));
-  return orderparam_refresh(b);
+  return orderparam_refresh(b, slot);
 }
 
 // the reference's z direction: the longest, of equal ones the last
@@ -1023,7 +1081,7 @@ int su2hip_wall_start(su2hip_batch_t *b, int slot, int dir) {
   if (rc) return rc;
   HIPTRY(launch_wall(ctx_of(b), slot, dir));
   HIPTRY(hipStreamSynchronize(b->stream));
-  return orderparam_refresh(b);
+  return orderparam_refresh(b, slot);
 }
 
 int su2hip_zprofile(su2hip_batch_t *b, int slot, int dir, double *out, int max_rows, int *nrows) {
@@ -1122,7 +1180,7 @@ int su2hip_pass(su2hip_batch_t *b, int kind, int parity, int dir) {
   bool muca = false;
   for (auto &s : b->slots)
     if (s.used && field && muca_on_field(s.hm, field)) muca = true;
-  hipError_t e = enqueue_step(b, Step{kind, parity, dir, 0, muca});
+  hipError_t e = enqueue_step(b, Step{kind, parity, dir, 0, muca, 0}, false);
   if (e == hipErrorInvalidValue)
     return fail(SU2HIP_EINVAL, "kind %d is not part of this model", kind);
   if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "pass launch: %s", hipGetErrorString(e));
@@ -1138,6 +1196,78 @@ int su2hip_use_graph(su2hip_batch_t *b, int enable) {
   if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
   b->use_graph = enable != 0;
   return SU2HIP_OK;
+}
+
+int su2hip_set_random_order(su2hip_batch_t *b, int enable) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  if (b->random_order != (enable != 0)) {
+    b->random_order = enable != 0;
+    b->sched_dirty = true;
+    b->graph_valid = false;
+  }
+  return prepare(b);
+}
+
+int su2hip_get_random_order(su2hip_batch_t *b, int *enabled) {
+  if (!b || !enabled) return fail(SU2HIP_EINVAL, "NULL argument");
+  *enabled = b->random_order ? 1 : 0;
+  return SU2HIP_OK;
+}
+
+int su2hip_slot_params(su2hip_batch_t *b, int slot, su2hip_params *p) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!p) return fail(SU2HIP_EINVAL, "NULL params");
+  *p = b->slots[slot].p;
+  return SU2HIP_OK;
+}
+
+int su2hip_muca_acceptance(su2hip_batch_t *b, int slot, int enable) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  Slot &s = b->slots[slot];
+  if (!s.hm.enabled) return fail(SU2HIP_ESTATE, "slot %d has no multicanonical weight", slot);
+  s.hm.do_acceptance = enable != 0;
+  rc = push_muca(b, slot);
+  if (rc) return rc;
+  b->sched_dirty = true;
+  b->graph_valid = false;
+  return prepare(b);
+}
+
+int su2hip_muca_orderparam(su2hip_batch_t *b, int slot, int recompute, double *pv, double *value) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!b->slots[slot].hm.enabled)
+    return fail(SU2HIP_ESTATE, "slot %d has no multicanonical weight", slot);
+  if (recompute) {
+    rc = orderparam_refresh(b, slot);
+    if (rc) return rc;
+  }
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  const MucaDev &h = b->slots[slot].hm;
+  if (pv) {
+    pv[0] = h.pv[0];
+    pv[1] = h.pv[1];
+  }
+  if (value) *value = h.pv[0] + h.pv[1];
+  return SU2HIP_OK;
+}
+
+int su2hip_muca_set_orderparam(su2hip_batch_t *b, int slot, const double *pv) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (!pv) return fail(SU2HIP_EINVAL, "NULL pv");
+  rc = pull_muca(b, slot);
+  if (rc) return rc;
+  Slot &s = b->slots[slot];
+  if (!s.hm.enabled) return fail(SU2HIP_ESTATE, "slot %d has no multicanonical weight", slot);
+  s.hm.pv[0] = pv[0];
+  s.hm.pv[1] = pv[1];
+  return push_muca(b, slot);
 }
 
 int su2hip_sync(su2hip_batch_t *b) {
@@ -1375,7 +1505,7 @@ int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m, const dou
   if (rc) return rc;
   rc = push_desc(b);
   if (rc) return rc;
-  return orderparam_refresh(b);
+  return orderparam_refresh(b, slot);
 }
 
 int su2hip_muca_get(su2hip_batch_t *b, int slot, su2hip_muca *m, double *pos, double *W,

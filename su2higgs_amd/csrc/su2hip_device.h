@@ -65,6 +65,14 @@ __device__ __forceinline__ double rng_u53(Rng &r) {
   return (double)(x >> 11) * 0x1.0p-53;
 }
 
+// the 64-bit word of the next draw (the reference's iran(); rng_u53 is its
+// top 53 bits)
+__device__ __forceinline__ uint64_t rng_u64(Rng &r) {
+  uint32_t k = r.k++;
+  rng_block(r, k >> 1);
+  return (k & 1) ? (((uint64_t)r.w3 << 32) | r.w2) : (((uint64_t)r.w1 << 32) | r.w0);
+}
+
 __device__ __forceinline__ float rng_u24(Rng &r) {
   uint32_t k = r.k++;
   rng_block(r, k >> 2);

@@ -32,6 +32,15 @@ constexpr int CNT_DOUBLET2_METRO = 11;
 constexpr uint32_t MUCA_SITE = 0xFFFFFFFFu;
 constexpr uint64_t HOTSTART_PASS = 0xFFFFFFFFFFFFFFFFull;
 constexpr uint64_t WALL_PASS = 0xFFFFFFFFFFFFFFFEull;
+constexpr uint32_t SCHED_SITE = 0xFFFFFFFEu;  // site of a sweep's pass-order draws
+
+// Step of a sweep for the random pass order: bit 15 set for a scalar
+// round, bits 8-14 the round, bits 0-7 the position (0-5 in a link round's
+// parity/direction arrays, 0-1 in a scalar round's parities)
+constexpr uint16_t STEP_SCALAR = 0x8000;
+inline uint16_t step_info(bool scalar, int round, int pos) {
+  return (uint16_t)((scalar ? STEP_SCALAR : 0) | ((round & 0x7F) << 8) | (pos & 0xFF));
+}
 
 // model bits; M_H2 (two doublets) always comes with M_NH
 enum : int { M_NH = 1, M_U1 = 2, M_TR = 4, M_SG = 8, M_H2 = 16 };
@@ -127,6 +136,10 @@ struct LaunchCtx {
   // threads per workgroup of pass kernels = sites per block-table chunk
   // (64, 128 or 256); blk/nblk/cnt then belong to the table of that chunk
   int block = BLK;
+  // random pass order: per lattice and sweep step, parity | direction << 1
+  // ([max_slots][npass]); null = the launch's parity and direction
+  const uint8_t *order = nullptr;
+  int npass = 0;
 };
 
 // One checkerboard pass (dir: the direction of a link kind, the doublet of
@@ -138,10 +151,16 @@ hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir,
 // doublet of a doublet kind)
 hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int hid, int parity,
                               int pass_off, int muca_step);
+// random pass order of the coming sweep of every active lattice
+// (src/update.c:292-300, :341-345) into order[max_slots][npass], from the
+// lattice's Philox stream at (pass counter, SCHED_SITE)
+hipError_t launch_order(const LaunchCtx &c, uint8_t *order, const uint16_t *info, int npass,
+                        int link_rounds, int scalar_rounds);
 // sweep epilogue: pass counter += passes, budget -= 1
 hipError_t launch_end_sweep(const LaunchCtx &c, int passes);
 hipError_t launch_measure(const LaunchCtx &c);
-hipError_t launch_orderparam_init(const LaunchCtx &c);
+// both parities' order-parameter contributions of lattice `only` (< 0: all)
+hipError_t launch_orderparam_init(const LaunchCtx &c, int only = -1);
 hipError_t launch_cold(const LaunchCtx &c, int slot, double phi0, double sigma0,
                        double singlet0);
 hipError_t launch_hot(const LaunchCtx &c, int slot);

@@ -776,14 +776,22 @@ __device__ __forceinline__ double orderparam_at(const LatDesc &L, int op, int pa
 template <typename T, int M, int KIND>
 __global__ __launch_bounds__(BLK) void k_pass(const LatDesc *__restrict__ lats,
                                               const MucaDev *__restrict__ mucas,
-                                              const BlockMap *__restrict__ blk, int par, int mu,
-                                              int pass_off, int step, int reunit,
+                                              const BlockMap *__restrict__ blk, int par_in,
+                                              int mu_in, int pass_off, int step, int reunit,
                                               long long *__restrict__ cnt, int nblk,
-                                              double *__restrict__ partial) {
+                                              double *__restrict__ partial,
+                                              const uint8_t *__restrict__ order, int npass) {
   __shared__ double lds[BLK / 64];
   const BlockMap bm = blk[blockIdx.x];
   const LatDesc &L = lats[bm.lat];
   if (!L.active || L.budget == 0) return;
+  int par = par_in, mu = mu_in;
+  if (order) {
+    // random pass order: this lattice's parity (and link direction) of the step
+    uint8_t o = order[(size_t)bm.lat * npass + pass_off];
+    par = o & 1;
+    if (KIND <= SU2HIP_K_U1_METROPOLIS) mu = o >> 1;
+  }
   const int s = bm.s0 + (int)threadIdx.x;
   const bool valid = s < L.Vh;
   int lo = 0, hi = L.Vh;
@@ -988,11 +996,13 @@ __device__ void muca_update_slow(MucaDev &m) {
 __global__ __launch_bounds__(BLK) void k_muca_accept(const LatDesc *__restrict__ lats,
                                                      MucaDev *__restrict__ mucas,
                                                      const double *__restrict__ partial, int kind,
-                                                     int hid, int par, int pass_off, int step) {
+                                                     int hid, int par_in, int pass_off, int step,
+                                                     const uint8_t *__restrict__ order, int npass) {
   __shared__ double lds[BLK / 64];
   const LatDesc &L = lats[blockIdx.x];
   MucaDev &m = mucas[blockIdx.x];
   if (!L.active || L.budget == 0 || !muca_field_on(m, kind, hid)) return;
+  const int par = order ? (order[(size_t)blockIdx.x * npass + pass_off] & 1) : par_in;
   int lo, hi;
   if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
   // fixed-order sum of this lattice's block partials
@@ -1033,11 +1043,13 @@ template <typename T>
 __global__ __launch_bounds__(BLK) void k_muca_restore(const LatDesc *__restrict__ lats,
                                                       const MucaDev *__restrict__ mucas,
                                                       const BlockMap *__restrict__ blk, int kind,
-                                                      int hid, int par, int step) {
+                                                      int hid, int par_in, int pass_off, int step,
+                                                      const uint8_t *__restrict__ order, int npass) {
   const BlockMap bm = blk[blockIdx.x];
   const LatDesc &L = lats[bm.lat];
   const MucaDev &m = mucas[bm.lat];
   if (!L.active || L.budget == 0 || !muca_field_on(m, kind, hid) || !m.rejected) return;
+  const int par = order ? (order[(size_t)bm.lat * npass + pass_off] & 1) : par_in;
   int lo, hi;
   if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
   int s = bm.s0 + (int)threadIdx.x;
@@ -1055,12 +1067,12 @@ template <typename T, int M>
 __global__ __launch_bounds__(BLK) void k_orderparam(const LatDesc *__restrict__ lats,
                                                     const MucaDev *__restrict__ mucas,
                                                     const BlockMap *__restrict__ blk, int nblk,
-                                                    double *__restrict__ partial) {
+                                                    double *__restrict__ partial, int only) {
   __shared__ double lds[BLK / 64];
   const BlockMap bm = blk[blockIdx.x];
   const LatDesc &L = lats[bm.lat];
   const MucaDev &m = mucas[bm.lat];
-  if (!L.active || !m.enabled) return;
+  if (!L.active || !m.enabled || (only >= 0 && bm.lat != only)) return;
   int s = bm.s0 + (int)threadIdx.x;
   for (int par = 0; par < 2; par++) {
     double v = s < L.Vh ? orderparam_at<T, M>(L, m.orderparam, par, s) : 0.0;
@@ -1071,16 +1083,63 @@ __global__ __launch_bounds__(BLK) void k_orderparam(const LatDesc *__restrict__ 
 
 __global__ __launch_bounds__(BLK) void k_orderparam_final(const LatDesc *__restrict__ lats,
                                                           MucaDev *__restrict__ mucas, int nblk,
-                                                          const double *__restrict__ partial) {
+                                                          const double *__restrict__ partial,
+                                                          int only) {
   __shared__ double lds[BLK / 64];
   const LatDesc &L = lats[blockIdx.x];
   MucaDev &m = mucas[blockIdx.x];
-  if (!L.active || !m.enabled) return;
+  if (!L.active || !m.enabled || (only >= 0 && (int)blockIdx.x != only)) return;
   for (int par = 0; par < 2; par++) {
     double v = 0.0;
     for (int i = threadIdx.x; i < L.nblk; i += BLK) v += partial[(size_t)par * nblk + L.blk_base + i];
     double tot = block_sum(v, lds);
     if (threadIdx.x == 0) m.pv[par] = tot / L.vol;
+  }
+}
+
+// shuffle (src/update.c:461-473) with iran() the next 64-bit draw
+__device__ __forceinline__ void shuffle6(int *a, Rng &r) {
+  for (int i = 5; i > 0; i--) {
+    unsigned int j = (unsigned int)(rng_u64(r) % (unsigned long long)(i + 1));
+    int t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+  }
+}
+
+// the random pass order of one sweep of every lattice (one thread per
+// lattice): per link round the parities and then the directions shuffled,
+// per scalar round a uniform for the first parity, in the reference's draw
+// order (src/update.c:278-300, :337-345)
+__global__ void k_order(const LatDesc *__restrict__ lats, int n, uint8_t *__restrict__ order,
+                        const uint16_t *__restrict__ info, int npass, int link_rounds,
+                        int scalar_rounds) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const LatDesc &L = lats[i];
+  if (!L.active || L.budget == 0) return;
+  Rng r;
+  rng_init(r, L.key0, L.key1, L.pass, SCHED_SITE);
+  uint8_t *o = order + (size_t)i * npass;
+  for (int k = 0; k < link_rounds; k++) {
+    int pa[6] = {0, 1, 0, 1, 0, 1}, da[6] = {0, 0, 1, 1, 2, 2};
+    shuffle6(pa, r);
+    shuffle6(da, r);
+    for (int j = 0; j < npass; j++) {
+      uint16_t in = info[j];
+      if (!(in & STEP_SCALAR) && ((in >> 8) & 0x7F) == k) {
+        int pos = in & 0xFF;
+        o[j] = (uint8_t)(pa[pos] | (da[pos] << 1));
+      }
+    }
+  }
+  for (int s = 0; s < scalar_rounds; s++) {
+    int p0 = rng_u53(r) < 0.5 ? 0 : 1;
+    for (int j = 0; j < npass; j++) {
+      uint16_t in = info[j];
+      if ((in & STEP_SCALAR) && ((in >> 8) & 0x7F) == s)
+        o[j] = (uint8_t)((in & 0xFF) == 0 ? p0 : 1 - p0);
+    }
   }
 }
 
@@ -1898,11 +1957,11 @@ static hipError_t pass_one(const LaunchCtx &c, int parity, int dir, int pass_off
     if (c.ev_start || c.ev_stop) {
       hipExtLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream,
                             c.ev_start, c.ev_stop, 0, c.desc, c.muca, c.blk, parity, dir, pass_off,
-                            step, reunit, c.cnt, c.nblk, c.partial);
+                            step, reunit, c.cnt, c.nblk, c.partial, c.order, c.npass);
     } else {
       hipLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream, c.desc,
                          c.muca, c.blk, parity, dir, pass_off, step, reunit, c.cnt, c.nblk,
-                         c.partial);
+                         c.partial, c.order, c.npass);
     }
     return hipGetLastError();
   }
@@ -1936,31 +1995,39 @@ hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir, int pa
 hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int hid, int parity, int pass_off,
                               int muca_step) {
   hipLaunchKernelGGL(k_muca_accept, dim3(c.max_slots), dim3(BLK), 0, c.stream, c.desc, c.muca,
-                     c.partial, kind, hid, parity, pass_off, muca_step);
+                     c.partial, kind, hid, parity, pass_off, muca_step, c.order, c.npass);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || c.nblk == 0) return e;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
     hipLaunchKernelGGL(k_muca_restore<T>, dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.muca,
-                       c.blk, kind, hid, parity, muca_step);
+                       c.blk, kind, hid, parity, pass_off, muca_step, c.order, c.npass);
     return hipGetLastError();
   });
 }
 
-hipError_t launch_orderparam_init(const LaunchCtx &c) {
+hipError_t launch_orderparam_init(const LaunchCtx &c, int only) {
   if (c.nblk == 0) return hipSuccess;
   hipError_t e = with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
     return with_model(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_orderparam<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc,
-                         c.muca, c.blk, c.nblk, c.partial);
+                         c.muca, c.blk, c.nblk, c.partial, only);
       return hipGetLastError();
     });
   });
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_orderparam_final, dim3(c.max_slots), dim3(BLK), 0, c.stream, c.desc,
-                     c.muca, c.nblk, c.partial);
+                     c.muca, c.nblk, c.partial, only);
+  return hipGetLastError();
+}
+
+hipError_t launch_order(const LaunchCtx &c, uint8_t *order, const uint16_t *info, int npass,
+                        int link_rounds, int scalar_rounds) {
+  int nb = (c.max_slots + 63) / 64;
+  hipLaunchKernelGGL(k_order, dim3(nb), dim3(64), 0, c.stream, c.desc, c.max_slots, order, info,
+                     npass, link_rounds, scalar_rounds);
   return hipGetLastError();
 }
 

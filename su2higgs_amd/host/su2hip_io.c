@@ -173,6 +173,7 @@ static int parse_model(const char *s, su2hip_run_config *c) {
   c->correlators = 0;
   c->measure_z = 0;
   c->blocking = 0;
+  c->trajectory = 0;
   char buf[MAXLINE];
   snprintf(buf, sizeof(buf), "%s", s);
   for (char *tok = strtok(buf, "+,"); tok; tok = strtok(NULL, "+,")) {
@@ -194,6 +195,8 @@ static int parse_model(const char *s, su2hip_run_config *c) {
       c->measure_z = 1;
     else if (!strcasecmp(tok, "blocking"))
       c->blocking = c->correlators = 1;
+    else if (!strcasecmp(tok, "trajectory"))
+      c->trajectory = 1;
     else if (!strcasecmp(tok, "su2") || !strcasecmp(tok, "gauge"))
       ;
     else
@@ -318,6 +321,7 @@ int su2hip_config_read(const char *path, const char *model, su2hip_run_config *o
     GETI(&f, "correlator_interval", c.correlator_interval);
   }
   if (c.blocking) GETI(&f, "blocks", c.blocks);
+  if (c.trajectory) GETI(&f, "do_trajectory", c.do_trajectory);
   c.flow_scheme = SU2HIP_FLOW_EULER;
   if (c.gradflow) {
     GETI(&f, "do_flow", c.do_flow);

@@ -15,6 +15,7 @@
 
 #include "../../include/su2hip.h"
 #include "../../include/su2hip_io.h"
+#include "../../include/su2hip_traj.h"
 
 #define MAXBINS 100000
 
@@ -260,6 +261,28 @@ int main(int argc, char **argv) {
     printf("Using weight function with %d bins in range %lf, %lf\n", cfg.muca.bins, cfg.muca.min, cfg.muca.max);
   }
 
+  /* heatbath trajectories of the HB_TRAJECTORY build (src/main.c:223-238) */
+  su2hip_trajectory_config traj;
+  memset(&traj, 0, sizeof(traj));
+  int do_traj = cfg.trajectory && cfg.do_trajectory;
+  if (do_traj) {
+    printf("\nReal time simulation: reading file \"realtime_config\"\n");
+    TRY("realtime_config", su2hip_realtime_config_read("realtime_config", &traj));
+    /* print_realtime_params (src/hb_trajectory.c:166-178) */
+    printf("Heatbath trajectory mode every %d iterations, %d trajectories each\n", traj.mode_interval,
+           traj.n_traj);
+    printf("--- min %lf, max %lf, measure interval %d ---\n", traj.min, traj.max, traj.interval);
+    if (p->algorithm_su2link != SU2HIP_HEATBATH) printf("\n--- WARNING: SU(2) update not using heatbath!\n");
+    if (p->u1 && p->algorithm_u1link != SU2HIP_HEATBATH)
+      printf("\n--- WARNING: U(1) update not using heatbath!\n");
+    fflush(stdout);
+    if (!cfg.multicanonical) {
+      printf("\nError: Need multicanonical for heatbath trajectories!! Exiting...\n");
+      return 1;
+    }
+    TRY("trajectory_header", su2hip_trajectory_header("trajectory", &traj));
+  }
+
   if (cfg.reset) {
     /* thermalise without modifying the weight (src/main.c:253-282) */
     su2hip_muca m;
@@ -298,6 +321,7 @@ int main(int argc, char **argv) {
   double start = now();
   int flow_id = 1;
   int correlator_id = 1;
+  int traj_id = 1;
   double *corr = NULL;
   if (cfg.correlators && cfg.do_correlators) {
     /* main.c:98-104: labels, the longest direction */
@@ -327,6 +351,8 @@ int main(int argc, char **argv) {
       TRY("zprof_append", su2hip_zprof_append("measure_z", p, zprof, nz));
     }
     if (cfg.gradflow && cfg.do_flow && iter % cfg.flow_interval == 0) flow(b, &cfg, flow_id++);
+    if (do_traj && iter % traj.mode_interval == 0)
+      TRY("trajectories", su2hip_trajectories(b, 0, &traj, traj_id++, "trajectory"));
     if (corr && iter % cfg.correlator_interval == 0) {
       /* measure_correlators("correl0", ...) (src/main.c:330-349) */
       int nd = 0;

@@ -79,6 +79,24 @@ class Batch:
     def use_graph(self, on=True):
         check(self.lib.su2hip_use_graph(self.h, 1 if on else 0))
 
+    def set_random_order(self, on=True):
+        check(self.lib.su2hip_set_random_order(self.h, 1 if on else 0))
+
+    def muca_acceptance(self, slot, on):
+        check(self.lib.su2hip_muca_acceptance(self.h, slot, 1 if on else 0))
+
+    def muca_orderparam(self, slot, recompute=True):
+        """(param_value[EVEN], param_value[ODD]) after an optional recompute"""
+        pv = np.zeros(2)
+        v = C.c_double()
+        check(self.lib.su2hip_muca_orderparam(self.h, slot, 1 if recompute else 0,
+                                              pv.ctypes.data_as(C.POINTER(C.c_double)), C.byref(v)))
+        return pv
+
+    def muca_set_orderparam(self, slot, pv):
+        pv = np.ascontiguousarray(pv, dtype=np.float64)
+        check(self.lib.su2hip_muca_set_orderparam(self.h, slot, pv.ctypes.data_as(C.POINTER(C.c_double))))
+
     def set_budget(self, slot, n):
         check(self.lib.su2hip_set_budget(self.h, slot, n))
 

@@ -57,6 +57,9 @@ def oracle_lib():
             "orc_site_update": (C.c_int, [h, C.c_int, C.c_long, C.c_int, C.c_uint64]),
             "orc_pass": (None, [h, C.c_int, C.c_int, C.c_int]),
             "orc_update": (None, [h, C.c_int]),
+            "orc_set_random_order": (None, [h, C.c_int]),
+            "orc_muca_set_acceptance": (None, [h, C.c_int]),
+            "orc_word": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]),
             "orc_measure": (None, [h, dp]),
             "orc_muca_set": (None, [h, P(Muca), dp, dp]),
             "orc_muca_get": (None, [h, P(Muca), dp, dp, P(C.c_int32)]),
@@ -233,13 +236,18 @@ _refs = {}
 
 REF_FLAGS = {1: "h", 3: "hu", 4: "t", 5: "ht", 17: "hg", 20: "tg", 11: "hsu",
              33: "hc", 36: "tc", 37: "htc", 65: "hz", 68: "tz", 75: "hsuz", 164: "tcb",
-             257: "d", 259: "du", 275: "dug"}
+             257: "d", 259: "du", 275: "dug", 513: "hr", 516: "tr"}
+# builds that do not link as shipped (check_set is undefined in
+# src/hb_trajectory.c): their harness libraries load with lazy binding
+LAZY = ("hr", "tr")
 
 
-def ref_tag(p, gradflow=False, correlators=False, measure_z=False, blocking=False):
+def ref_tag(p, gradflow=False, correlators=False, measure_z=False, blocking=False,
+            trajectory=False):
     m = (1 if p.nhiggs else 0) | (2 if p.u1 else 0) | (4 if p.triplet else 0) \
         | (8 if p.singlet else 0) | (16 if gradflow else 0) | (32 if correlators else 0) \
-        | (64 if measure_z else 0) | (128 if blocking else 0) | (256 if p.nhiggs == 2 else 0)
+        | (64 if measure_z else 0) | (128 if blocking else 0) | (256 if p.nhiggs == 2 else 0) \
+        | (512 if trajectory else 0)
     return REF_FLAGS.get(m)
 
 
@@ -251,7 +259,7 @@ def ref_lib(tag):
     if tag in _refs:
         return _refs[tag]
     path = os.path.join(REFDIR, "libref_%s.so" % tag)
-    lib = C.CDLL(path)
+    lib = C.CDLL(path, mode=os.RTLD_LAZY) if tag in LAZY else C.CDLL(path)
     sig = {
         "ref_model": (C.c_int, []),
         "ref_setup": (C.c_int, [P(Params)]),
@@ -283,6 +291,8 @@ def ref_lib(tag):
         "ref_save_lattice": (None, [C.c_char_p, C.c_long]),
         "ref_load_lattice": (C.c_long, [C.c_char_p]),
         "ref_get_parameters": (C.c_int, [C.c_char_p, P(Params), lp, P(Muca), C.c_char_p, C.c_char_p]),
+        "ref_set_random_sweeps": (None, [C.c_int]),
+        "ref_muca_set_acceptance": (None, [C.c_int]),
     }
     opt = {
         "ref_doublet_staple": (None, [C.c_long, dp]),
@@ -307,6 +317,7 @@ def ref_lib(tag):
         "ref_blocked_setup": (C.c_int, [C.c_int]),
         "ref_blocked_correlators": (None, [C.c_int]),
         "ref_blocked_values": (C.c_int, [dp]),
+        "ref_trajectories": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]),
     }
     for n, (r, a) in sig.items():
         fn = getattr(lib, n)
@@ -324,8 +335,9 @@ def ref_lib(tag):
 class Ref:
     """The reference compiled from source, one global lattice per tag."""
 
-    def __init__(self, p, gradflow=False, correlators=False, measure_z=False, blocking=False):
-        self.tag = ref_tag(p, gradflow, correlators or blocking, measure_z, blocking)
+    def __init__(self, p, gradflow=False, correlators=False, measure_z=False, blocking=False,
+                 trajectory=False):
+        self.tag = ref_tag(p, gradflow, correlators or blocking, measure_z, blocking, trajectory)
         self.lib = ref_lib(self.tag)
         self.p = p
         self.V = volume(p)

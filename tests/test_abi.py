@@ -52,6 +52,7 @@ STRUCTS = {
     "su2hip_result": abi.Result,
     "su2hip_serve_config": abi.ServeConfig,
     "su2hip_serve_stats": abi.ServeStats,
+    "su2hip_trajectory_config": abi.TrajectoryConfig,
 }
 RENAME = {"pass_": "pass"}
 
@@ -59,7 +60,8 @@ RENAME = {"pass_": "pass"}
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc absent")
 def test_struct_layouts_match_the_headers(tmp_path):
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "su2hip.h"',
-             '#include "su2hip_io.h"', '#include "su2hip_serve.h"', "int main(void) {"]
+             '#include "su2hip_io.h"', '#include "su2hip_serve.h"', '#include "su2hip_traj.h"',
+             "int main(void) {"]
     expect = []
     for cname, py in STRUCTS.items():
         lines.append('printf("%%zu\\n", sizeof(%s));' % cname)

@@ -438,3 +438,27 @@ def test_wall_start_structure_matches_reference(tag):
     assert np.allclose((a["su2link"] ** 2).sum(-1), 1)
     if have["doublet"] is not None:  # the two phases sit on the same sides
         assert np.array_equal(a["doublet"][:, 0] > 0.6, b["doublet"][:, 0] > 0.6)
+
+
+def test_realtime_config_reader(lib, tmp_path):
+    """read_realtime_config's format (src/hb_trajectory.c:99-163): the first
+    two words of each line, '#' keys skipped, keys in any case, the last
+    occurrence wins; a missing key or file is an error.  (The reference's
+    own reader cannot run: check_set, which it calls, is defined in none of
+    its sources.)"""
+    path = str(tmp_path / "realtime_config")
+    with open(path, "w") as f:
+        f.write("# realtime trajectories\nN_TRAJ 5 extra words\ninterval 3\n\nmode_interval 100\n"
+                "min 0.25\n#max 9\nmax 1.75\ninterval 4\n")
+    t = abi.TrajectoryConfig()
+    assert lib.su2hip_realtime_config_read(path.encode(), C.byref(t)) == 0
+    assert (t.n_traj, t.interval, t.mode_interval, t.min, t.max) == (5, 4, 100, 0.25, 1.75)
+    with open(path, "w") as f:
+        f.write("n_traj 5\ninterval 3\nmode_interval 100\nmin 0.25\n")
+    assert lib.su2hip_realtime_config_read(path.encode(), C.byref(t)) == abi.EIO
+    assert b"max" in lib.su2hip_last_error()
+    assert lib.su2hip_realtime_config_read(str(tmp_path / "absent").encode(), C.byref(t)) == abi.EIO
+    hp = str(tmp_path / "trajectory")
+    t.n_traj, t.min, t.max = 3, 0.25, 1.75
+    assert lib.su2hip_trajectory_header(hp.encode(), C.byref(t)) == 0
+    assert open(hp).read() == "===== Realtime trajectories: min=0.25, max=1.75, n_traj=3 ===== \n"

@@ -175,6 +175,46 @@ def test_full_sweeps(tag, variant):
     assert o.rng_get()[1] == r.lib.ref_rng_pass()
 
 
+@pytest.mark.parametrize("tag", ["h", "hu", "t", "d"])
+def test_random_order_sweeps(tag):
+    """The reference's random_sweeps (shuffled link passes, src/update.c:
+    292-300, a random first parity per scalar round, :341-345) against the
+    oracle's random order: same shuffles from the sweep's schedule draws,
+    same passes, same fields."""
+    need_ref(tag)
+    p, o, r, f = setup(tag, link_overrelax=0)
+    o.rng_set(7, 0)
+    r.rng_set(7, 0)
+    o.lib.orc_set_random_order(o.h, 1)
+    r.lib.ref_set_random_sweeps(1)
+    o.update(3)
+    r.update(3)
+    fo, fr = o.get_fields(), r.get_fields()
+    tol = 1e-11 if p.nhiggs < 2 else 1e-9
+    for k in orc.FIELDS:
+        if fr[k] is not None and np.abs(fr[k]).max() > 0:
+            assert np.abs(fo[k] - fr[k]).max() < tol, k
+    co, cr = o.counters(), r.counters()
+    for k in cr:
+        if k not in ("total_link_or", "accepted_link_or"):
+            assert co[k] == cr[k], k
+    # the order really changed: the same start without it ends elsewhere
+    o2 = orc.Oracle(p)
+    o2.set_fields(f)
+    o2.rng_set(7, 0)
+    o2.update(3)
+    assert np.abs(o2.get_fields()["su2link"] - fo["su2link"]).max() > 1e-3
+
+
+def test_shuffle_draws_are_the_iran_words():
+    """iran() is the full 64-bit word of a draw and dran() its top 53 bits,
+    as genrand64_real2 derives from genrand64_int64."""
+    lib = orc.oracle_lib()
+    for k in range(8):
+        w = lib.orc_word(11, 5, 0xFFFFFFFE, k)
+        assert orc.uniform(11, 5, 0xFFFFFFFE, k) == (w >> 11) * 2.0 ** -53
+
+
 def test_higgs2_rounding_growth():
     """The tolerance of the two-doublet sweep comparison: a 1e-14 relative
     change of the doublets grows by more than 10x per sweep under the
