@@ -161,7 +161,7 @@ def test_trajectories_match_reference(tmp_path, tag, kw):
         x, y = a.split(), c.split()
         assert len(x) == len(y) == 2 and x[0] == y[0], (a, c)
         assert abs(float(x[1]) - float(y[1])) <= 1.01e-8, (a, c)
-    assert maxdiff(b.get_fields(0), r.get_fields()) < 1e-10
+    assert maxdiff(r.get_fields(), b.get_fields(0)) < 1e-10
     co, cg = r.counters(), b.counters(0)
     for k in co:
         assert co[k] == cg[k], k
