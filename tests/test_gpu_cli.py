@@ -160,6 +160,56 @@ def test_cli_two_doublets_match_reference(tmp_path):
         assert np.abs(g[k] - f[k]).max() < 1e-10, k
 
 
+def test_cli_heatbath_trajectories_match_reference(tmp_path):
+    """The HB_TRAJECTORY build's main loop (src/main.c:321-328): a
+    multicanonical run with a set of trajectories at iteration 3; the
+    measure lines and the trajectory file against the reference's own
+    make_realtime_trajectories on the same Philox stream."""
+    if not orc.ref_available("hr"):
+        pytest.skip("reference build absent")
+    from test_gpu_traj import write_weight
+    d = str(tmp_path)
+    os.chdir(d)
+    write_weight("weight", 40, 0.0, 2.0)
+    cfg = CONFIG.format(model="higgs+trajectory", it=4, do_flow=0).replace(
+        "multicanonical 0", "multicanonical 1")
+    cfg += ("do_trajectory 1\nweightfile weight\nbins 40\nmin 0.0\nmax 2.0\nchecks_per_sweep 2\n"
+            "orderparam phisq\nmuca_mode 0\nmuca_delta 0.3\n")
+    with open("realtime_config", "w") as f:
+        f.write("n_traj 3\ninterval 1\nmode_interval 3\nmin 0.7\nmax 0.95\n")
+    run_cli(d, cfg)
+    ours_m = open("measure").read().splitlines(keepends=True)
+    ours_t = open("trajectory").read()
+    os.rename("trajectory", "trajectory_ours")
+    p = make_params(L=(4, 6, 8), nhiggs=1, betasu2=8.0, msq_phi=-0.25, lambda_phi=0.02,
+                    scalar_sweeps=3, seed=4242)
+    r = orc.Ref(p, trajectory=True)
+    o = orc.Oracle(p)
+    o.cold_start(phi0=1.0)
+    r.set_fields(o.get_fields())
+    r.rng_set(p.seed, 0)
+    m = abi.Muca()
+    m.enabled, m.orderparam, m.mode, m.checks_per_sweep = 1, abi.PHISQ, abi.READONLY, 2
+    m.bins, m.min, m.max, m.delta = 40, 0.0, 2.0, 0.3
+    assert r.lib.ref_muca_load(b"weight", C.byref(m)) == 0
+    lines = []
+    for it in range(1, 5):
+        lines.append(r.measure_text())
+        if it % 3 == 0:
+            r.lib.ref_trajectories(3, 1, 0.7, 0.95, it // 3)
+        r.update(1)
+    same_lines(ours_m, lines)
+    ref_t = "===== Realtime trajectories: min=0.7, max=0.95, n_traj=3 ===== \n" + \
+        open("trajectory").read()
+    a, c = ours_t.splitlines(), ref_t.splitlines()
+    assert len(a) == len(c) > 10
+    for x, y in zip(a, c):
+        if x != y:
+            u, v = x.split(), y.split()
+            assert len(u) == len(v) == 2 and u[0] == v[0], (x, y)
+            assert abs(float(u[1]) - float(v[1])) <= 1.01e-8, (x, y)
+
+
 def test_cli_resume_is_exact(tmp_path):
     """6 sweeps in one run = 3 + 3 with a restart from the final checkpoint
     (the Philox position travels in lattice.rng).  As in the reference, the
