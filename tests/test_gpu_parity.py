@@ -189,8 +189,9 @@ def test_budgets():
     assert maxdiff(f1, b.get_fields(1)) == 0.0
 
 
-def test_flow_force_and_euler():
-    p = make_params(L=(4, 6, 8), nhiggs=1)
+@pytest.mark.parametrize("model", [dict(nhiggs=1), dict(nhiggs=2, u1=1)])
+def test_flow_force_and_euler(model):
+    p = make_params(L=(4, 6, 8), **model)
     o = orc.Oracle(p)
     o.hot_start()
     b = Batch(p)

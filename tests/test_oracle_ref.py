@@ -317,3 +317,36 @@ def test_blocked_correlators(L):
     assert ours.shape == ref.shape
     scale = np.abs(ref).max(1, keepdims=True) + 1e-300
     assert (np.abs(ours - ref) <= 1e-12 * scale).all(), np.abs(ours - ref).max((0, 1))
+
+
+def test_flow_with_two_doublets(tmp_path):
+    """grad_flow of the -DNHIGGS=2 -DU1 -DGRADFLOW build: the links flow,
+    both doublets stay, measure_flow rows through the two-doublet columns."""
+    need_ref("dug")
+    import ctypes as C
+    import os
+    from su2higgs_amd import abi
+    p = make_params(L=(4, 4, 6), nhiggs=2, u1=1, seed=77)
+    o = orc.Oracle(p)
+    o.hot_start()
+    r = orc.Ref(p, gradflow=True)
+    r.set_fields(o.get_fields())
+    F, _ = o.flow_force()
+    G = np.zeros((o.V, 3, 4))
+    r.lib.ref_flow_force(orc.ptr(G), None)
+    assert rel(F, G) < 1e-12
+    os.chdir(tmp_path)
+    r.lib.ref_grad_flow(0.25, 0.05, 2, 1)
+    rows = o.flow(abi.FLOW_EULER, 0.05, 0.25, 2)
+    text = open("measure_flow").read().strip().splitlines()[1:]
+    assert len(text) == len(rows)
+    lib = abi.load()
+    for line, row in zip(text, rows):
+        vals = [float(x) for x in line.split()]
+        cols = (C.c_double * 64)()
+        n = C.c_int()
+        lib.su2hip_measure_columns(C.byref(p), orc.ptr(np.ascontiguousarray(row[1:])), -0.0, 0.0,
+                                   cols, C.byref(n))
+        assert len(vals) == 1 + n.value
+        for a, b in zip(vals[1:], list(cols)[: n.value]):
+            assert abs(a - b) <= 1e-5 * max(1.0, abs(b)), (a, b)
