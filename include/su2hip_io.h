@@ -68,11 +68,22 @@ int su2hip_lattice_write(const char *path, const su2hip_params *p, int64_t iter,
                          double total_time, double comms_time, const double *su2link,
                          const double *u1link, const double *doublet,
                          const double *triplet, const double *singlet);
-/* Checks size, dim and extents against p (the reference dies on a mismatch,
- * src/checkpoint.c:154-179; here SU2HIP_EIO). */
+/* Checks dim and extents against p (the reference dies on a mismatch,
+ * src/checkpoint.c:154-179; here SU2HIP_EIO).  A file written by an MPI run
+ * (size > 1, rank after rank, src/checkpoint.c:226-269) is read too: its
+ * sites are put back into the single-lattice order (su2hip_mpi_site_order).
+ * The reference itself only reads a file of its own rank count. */
 int su2hip_lattice_read(const char *path, const su2hip_params *p, int64_t *iter,
                         double *total_time, double *comms_time, double *su2link,
                         double *u1link, double *doublet, double *triplet, double *singlet);
+/* Node grid (nslices) and node box (slice) of an MPI run of `size` ranks on
+ * extents L (make_slices, src/layout.c:118-237). */
+int su2hip_mpi_slices(const int32_t *L, int size, int32_t *nslices, int32_t *slice);
+/* order[q] = single-lattice site index of the q-th site of an MPI-written
+ * field block (V entries): ranks in turn, each its node box's sites
+ * x-fastest, EVEN before ODD (sitemap and paritymap, src/layout.c:245-402,
+ * :826-851). */
+int su2hip_mpi_site_order(const int32_t *L, int size, int64_t *order);
 
 /* Weight file (load_weight / save_weight, src/multicanonical.c:82-288).
  * On input m holds the config's bins/min/max/mode/delta/orderparam/

@@ -268,6 +268,8 @@ def load():
         "su2hip_pass": (C.c_int, [h, C.c_int, C.c_int, C.c_int]),
         "su2hip_use_graph": (C.c_int, [h, C.c_int]),
         "su2hip_set_random_order": (C.c_int, [h, C.c_int]),
+        "su2hip_mpi_slices": (C.c_int, [P(C.c_int32), C.c_int, P(C.c_int32), P(C.c_int32)]),
+        "su2hip_mpi_site_order": (C.c_int, [P(C.c_int32), C.c_int, P(C.c_int64)]),
         "su2hip_get_random_order": (C.c_int, [h, P(C.c_int)]),
         "su2hip_slot_params": (C.c_int, [h, C.c_int, P(Params)]),
         "su2hip_realtime_config_read": (C.c_int, [C.c_char_p, P(TrajectoryConfig)]),

@@ -444,6 +444,84 @@ int su2hip_lattice_write(const char *path, const su2hip_params *p, int64_t iter,
   return SU2HIP_OK;
 }
 
+/* Node grid of an MPI run (make_slices, src/layout.c:118-237): factor the
+ * rank count into the primes 2..29 and, from the largest prime down, cut the
+ * longest node side that the prime divides (the first such side when equal).
+ * The rank count and the lattice must allow it, as the reference dies
+ * otherwise (src/layout.c:122-151). */
+int su2hip_mpi_slices(const int32_t *L, int size, int32_t *nslices, int32_t *slice) {
+  if (!L || !nslices || !slice) return ioerr(SU2HIP_EINVAL, "NULL argument");
+  static const int prime[10] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29};
+  long vol = (long)L[0] * L[1] * L[2];
+  if (size < 1 || vol % size != 0)
+    return ioerr(SU2HIP_EINVAL, "Can't lay out %ld sites using %d nodes!", vol, size);
+  int nf[10], i = size;
+  for (int n = 0; n < 10; n++) {
+    nf[n] = 0;
+    while (i % prime[n] == 0) {
+      i /= prime[n];
+      nf[n]++;
+    }
+  }
+  if (i != 1) return ioerr(SU2HIP_EINVAL, "Can't factorize %d nodes with primes up to 29", size);
+  long sl[3] = {L[0], L[1], L[2]};
+  int ns[3] = {1, 1, 1};
+  for (int n = 9; n >= 0; n--)
+    for (int f = 0; f < nf[n]; f++) {
+      long largest = 1;
+      for (int d = 0; d < 3; d++)
+        if (sl[d] > largest && sl[d] % prime[n] == 0) largest = sl[d];
+      if (largest == 1)
+        return ioerr(SU2HIP_EINVAL, "no side of %d x %d x %d divides into %d nodes", L[0], L[1],
+                     L[2], size);
+      for (int d = 0; d < 3; d++)
+        if (sl[d] == largest) {
+          sl[d] /= prime[n];
+          ns[d] *= prime[n];
+          break;
+        }
+    }
+  for (int d = 0; d < 3; d++) {
+    nslices[d] = ns[d];
+    slice[d] = (int32_t)sl[d];
+  }
+  return SU2HIP_OK;
+}
+
+/* Where the sites of an MPI-written checkpoint go.  write_field
+ * (src/checkpoint.c:226-269) writes each field rank after rank; a rank's
+ * sites are its node box (ranks numbered x-fastest over the node grid, the
+ * sites' own indexing logic, doc/doc.tex:653; indexToCoords,
+ * src/layout.c:945-963) in the node-local order: the box's sites x-fastest
+ * (sitemap puts real sites before halos in that order, src/layout.c:
+ * 245-402), then EVEN before ODD with the order kept (paritymap,
+ * src/layout.c:826-851), parity from the global coordinates
+ * (src/layout.c:730-756).  order[q] is the single-lattice site index
+ * (EVEN first, natural order) of the q-th site of the file's field blocks. */
+int su2hip_mpi_site_order(const int32_t *L, int size, int64_t *order) {
+  if (!order) return ioerr(SU2HIP_EINVAL, "NULL argument");
+  int32_t ns[3], sl[3];
+  int rc = su2hip_mpi_slices(L, size, ns, sl);
+  if (rc) return rc;
+  long V = (long)L[0] * L[1] * L[2], Vh = V / 2;
+  long per = (long)sl[0] * sl[1] * sl[2];
+  long q = 0;
+  for (int r = 0; r < size; r++) {
+    long n0 = r % ns[0], n1 = (r / ns[0]) % ns[1], n2 = r / ((long)ns[0] * ns[1]);
+    long o0 = n0 * sl[0], o1 = n1 * sl[1], o2 = n2 * sl[2];
+    for (int par = 0; par < 2; par++)
+      for (long k = 0; k < per; k++) {
+        long x = o0 + k % sl[0], y = o1 + (k / sl[0]) % sl[1], z = o2 + k / ((long)sl[0] * sl[1]);
+        if ((int)((x + y + z) & 1) != par) continue;
+        long nat = x + (long)L[0] * (y + (long)L[1] * z);
+        /* single-lattice order: parity block, then natural order inside it
+         * (L[0] even: position nat >> 1) */
+        order[q++] = par * Vh + (nat >> 1);
+      }
+  }
+  return q == V ? SU2HIP_OK : ioerr(SU2HIP_EINVAL, "MPI site order: %ld of %ld sites", q, V);
+}
+
 int su2hip_lattice_read(const char *path, const su2hip_params *p, int64_t *iter,
                         double *total_time, double *comms_time, double *su2link,
                         double *u1link, double *doublet, double *triplet, double *singlet) {
@@ -456,13 +534,13 @@ int su2hip_lattice_read(const char *path, const su2hip_params *p, int64_t *iter,
     fclose(fp);
     return ioerr(SU2HIP_EIO, "Error reading latticefile %s", path);
   }
-  int ok = (size == 1 && dim == 3);
+  int ok = (size >= 1 && dim == 3);
   for (int d = 0; ok && d < 3; d++) ok = L[d] == p->L[d];
   if (!ok) {
     fclose(fp);
     return ioerr(SU2HIP_EIO,
                  "Dimensions in latticefile do not match! Got MPI size %d, dimension %d, volume "
-                 "%d x %d x %d; expected MPI size 1, dimension 3, volume %d x %d x %d",
+                 "%d x %d x %d; expected dimension 3, volume %d x %d x %d",
                  size, dim, L[0], L[1], L[2], p->L[0], p->L[1], p->L[2]);
   }
   long it = 0;
@@ -473,22 +551,53 @@ int su2hip_lattice_read(const char *path, const su2hip_params *p, int64_t *iter,
     return ioerr(SU2HIP_EIO, "Error reading latticefile header of %s", path);
   }
   long V = vol_of(p);
+  /* a file of an MPI run: the rank-major site order, mapped back */
+  int64_t *order = NULL;
+  double *buf = NULL;
+  if (size > 1) {
+    order = malloc(sizeof(int64_t) * V);
+    buf = malloc(sizeof(double) * V * 12);
+    if (!order || !buf) {
+      free(order);
+      free(buf);
+      fclose(fp);
+      return ioerr(SU2HIP_ENOMEM, "host allocation");
+    }
+    int32_t L32[3] = {L[0], L[1], L[2]};
+    int rc = su2hip_mpi_site_order(L32, size, order);
+    if (rc) {
+      free(order);
+      free(buf);
+      fclose(fp);
+      return rc;
+    }
+  }
   double *dst[5] = {su2link, u1link, doublet, triplet, singlet};
   int have[5] = {1, p->u1, p->nhiggs, p->triplet, p->singlet};
-  long n[5] = {V * 12, V * 3, V * 4 * p->nhiggs, V * 3, V};
-  for (int k = 0; k < 5; k++) {
+  int comps[5] = {12, 3, 4, 3, 1};
+  int err = 0;
+  for (int k = 0; k < 5 && !err; k++) {
     if (!have[k]) continue;
-    if (dst[k]) {
-      if (fread(dst[k], sizeof(double), n[k], fp) != (size_t)n[k]) {
-        fclose(fp);
-        return ioerr(SU2HIP_EIO, "Error reading field!");
+    /* doublets one after the other, each its own field block */
+    int blocks = k == 2 ? p->nhiggs : 1;
+    for (int bl = 0; bl < blocks && !err; bl++) {
+      long n = V * comps[k];
+      double *d = dst[k] ? dst[k] + (long)bl * n : NULL;
+      if (!d) {
+        err = fseek(fp, n * (long)sizeof(double), SEEK_CUR) != 0;
+      } else if (!order) {
+        err = fread(d, sizeof(double), n, fp) != (size_t)n;
+      } else {
+        err = fread(buf, sizeof(double), n, fp) != (size_t)n;
+        for (long q = 0; q < V && !err; q++)
+          memcpy(d + order[q] * comps[k], buf + q * comps[k], sizeof(double) * comps[k]);
       }
-    } else if (fseek(fp, n[k] * (long)sizeof(double), SEEK_CUR) != 0) {
-      fclose(fp);
-      return ioerr(SU2HIP_EIO, "Error reading field!");
     }
   }
   fclose(fp);
+  free(order);
+  free(buf);
+  if (err) return ioerr(SU2HIP_EIO, "Error reading field!");
   if (iter) *iter = it;
   if (total_time) *total_time = tt;
   if (comms_time) *comms_time = ct;

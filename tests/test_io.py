@@ -462,3 +462,98 @@ def test_realtime_config_reader(lib, tmp_path):
     t.n_traj, t.min, t.max = 3, 0.25, 1.75
     assert lib.su2hip_trajectory_header(hp.encode(), C.byref(t)) == 0
     assert open(hp).read() == "===== Realtime trajectories: min=0.25, max=1.75, n_traj=3 ===== \n"
+
+
+I32 = C.c_int32 * 3
+
+
+def mpi_slices(lib, L, size):
+    ns, sl = I32(), I32()
+    rc = lib.su2hip_mpi_slices(I32(*L), size, ns, sl)
+    return rc, tuple(ns), tuple(sl)
+
+
+def test_mpi_slices(lib):
+    """make_slices (src/layout.c:118-237) on cases worked by hand from its
+    rule: primes of the rank count from the largest down, each cutting the
+    longest side it divides (the first of equal ones)."""
+    assert mpi_slices(lib, (16, 16, 16), 8) == (0, (2, 2, 2), (8, 8, 8))
+    assert mpi_slices(lib, (4, 6, 8), 2) == (0, (1, 1, 2), (4, 6, 4))
+    assert mpi_slices(lib, (6, 6, 6), 6) == (0, (3, 2, 1), (2, 3, 6))
+    assert mpi_slices(lib, (12, 8, 10), 30) == (0, (3, 2, 5), (4, 4, 2))
+    assert mpi_slices(lib, (8, 8, 8), 1) == (0, (1, 1, 1), (8, 8, 8))
+    assert mpi_slices(lib, (4, 4, 4), 7)[0] == abi.EINVAL      # 64 sites on 7 ranks
+    assert mpi_slices(lib, (62, 2, 2), 31)[0] == abi.EINVAL    # 31 is not among the primes
+
+
+MPI_CASES = [((4, 6, 8), 2), ((6, 6, 6), 6), ((16, 16, 16), 8), ((12, 8, 10), 30), ((8, 4, 6), 4)]
+
+
+@pytest.mark.parametrize("L,size", MPI_CASES)
+def test_mpi_site_order_is_the_documented_layout(lib, L, size):
+    """The node-local order of an MPI checkpoint, by its documented
+    properties: a permutation of the sites; rank r holds the node box at
+    node coordinates r taken x-fastest (doc/doc.tex:653); inside a rank the
+    EVEN sites come first, then the ODD ones (paritymap), each in x-fastest
+    order of the box (sitemap).  (The reference's MPI layout code needs
+    mpi.h and does not build here, so these properties, not the reference,
+    pin it.)"""
+    rc, ns, sl = mpi_slices(lib, L, size)
+    assert rc == 0
+    V = L[0] * L[1] * L[2]
+    order = np.zeros(V, dtype=np.int64)
+    assert lib.su2hip_mpi_site_order(I32(*L), size, order.ctypes.data_as(C.POINTER(C.c_int64))) == 0
+    assert sorted(order) == list(range(V))
+    # single-lattice index -> coordinates: parity block, natural order inside
+    nat = np.array([n for par in (0, 1) for n in range(V)
+                    if (n % L[0] + (n // L[0]) % L[1] + n // (L[0] * L[1])) % 2 == par])
+    x, y, z = nat[order] % L[0], (nat[order] // L[0]) % L[1], nat[order] // (L[0] * L[1])
+    per = V // size
+    for r in range(size):
+        sl_r = slice(r * per, (r + 1) * per)
+        node = (r % ns[0], (r // ns[0]) % ns[1], r // (ns[0] * ns[1]))
+        for c, n, s in zip((x, y, z), node, sl):
+            assert (c[sl_r] >= n * s).all() and (c[sl_r] < (n + 1) * s).all()
+        par = (x[sl_r] + y[sl_r] + z[sl_r]) % 2
+        ne = int((par == 0).sum())
+        assert (par[:ne] == 0).all() and (par[ne:] == 1).all()
+        lx, ly, lz = x[sl_r] - node[0] * sl[0], y[sl_r] - node[1] * sl[1], z[sl_r] - node[2] * sl[2]
+        local = lx + sl[0] * (ly + sl[1] * lz)
+        assert (np.diff(local[:ne]) > 0).all() and (np.diff(local[ne:]) > 0).all()
+
+
+@pytest.mark.parametrize("model", [dict(nhiggs=1, u1=1), dict(nhiggs=2), dict(nhiggs=0, triplet=1)])
+def test_checkpoint_of_an_mpi_run_is_read_into_single_order(lib, tmp_path, model):
+    """A lattice file with MPI size 4: header, then every field block rank
+    after rank in the node-local order (each doublet its own block); the
+    reader returns the fields in the single-lattice order."""
+    L, size = (8, 4, 6), 4
+    p = make_params(L=L, **model)
+    f = hot_fields(p)
+    V = abi.volume(p)
+    order = np.zeros(V, dtype=np.int64)
+    assert lib.su2hip_mpi_site_order(I32(*L), size, order.ctypes.data_as(C.POINTER(C.c_int64))) == 0
+    blocks = [f["su2link"].reshape(V, 12)]
+    if p.u1:
+        blocks.append(f["u1link"].reshape(V, 3))
+    if p.nhiggs == 2:
+        blocks += [f["doublet"][0], f["doublet"][1]]
+    elif p.nhiggs:
+        blocks.append(f["doublet"])
+    if p.triplet:
+        blocks.append(f["triplet"])
+    path = tmp_path / "lattice"
+    with open(path, "wb") as fh:
+        fh.write(np.array([size, 3, *L], dtype=np.int32).tobytes())
+        fh.write(np.array([41], dtype=np.int64).tobytes())
+        fh.write(np.array([1.5, 0.25]).tobytes())
+        for b in blocks:
+            fh.write(np.ascontiguousarray(b[order]).tobytes())
+    g = abi.field_arrays(p)
+    it = C.c_int64()
+    assert lib.su2hip_lattice_read(str(path).encode(), C.byref(p), C.byref(it), None, None,
+                                   *[ptr(g[k]) for k in orc.FIELDS]) == 0
+    assert it.value == 41
+    for k in orc.FIELDS:
+        if g[k] is not None:
+            assert np.array_equal(g[k], f[k]), k
