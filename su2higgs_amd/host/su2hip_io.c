@@ -499,7 +499,9 @@ int su2hip_mpi_slices(const int32_t *L, int size, int32_t *nslices, int32_t *sli
  * (src/layout.c:730-756).  order[q] is the single-lattice site index
  * (EVEN first, natural order) of the q-th site of the file's field blocks. */
 int su2hip_mpi_site_order(const int32_t *L, int size, int64_t *order) {
-  if (!order) return ioerr(SU2HIP_EINVAL, "NULL argument");
+  if (!order || !L) return ioerr(SU2HIP_EINVAL, "NULL argument");
+  for (int d = 0; d < 3; d++)
+    if (L[d] < 2 || (L[d] & 1)) return ioerr(SU2HIP_EINVAL, "extent L%d = %d must be even", d + 1, L[d]);
   int32_t ns[3], sl[3];
   int rc = su2hip_mpi_slices(L, size, ns, sl);
   if (rc) return rc;

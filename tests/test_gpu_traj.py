@@ -175,3 +175,44 @@ def test_trajectories_match_reference(tmp_path, tag, kw):
     assert mg.do_acceptance == 1
     en = C.c_int()
     assert lib.su2hip_get_random_order(b.h, C.byref(en)) == 0 and en.value == 0
+
+
+def test_trajectories_leave_other_lattices_alone(tmp_path):
+    """Trajectories of one slot of a batch: the other lattice neither sweeps
+    nor changes, its budget comes back, and the slot ends as a run alone."""
+    from test_gpu_parity import muca_weight
+    os.chdir(tmp_path)
+    ps = [make_params(L=(4, 4, 6), nhiggs=1, seed=41), make_params(L=(4, 6, 6), nhiggs=1, seed=42)]
+    m = abi.Muca()
+    m.enabled, m.orderparam, m.mode, m.checks_per_sweep = 1, abi.PHISQ, abi.READONLY, 1
+    m.bins, m.min, m.max, m.wrk_min, m.wrk_max, m.delta = 40, 0.0, 4.0, 0.0, 4.0, 0.3
+    m.do_acceptance = 1
+    pos, W = muca_weight(hi=4.0)
+    t = TrajectoryConfig()
+    t.n_traj, t.interval, t.mode_interval = 2, 1, 1
+    lib = abi.load()
+    mix = Batch(ps)
+    for i in range(2):
+        mix.hot_start(i)
+        mix.muca_set(i, m, pos, W)
+    mix.update(3)
+    v = mix.muca_orderparam(0).sum()
+    t.min, t.max = v - 0.1, v + 0.1
+    before = mix.get_fields(1)
+    c1 = mix.counters(1)
+    mix.set_budget(1, 7)
+    assert lib.su2hip_trajectories(mix.h, 0, C.byref(t), 1, b"traj_mix") == 0
+    after = mix.get_fields(1)
+    for k in present(before):
+        assert np.array_equal(before[k], after[k]), k
+    assert mix.counters(1) == c1
+    assert mix.budgets()[1] == 7
+    one = Batch(ps[0])
+    one.hot_start(0)
+    one.muca_set(0, m, pos, W)
+    one.update(3)
+    assert lib.su2hip_trajectories(one.h, 0, C.byref(t), 1, b"traj_one") == 0
+    assert open("traj_mix").read() == open("traj_one").read()
+    a, g = one.get_fields(0), mix.get_fields(0)
+    for k in present(a):
+        assert np.array_equal(a[k], g[k]), k
