@@ -88,9 +88,10 @@ OBS_THERM = 1000   # sweeps from the cold start before the observables are sampl
 OBS_SAMPLES = 60
 
 
-def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100):
+def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100, graph=True):
     """Timed region: `steps` compound sweeps replayed as a hipGraph on the
-    library's stream (wall clock between stream synchronisations).  Kernel
+    library's stream (wall clock between stream synchronisations; with
+    graph=False launched kernel by kernel from the host).  Kernel
     region: `timing_steps` sweeps launched one by one with a HIP event pair
     around every launch on the same stream (per-kernel durations)."""
     from su2higgs_amd import abi
@@ -103,7 +104,7 @@ def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100):
     # workgroup chunk per pass kind picked by the autotuner (state restored)
     tune = b.autotune(repeats=50)
     picks = {name: b.chunk(abi.KIND_NAMES.index(name)) for name in tune}
-    b.use_graph(True)
+    b.use_graph(graph)
     b.update(warmup)
     b.sync()
     t0 = time.perf_counter()
@@ -137,7 +138,7 @@ def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100):
             res["kernel_avg_ms"][name], res["kernel_span_ms"][name] = b.time_pass(k, 200)
     if measure_obs:
         # both precisions: thermalise to the same sweep count, then sample
-        b.use_graph(True)
+        b.use_graph(graph)
         done = warmup + steps + ts + 1
         b.update(max(0, OBS_THERM - done))
         plaq, phi2, u1 = [], [], []
@@ -360,6 +361,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the serving stream (0 skips it)")
     ap.add_argument("--no-obs", action="store_true", help="skip the observable comparison")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the timed sweeps kernel by kernel instead of replaying a "
+                         "captured hipGraph (for tools that do not follow graph launches)")
     ap.add_argument("--probe-bytes", type=int, default=0,
                     help="only run the streaming copy kernel over this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -377,11 +381,12 @@ def main():
         return
     obs = not args.no_obs
     log("[bench] FP32-mixed 96^3, %d warmup + %d timed sweeps" % (args.warmup, args.steps))
-    r32 = run_precision(abi.FP32, args.steps, args.warmup, measure_obs=obs)
+    r32 = run_precision(abi.FP32, args.steps, args.warmup, measure_obs=obs, graph=not args.no_graph)
     log("[bench] fp32 %.3f ms/sweep, %.3e site-updates/s" % (r32["ms_per_sweep"], r32["site_updates_per_s"]))
     s64 = max(1, min(args.steps, 200))
     log("[bench] FP64 plain 96^3, %d timed sweeps" % s64)
-    r64 = run_precision(abi.FP64, s64, max(1, args.warmup // 2), measure_obs=obs)
+    r64 = run_precision(abi.FP64, s64, max(1, args.warmup // 2), measure_obs=obs,
+                        graph=not args.no_graph)
     log("[bench] fp64 %.3f ms/sweep, %.3e site-updates/s" % (r64["ms_per_sweep"], r64["site_updates_per_s"]))
     roof = roofline(r32)
     roof64 = roofline(r64)
@@ -426,6 +431,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: cold start phi=(1,0,0,0), Philox4x32-10 draws",
+        "timed_launch": "hipGraph replay" if not args.no_graph else "kernel by kernel",
         "config": {"workload": "96^3 SU(2)xU(1)+fundamental Higgs, betaG=9 betaY=27 msq=-0.20 "
                                "lambda=0.015, sweep = 1 link heatbath + 4 link overrelax + U(1) "
                                "Metropolis + 4 Higgs overrelax + 1 Higgs Metropolis",
