@@ -215,29 +215,43 @@ __device__ __forceinline__ Site shift(const LatDesc &L, const Site &c, int d, in
 // 4-component site vectors (one 16-byte load per site in FP32), the others
 // component-major
 
+// Field arrays live in device global memory.  Their pointers reach the
+// kernels through LatDesc (generic pointers), and a generic-to-global cast is
+// folded away as a no-op, which leaves FLAT loads that count against both the
+// vector-memory and the LDS/scalar wait counters.  Going through the integer
+// value makes the loads and stores GLOBAL ones (SGPR base + VGPR offset).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef double vd2 __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ gptr<T> gmem(T *p) {
+  return reinterpret_cast<gptr<T>>(reinterpret_cast<unsigned long>(p));
+}
+
 template <typename T>
 __device__ __forceinline__ Q<T> ld4(const T *p);
 template <>
 __device__ __forceinline__ Q<float> ld4<float>(const float *p) {
-  float4 v = *reinterpret_cast<const float4 *>(p);
+  vf4 v = *gmem(reinterpret_cast<const vf4 *>(p));
   return Q<float>{v.x, v.y, v.z, v.w};
 }
 template <>
 __device__ __forceinline__ Q<double> ld4<double>(const double *p) {
-  double2 a = *reinterpret_cast<const double2 *>(p);
-  double2 b = *reinterpret_cast<const double2 *>(p + 2);
+  vd2 a = *gmem(reinterpret_cast<const vd2 *>(p));
+  vd2 b = *gmem(reinterpret_cast<const vd2 *>(p + 2));
   return Q<double>{a.x, a.y, b.x, b.y};
 }
 template <typename T>
 __device__ __forceinline__ void st4(T *p, const Q<T> &q);
 template <>
 __device__ __forceinline__ void st4<float>(float *p, const Q<float> &q) {
-  *reinterpret_cast<float4 *>(p) = make_float4(q.a, q.b, q.c, q.d);
+  *gmem(reinterpret_cast<vf4 *>(p)) = vf4{q.a, q.b, q.c, q.d};
 }
 template <>
 __device__ __forceinline__ void st4<double>(double *p, const Q<double> &q) {
-  *reinterpret_cast<double2 *>(p) = make_double2(q.a, q.b);
-  *reinterpret_cast<double2 *>(p + 2) = make_double2(q.c, q.d);
+  *gmem(reinterpret_cast<vd2 *>(p)) = vd2{q.a, q.b};
+  *gmem(reinterpret_cast<vd2 *>(p + 2)) = vd2{q.c, q.d};
 }
 
 template <typename T>
@@ -258,13 +272,13 @@ __device__ __forceinline__ void stF(T *__restrict__ F, int Vh, int par, int s, c
 }
 template <typename T>
 __device__ __forceinline__ V3<T> ldG(const T *__restrict__ G, int Vh, int par, int s) {
-  const T *b = G + (size_t)par * Vh + s;
+  gptr<const T> b = gmem(G + (size_t)par * Vh + s);
   size_t st = (size_t)2 * Vh;
   return V3<T>{b[0], b[st], b[2 * st]};
 }
 template <typename T>
 __device__ __forceinline__ void stG(T *__restrict__ G, int Vh, int par, int s, const V3<T> &q) {
-  T *b = G + (size_t)par * Vh + s;
+  gptr<T> b = gmem(G + (size_t)par * Vh + s);
   size_t st = (size_t)2 * Vh;
   b[0] = q.x;
   b[st] = q.y;
@@ -272,11 +286,11 @@ __device__ __forceinline__ void stG(T *__restrict__ G, int Vh, int par, int s, c
 }
 template <typename T>
 __device__ __forceinline__ T ldA(const T *__restrict__ A, int Vh, int d, int par, int s) {
-  return A[(size_t)(d * 2 + par) * Vh + s];
+  return *gmem(A + (size_t)(d * 2 + par) * Vh + s);
 }
 template <typename T>
 __device__ __forceinline__ T ldS(const T *__restrict__ S, int Vh, int par, int s) {
-  return S[(size_t)par * Vh + s];
+  return *gmem(S + (size_t)par * Vh + s);
 }
 
 // load by site

@@ -529,7 +529,7 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       }
     }
     if (!metro_accept(dS, r)) return 0;
-    A[(size_t)(mu * 2 + par) * Vh + s] = na;
+    *gmem(A + (size_t)(mu * 2 + par) * Vh + s) = na;
     return 1;
   } else if constexpr ((KIND == SU2HIP_K_DOUBLET_OVERRELAX ||
                         KIND == SU2HIP_K_DOUBLET_METROPOLIS) && (M & M_H2) != 0) {
@@ -717,7 +717,7 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       double diff = 3.0 * (nS * nS - Sx * Sx) - (nS - Sx) * nb + (pot(nS) - pot(Sx));
       if (!metro_accept((T)diff, r)) return 0;
     }
-    S[(size_t)par * Vh + s] = (T)nS;
+    *gmem(S + (size_t)par * Vh + s) = (T)nS;
     return 1;
   }
   return 0;
