@@ -317,6 +317,131 @@ __device__ __forceinline__ void higgs2_or(const LatDesc &L, int h, const Q<doubl
 }
 
 // ------------------------------------------------------------------------
+// A rejection loop shared by the lanes of a wavefront (the direction of the
+// FP64 link heatbath, accepted with probability pi/6 per try).
+//
+// A lane's rejection loop runs until one of its tries accepts, and a
+// wavefront runs until its slowest lane is done (about 7 tries).  Here, in
+// each round, the 64 lanes make tries for the lanes that are still pending:
+// pending lane of rank q (among the pending ones, P of them) gets the
+// contiguous run of workers w with (w P) >> 6 == q, worker w making its try
+// number tries_q + (w - lo(q)), lo(q) = ceil(64 q / P).  A pending lane
+// takes the first accepted try of its run.  Draws are addressed by (pass,
+// site, index), so a worker can make another site's try; every lane ends
+// with the draws, the accepted try and the stream position of its own
+// sequential loop, in about 3 rounds.  Callers use this only when all 64
+// lanes are active.  (The Kennedy-Pendleton loop before it accepts most
+// tries at once; sharing its rounds cost more than the waiting it saves.)
+
+// draws n .. n+2: always two Philox blocks (words 0-2 of n>>1 and n>>1 + 1,
+// or words 1-3 for odd n)
+__device__ __forceinline__ void rng_u53_run3(const Rng &r, uint32_t site, uint32_t n, double u[3]) {
+  uint32_t a0 = n >> 1, a1 = site, a2 = r.plo, a3 = r.phi;
+  philox10(a0, a1, a2, a3, r.k0, r.k1);
+  uint32_t b0 = (n >> 1) + 1, b1 = site, b2 = r.plo, b3 = r.phi;
+  philox10(b0, b1, b2, b3, r.k0, r.k1);
+  uint64_t w0 = ((uint64_t)a1 << 32) | a0, w1 = ((uint64_t)a3 << 32) | a2;
+  uint64_t w2 = ((uint64_t)b1 << 32) | b0, w3 = ((uint64_t)b3 << 32) | b2;
+  bool odd = n & 1;
+  u[0] = (double)((odd ? w1 : w0) >> 11) * 0x1.0p-53;
+  u[1] = (double)((odd ? w2 : w1) >> 11) * 0x1.0p-53;
+  u[2] = (double)((odd ? w3 : w2) >> 11) * 0x1.0p-53;
+}
+
+// lane of the q-th (from 0) set bit of m; q < popcount(m)
+__device__ __forceinline__ int kth_set_bit(uint64_t m, int q) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    int c = __popcll(m & ((1ull << w) - 1));
+    if (q >= c) {
+      q -= c;
+      m >>= w;
+      pos += w;
+    }
+  }
+  return pos;
+}
+
+// this lane's place in a round: its target lane and try offset as a worker,
+// its run [lo, hi) of workers as a pending target
+struct CoopRound {
+  int target, offset;
+  uint64_t run;
+};
+__device__ __forceinline__ CoopRound coop_round(uint64_t pending, int lane) {
+  const int P = __popcll(pending);
+  CoopRound c;
+  int q = (lane * P) >> 6;
+  c.target = kth_set_bit(pending, q);
+  c.offset = lane - (q * 64 + P - 1) / P;
+  int mq = __popcll(pending & ((1ull << lane) - 1));
+  int lo = (mq * 64 + P - 1) / P, hi = ((mq + 1) * 64 + P - 1) / P;
+  uint64_t below_hi = hi >= 64 ? ~0ull : (1ull << hi) - 1;
+  c.run = below_hi & ~((1ull << lo) - 1);
+  return c;
+}
+
+// The direction's cube rejection of heatbath_su2link (src/heatbath.c:82-93)
+// for the lanes with `want`, starting at draw r.k: the accepted try's
+// r1, r2, r3, d and the number of tries.
+__device__ void hb_dir_wave(const Rng &r, bool want, double &r1o, double &r2o, double &r3o,
+                            double &dout, int &tries_out) {
+  const int lane = __lane_id();
+  bool pending = want;
+  int tries = 0;
+  for (;;) {
+    uint64_t pm = __ballot(pending);
+    if (pm == 0) break;
+    const bool all = pm == ~0ull;  // every lane on its own try
+    CoopRound c;
+    int t;
+    uint32_t tsite, tn;
+    if (all) {
+      c = CoopRound{lane, 0, 1ull << lane};
+      t = tries;
+      tsite = r.site;
+      tn = r.k;
+    } else {
+      c = coop_round(pm, lane);
+      t = __shfl(tries, c.target, 64) + c.offset;
+      tsite = __shfl(r.site, c.target, 64);
+      tn = __shfl(r.k, c.target, 64);
+    }
+    double u[3];
+    rng_u53_run3(r, tsite, tn + 3u * (uint32_t)t, u);
+    double r1 = 2.0 * u[0] - 1.0;
+    double r2 = 2.0 * u[1] - 1.0;
+    double r3 = 2.0 * u[2] - 1.0;
+    double d = r1 * r1 + r2 * r2 + r3 * r3;
+    bool acc = !(d > 1.0 || d == 0.0);
+    uint64_t mine = __ballot(acc) & c.run;
+    int win = mine ? __ffsll((unsigned long long)mine) - 1 : lane;
+    double w1 = r1, w2 = r2, w3 = r3, wd = d;
+    if (!all) {
+      w1 = __shfl(r1, win, 64);
+      w2 = __shfl(r2, win, 64);
+      w3 = __shfl(r3, win, 64);
+      wd = __shfl(d, win, 64);
+    }
+    if (pending) {
+      int lo = __ffsll((unsigned long long)c.run) - 1;
+      if (mine) {
+        r1o = w1;
+        r2o = w2;
+        r3o = w3;
+        dout = wd;
+        tries += win - lo + 1;
+        pending = false;
+      } else {
+        tries += __popcll(c.run);
+      }
+    }
+  }
+  tries_out = tries;
+}
+
+// ------------------------------------------------------------------------
 // per-site updates; each returns 1 if accepted and stores the new value
 
 struct Fields {
@@ -347,6 +472,9 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     Q<T> V = link_staple<T, M>(L, U, A, F, x, mu);
     T r1, r2, r3, d, a0;
     int loop = 1;
+    // a full wavefront shares the FP64 direction's rejection tries
+    // (hb_dir_wave); the results are those of the per-lane loop
+    const bool wave = sizeof(T) == 8 && __ballot(1) == ~0ull;
     if constexpr (sizeof(T) == 8) {
       T a = ::sqrt(qdot(V, V));
       V = qscale(V, (T)-1 / a);
@@ -377,22 +505,32 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     }
     a0 = (loop > 200) ? (T)1e-9 : (T)1 - d;
     T rad = (T)1 - a0 * a0;
-    if (rad < (T)0) return 0;
     Q<T> nu;
     if constexpr (sizeof(T) == 8) {
       // isotropic direction by cube rejection, the reference's draw order
       // (src/heatbath.c:82-93)
-      d = (T)2;
-      while (d > (T)1 || d == (T)0) {
-        r1 = (T)2 * uni<T>(r) - (T)1;
-        r2 = (T)2 * uni<T>(r) - (T)1;
-        r3 = (T)2 * uni<T>(r) - (T)1;
-        d = r1 * r1 + r2 * r2 + r3 * r3;
+      if (wave) {
+        // every lane stays in the wavefront's rounds, as a worker at least
+        int tries;
+        hb_dir_wave(r, !(rad < (T)0), r1, r2, r3, d, tries);
+        if (rad < (T)0) return 0;
+        r.k += 3u * (uint32_t)tries;
+        r.blk = 0xFFFFFFFFu;
+      } else {
+        if (rad < (T)0) return 0;
+        d = (T)2;
+        while (d > (T)1 || d == (T)0) {
+          r1 = (T)2 * uni<T>(r) - (T)1;
+          r2 = (T)2 * uni<T>(r) - (T)1;
+          r3 = (T)2 * uni<T>(r) - (T)1;
+          d = r1 * r1 + r2 * r2 + r3 * r3;
+        }
       }
       rad = ::sqrt(rad);
       d = ::sqrt(d);
       nu = Q<T>{a0, r1 * (rad / d), r2 * (rad / d), r3 * (rad / d)};
     } else {
+      if (rad < (T)0) return 0;
       // FP32: the same uniform direction on S^2 without rejection
       // (cos(theta) uniform in [-1,1], phi uniform), so a wavefront does
       // not wait for its slowest lane

@@ -103,6 +103,25 @@ def test_passes_site_by_site(tag):
     assert o.rng_get()[1] == b.rng_get(0)[1]
 
 
+@pytest.mark.parametrize("L", [(4, 6, 8), (8, 8, 8), (16, 8, 12)])
+@pytest.mark.parametrize("beta", [1.0, 8.0])
+def test_link_heatbath_wavefront_rounds(L, beta):
+    """FP64 link heatbath: full wavefronts share the direction's rejection
+    tries across lanes, partial ones (the 32-site tail at 4x6x8) loop per
+    lane; both give the oracle's sites, accepts and stream positions."""
+    p, o, b = pair("hu", L=L, betasu2=beta)
+    for sweep in range(2):
+        for d in range(3):
+            for par in (0, 1):
+                o.pass_(K_LINK_HEATBATH, par, d)
+                b.pass_(K_LINK_HEATBATH, par, d)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-12
+    co, cg = o.counters(), b.counters(0)
+    for k in co:
+        assert co[k] == cg[k], k
+    assert o.rng_get()[1] == b.rng_get(0)[1]
+
+
 @pytest.mark.parametrize("tag", list(MODELS))
 def test_compound_sweeps(tag):
     p, o, b = pair(tag, link_overrelax=2)
