@@ -55,17 +55,26 @@ static int cfg_open(const char *path, cfgfile *f) {
   int cap = 64;
   f->lines = (char **)malloc(sizeof(char *) * cap);
   f->last_nl = 1;
+  if (!f->lines) goto oom;
   while (fgets(buf, MAXLINE, fp)) {
     if (f->n == cap) {
+      char **grown = (char **)realloc(f->lines, sizeof(char *) * cap * 2);
+      if (!grown) goto oom;
+      f->lines = grown;
       cap *= 2;
-      f->lines = (char **)realloc(f->lines, sizeof(char *) * cap);
     }
-    f->lines[f->n++] = strdup(buf);
+    char *line = strdup(buf);
+    if (!line) goto oom;
+    f->lines[f->n++] = line;
     size_t l = strlen(buf);
     f->last_nl = l > 0 && buf[l - 1] == '\n';
   }
   fclose(fp);
   return SU2HIP_OK;
+oom:
+  fclose(fp);
+  cfg_free(f);
+  return ioerr(SU2HIP_ENOMEM, "host allocation reading %s", path);
 }
 
 /* One lookup: search on from the current position to the end, then once

@@ -47,7 +47,8 @@ static void alloc_fields(const su2hip_params *p, hostfields *h) {
   h->db = p->nhiggs ? calloc(V * 4 * p->nhiggs, sizeof(double)) : NULL;
   h->tr = p->triplet ? calloc(V * 3, sizeof(double)) : NULL;
   h->sg = p->singlet ? calloc(V, sizeof(double)) : NULL;
-  if (!h->link) {
+  if (!h->link || (p->u1 && !h->u1) || (p->nhiggs && !h->db) || (p->triplet && !h->tr) ||
+      (p->singlet && !h->sg)) {
     fprintf(stderr, "su2hip: host allocation failed\n");
     exit(1);
   }
