@@ -118,3 +118,46 @@ def test_stream_is_the_stated_distribution():
     # deterministic in the seed
     r2 = serve.stream(7, 20.0, 50, flow_every=10)
     assert [(a.arrival, a.seed) for a in r2] == [(a.arrival, a.seed) for a in list(r)[:50]]
+
+
+@pytest.mark.parametrize("kw,code,msg", [
+    (dict(L=(7, 8, 8)), abi.EINVAL, b"even"),
+    (dict(L=(0, 8, 8)), abi.EINVAL, b"even"),
+    (dict(L=(1024, 1024, 512)), abi.ERANGE, b"too large"),       # 2^29 sites
+    (dict(nhiggs=3), abi.EUNSUPPORTED, b"nhiggs"),
+    (dict(nhiggs=2, triplet=1), abi.EUNSUPPORTED, b"not supported"),
+    (dict(nhiggs=0, u1=1), abi.EUNSUPPORTED, b"not supported"),
+    (dict(precision=16), abi.EINVAL, b"precision"),
+    (dict(algorithm_su2link=abi.OVERRELAX), abi.EINVAL, b"algorithm_su2link"),
+    (dict(scalar_sweeps=-1), abi.EINVAL, b"negative"),
+    (dict(betasu2=0.0), abi.EINVAL, b"betasu2"),
+])
+def test_invalid_parameters_rejected_before_any_device_call(kw, code, msg):
+    """su2hip_create and su2hip_footprint check the parameters before they
+    touch a device: the error code and message, and no handle."""
+    lib = abi.load()
+    p = abi.make_params(**kw)
+    h = C.c_void_p()
+    assert lib.su2hip_create(1, C.byref(p), 1, 0, C.byref(h)) == code
+    assert msg in lib.su2hip_last_error()
+    assert not h.value
+    nb = C.c_uint64()
+    assert lib.su2hip_footprint(C.byref(p), 0, C.byref(nb)) == code
+
+
+def test_batch_rules_and_largest_volume_without_a_device():
+    lib = abi.load()
+    h = C.c_void_p()
+    # lattices of one batch share field content, precision and schedule
+    ps = (abi.Params * 2)(abi.make_params(), abi.make_params(precision=abi.FP32))
+    assert lib.su2hip_create(2, ps, 2, 0, C.byref(h)) == abi.EINVAL
+    assert b"differs" in lib.su2hip_last_error()
+    # an empty handle needs room for a lattice
+    assert lib.su2hip_create(0, ps, 0, 0, C.byref(h)) == abi.EINVAL
+    # the largest accepted volume (640^3 <= 2^28 sites): its FP64 fields
+    # (12 link + 4 doublet + 3 U(1) words per site) fit one GPU's 288 GB
+    p = abi.make_params(L=(640, 640, 640), u1=1)
+    nb = C.c_uint64()
+    assert lib.su2hip_footprint(C.byref(p), 1, C.byref(nb)) == abi.OK
+    V = 640 ** 3
+    assert 19 * 8 * V <= nb.value < 288e9
