@@ -6,7 +6,9 @@ is compared with it here directly (FP64, same Philox draws): every pass kind
 in every direction and parity, and one whole compound sweep.  Beside that,
 properties that hold at any size: link overrelaxation keeps the total action
 and undoes itself, graph replay and eager launches give the same bits, and
-the links stay on SU(2).
+the links stay on SU(2).  The batched scan's 96 lattices (16^3..48^3) at
+their full shape: each sampled lattice bitwise the same alone and in the
+mix (FP32), and its smallest and largest lattices against the oracle (FP64).
 
 Tolerances as in test_gpu_parity: fields to rounding (1e-11 absolute on O(1)
 values after single passes, 1e-10 after a sweep), sums 1e-12 relative,
@@ -124,3 +126,64 @@ def test_graph_replay_equals_eager_and_links_stay_unitary(precision):
     assert np.abs(norm - 1).max() < (1e-12 if precision == FP64 else 1e-5)
     assert np.isfinite(res[0]["doublet"]).all()
     assert np.abs(res[0]["u1link"]).max() <= np.pi * (1 + 1e-6) or precision == FP64
+
+
+def scan_params(precision):
+    """The batched scan's 96 lattices: volumes drawn from {16^3, 24^3, 32^3,
+    48^3}, beta 8, lambda 0.02, m^2 spaced across [-0.30, -0.15], a Philox key
+    per lattice."""
+    rng = np.random.default_rng(96)
+    Ls = rng.choice([16, 24, 32, 48], size=96)
+    return [make_params(L=(int(L),) * 3, nhiggs=1, precision=precision, betasu2=8.0,
+                        lambda_phi=0.02, msq_phi=-0.30 + 0.15 * k / 95, link_overrelax=4,
+                        scalar_sweeps=4, seed=1000 + k) for k, L in enumerate(Ls)]
+
+
+def test_batched_scan_full_shape_same_alone_and_in_mix():
+    """FP32, the whole 96-lattice batch in one launch per pass: a lattice's
+    chain is bitwise the one it follows alone."""
+    ps = scan_params(FP32)
+    b = Batch(ps)
+    for i in range(len(ps)):
+        b.hot_start(i)
+    b.use_graph(True)
+    b.update(3)
+    vols = [abi.volume(p) for p in ps]
+    picks = sorted({0, 95, int(np.argmax(vols)), int(np.argmin(vols))})
+    mixed = {k: b.get_fields(k) for k in picks}
+    b.close()
+    for k in picks:
+        a = Batch(ps[k])
+        a.hot_start(0)
+        a.update(3)
+        f = a.get_fields(0)
+        a.close()
+        for name in ("su2link", "doublet"):
+            assert np.array_equal(f[name], mixed[k][name]), (k, name)
+
+
+def test_batched_scan_full_shape_fp64_matches_oracle():
+    """FP64, the whole batch swept twice; its smallest and largest lattices
+    against the oracle run alone from the same fields."""
+    ps = scan_params(FP64)
+    vols = [abi.volume(p) for p in ps]
+    picks = [int(np.argmin(vols)), int(np.argmax(vols))]
+    b = Batch(ps)
+    oracles = {}
+    for i in range(len(ps)):
+        b.hot_start(i)
+    for k in picks:
+        o = orc.Oracle(ps[k])
+        o.hot_start()
+        b.set_fields(k, o.get_fields())
+        oracles[k] = o
+    b.update(2)
+    for k in picks:
+        o = oracles[k]
+        o.update(2)
+        fo, fg = o.get_fields(), b.get_fields(k)
+        assert max(np.abs(fo[n] - fg[n]).max() for n in ("su2link", "doublet")) < 1e-10, k
+        co, cg = o.counters(), b.counters(k)
+        for c in co:
+            assert co[c] == cg[c], (k, c)
+    b.close()
