@@ -182,6 +182,21 @@ def roofline(res):
             "words_per_site_per_pass": words}
 
 
+def kernel_rooflines(res):
+    """Every pass kind of the sweep against the HBM peak: algorithmic bytes
+    per launch over the kind's average launch duration."""
+    from su2higgs_amd import abi
+    p = res["p"]
+    w = 8 if p.precision == abi.FP64 else 4
+    out = {}
+    for kind, avg_ms in res["kernel_avg_ms"].items():
+        byts = algo_words(abi.KIND_NAMES.index(kind), p) * res["V"] * w
+        ach = byts / (avg_ms * 1e-3) / 1e9
+        out[kind] = {"avg_launch_us": avg_ms * 1e3, "algorithmic_MB": byts / 1e6,
+                     "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
+    return out
+
+
 def roofline_beyond_cache(L=192):
     """The dominant kernel (FP32 link overrelaxation) on an L^3 lattice whose
     fields (16 L^3 words, 537 MB at 192^3) exceed the 256 MB Infinity Cache
@@ -452,6 +467,8 @@ def main():
         "roofline": roof,
         "roofline_fp64": roof64,
         "roofline_beyond_cache": roof_big,
+        "kernel_roofline_fp32": kernel_rooflines(r32),
+        "kernel_roofline_fp64": kernel_rooflines(r64),
         "kernel_avg_ms_fp32": r32["kernel_avg_ms"],
         "kernel_avg_ms_fp64": r64["kernel_avg_ms"],
         "chunk_fp32": r32["chunk"],
