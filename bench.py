@@ -383,8 +383,9 @@ def main():
                     help="only run the streaming copy kernel over this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(lattices would be sharded across GPUs with no communication; see DESIGN.md)",
+        print("bench.py measures one GPU: there is no multi-GPU path (a batch or scan "
+              "would shard its lattices across GPUs with no communication; one large "
+              "lattice would split into z slabs with a halo plane per pass)",
               file=sys.stderr)
         sys.exit(2)
     from su2higgs_amd import abi

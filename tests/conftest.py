@@ -25,6 +25,6 @@ def gpu_present():
 
 @pytest.fixture(scope="session")
 def su2lib():
-    """The built HIP library; on a GPU box its absence is an error."""
+    """The built HIP library; on a machine with a GPU its absence is an error."""
     from su2higgs_amd import abi
     return abi.load()

@@ -1,7 +1,7 @@
 """HIP path against the oracle on the same inputs and Philox draws (FP64).
 
-Tolerances: deterministic quantities 1e-12 relative (the north-star bar for
-staples, action density, overrelaxation and flow steps); after stochastic
+Tolerances: deterministic quantities 1e-12 relative (the bar for staples,
+action density, overrelaxation and flow steps); after stochastic
 passes the fields agree to rounding (1e-11 absolute on O(1) values) because
 every accept/reject decision is the same; counters are exact."""
 import numpy as np
