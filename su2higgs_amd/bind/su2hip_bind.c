@@ -1,0 +1,109 @@
+/* su2hip_bind.c -- drive su2higgs's sweep on an MI355X through libsu2hip. */
+#include <stdlib.h>
+#include <string.h>
+
+#include "su2.h"
+#include "su2hip.h"
+
+static su2hip_batch_t *H;
+
+static su2hip_params to_su2hip(lattice const *l, params const *p) {
+  su2hip_params q = {0};
+  for (int d = 0; d < 3; d++) q.L[d] = (int)l->L[d];
+  q.nhiggs = NHIGGS;
+#ifdef U1
+  q.u1 = 1; q.betau1 = p->betau1; q.r_u1 = p->r_u1;
+#endif
+#ifdef TRIPLET
+  q.triplet = 1; q.msq_triplet = p->msq_triplet; q.b4 = p->b4; q.a2 = p->a2;
+  q.algorithm_su2triplet = p->algorithm_su2triplet;
+  q.update_su2triplet = p->update_su2triplet;
+#endif
+#ifdef SINGLET
+  q.singlet = 1; q.update_singlet = p->update_singlet;
+  q.algorithm_singlet = p->algorithm_singlet;
+  q.b1_s = p->b1_s; q.msq_s = p->msq_s; q.b3_s = p->b3_s; q.b4_s = p->b4_s;
+  q.a1_s = p->a1_s; q.a2_s = p->a2_s;
+#endif
+  q.precision = SU2HIP_FP64;          /* or SU2HIP_FP32 */
+  q.algorithm_su2link = p->algorithm_su2link;
+  q.algorithm_u1link = SU2HIP_METROPOLIS;
+  q.algorithm_su2doublet = p->algorithm_su2doublet;
+  q.update_links = p->update_links;
+  q.scalar_sweeps = p->scalar_sweeps;
+  q.update_su2doublet = p->update_su2doublet;
+  q.betasu2 = p->betasu2;
+  q.msq_phi = p->msq_phi;
+  q.lambda_phi = p->lambda_phi;
+#if (NHIGGS == 2)
+  q.msq_phi2 = p->msq_phi2;
+  q.m12sq_re = p->m12sq.re; q.m12sq_im = p->m12sq.im;
+  q.lam2 = p->lam2; q.lam3 = p->lam3; q.lam4 = p->lam4;
+  q.lam5_re = p->lam5.re; q.lam5_im = p->lam5.im;
+  q.lam6_re = p->lam6.re; q.lam6_im = p->lam6.im;
+  q.lam7_re = p->lam7.re; q.lam7_im = p->lam7.im;
+#endif
+  q.seed = 1234;                      /* Philox key */
+  return q;
+}
+
+/* the field blocks of `f` in su2hip_set_fields/get_fields order; NULL for
+ * a field this build does not have */
+#ifdef U1
+#define U1_OF(f) ((f)->u1link[0])
+#else
+#define U1_OF(f) NULL
+#endif
+#if (NHIGGS == 2)
+/* the two doublets are two allocations (src/alloc.c:109); su2hip takes them
+ * as one array, the first doublet first: staged through Dstage */
+static double *Dstage;
+static long Vsites;
+#define DOUBLET_OF(f) Dstage
+#elif (NHIGGS == 1)
+#define DOUBLET_OF(f) ((f)->su2doublet[0][0])
+#else
+#define DOUBLET_OF(f) NULL
+#endif
+#ifdef TRIPLET
+#define TRIPLET_OF(f) ((f)->su2triplet[0])
+#else
+#define TRIPLET_OF(f) NULL
+#endif
+#ifdef SINGLET
+#define SINGLET_OF(f) ((f)->singlet[0])
+#else
+#define SINGLET_OF(f) NULL
+#endif
+
+/* once, after alloc_fields/setfields or load_lattice (src/main.c:160-200) */
+int su2hip_bind_init(lattice const *l, fields const *f, params const *p) {
+  su2hip_params q = to_su2hip(l, p);
+  int rc = su2hip_create(1, &q, 1, 0, &H);
+  if (rc) return rc;
+  su2hip_use_graph(H, 1);
+#if (NHIGGS == 2)
+  Vsites = l->sites;
+  Dstage = malloc(sizeof(double) * 2 * SU2DB * Vsites);
+  if (!Dstage) return SU2HIP_ENOMEM;
+  for (int db = 0; db < 2; db++)
+    memcpy(Dstage + db * SU2DB * Vsites, f->su2doublet[db][0], sizeof(double) * SU2DB * Vsites);
+#endif
+  return su2hip_set_fields(H, 0, f->su2link[0][0], U1_OF(f), DOUBLET_OF(f), TRIPLET_OF(f),
+                           SINGLET_OF(f));
+}
+
+/* replaces update_lattice(&l, f, &p, &c, &w) (src/su2.h:426, src/main.c:353) */
+int su2hip_bind_update(void) { return su2hip_update(H, 1); }
+
+/* before measure() or save_lattice(): bring the fields back into `f`
+ * (or call su2hip_measure + su2hip_measure_columns and print them instead). */
+int su2hip_bind_download(fields *f) {
+  int rc = su2hip_get_fields(H, 0, f->su2link[0][0], U1_OF(f), DOUBLET_OF(f), TRIPLET_OF(f),
+                             SINGLET_OF(f));
+#if (NHIGGS == 2)
+  for (int db = 0; rc == 0 && db < 2; db++)
+    memcpy(f->su2doublet[db][0], Dstage + db * SU2DB * Vsites, sizeof(double) * SU2DB * Vsites);
+#endif
+  return rc;
+}

@@ -1,0 +1,74 @@
+"""The reference's own driver with its sweep on the GPU.
+
+oracle/_ref/su2_<tag>_gpu is the reference's unmodified main loop
+(src/main.c) linked with the reference-side binding
+su2higgs_amd/bind/su2hip_bind.c: update_lattice is routed to libsu2hip
+(oracle/bind_shim.c).  On the same config it must write the `measure` file
+the su2hip driver writes: both run the same GPU sweeps (Philox seed 1234,
+pass counter from 0); the reference measures on the CPU with its own
+measure(), su2hip on the GPU.  The final `lattice` files hold bitwise the
+same fields."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import orc
+from su2higgs_amd import abi
+from su2higgs_amd.abi import make_params, ptr
+from test_gpu_cli import CONFIG, TWO_HIGGS, run_cli, same_lines
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BINDING_SEED = 1234  # the Philox key su2hip_bind.c gives the lattice
+
+
+def run_ref_gpu(tag, d, cfg):
+    exe = os.path.join(ROOT, "oracle", "_ref", "su2_%s_gpu" % tag)
+    if not os.path.exists(exe):
+        pytest.skip("reference build with the binding absent")
+    with open(os.path.join(d, "config"), "w") as f:
+        f.write(cfg)
+    r = subprocess.run([exe, "config"], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:]
+    return r.stdout
+
+
+def lattice_fields(path, p):
+    lib = abi.load()
+    g = abi.field_arrays(p)
+    it = C.c_int64()
+    assert lib.su2hip_lattice_read(path.encode(), C.byref(p), C.byref(it), None, None,
+                                   *[ptr(g[k]) for k in orc.FIELDS]) == 0
+    return it.value, g
+
+
+@pytest.mark.parametrize("tag,model,extra,p_kw", [
+    ("hu", "higgs+u1", {}, dict(nhiggs=1, u1=1)),
+    ("d", "2hdm", TWO_HIGGS, dict(nhiggs=2, **TWO_HIGGS)),
+])
+def test_reference_main_with_gpu_sweep_matches_su2hip(tag, model, extra, p_kw, tmp_path):
+    cfg = CONFIG.format(model=model, it=8, do_flow=0).replace(
+        "seed 4242", "seed %d" % BINDING_SEED)
+    cfg += "".join("%s %r\n" % kv for kv in extra.items())
+    dr, dc = tmp_path / "ref", tmp_path / "su2hip"
+    dr.mkdir()
+    dc.mkdir()
+    run_ref_gpu(tag, str(dr), cfg)
+    run_cli(str(dc), cfg)
+    ref = open(dr / "measure").read().splitlines(keepends=True)
+    ours = open(dc / "measure").read().splitlines(keepends=True)
+    assert len(ref) == 8
+    same_lines(ours, ref)
+    p = make_params(L=(4, 6, 8), betasu2=8.0, betau1=9.0, msq_phi=-0.25, lambda_phi=0.02,
+                    scalar_sweeps=3, seed=BINDING_SEED, **p_kw)
+    itr, fr = lattice_fields(str(dr / "lattice"), p)
+    itc, fc = lattice_fields(str(dc / "lattice"), p)
+    assert itr == itc
+    for k in orc.FIELDS:
+        if fr.get(k) is not None:
+            assert np.array_equal(fr[k], fc[k]), k

@@ -1,8 +1,9 @@
-"""The reference-side binding shown in INTEGRATION.md (su2hip_bind.c) compiles
-against the reference's own headers and include/su2hip.h for the builds it
-covers: one doublet with U(1), a singlet or a triplet, two doublets, and the
-adjoint-only build.  Compile only (gcc -c -Wall -Werror); skipped where the
-reference sources are absent."""
+"""The reference-side binding su2higgs_amd/bind/su2hip_bind.c, which
+INTEGRATION.md shows in full, compiles against the reference's own headers and
+include/su2hip.h for the builds it covers: one doublet with U(1), a singlet or
+a triplet, two doublets, and the adjoint-only build.  Compile only (gcc -c
+-Wall -Werror); skipped where the reference sources are absent.  Running it
+inside the reference's main loop is tests/test_gpu_binding.py."""
 import os
 import re
 import shutil
@@ -16,21 +17,21 @@ BUILDS = ["-DNHIGGS=1 -DU1", "-DNHIGGS=2", "-DNHIGGS=2 -DU1", "-DNHIGGS=1 -DTRIP
           "-DNHIGGS=1 -DSINGLET -DU1", "-DNHIGGS=0 -DTRIPLET"]
 
 
-def stub_source():
+BINDING = os.path.join(ROOT, "su2higgs_amd", "bind", "su2hip_bind.c")
+
+
+def test_integration_shows_the_binding_verbatim():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     blocks = [b for b in re.findall(r"```c\n(.*?)```", text, re.S) if "su2hip_bind_init" in b]
-    assert len(blocks) == 1
-    return blocks[0]
+    assert blocks == [open(BINDING).read()]
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF_SRC, "su2.h")) or not shutil.which("gcc"),
                     reason="reference sources or gcc absent")
 @pytest.mark.parametrize("flags", BUILDS)
 def test_binding_stub_compiles_against_reference_headers(flags, tmp_path):
-    src = tmp_path / "su2hip_bind.c"
-    src.write_text(stub_source())
     cmd = (["gcc", "-std=gnu11", "-Wall", "-Werror", "-c"] + flags.split() +
-           ["-I" + REF_SRC, "-I" + os.path.join(ROOT, "include"), str(src), "-o",
+           ["-I" + REF_SRC, "-I" + os.path.join(ROOT, "include"), BINDING, "-o",
             str(tmp_path / "bind.o")])
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 0, r.stdout
