@@ -47,14 +47,23 @@ def lattice_fields(path, p):
     return it.value, g
 
 
+TRIPLET_KEYS = ("msq_triplet -0.3\nb4 0.1\nsigma0 0.4\nalgorithm_su2triplet overrelax\n"
+                "update_triplet 1\n")
+SINGLET_KEYS = ("update_singlet 1\nsinglet0 0.3\nalgorithm_singlet overrelax\nmsq_s -0.1\n"
+                "b1_s 0.2\nb3_s 0.1\nb4_s 0.3\na1_s 0.3\na2_s 0.25\n")
+
+
 @pytest.mark.parametrize("tag,model,extra,p_kw", [
-    ("hu", "higgs+u1", {}, dict(nhiggs=1, u1=1)),
-    ("d", "2hdm", TWO_HIGGS, dict(nhiggs=2, **TWO_HIGGS)),
-])
+    ("hu", "higgs+u1", "", dict(nhiggs=1, u1=1)),
+    ("d", "2hdm", "".join("%s %r\n" % kv for kv in TWO_HIGGS.items()), dict(nhiggs=2)),
+    ("t", "triplet", TRIPLET_KEYS, dict(nhiggs=0, triplet=1)),
+    ("hsu", "higgs+u1+singlet", SINGLET_KEYS, dict(nhiggs=1, u1=1, singlet=1)),
+], ids=["higgs_u1", "two_doublets", "adjoint", "higgs_u1_singlet"])
 def test_reference_main_with_gpu_sweep_matches_su2hip(tag, model, extra, p_kw, tmp_path):
+    """One doublet with U(1), two doublets, the adjoint alone, and a doublet
+    with U(1) and a singlet: every field path of the binding."""
     cfg = CONFIG.format(model=model, it=8, do_flow=0).replace(
-        "seed 4242", "seed %d" % BINDING_SEED)
-    cfg += "".join("%s %r\n" % kv for kv in extra.items())
+        "seed 4242", "seed %d" % BINDING_SEED) + extra
     dr, dc = tmp_path / "ref", tmp_path / "su2hip"
     dr.mkdir()
     dc.mkdir()
@@ -64,8 +73,7 @@ def test_reference_main_with_gpu_sweep_matches_su2hip(tag, model, extra, p_kw, t
     ours = open(dc / "measure").read().splitlines(keepends=True)
     assert len(ref) == 8
     same_lines(ours, ref)
-    p = make_params(L=(4, 6, 8), betasu2=8.0, betau1=9.0, msq_phi=-0.25, lambda_phi=0.02,
-                    scalar_sweeps=3, seed=BINDING_SEED, **p_kw)
+    p = make_params(L=(4, 6, 8), **p_kw)  # the field content is what the reader needs
     itr, fr = lattice_fields(str(dr / "lattice"), p)
     itc, fc = lattice_fields(str(dc / "lattice"), p)
     assert itr == itc
