@@ -6,8 +6,11 @@
  * (src/update.c:270, from src/main.c:265 and :353) lands here.  The first
  * call uploads the fields the reference initialised; every call then runs
  * one compound sweep on the GPU and brings the fields back, so that the
- * reference's measure() and save_lattice() see them.  Multicanonical runs
- * are not routed (the binding does not pass the weight), and the counters
+ * reference's measure() and save_lattice() see them.  In a multicanonical
+ * run the weight moves to the device at the first call, again whenever the
+ * main loop changes its mode (READONLY during thermalization,
+ * src/main.c:255-278), and its state comes back after every sweep, so that
+ * measure() prints the device's order parameter and weight.  The counters
  * are not filled in.
  */
 #include <stdio.h>
@@ -19,24 +22,26 @@
 int su2hip_bind_init(lattice const *l, fields const *f, params const *p);
 int su2hip_bind_update(void);
 int su2hip_bind_download(fields *f);
+int su2hip_bind_muca(weight const *w);
+int su2hip_bind_muca_download(weight *w);
+
+static void die_with(const char *what) {
+  fprintf(stderr, "bind_shim: %s: %s\n", what, su2hip_last_error());
+  exit(1);
+}
 
 void __wrap_update_lattice(lattice *l, fields *f, params const *p, counters *c, weight *w) {
-  static int ready = 0;
+  static int ready = 0, mode = -1;
   (void)c;
-  (void)w;
-  if (p->multicanonical) {
-    fprintf(stderr, "bind_shim: multicanonical runs are not routed to the GPU\n");
-    exit(1);
-  }
   if (!ready) {
-    if (su2hip_bind_init(l, f, p)) {
-      fprintf(stderr, "bind_shim: su2hip_bind_init: %s\n", su2hip_last_error());
-      exit(1);
-    }
+    if (su2hip_bind_init(l, f, p)) die_with("su2hip_bind_init");
     ready = 1;
   }
-  if (su2hip_bind_update() || su2hip_bind_download(f)) {
-    fprintf(stderr, "bind_shim: %s\n", su2hip_last_error());
-    exit(1);
+  if (p->multicanonical && w->mode != mode) {
+    if (su2hip_bind_muca(w)) die_with("su2hip_bind_muca");
+    mode = w->mode;
   }
+  if (su2hip_bind_update()) die_with("su2hip_bind_update");
+  if (su2hip_bind_download(f)) die_with("su2hip_bind_download");
+  if (p->multicanonical && su2hip_bind_muca_download(w)) die_with("su2hip_bind_muca_download");
 }

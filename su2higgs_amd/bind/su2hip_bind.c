@@ -107,3 +107,46 @@ int su2hip_bind_download(fields *f) {
 #endif
   return rc;
 }
+
+/* multicanonical runs: after load_weight and calc_orderparam
+ * (src/main.c:89, :216-217), the weight and its state move to the device,
+ * which then does the global accept/reject inside every scalar pass
+ * (multicanonical_acceptance, src/update.c:13-30) and, in FAST and SLOW
+ * modes, the weight recursion.  Call again when the main loop changes
+ * w->mode (src/main.c:255-278). */
+int su2hip_bind_muca(weight const *w) {
+  su2hip_muca m = {0};
+  m.enabled = 1;
+  m.orderparam = w->orderparam;      /* PHISQ .. PHI2SQ: the same codes */
+  m.mode = w->mode;                  /* READONLY, FAST, SLOW: the same codes */
+  m.checks_per_sweep = w->checks_per_sweep;
+  m.bins = w->bins;
+  m.last_max = w->last_max;
+  m.do_acceptance = w->do_acceptance;
+  m.muca_count = w->muca_count;
+  m.update_interval = w->update_interval;
+  m.delta = w->delta;
+  m.min = w->min;
+  m.max = w->max;
+  m.wrk_min = w->wrk_min;
+  m.wrk_max = w->wrk_max;
+  m.param_value[0] = w->param_value[EVEN];
+  m.param_value[1] = w->param_value[ODD];
+  int rc = su2hip_muca_set(H, 0, &m, w->pos, w->W);
+  if (rc == 0 && w->mode == SLOW) rc = su2hip_muca_set_sums(H, 0, w->gsum, w->nsum);
+  return rc;
+}
+
+/* before measure() or save_weight(): the device's weight state back into w */
+int su2hip_bind_muca_download(weight *w) {
+  su2hip_muca m;
+  int rc = su2hip_muca_get(H, 0, &m, w->pos, w->W, w->hits);
+  if (rc) return rc;
+  if (w->mode == SLOW) rc = su2hip_muca_get_sums(H, 0, w->gsum, w->nsum);
+  w->param_value[EVEN] = m.param_value[0];
+  w->param_value[ODD] = m.param_value[1];
+  w->last_max = m.last_max;
+  w->muca_count = m.muca_count;
+  linearize_weight(w);
+  return rc;
+}

@@ -80,3 +80,27 @@ def test_reference_main_with_gpu_sweep_matches_su2hip(tag, model, extra, p_kw, t
     for k in orc.FIELDS:
         if fr.get(k) is not None:
             assert np.array_equal(fr[k], fc[k]), k
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["readonly", "fast"])
+def test_reference_main_multicanonical_with_gpu_sweep_matches_su2hip(mode, tmp_path):
+    """A multicanonical run (order parameter phi^2, 40 bins on [0, 2], two
+    global checks per parity pass): the weight moves to the device through
+    su2hip_bind_muca and its state comes back after every sweep, so the
+    reference's measure() prints the device's order parameter and weight.
+    In FAST mode the weight is updated on the device as it goes."""
+    from test_gpu_traj import write_weight
+    cfg = CONFIG.format(model="higgs+u1", it=8, do_flow=0).replace(
+        "seed 4242", "seed %d" % BINDING_SEED).replace("multicanonical 0", "multicanonical 1")
+    cfg += ("weightfile weight\nbins 40\nmin 0.0\nmax 2.0\nchecks_per_sweep 2\n"
+            "orderparam phisq\nmuca_mode %d\nmuca_delta 0.3\n" % mode)
+    dr, dc = tmp_path / "ref", tmp_path / "su2hip"
+    for d in (dr, dc):
+        d.mkdir()
+        write_weight(str(d / "weight"), 40, 0.0, 2.0)
+    run_ref_gpu("hu", str(dr), cfg)
+    run_cli(str(dc), cfg)
+    ref = open(dr / "measure").read().splitlines(keepends=True)
+    ours = open(dc / "measure").read().splitlines(keepends=True)
+    assert len(ref) == 8
+    same_lines(ours, ref)
