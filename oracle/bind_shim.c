@@ -10,8 +10,9 @@
  * run the weight moves to the device at the first call, again whenever the
  * main loop changes its mode (READONLY during thermalization,
  * src/main.c:255-278), and its state comes back after every sweep, so that
- * measure() prints the device's order parameter and weight.  The counters
- * are not filled in.
+ * measure() prints the device's order parameter and weight.  The
+ * acceptance counts since the previous call are added into the reference's
+ * counters, for print_acceptance at checkpoints.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -24,6 +25,7 @@ int su2hip_bind_update(void);
 int su2hip_bind_download(fields *f);
 int su2hip_bind_muca(weight const *w);
 int su2hip_bind_muca_download(weight *w);
+int su2hip_bind_counters(counters *c);
 
 static void die_with(const char *what) {
   fprintf(stderr, "bind_shim: %s: %s\n", what, su2hip_last_error());
@@ -32,7 +34,6 @@ static void die_with(const char *what) {
 
 void __wrap_update_lattice(lattice *l, fields *f, params const *p, counters *c, weight *w) {
   static int ready = 0, mode = -1;
-  (void)c;
   if (!ready) {
     if (su2hip_bind_init(l, f, p)) die_with("su2hip_bind_init");
     ready = 1;
@@ -44,4 +45,5 @@ void __wrap_update_lattice(lattice *l, fields *f, params const *p, counters *c, 
   if (su2hip_bind_update()) die_with("su2hip_bind_update");
   if (su2hip_bind_download(f)) die_with("su2hip_bind_download");
   if (p->multicanonical && su2hip_bind_muca_download(w)) die_with("su2hip_bind_muca_download");
+  if (su2hip_bind_counters(c)) die_with("su2hip_bind_counters");
 }

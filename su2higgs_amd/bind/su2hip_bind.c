@@ -150,3 +150,45 @@ int su2hip_bind_muca_download(weight *w) {
   linearize_weight(w);
   return rc;
 }
+
+/* after each su2hip_bind_update: the device's acceptance counts since the
+ * previous call, added into the reference's counters (which main() resets
+ * after thermalization, src/main.c:273), for print_acceptance */
+int su2hip_bind_counters(counters *c) {
+  static su2hip_counters last;
+  su2hip_counters now;
+  int rc = su2hip_get_counters(H, 0, &now);
+  if (rc) return rc;
+#define ADD(dst, member) ((dst) += now.member - last.member)
+  ADD(c->total_su2link, total_su2link);
+  ADD(c->accepted_su2link, accepted_su2link);
+  ADD(c->total_u1link, total_u1link);
+  ADD(c->accepted_u1link, accepted_u1link);
+#if (NHIGGS > 0)
+  ADD(c->total_doublet[0], total_doublet);
+  ADD(c->accepted_doublet[0], accepted_doublet);
+  ADD(c->total_overrelax_doublet[0], total_overrelax_doublet);
+  ADD(c->acc_overrelax_doublet[0], acc_overrelax_doublet);
+#endif
+#if (NHIGGS == 2)
+  ADD(c->total_doublet[1], total_doublet2);
+  ADD(c->accepted_doublet[1], accepted_doublet2);
+  ADD(c->total_overrelax_doublet[1], total_overrelax_doublet2);
+  ADD(c->acc_overrelax_doublet[1], acc_overrelax_doublet2);
+#endif
+#ifdef SINGLET
+  ADD(c->total_singlet, total_singlet);
+  ADD(c->accepted_singlet, accepted_singlet);
+  ADD(c->total_overrelax_singlet, total_overrelax_singlet);
+  ADD(c->acc_overrelax_singlet, acc_overrelax_singlet);
+#endif
+  ADD(c->total_triplet, total_triplet);
+  ADD(c->accepted_triplet, accepted_triplet);
+  ADD(c->total_overrelax_triplet, total_overrelax_triplet);
+  ADD(c->acc_overrelax_triplet, acc_overrelax_triplet);
+  ADD(c->total_muca, total_muca);
+  ADD(c->accepted_muca, accepted_muca);
+#undef ADD
+  last = now;
+  return SU2HIP_OK;
+}

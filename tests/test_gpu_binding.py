@@ -38,6 +38,13 @@ def run_ref_gpu(tag, d, cfg):
     return r.stdout
 
 
+def acceptance_lines(stdout):
+    """The rate line after each "-- Acceptance rates --" (print_acceptance,
+    src/checkpoint.c:17-78, printed at every checkpoint)."""
+    lines = stdout.splitlines()
+    return [lines[i + 1].strip() for i, l in enumerate(lines) if l.strip() == "-- Acceptance rates --"]
+
+
 def lattice_fields(path, p):
     lib = abi.load()
     g = abi.field_arrays(p)
@@ -63,16 +70,19 @@ def test_reference_main_with_gpu_sweep_matches_su2hip(tag, model, extra, p_kw, t
     """One doublet with U(1), two doublets, the adjoint alone, and a doublet
     with U(1) and a singlet: every field path of the binding."""
     cfg = CONFIG.format(model=model, it=8, do_flow=0).replace(
-        "seed 4242", "seed %d" % BINDING_SEED) + extra
+        "seed 4242", "seed %d" % BINDING_SEED).replace("checkpoint 1000", "checkpoint 4") + extra
     dr, dc = tmp_path / "ref", tmp_path / "su2hip"
     dr.mkdir()
     dc.mkdir()
-    run_ref_gpu(tag, str(dr), cfg)
-    run_cli(str(dc), cfg)
+    out_ref = run_ref_gpu(tag, str(dr), cfg)
+    out_cli = run_cli(str(dc), cfg)
     ref = open(dr / "measure").read().splitlines(keepends=True)
     ours = open(dc / "measure").read().splitlines(keepends=True)
     assert len(ref) == 8
     same_lines(ours, ref)
+    # acceptance rates at the checkpoints of iterations 4 and 8
+    acc = acceptance_lines(out_ref)
+    assert len(acc) == 2 and acc == acceptance_lines(out_cli), (acc, acceptance_lines(out_cli))
     p = make_params(L=(4, 6, 8), **p_kw)  # the field content is what the reader needs
     itr, fr = lattice_fields(str(dr / "lattice"), p)
     itc, fc = lattice_fields(str(dc / "lattice"), p)
@@ -92,15 +102,18 @@ def test_reference_main_multicanonical_with_gpu_sweep_matches_su2hip(mode, tmp_p
     from test_gpu_traj import write_weight
     cfg = CONFIG.format(model="higgs+u1", it=8, do_flow=0).replace(
         "seed 4242", "seed %d" % BINDING_SEED).replace("multicanonical 0", "multicanonical 1")
+    cfg = cfg.replace("checkpoint 1000", "checkpoint 4")
     cfg += ("weightfile weight\nbins 40\nmin 0.0\nmax 2.0\nchecks_per_sweep 2\n"
             "orderparam phisq\nmuca_mode %d\nmuca_delta 0.3\n" % mode)
     dr, dc = tmp_path / "ref", tmp_path / "su2hip"
     for d in (dr, dc):
         d.mkdir()
         write_weight(str(d / "weight"), 40, 0.0, 2.0)
-    run_ref_gpu("hu", str(dr), cfg)
-    run_cli(str(dc), cfg)
+    out_ref = run_ref_gpu("hu", str(dr), cfg)
+    out_cli = run_cli(str(dc), cfg)
     ref = open(dr / "measure").read().splitlines(keepends=True)
     ours = open(dc / "measure").read().splitlines(keepends=True)
     assert len(ref) == 8
     same_lines(ours, ref)
+    acc = acceptance_lines(out_ref)
+    assert len(acc) == 2 and acc == acceptance_lines(out_cli), (acc, acceptance_lines(out_cli))
