@@ -314,39 +314,65 @@ measure_local 0
 """
 
 
+def _ref_wall(exe, dirs, it):
+    """Wall time of len(dirs) copies of the reference binary run side by side,
+    one per directory, each for `it` sweeps of the 96^3 workload."""
+    procs = []
+    t0 = time.perf_counter()
+    for d in dirs:
+        with open(os.path.join(d, "config"), "w") as f:
+            f.write(REF_CONFIG.format(L=96, it=it))
+        for fn in ("lattice", "measure", "labels"):
+            try:
+                os.remove(os.path.join(d, fn))
+            except OSError:
+                pass
+        procs.append(subprocess.Popen([exe, "config"], cwd=d, stdout=subprocess.DEVNULL,
+                                      stderr=subprocess.DEVNULL))
+    ok = True
+    for pr in procs:
+        try:
+            ok = pr.wait(timeout=300) == 0 and ok
+        except subprocess.TimeoutExpired:
+            pr.kill()
+            ok = False
+    return (time.perf_counter() - t0) if ok else None
+
+
 def cpu_baseline():
     """The reference's serial CPU build (oracle/_ref/su2_hu, -DNHIGGS=1 -DU1)
-    on this host: wall(4 sweeps) - wall(1 sweep) of 96^3, cold start, the
-    reference's own mix (heatbath + U(1) Metropolis + 4 OR + 1 Metropolis =
-    11 V variable updates per sweep; it has no link overrelaxation)."""
+    on this host's cores: wall(4 sweeps) - wall(1 sweep) of 96^3, cold start,
+    the reference's own mix (heatbath + U(1) Metropolis + 4 OR + 1 Metropolis
+    = 11 V variable updates per sweep; it has no link overrelaxation).  One
+    copy alone, then one independent copy per core side by side (the MPI
+    build cannot run here; independent chains are its best case), whose
+    aggregate rate is the value."""
     exe = os.path.join(ROOT, "oracle", "_ref", "su2_hu")
     if not os.path.exists(exe):
         return oracle_baseline()
-    walls = {}
-    d = tempfile.mkdtemp(prefix="su2ref_")
     try:
-        for it in (1, 4):
-            with open(os.path.join(d, "config"), "w") as f:
-                f.write(REF_CONFIG.format(L=96, it=it))
-            for fn in ("lattice", "measure", "labels"):
-                try:
-                    os.remove(os.path.join(d, fn))
-                except OSError:
-                    pass
-            t0 = time.perf_counter()
-            r = subprocess.run([exe, "config"], cwd=d, stdout=subprocess.DEVNULL,
-                               stderr=subprocess.DEVNULL, timeout=300)
-            walls[it] = time.perf_counter() - t0
-            if r.returncode != 0:
-                return oracle_baseline()
-    finally:
-        shutil.rmtree(d, ignore_errors=True)
-    per_sweep = (walls[4] - walls[1]) / 3
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(16, ncores))
     V = 96 ** 3
-    return {"value": 11 * V / per_sweep, "unit": "site-updates/s", "cores": 1, "kind": "reference",
+    dirs = [tempfile.mkdtemp(prefix="su2ref_") for _ in range(ncores)]
+    try:
+        rates = {}
+        for n in sorted({1, ncores}):
+            w1, w4 = _ref_wall(exe, dirs[:n], 1), _ref_wall(exe, dirs[:n], 4)
+            if w1 is None or w4 is None:
+                return oracle_baseline()
+            rates[n] = (n * 11 * V / ((w4 - w1) / 3), (w4 - w1) / 3)
+    finally:
+        for d in dirs:
+            shutil.rmtree(d, ignore_errors=True)
+    return {"value": rates[ncores][0], "unit": "site-updates/s", "cores": ncores,
+            "kind": "reference", "value_one_core": rates[1][0],
             "sample": "reference serial build (gcc -O3 -fcommon -DNHIGGS=1 -DU1), 96^3 cold start, "
-                      "wall(4 sweeps)-wall(1 sweep) = %.2f s/sweep, 11 V variable updates per sweep"
-                      % per_sweep}
+                      "wall(4 sweeps)-wall(1 sweep), 11 V variable updates per sweep: %d "
+                      "independent copies side by side, %.2f s/sweep each (one copy alone: "
+                      "%.2f s/sweep)" % (ncores, rates[ncores][1], rates[1][1])}
 
 
 def oracle_baseline():
