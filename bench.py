@@ -255,10 +255,11 @@ def multicanonical():
     return c
 
 
-def batched_scan(steps):
+def batched_scan(steps, graph=True):
     """96 independent SU(2)+Higgs lattices, volumes drawn from
     {16^3,24^3,32^3,48^3}, msq spaced across [-0.30,-0.15], FP32, one launch per
-    pass for the whole batch, compound sweeps replayed as a hipGraph."""
+    pass for the whole batch, compound sweeps replayed as a hipGraph (kernel by
+    kernel with graph=False, for profilers)."""
     from su2higgs_amd import abi
     from su2higgs_amd.abi import FP32, make_params
     from su2higgs_amd.lattice import Batch
@@ -271,7 +272,7 @@ def batched_scan(steps):
     for i in range(len(ps)):
         b.hot_start(i)
     b.autotune(repeats=20)
-    b.use_graph(True)
+    b.use_graph(graph)
     b.update(3)
     b.sync()
     t0 = time.perf_counter()
@@ -439,7 +440,7 @@ def main():
     batched = None
     if not args.no_batch:
         log("[bench] batched scan, 96 lattices")
-        batched = batched_scan(max(5, 2 * args.steps))
+        batched = batched_scan(max(5, 2 * args.steps), graph=not args.no_graph)
         log("[bench] batched %.3e site-updates/s" % batched["site_updates_per_s"])
     served = None
     if args.serve_requests > 0:
