@@ -281,9 +281,19 @@ def batched_scan(steps, graph=True):
     wall = time.perf_counter() - t0
     Vtot = sum(abi.volume(p) for p in ps)
     _, upd = passes_and_updates(ps[0])
+    # every pass kind of the sweep over the whole batch against the HBM peak
+    kr = {}
+    for k in (abi.K_LINK_HEATBATH, abi.K_LINK_OVERRELAX, abi.K_DOUBLET_OVERRELAX,
+              abi.K_DOUBLET_METROPOLIS):
+        ms, _ = b.time_pass(k, 50)
+        byts = algo_words(k, ps[0]) * Vtot * 4
+        ach = byts / (ms * 1e-3) / 1e9
+        kr[abi.KIND_NAMES[k]] = {"avg_launch_us": ms * 1e3, "algorithmic_MB": byts / 1e6,
+                                 "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
     b.close()
     return {"lattices": 96, "total_sites": int(Vtot), "sweeps": steps,
-            "ms_per_sweep": 1e3 * wall / steps, "site_updates_per_s": upd * Vtot * steps / wall}
+            "ms_per_sweep": 1e3 * wall / steps, "site_updates_per_s": upd * Vtot * steps / wall,
+            "kernel_roofline": kr}
 
 
 REF_CONFIG = """dim 3
