@@ -1,8 +1,10 @@
 /* su2hip_bind.c -- drive su2higgs's sweep on an MI355X through libsu2hip. */
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "su2.h"
+#include "generic/mersenne.h"
 #include "su2hip.h"
 
 static su2hip_batch_t *H;
@@ -43,7 +45,10 @@ static su2hip_params to_su2hip(lattice const *l, params const *p) {
   q.lam6_re = p->lam6.re; q.lam6_im = p->lam6.im;
   q.lam7_re = p->lam7.re; q.lam7_im = p->lam7.im;
 #endif
-  q.seed = 1234;                      /* Philox key */
+  /* Philox key: the next word of the reference's own generator, which
+   * main() seeds from the clock (src/main.c:59-78), so every run is a new
+   * chain.  A resumed run takes its key back from <latticefile>.rng. */
+  q.seed = iran();
   return q;
 }
 
@@ -76,12 +81,31 @@ static long Vsites;
 #define SINGLET_OF(f) NULL
 #endif
 
-/* once, after alloc_fields/setfields or load_lattice (src/main.c:160-200) */
+/* <latticefile>.rng: the Philox key and pass counter beside the lattice file */
+static void rng_path(char *out, size_t n, const char *latticefile) {
+  snprintf(out, n, "%s.rng", latticefile);
+}
+
+/* once, after alloc_fields/setfields or load_lattice (src/main.c:160-200).
+ * A run that continues from a lattice file (reset 0, src/main.c:280-282)
+ * continues its Philox stream from <latticefile>.rng when that exists. */
 int su2hip_bind_init(lattice const *l, fields const *f, params const *p) {
   su2hip_params q = to_su2hip(l, p);
   int rc = su2hip_create(1, &q, 1, 0, &H);
   if (rc) return rc;
   su2hip_use_graph(H, 1);
+  char rp[sizeof(p->latticefile) + 8];
+  rng_path(rp, sizeof(rp), p->latticefile);
+  FILE *fr = p->reset ? NULL : fopen(rp, "r");
+  unsigned long long key = q.seed, pass = 0;
+  if (fr) {
+    int got = fscanf(fr, "%llu %llu", &key, &pass);
+    fclose(fr);
+    if (got != 2) return SU2HIP_EIO;
+    rc = su2hip_rng_set(H, 0, key, pass);
+    if (rc) return rc;
+  }
+  printf("su2hip: Philox key %llu, pass %llu\n", key, pass);
 #if (NHIGGS == 2)
   Vsites = l->sites;
   Dstage = malloc(sizeof(double) * 2 * SU2DB * Vsites);
@@ -91,6 +115,21 @@ int su2hip_bind_init(lattice const *l, fields const *f, params const *p) {
 #endif
   return su2hip_set_fields(H, 0, f->su2link[0][0], U1_OF(f), DOUBLET_OF(f), TRIPLET_OF(f),
                            SINGLET_OF(f));
+}
+
+/* after save_lattice(l, f, c, fname) (src/su2.h:458, src/main.c:372,391):
+ * the Philox position, so that a resumed run replays nothing */
+int su2hip_bind_save_rng(const char *latticefile) {
+  if (!H) return SU2HIP_OK; /* no sweep yet: nothing drawn */
+  uint64_t key, pass;
+  int rc = su2hip_rng_get(H, 0, &key, &pass);
+  if (rc) return rc;
+  char rp[256];
+  rng_path(rp, sizeof(rp), latticefile);
+  FILE *fw = fopen(rp, "w");
+  if (!fw) return SU2HIP_EIO;
+  fprintf(fw, "%llu %llu\n", (unsigned long long)key, (unsigned long long)pass);
+  return fclose(fw) ? SU2HIP_EIO : SU2HIP_OK;
 }
 
 /* replaces update_lattice(&l, f, &p, &c, &w) (src/su2.h:426, src/main.c:353) */

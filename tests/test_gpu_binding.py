@@ -4,12 +4,13 @@ oracle/_ref/su2_<tag>_gpu is the reference's unmodified main loop
 (src/main.c) linked with the reference-side binding
 su2higgs_amd/bind/su2hip_bind.c: update_lattice is routed to libsu2hip
 (oracle/bind_shim.c).  On the same config it must write the `measure` file
-the su2hip driver writes: both run the same GPU sweeps (Philox seed 1234,
-pass counter from 0); the reference measures on the CPU with its own
-measure(), su2hip on the GPU.  The final `lattice` files hold bitwise the
-same fields."""
+the su2hip driver writes: both run the same GPU sweeps (the Philox key the
+binding drew from the reference's generator and printed, pass counter from
+0); the reference measures on the CPU with its own measure(), su2hip on the
+GPU.  The final `lattice` files hold bitwise the same fields."""
 import ctypes as C
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -23,19 +24,35 @@ from test_gpu_cli import CONFIG, TWO_HIGGS, run_cli, same_lines
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BINDING_SEED = 1234  # the Philox key su2hip_bind.c gives the lattice
 
 
-def run_ref_gpu(tag, d, cfg):
+def run_ref_gpu(tag, d, cfg, mt_seed=None):
+    """Run the reference's main loop with the GPU sweep in directory d;
+    mt_seed seeds the reference's generator (through the test shim) before
+    the binding draws its Philox key from it."""
     exe = os.path.join(ROOT, "oracle", "_ref", "su2_%s_gpu" % tag)
     if not os.path.exists(exe):
         pytest.skip("reference build with the binding absent")
     with open(os.path.join(d, "config"), "w") as f:
         f.write(cfg)
+    env = dict(os.environ)
+    if mt_seed is not None:
+        env["SU2HIP_SHIM_MT_SEED"] = str(mt_seed)
     r = subprocess.run([exe, "config"], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                       text=True, timeout=120)
+                       text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stdout[-3000:]
     return r.stdout
+
+
+def philox_key(stdout):
+    """The key (and starting pass) su2hip_bind_init printed."""
+    m = re.search(r"su2hip: Philox key (\d+), pass (\d+)", stdout)
+    assert m, stdout[-2000:]
+    return int(m.group(1)), int(m.group(2))
+
+
+def with_key(cfg, key):
+    return cfg.replace("seed 4242", "seed %d" % key)
 
 
 def acceptance_lines(stdout):
@@ -69,13 +86,14 @@ SINGLET_KEYS = ("update_singlet 1\nsinglet0 0.3\nalgorithm_singlet overrelax\nms
 def test_reference_main_with_gpu_sweep_matches_su2hip(tag, model, extra, p_kw, tmp_path):
     """One doublet with U(1), two doublets, the adjoint alone, and a doublet
     with U(1) and a singlet: every field path of the binding."""
-    cfg = CONFIG.format(model=model, it=8, do_flow=0).replace(
-        "seed 4242", "seed %d" % BINDING_SEED).replace("checkpoint 1000", "checkpoint 4") + extra
+    cfg = CONFIG.format(model=model, it=8, do_flow=0).replace("checkpoint 1000", "checkpoint 4") + extra
     dr, dc = tmp_path / "ref", tmp_path / "su2hip"
     dr.mkdir()
     dc.mkdir()
     out_ref = run_ref_gpu(tag, str(dr), cfg)
-    out_cli = run_cli(str(dc), cfg)
+    key, pass0 = philox_key(out_ref)
+    assert pass0 == 0
+    out_cli = run_cli(str(dc), with_key(cfg, key))
     ref = open(dr / "measure").read().splitlines(keepends=True)
     ours = open(dc / "measure").read().splitlines(keepends=True)
     assert len(ref) == 8
@@ -100,8 +118,7 @@ def test_reference_main_multicanonical_with_gpu_sweep_matches_su2hip(mode, tmp_p
     reference's measure() prints the device's order parameter and weight.
     In FAST mode the weight is updated on the device as it goes."""
     from test_gpu_traj import write_weight
-    cfg = CONFIG.format(model="higgs+u1", it=8, do_flow=0).replace(
-        "seed 4242", "seed %d" % BINDING_SEED).replace("multicanonical 0", "multicanonical 1")
+    cfg = CONFIG.format(model="higgs+u1", it=8, do_flow=0).replace("multicanonical 0", "multicanonical 1")
     cfg = cfg.replace("checkpoint 1000", "checkpoint 4")
     cfg += ("weightfile weight\nbins 40\nmin 0.0\nmax 2.0\nchecks_per_sweep 2\n"
             "orderparam phisq\nmuca_mode %d\nmuca_delta 0.3\n" % mode)
@@ -110,10 +127,60 @@ def test_reference_main_multicanonical_with_gpu_sweep_matches_su2hip(mode, tmp_p
         d.mkdir()
         write_weight(str(d / "weight"), 40, 0.0, 2.0)
     out_ref = run_ref_gpu("hu", str(dr), cfg)
-    out_cli = run_cli(str(dc), cfg)
+    out_cli = run_cli(str(dc), with_key(cfg, philox_key(out_ref)[0]))
     ref = open(dr / "measure").read().splitlines(keepends=True)
     ours = open(dc / "measure").read().splitlines(keepends=True)
     assert len(ref) == 8
     same_lines(ours, ref)
     acc = acceptance_lines(out_ref)
     assert len(acc) == 2 and acc == acceptance_lines(out_cli), (acc, acceptance_lines(out_cli))
+
+
+def test_binding_keys_differ_between_runs(tmp_path):
+    """Two runs whose reference generators are seeded differently (as two
+    runs started at different times are) draw different Philox keys and so
+    different chains."""
+    cfg = CONFIG.format(model="higgs+u1", it=4, do_flow=0)
+    outs = []
+    for k, mt in enumerate((11, 12)):
+        d = tmp_path / ("run%d" % k)
+        d.mkdir()
+        outs.append((philox_key(run_ref_gpu("hu", str(d), cfg, mt_seed=mt))[0],
+                     open(d / "measure").read().splitlines()))
+    (k0, m0), (k1, m1) = outs
+    assert k0 != k1
+    assert m0[0] == m1[0]  # the cold start, measured before the first sweep
+    assert m0[1:] != m1[1:]
+
+
+def test_binding_resume_continues_the_stream(tmp_path):
+    """A run of 4 iterations resumed to iteration 8 (reset 0: the reference
+    continues at the stored iteration + 1, src/main.c:280-282, so 7 sweeps
+    in all) ends on the same fields, bit for bit, and writes the same
+    measure lines as one run of 7 iterations: the binding saved the Philox
+    position with the checkpoint and took it back.  Without the .rng file
+    the resumed run would replay the stream from pass 0."""
+    base = CONFIG.format(model="higgs+u1", it=7, do_flow=0)
+    d1, d2, d3 = tmp_path / "straight", tmp_path / "resumed", tmp_path / "replayed"
+    for d in (d1, d2, d3):
+        d.mkdir()
+    run_ref_gpu("hu", str(d1), base, mt_seed=5)
+    for d in (d2, d3):
+        run_ref_gpu("hu", str(d), base.replace("iterations 7", "iterations 4"), mt_seed=5)
+        assert os.path.exists(d / "lattice.rng")
+    os.remove(d3 / "lattice.rng")
+    resume = base.replace("iterations 7", "iterations 8").replace("reset 1", "reset 0")
+    out2 = run_ref_gpu("hu", str(d2), resume, mt_seed=99)
+    run_ref_gpu("hu", str(d3), resume, mt_seed=5)
+    k1 = open(d1 / "lattice.rng").read().split()
+    k2 = open(d2 / "lattice.rng").read().split()
+    assert k1 == k2  # same key, same pass count at the end
+    assert "pass 0" not in out2
+    p = make_params(L=(4, 6, 8), nhiggs=1, u1=1)
+    _, f1 = lattice_fields(str(d1 / "lattice"), p)
+    _, f2 = lattice_fields(str(d2 / "lattice"), p)
+    _, f3 = lattice_fields(str(d3 / "lattice"), p)
+    for k in ("su2link", "u1link", "doublet"):
+        assert np.array_equal(f1[k], f2[k]), k
+    assert not np.array_equal(f1["su2link"], f3["su2link"])
+    assert open(d1 / "measure").read() == open(d2 / "measure").read()

@@ -21,9 +21,16 @@ BINDING = os.path.join(ROOT, "su2higgs_amd", "bind", "su2hip_bind.c")
 
 
 def test_integration_shows_the_binding_verbatim():
+    """The one ```c block that opens with the file's own header line is the
+    file, byte for byte; the other blocks are call sequences."""
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
-    blocks = [b for b in re.findall(r"```c\n(.*?)```", text, re.S) if "su2hip_bind_init" in b]
+    blocks = [b for b in re.findall(r"```c\n(.*?)```", text, re.S)
+              if b.startswith("/* su2hip_bind.c")]
     assert blocks == [open(BINDING).read()]
+    # the call sequence names only functions the binding defines
+    defined = set(re.findall(r"^int (su2hip_bind_\w+)\(", blocks[0], re.M))
+    called = set(re.findall(r"(su2hip_bind_\w+)\(", text))
+    assert called <= defined, called - defined
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF_SRC, "su2.h")) or not shutil.which("gcc"),
