@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+CPU_SHARE = 16  # host cores that go with one GPU of the box
 
 
 def log(*a):
@@ -197,24 +198,40 @@ def kernel_rooflines(res):
     return out
 
 
-def roofline_beyond_cache(L=192):
-    """The dominant kernel (FP32 link overrelaxation) on an L^3 lattice whose
-    fields (16 L^3 words, 537 MB at 192^3) exceed the 256 MB Infinity Cache
-    that holds the 96^3 workload: the HBM-only rate."""
+def roofline_beyond_cache(L=192, precisions=None):
+    """Every pass kind of the workload's sweep on an L^3 lattice whose fields
+    (19 L^3 words: 537 MB FP32, 1.07 GB FP64 at 192^3) exceed the 256 MB
+    Infinity Cache that holds the 96^3 fields: HBM-only rates.  The first
+    entry of each precision's table keeps the dominant kernel (link
+    overrelaxation) for the summary fields."""
     from su2higgs_amd import abi
     from su2higgs_amd.lattice import Batch
-    p = workload_params(abi.FP32, L=L)
-    b = Batch(p)
-    b.cold_start(0, phi0=1.0)
-    b.update(2)
-    ms, span = b.time_pass(abi.K_LINK_OVERRELAX, 100)
-    b.close()
-    V = abi.volume(p)
-    byts = algo_words(abi.K_LINK_OVERRELAX, p) * V * 4
-    ach = byts / (ms * 1e-3) / 1e9
-    return {"workload": "%d^3 SU(2)xU(1)+Higgs fp32" % L, "kernel": "link_overrelax",
-            "avg_launch_ms": ms, "algorithmic_bytes_per_launch": byts, "achieved": ach,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+    out = {}
+    for prec in precisions or (abi.FP32, abi.FP64):
+        p = workload_params(prec, L=L)
+        b = Batch(p)
+        b.cold_start(0, phi0=1.0)
+        b.update(2)
+        V = abi.volume(p)
+        w = 4 if prec == abi.FP32 else 8
+        kinds = {}
+        for k in (abi.K_LINK_OVERRELAX, abi.K_LINK_HEATBATH, abi.K_U1_METROPOLIS,
+                  abi.K_DOUBLET_OVERRELAX, abi.K_DOUBLET_METROPOLIS):
+            ms, span = b.time_pass(k, 60)
+            byts = algo_words(k, p) * V * w
+            ach = byts / (ms * 1e-3) / 1e9
+            kinds[abi.KIND_NAMES[k]] = {"avg_launch_us": ms * 1e3, "algorithmic_MB": byts / 1e6,
+                                        "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
+        b.close()
+        out["fp32" if prec == abi.FP32 else "fp64"] = kinds
+    lo = out["fp32"]["link_overrelax"] if "fp32" in out else None
+    res = {"workload": "%d^3 SU(2)xU(1)+Higgs" % L, "kernels": out}
+    if lo:
+        res.update({"kernel": "link_overrelax", "avg_launch_ms": lo["avg_launch_us"] * 1e-3,
+                    "algorithmic_bytes_per_launch": lo["algorithmic_MB"] * 1e6,
+                    "achieved": lo["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": lo["frac"]})
+    return res
 
 
 def serving(n, rate=20.0, seed=2026):
@@ -245,7 +262,8 @@ def multicanonical():
     per parity half-sweep, 500 + 2000 sweeps, FP32-mixed (FP64 order
     parameter, weight and accept), through the su2hip driver; compared with
     the reference's own chains of the same run (tests/golden)."""
-    from su2higgs_amd import muca48
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import muca48  # the comparison with the reference's recorded chains
     if not os.path.exists(muca48.FIXTURE):
         return None
     ours = muca48.run(fp32=True)
@@ -255,23 +273,29 @@ def multicanonical():
     return c
 
 
-def batched_scan(steps, graph=True):
-    """96 independent SU(2)+Higgs lattices, volumes drawn from
-    {16^3,24^3,32^3,48^3}, msq spaced across [-0.30,-0.15], FP32, one launch per
-    pass for the whole batch, compound sweeps replayed as a hipGraph (kernel by
-    kernel with graph=False, for profilers)."""
-    from su2higgs_amd import abi
+def scan_params(precision):
+    """The 96 lattices of the scan: volumes drawn from {16^3,24^3,32^3,48^3},
+    betaG=8, lambda=0.02, msq spaced across [-0.30,-0.15], per-lattice seeds."""
     from su2higgs_amd.abi import FP32, make_params
-    from su2higgs_amd.lattice import Batch
     rng = np.random.default_rng(96)
     Ls = rng.choice([16, 24, 32, 48], size=96)
-    ps = [make_params(L=(int(L),) * 3, nhiggs=1, precision=FP32, betasu2=8.0, lambda_phi=0.02,
-                      msq_phi=-0.30 + 0.15 * k / 95, link_overrelax=4, scalar_sweeps=4,
-                      reunitarize=1, seed=1000 + k) for k, L in enumerate(Ls)]
+    return [make_params(L=(int(L),) * 3, nhiggs=1, precision=precision, betasu2=8.0,
+                        lambda_phi=0.02, msq_phi=-0.30 + 0.15 * k / 95, link_overrelax=4,
+                        scalar_sweeps=4, reunitarize=1 if precision == FP32 else 0,
+                        seed=1000 + k) for k, L in enumerate(Ls)]
+
+
+def _scan_rate(ps, steps, graph, tune=True, kinds=()):
+    """Site-updates/s of one batch holding every lattice of ps (hot starts,
+    autotuned chunks, `steps` sweeps replayed as a hipGraph), and the
+    per-kind roofline over the whole batch."""
+    from su2higgs_amd import abi
+    from su2higgs_amd.lattice import Batch
     b = Batch(ps)
     for i in range(len(ps)):
         b.hot_start(i)
-    b.autotune(repeats=20)
+    if tune:
+        b.autotune(repeats=20)
     b.use_graph(graph)
     b.update(3)
     b.sync()
@@ -281,19 +305,54 @@ def batched_scan(steps, graph=True):
     wall = time.perf_counter() - t0
     Vtot = sum(abi.volume(p) for p in ps)
     _, upd = passes_and_updates(ps[0])
-    # every pass kind of the sweep over the whole batch against the HBM peak
+    w = 4 if ps[0].precision == abi.FP32 else 8
     kr = {}
-    for k in (abi.K_LINK_HEATBATH, abi.K_LINK_OVERRELAX, abi.K_DOUBLET_OVERRELAX,
-              abi.K_DOUBLET_METROPOLIS):
+    for k in kinds:
         ms, _ = b.time_pass(k, 50)
-        byts = algo_words(k, ps[0]) * Vtot * 4
+        byts = algo_words(k, ps[0]) * Vtot * w
         ach = byts / (ms * 1e-3) / 1e9
         kr[abi.KIND_NAMES[k]] = {"avg_launch_us": ms * 1e3, "algorithmic_MB": byts / 1e6,
                                  "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
     b.close()
+    return upd * Vtot * steps / wall, 1e3 * wall / steps, kr
+
+
+def batched_scan(steps, graph=True, unbatched_steps=50):
+    """The batched scan three ways on the same 96 lattices:
+    - FP32-mixed batched: every lattice in one handle, one launch per pass
+      for the whole batch (the headline form of the scan);
+    - FP64 batched: the same, in the reference's arithmetic (496 MB of
+      fields, beyond the 256 MB Infinity Cache);
+    - FP32 one lattice at a time: each lattice alone in its own handle, as
+      the reference runs one process per lattice (src/main.c:251-379),
+      `unbatched_steps` sweeps each, its own hipGraph.
+    Site-updates/s = 20 V per sweep summed over the lattices / wall time."""
+    from su2higgs_amd import abi
+    from su2higgs_amd.abi import FP32, FP64
+    kinds = (abi.K_LINK_HEATBATH, abi.K_LINK_OVERRELAX, abi.K_DOUBLET_OVERRELAX,
+             abi.K_DOUBLET_METROPOLIS)
+    p32, p64 = scan_params(FP32), scan_params(FP64)
+    Vtot = sum(abi.volume(p) for p in p32)
+    v32, ms32, kr32 = _scan_rate(p32, steps, graph, kinds=kinds)
+    log("[bench] batched fp32 %.3e site-updates/s" % v32)
+    v64, ms64, kr64 = _scan_rate(p64, max(5, steps // 2), graph, kinds=kinds)
+    log("[bench] batched fp64 %.3e site-updates/s" % v64)
+    # one lattice at a time: total sweep work over total wall time
+    _, upd = passes_and_updates(p32[0])
+    tot_upd, tot_wall = 0.0, 0.0
+    for p in p32:
+        r, ms, _ = _scan_rate([p], unbatched_steps, graph, tune=False)
+        tot_upd += upd * abi.volume(p) * unbatched_steps
+        tot_wall += ms * 1e-3 * unbatched_steps
+    vun = tot_upd / tot_wall
+    log("[bench] fp32 one lattice at a time %.3e site-updates/s" % vun)
     return {"lattices": 96, "total_sites": int(Vtot), "sweeps": steps,
-            "ms_per_sweep": 1e3 * wall / steps, "site_updates_per_s": upd * Vtot * steps / wall,
-            "kernel_roofline": kr}
+            "ms_per_sweep": ms32, "site_updates_per_s": v32, "kernel_roofline": kr32,
+            "fp64_value": v64, "fp64_ms_per_sweep": ms64, "fp64_sweeps": max(5, steps // 2),
+            "fp64_kernel_roofline": kr64, "fp64_field_MB": Vtot * 16 * 8 / 1e6,
+            "unbatched_value": vun, "unbatched_sweeps_per_lattice": unbatched_steps,
+            "fp32_batched_over_fp64_batched": v32 / v64,
+            "fp32_batched_over_fp32_unbatched": v32 / vun}
 
 
 REF_CONFIG = """dim 3
@@ -362,10 +421,12 @@ def cpu_baseline():
     if not os.path.exists(exe):
         return oracle_baseline()
     try:
-        ncores = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        ncores = os.cpu_count() or 1
-    ncores = max(1, min(16, ncores))
+        affinity = os.cpu_count() or 1
+    # A one-GPU share of the host is 16 cores, whatever the affinity set
+    # shows: the copies run on at most that many.
+    ncores = max(1, min(CPU_SHARE, affinity))
     V = 96 ** 3
     dirs = [tempfile.mkdtemp(prefix="su2ref_") for _ in range(ncores)]
     try:
@@ -380,6 +441,8 @@ def cpu_baseline():
             shutil.rmtree(d, ignore_errors=True)
     return {"value": rates[ncores][0], "unit": "site-updates/s", "cores": ncores,
             "kind": "reference", "value_one_core": rates[1][0],
+            "host_affinity_cores": affinity, "host_cpu_count": os.cpu_count(),
+            "cores_cap": "%d: the host share of one GPU" % CPU_SHARE,
             "sample": "reference serial build (gcc -O3 -fcommon -DNHIGGS=1 -DU1), 96^3 cold start, "
                       "wall(4 sweeps)-wall(1 sweep), 11 V variable updates per sweep: %d "
                       "independent copies side by side, %.2f s/sweep each (one copy alone: "
@@ -443,10 +506,10 @@ def main():
     log("[bench] fp64 %.3f ms/sweep, %.3e site-updates/s" % (r64["ms_per_sweep"], r64["site_updates_per_s"]))
     roof = roofline(r32)
     roof64 = roofline(r64)
-    log("[bench] link overrelaxation beyond the Infinity Cache (192^3)")
+    log("[bench] every pass kind beyond the Infinity Cache (192^3)")
     roof_big = roofline_beyond_cache()
-    log("[bench] 192^3 link OR %.2f us, %.0f GB/s (%.2f of peak)"
-        % (roof_big["avg_launch_ms"] * 1e3, roof_big["achieved"], roof_big["frac"]))
+    for pr, t in roof_big["kernels"].items():
+        log("[bench] 192^3 %s: %s" % (pr, ", ".join("%s %.2f" % (k, v["frac"]) for k, v in t.items())))
     batched = None
     if not args.no_batch:
         log("[bench] batched scan, 96 lattices")

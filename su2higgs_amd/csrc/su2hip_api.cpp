@@ -1423,28 +1423,28 @@ int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m, const dou
     if (!ok) return fail(SU2HIP_EINVAL, "order parameter %d needs a field this model lacks", op);
   }
   HIPTRY(hipStreamSynchronize(b->stream));
-  if (s.muca_mem) {
-    HIPTRY(hipFree(s.muca_mem));
-    s.muca_mem = nullptr;
-  }
+  // Everything new is allocated and filled first; the slot's previous
+  // weight (or its disabled state) is replaced only once that succeeded, so
+  // a failure leaves the slot exactly as it was.
   MucaDev h{};
+  void *newmem = nullptr, *newbu = nullptr;
+  size_t bytes = 0, bub = 0;
   if (m->enabled) {
     // backup of the order-parameter field (dofs of the doublet or triplet)
     size_t Vh = (size_t)vol_of(s.p) / 2;
-    size_t bub = align256((s.p.nhiggs ? 8 : 6) * Vh * wsize(s.p));
+    bub = align256((s.p.nhiggs ? 8 : 6) * Vh * wsize(s.p));
     if (!s.bu_mem) {
       if (b->hbm_budget && b->used + bub > b->hbm_budget)
         return fail(SU2HIP_ENOMEM, "HBM budget exceeded by the multicanonical backup");
-      if (hipMalloc(&s.bu_mem, bub) != hipSuccess) return fail(SU2HIP_ENOMEM, "backup allocation");
-      s.bu_bytes = bub;
-      b->used += bub;
-      s.hd.B = s.bu_mem;
+      if (hipMalloc(&newbu, bub) != hipSuccess) return fail(SU2HIP_ENOMEM, "backup allocation");
     }
     int n = m->bins + 2;
-    size_t bytes = sizeof(double) * n * 6 + sizeof(int) * n + sizeof(long long) * n * 2 + 1024;
-    if (hipMalloc(&s.muca_mem, bytes) != hipSuccess) return fail(SU2HIP_ENOMEM, "weight arrays");
-    s.muca_bytes = bytes;
-    char *q = (char *)s.muca_mem;
+    bytes = sizeof(double) * n * 6 + sizeof(int) * n + sizeof(long long) * n * 2 + 1024;
+    if (hipMalloc(&newmem, bytes) != hipSuccess) {
+      if (newbu) (void)hipFree(newbu);
+      return fail(SU2HIP_ENOMEM, "weight arrays");
+    }
+    char *q = (char *)newmem;
     auto take = [&](size_t sz) {
       void *r = q;
       q += (sz + 15) & ~(size_t)15;
@@ -1472,14 +1472,22 @@ int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m, const dou
     }
     if (m->mode == SU2HIP_SLOW)
       for (int i = 0; i < n; i++) hg[i] = exp(W[i]);
-    HIPTRY(hipMemcpy(h.pos, pos, sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(h.W, W, sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(h.slope, slope.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(h.b, bb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(h.hgram, hg.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(h.hits, hits.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(h.gsum, gs.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(h.nsum, ns.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    const struct {
+      void *dst;
+      const void *src;
+      size_t sz;
+    } up[] = {{h.pos, pos, sizeof(double) * n},           {h.W, W, sizeof(double) * n},
+              {h.slope, slope.data(), sizeof(double) * n}, {h.b, bb.data(), sizeof(double) * n},
+              {h.hgram, hg.data(), sizeof(double) * 2 * n}, {h.hits, hits.data(), sizeof(int) * n},
+              {h.gsum, gs.data(), sizeof(long long) * n},  {h.nsum, ns.data(), sizeof(long long) * n}};
+    for (const auto &u : up) {
+      hipError_t e = hipMemcpy(u.dst, u.src, u.sz, hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        (void)hipFree(newmem);
+        if (newbu) (void)hipFree(newbu);
+        return fail(SU2HIP_EDEVICE, "weight upload: %s", hipGetErrorString(e));
+      }
+    }
     h.enabled = 1;
     h.orderparam = m->orderparam;
     h.mode = m->mode;
@@ -1495,6 +1503,16 @@ int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m, const dou
     h.max = m->max;
     h.wrk_min = m->wrk_min;
     h.wrk_max = m->wrk_max;
+  }
+  // commit: nothing below allocates
+  if (s.muca_mem) (void)hipFree(s.muca_mem);
+  s.muca_mem = newmem;
+  s.muca_bytes = bytes;
+  if (newbu) {
+    s.bu_mem = newbu;
+    s.bu_bytes = bub;
+    b->used += bub;
+    s.hd.B = s.bu_mem;
   }
   s.hm = h;
   rc = push_muca(b, slot);
@@ -2227,30 +2245,35 @@ int su2hip_copy_probe(uint64_t bytes, int repeats, double *ms) {
   if (!ms || bytes < 16 || repeats < 1) return fail(SU2HIP_EINVAL, "copy probe arguments");
   size_t n = bytes / sizeof(double);
   double *a = nullptr, *c = nullptr;
-  if (hipMalloc(&a, n * sizeof(double)) != hipSuccess) return fail(SU2HIP_ENOMEM, "probe src");
-  if (hipMalloc(&c, n * sizeof(double)) != hipSuccess) {
-    (void)hipFree(a);
-    return fail(SU2HIP_ENOMEM, "probe dst");
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  // every resource is released on every path: the first failure is kept in e
+  hipError_t e = hipMalloc(&a, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c, n * sizeof(double));
+  if (e != hipSuccess) {
+    if (a) (void)hipFree(a);
+    return fail(SU2HIP_ENOMEM, "probe buffers of %zu bytes", n * sizeof(double));
   }
-  hipStream_t s;
-  hipEvent_t e0, e1;
-  HIPTRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  HIPTRY(hipEventCreate(&e0));
-  HIPTRY(hipEventCreate(&e1));
-  HIPTRY(hipMemsetAsync(a, 0, n * sizeof(double), s));
-  HIPTRY(launch_copy_probe(a, c, n, s));
-  HIPTRY(hipEventRecord(e0, s));
-  for (int r = 0; r < repeats; r++) HIPTRY(launch_copy_probe(a, c, n, s));
-  HIPTRY(hipEventRecord(e1, s));
-  HIPTRY(hipEventSynchronize(e1));
+  const char *what = "stream";
   float t = 0.f;
-  HIPTRY(hipEventElapsedTime(&t, e0, e1));
-  *ms = t / repeats;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipStreamDestroy(s);
+  e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) { what = "events"; e = hipEventCreate(&e0); }
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) { what = "fill"; e = hipMemsetAsync(a, 0, n * sizeof(double), s); }
+  if (e == hipSuccess) { what = "copy kernel"; e = launch_copy_probe(a, c, n, s); }
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int r = 0; r < repeats && e == hipSuccess; r++) e = launch_copy_probe(a, c, n, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) { what = "wait"; e = hipEventSynchronize(e1); }
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (s) (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
   (void)hipFree(a);
   (void)hipFree(c);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "copy probe %s: %s", what, hipGetErrorString(e));
+  *ms = t / repeats;
   return SU2HIP_OK;
 }
 

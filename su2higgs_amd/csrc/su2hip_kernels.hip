@@ -2021,6 +2021,10 @@ __device__ __forceinline__ V3<T> triplet_force(const LatDesc &L, const T *U, con
   return V3<T>{-(res.x + k * g.x), -(res.y + k * g.y), -(res.z + k * g.z)};
 }
 
+// exp(i z.sigma) v.  FP32 renormalises the product: without it the link
+// norm random-walks over the 1200 stages of an RK3 flow to t = 4 and the
+// flowed plaquette drifts by 1e-2 .. 1e-1 of 1 - P; FP64 keeps the
+// reference's arithmetic (src/gradflow.c:361-401).
 template <typename T>
 __device__ __forceinline__ Q<T> expmul(const V3<T> &z, const Q<T> &v) {
   T mod = z.x * z.x + z.y * z.y + z.z * z.z;
@@ -2028,6 +2032,7 @@ __device__ __forceinline__ Q<T> expmul(const V3<T> &z, const Q<T> &v) {
   mod = msqrt(mod);
   T sn = msin(mod) / mod;
   Q<T> u{mcos(mod), sn * z.x, sn * z.y, sn * z.z};
+  if constexpr (sizeof(T) == 4) return normalise(qmul(u, v));
   return qmul(u, v);
 }
 

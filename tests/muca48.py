@@ -11,8 +11,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-EXE = os.path.join(HERE, "bin", "su2hip")
-FIXTURE = os.path.join(ROOT, "tests", "golden", "muca48_reference.npz")
+EXE = os.path.join(ROOT, "su2higgs_amd", "bin", "su2hip")
+FIXTURE = os.path.join(HERE, "golden", "muca48_reference.npz")
 
 
 def blocked(x, nblocks=20):

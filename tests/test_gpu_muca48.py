@@ -7,7 +7,7 @@ import os
 
 import pytest
 
-from su2higgs_amd import muca48
+import muca48
 
 pytestmark = pytest.mark.gpu
 

@@ -154,6 +154,59 @@ def test_graph_replay_matches_eager():
     assert b1.counters(0) == b2.counters(0)
 
 
+def test_graph_replay_after_every_reallocation_matches_eager():
+    """A captured sweep graph holds only kernel launches whose arguments are
+    device pointers and integers; every buffer they name (block tables and
+    their counters, partial sums, random-order tables, weight arrays) is
+    reallocated only together with dropping the graph.  Replays across
+    each such path are bitwise the eager launches: lattices added until
+    the block table grows past its capacity, a slot freed and refilled, a
+    chunk change (the other block table), random order switched on (its
+    tables) and a weight installed (its arrays)."""
+    base = dict(nhiggs=1, link_overrelax=1)
+    hs = [Batch(make_params(L=(8, 8, 8), seed=1, **base), max_slots=6) for _ in range(2)]
+    hs[1].use_graph(True)
+
+    def step(n=2):
+        for h in hs:
+            h.update(n)
+            h.sync()
+        for slot in hs[0].params:
+            f0, f1 = hs[0].get_fields(slot), hs[1].get_fields(slot)
+            for k in FIELDS:
+                if f0.get(k) is not None:
+                    assert np.array_equal(f0[k], f1[k]), (slot, k)
+            assert hs[0].counters(slot) == hs[1].counters(slot), slot
+
+    for h in hs:
+        h.hot_start(0)
+    step()
+    for L, seed in (((16, 16, 16), 2), ((24, 16, 32), 3), ((32, 32, 32), 4)):
+        for h in hs:
+            h.hot_start(h.add(make_params(L=L, seed=seed, **base)))
+        step()
+    for h in hs:
+        h.remove(1)
+        h.hot_start(h.add(make_params(L=(16, 8, 24), seed=5, **base)))
+    step()
+    for h in hs:
+        h.set_chunk(abi.K_LINK_OVERRELAX, 64)
+    step()
+    for h in hs:
+        h.set_random_order(True)
+    step()
+    m = abi.Muca()
+    m.enabled, m.orderparam, m.mode, m.checks_per_sweep, m.bins = 1, abi.PHISQ, abi.FAST, 2, 40
+    m.min, m.max, m.wrk_min, m.wrk_max, m.do_acceptance, m.delta = 0.0, 2.0, 0.0, 2.0, 1, 0.3
+    m.update_interval = 5
+    pos, W = muca_weight()
+    for h in hs:
+        h.muca_set(0, m, pos, W)
+    step(3)
+    for h in hs:
+        h.close()
+
+
 def test_batch_same_alone_and_in_mix():
     """Lattices of different volume swept in one launch give bitwise the
     results each gives alone."""
@@ -291,6 +344,38 @@ def test_muca_sweeps(tag, op, cps, slope, mode):
         go, no = o.muca_sums()
         gg, ng = b.muca_sums(0)
         assert (go[1:-1] == gg[1:-1]).all() and (no[1:-1] == ng[1:-1]).all()
+
+
+def test_muca_set_failure_leaves_the_slot_as_it_was():
+    """A weight whose backup does not fit the handle's HBM budget is
+    refused with SU2HIP_ENOMEM before anything is freed or installed: the
+    slot stays without a weight, and its sweeps are bitwise those of a
+    handle that was never asked."""
+    import ctypes as C
+    p = make_params(L=(4, 6, 8), **MODELS["h"])
+    lib = abi.load()
+    fp = C.c_uint64()
+    assert lib.su2hip_footprint(C.byref(p), 0, C.byref(fp)) == 0
+    b = Batch(p, hbm_budget=fp.value - 1)  # the fields fit, fields + backup do not
+    ref = Batch(p)
+    for h in (b, ref):
+        h.hot_start(0)
+    m = abi.Muca()
+    m.enabled, m.orderparam, m.mode, m.checks_per_sweep, m.bins = 1, abi.PHISQ, abi.READONLY, 1, 40
+    m.min, m.max, m.wrk_min, m.wrk_max, m.do_acceptance = 0.0, 2.0, 0.0, 2.0, 1
+    pos, W = muca_weight()
+    with pytest.raises(abi.Su2hipError, match="budget"):
+        b.muca_set(0, m, pos, W)
+    assert b.muca_get(0)[0].enabled == 0
+    for h in (b, ref):
+        h.update(2)
+    fb, fr = b.get_fields(0), ref.get_fields(0)
+    for k in FIELDS:
+        if fr.get(k) is not None:
+            assert np.array_equal(fb[k], fr[k]), k
+    assert b.counters(0) == ref.counters(0)
+    b.close()
+    ref.close()
 
 
 def test_muca_slow_recursion_from_loaded_sums():
