@@ -54,20 +54,24 @@ def passes_and_updates(p):
 
 
 def algo_words(kind, p):
-    """Compulsory HBM words per pass / V (each datum read or written once)."""
+    """Compulsory HBM words per pass / V (each datum read or written once).
+    A link is 4 words in FP64 and 3 in FP32 (the packed form: three
+    components, the fourth rebuilt from the unit norm)."""
     from su2higgs_amd import abi
     nh, u1, tr, sg = p.nhiggs, p.u1, p.triplet, p.singlet
+    lw = 1.0 if p.precision == abi.FP64 else 0.75  # link words / 4
     if kind in (abi.K_LINK_HEATBATH, abi.K_LINK_METROPOLIS, abi.K_LINK_OVERRELAX):
-        w = 2 + 8 + 2 + 4 * nh + 0.5 * u1 + 3 * tr  # U_mu(1-p), U_nu, write, phi, alpha_mu(p), Sigma
+        # U_mu(1-p) 2, U_nu 8, the write 2; phi, alpha_mu(p), Sigma
+        w = (2 + 8 + 2) * lw + 4 * nh + 0.5 * u1 + 3 * tr
         if kind != abi.K_LINK_HEATBATH:
-            w += 2  # the old link U_mu(p)
+            w += 2 * lw  # the old link U_mu(p)
         return w
     if kind == abi.K_U1_METROPOLIS:
-        return 3 + 4 * nh + 2 + 0.5
+        return 3 + 4 * nh + 2 * lw + 0.5
     if kind in (abi.K_DOUBLET_OVERRELAX, abi.K_DOUBLET_METROPOLIS):
-        return 4 + 12 + 2 + 3 * u1 + 1.5 * tr + 0.5 * sg
+        return 4 + 12 * lw + 2 + 3 * u1 + 1.5 * tr + 0.5 * sg
     if kind in (abi.K_TRIPLET_OVERRELAX, abi.K_TRIPLET_METROPOLIS):
-        return 3 + 12 + 1.5 + 2 * nh
+        return 3 + 12 * lw + 1.5 + 2 * nh
     if kind in (abi.K_SINGLET_OVERRELAX, abi.K_SINGLET_METROPOLIS):
         return 1 + 0.5 + 2 * nh
     return None
@@ -85,11 +89,71 @@ def load_traffic(kernel, workload):
         return None
 
 
-OBS_THERM = 1000   # sweeps from the cold start before the observables are sampled
-OBS_SAMPLES = 60
+OBS_L = 48          # extent of the observable chains (config-2 couplings and sweep mix)
+OBS_CHAINS = 32     # independent chains per precision, one batch
+OBS_THERM = 1500    # sweeps from the cold start before sampling (the plateau is reached by ~800)
+OBS_SWEEPS = 4000   # sampled sweeps
+OBS_EVERY = 20      # sweeps between measurements
 
 
-def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100, graph=True):
+def observables(chains=OBS_CHAINS, L=OBS_L, therm=OBS_THERM, sweeps=OBS_SWEEPS, every=OBS_EVERY):
+    """FP32-mixed against FP64 on the configuration-2 couplings and sweep mix:
+    `chains` independent chains per precision (their own Philox keys), cold
+    start, `therm` sweeps, then `sweeps` sweeps measured every `every`.  Per
+    observable: the mean over chains of each chain's time average, its error
+    from the chain-to-chain spread (chains are independent, so
+    autocorrelations need no modelling), FP32 - FP64 in standard deviations,
+    and the plateau check (first half of the sampling against the second
+    half, in standard deviations)."""
+    from su2higgs_amd import abi
+    from su2higgs_amd.lattice import Batch
+    out = {"L": L, "chains": chains, "thermalization_sweeps": therm, "sampled_sweeps": sweeps,
+           "every": every}
+    series = {}
+    for prec in (abi.FP32, abi.FP64):
+        ps = [workload_params(prec, L=L) for _ in range(chains)]
+        for k, p in enumerate(ps):
+            p.seed = 7000 + 100 * k + prec
+        b = Batch(ps)
+        for k in range(chains):
+            b.cold_start(k, phi0=1.0)
+        b.use_graph(True)
+        b.update(therm)
+        V = float(abi.volume(ps[0]))
+        rows = []
+        for _ in range(sweeps // every):
+            b.update(every)
+            rows.append(b.measure_all()[:chains])
+        b.close()
+        s_ = np.array(rows)  # [samples, chains, sums]
+        series[prec] = {
+            "plaquette": 1.0 - s_[:, :, abi.SUM["wilson"]] / (V * ps[0].betasu2) / 3.0,
+            "phi2": s_[:, :, abi.SUM["phi2"]] / V,
+            "u1_plaquette": 1.0 - s_[:, :, abi.SUM["u1wilson"]] / (V * ps[0].betau1) / 3.0}
+
+    def mean_err(x):
+        m = x.mean(0)  # per chain
+        return float(m.mean()), float(m.std(ddof=1) / np.sqrt(len(m)))
+
+    for name in ("plaquette", "phi2", "u1_plaquette"):
+        e = {}
+        for prec, tag in ((abi.FP32, "fp32"), (abi.FP64, "fp64")):
+            x = series[prec][name]
+            h = len(x) // 2
+            m, err = mean_err(x)
+            m1, e1 = mean_err(x[:h])
+            m2, e2 = mean_err(x[h:])
+            e[tag] = [m, err]
+            e[tag + "_plateau_sigma"] = (m2 - m1) / float(np.hypot(e1, e2))
+        d = e["fp32"][0] - e["fp64"][0]
+        se = float(np.hypot(e["fp32"][1], e["fp64"][1]))
+        e["fp32_minus_fp64"] = [d, se]
+        e["sigma"] = d / se
+        out[name] = e
+    return out
+
+
+def run_precision(precision, steps, warmup, timing_steps=100, graph=True):
     """Timed region: `steps` compound sweeps replayed as a hipGraph on the
     library's stream (wall clock between stream synchronisations; with
     graph=False launched kernel by kernel from the host).  Kernel
@@ -137,25 +201,6 @@ def run_precision(precision, steps, warmup, measure_obs=True, timing_steps=100, 
         k = abi.KIND_NAMES.index(name)
         if k < abi.K_MEASURE:
             res["kernel_avg_ms"][name], res["kernel_span_ms"][name] = b.time_pass(k, 200)
-    if measure_obs:
-        # both precisions: thermalise to the same sweep count, then sample
-        b.use_graph(graph)
-        done = warmup + steps + ts + 1
-        b.update(max(0, OBS_THERM - done))
-        plaq, phi2, u1 = [], [], []
-        for _ in range(OBS_SAMPLES):
-            b.update(5)
-            s = b.measure(0)
-            plaq.append(1.0 - s[abi.SUM["wilson"]] / (V * p.betasu2) / 3.0)
-            phi2.append(s[abi.SUM["phi2"]] / V)
-            u1.append(1.0 - s[abi.SUM["u1wilson"]] / (V * p.betau1) / 3.0)
-        res["observables"] = {"after_sweeps": max(OBS_THERM, done), "samples": OBS_SAMPLES,
-                              "every": 5}
-        for name, x in (("plaquette", plaq), ("phi2", phi2), ("u1_plaquette", u1)):
-            x = np.array(x)
-            nb = 10
-            blk = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
-            res["observables"][name] = [float(x.mean()), float(blk.std(ddof=1) / np.sqrt(nb))]
     res["p"] = p
     b.close()
     return res
@@ -465,6 +510,89 @@ def oracle_baseline():
             "sample": "oracle restatement, 32^3, 2 sweeps of the GPU mix (%.2f s/sweep)" % per}
 
 
+def muca48_weight(bins=200, lo=0.0, hi=2.0, seed=48):
+    """The multicanonical workload's READONLY weight: `bins` bins on [lo, hi],
+    slopes drawn from U[-5, 5] (seeded), W_0 = 0, W_{i+1} = W_i + s_i * width;
+    pos/W with the reference's virtual end bins (src/multicanonical.c:47-66)."""
+    rng = np.random.default_rng(seed)
+    edges = np.linspace(lo, hi, bins + 1)
+    W = np.concatenate([[0.0], np.cumsum(rng.uniform(-5, 5, bins) * (hi - lo) / bins)])
+    pos = np.concatenate([[lo - 1000.0 * (hi - lo)], edges])
+    Wf = np.concatenate([[W[0]], W])
+    return pos, Wf
+
+
+def profile_set(sweeps=10):
+    """Every kernel kind of the sweep at its workload's size, launched kernel
+    by kernel (no graphs), for kernel profilers: the 96^3 SU(2)xU(1)+Higgs
+    sweep with a measurement after each sweep; the 48^3 SU(2)+adjoint
+    multicanonical sweep (Sigmasq, READONLY weight of 200 bins on [0, 2], one
+    global check per parity half-sweep: k_muca_accept, k_muca_restore); the
+    RK3 flow of a 64^3 SU(2)+Higgs lattice (dt 0.01 to t 0.5, a measurement
+    every 10 steps); the 96^3 sweep's pass kinds at 192^3; the 96-lattice
+    scan.  Each in FP32-mixed and FP64.  Returns the wall time of each part."""
+    from su2higgs_amd import abi
+    from su2higgs_amd.abi import FP32, FP64, make_params
+    from su2higgs_amd.lattice import Batch
+    out = {}
+    for prec in (FP32, FP64):
+        tag = "fp32" if prec == FP32 else "fp64"
+        t0 = time.perf_counter()
+        b = Batch(workload_params(prec))
+        b.cold_start(0, phi0=1.0)
+        for _ in range(sweeps):
+            b.update(1)
+            b.measure(0)
+        b.sync()
+        b.close()
+        out["sweep96_" + tag] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p = make_params(L=(48, 48, 48), nhiggs=0, triplet=1, precision=prec, betasu2=10.0,
+                        msq_triplet=-0.12, b4=0.01, a2=0.0, link_overrelax=4, scalar_sweeps=4,
+                        reunitarize=1 if prec == FP32 else 0, seed=4848)
+        b = Batch(p)
+        b.cold_start(0, sigma0=0.5)
+        m = abi.Muca()
+        m.enabled, m.orderparam, m.mode, m.checks_per_sweep, m.bins = 1, abi.SIGMASQ, abi.READONLY, 1, 200
+        m.min, m.max, m.wrk_min, m.wrk_max, m.do_acceptance = 0.0, 2.0, 0.0, 2.0, 1
+        pos, W = muca48_weight()
+        b.update(50)  # into the weight's range
+        b.muca_set(0, m, pos, W)
+        for _ in range(sweeps):
+            b.update(1)
+            b.measure(0)
+        b.sync()
+        b.close()
+        out["muca48_" + tag] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p = make_params(L=(64, 64, 64), nhiggs=1, precision=prec, betasu2=9.0, msq_phi=-0.2,
+                        lambda_phi=0.02, link_overrelax=4, reunitarize=1 if prec == FP32 else 0,
+                        seed=6464)
+        b = Batch(p)
+        b.cold_start(0, phi0=1.0)
+        b.update(20)
+        b.flow(0, abi.FLOW_RK3, 0.01, 0.5, 10)
+        b.close()
+        out["flow64_" + tag] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b = Batch(workload_params(prec, L=192))
+        b.cold_start(0, phi0=1.0)
+        b.update(2)
+        b.sync()
+        b.close()
+        out["sweep192_" + tag] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ps = scan_params(prec)
+        b = Batch(ps)
+        for i in range(len(ps)):
+            b.hot_start(i)
+        b.update(3)
+        b.sync()
+        b.close()
+        out["scan96_" + tag] = time.perf_counter() - t0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,6 +607,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the timed sweeps kernel by kernel instead of replaying a "
                          "captured hipGraph (for tools that do not follow graph launches)")
+    ap.add_argument("--profile-set", action="store_true",
+                    help="only launch every kernel kind at its workload's size, kernel by kernel "
+                         "(for rocprofv3)")
     ap.add_argument("--probe-bytes", type=int, default=0,
                     help="only run the streaming copy kernel over this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -495,14 +626,16 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.probe_bytes, "ms": ms,
                           "GBps": 2 * args.probe_bytes / (ms * 1e-3) / 1e9}))
         return
+    if args.profile_set:
+        print(json.dumps({"profile_set_wall_s": profile_set()}))
+        return
     obs = not args.no_obs
     log("[bench] FP32-mixed 96^3, %d warmup + %d timed sweeps" % (args.warmup, args.steps))
-    r32 = run_precision(abi.FP32, args.steps, args.warmup, measure_obs=obs, graph=not args.no_graph)
+    r32 = run_precision(abi.FP32, args.steps, args.warmup, graph=not args.no_graph)
     log("[bench] fp32 %.3f ms/sweep, %.3e site-updates/s" % (r32["ms_per_sweep"], r32["site_updates_per_s"]))
     s64 = max(1, min(args.steps, 200))
     log("[bench] FP64 plain 96^3, %d timed sweeps" % s64)
-    r64 = run_precision(abi.FP64, s64, max(1, args.warmup // 2), measure_obs=obs,
-                        graph=not args.no_graph)
+    r64 = run_precision(abi.FP64, s64, max(1, args.warmup // 2), graph=not args.no_graph)
     log("[bench] fp64 %.3f ms/sweep, %.3e site-updates/s" % (r64["ms_per_sweep"], r64["site_updates_per_s"]))
     roof = roofline(r32)
     roof64 = roofline(r64)
@@ -533,7 +666,12 @@ def main():
         log("[bench] CPU baseline")
         cpu = cpu_baseline()
         log("[bench] cpu %.3e site-updates/s (%s)" % (cpu["value"], cpu["kind"]))
-    o32, o64 = r32.get("observables", {}), r64.get("observables", {})
+    obs_res = None
+    if obs:
+        log("[bench] observables: %d chains per precision, %d^3" % (OBS_CHAINS, OBS_L))
+        obs_res = observables()
+        log("[bench] fp32 - fp64: " + ", ".join("%s %.2f sigma" % (k, obs_res[k]["sigma"])
+                                                 for k in ("plaquette", "phi2", "u1_plaquette")))
     out = {
         "metric": "site-updates/s/GPU, FP32-mixed batched vs FP64 plain; p50/p99 scan latency",
         "value": r32["site_updates_per_s"],
@@ -562,9 +700,7 @@ def main():
         "batched_scan": batched,
         "serving": served,
         "multicanonical": muca,
-        "observables": {"fp32": o32, "fp64": o64, "fp32_minus_fp64_sigma": {
-            k: (o32[k][0] - o64[k][0]) / float(np.hypot(o32[k][1], o64[k][1]))
-            for k in ("plaquette", "phi2", "u1_plaquette") if k in o32 and k in o64}},
+        "observables": obs_res,
         "roofline": roof,
         "roofline_fp64": roof64,
         "roofline_beyond_cache": roof_big,

@@ -19,6 +19,7 @@ OBJDIR = os.path.join(ROOT, "_build")
 
 ARCH = os.environ.get("SU2HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 # no packed FP32 instructions in any HIP code object
 NOPK = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIPFLAGS = ["-O3", "--offload-arch=" + ARCH, "-fPIC", "-std=c++17", "-Wall",
@@ -64,7 +65,9 @@ def build(verbose=False):
                 print(o)
     lib = os.path.join(LIBDIR, "libsu2hip.so")
     tmp = lib + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
+    # roctx ranges on the host entry points (rocprofiler-sdk-roctx)
+    _run([HIPCC, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L" + os.path.join(ROCM, "lib"), "-lrocprofiler-sdk-roctx", "-lm"])
     os.replace(tmp, lib)
     main_c = os.path.join(HOST, "su2hip_main.c")
     if os.path.exists(main_c):

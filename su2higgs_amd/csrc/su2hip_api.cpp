@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "su2hip_internal.h"
 
 using namespace su2;
@@ -133,12 +135,30 @@ struct su2hip_batch {
 // ---------------------------------------------------------------------------
 // helpers
 
+// A roctx range around a host entry point (rocprofv3 --marker-trace shows
+// it; without a profiler the calls do nothing).  Sweeps, flows and
+// measurements are enqueued asynchronously, so a range covers the host's
+// enqueueing and any waits inside the call, not the kernels themselves.
+struct TraceRange {
+  explicit TraceRange(const char *what) { roctxRangePushA(what); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
+
 static size_t wsize(const su2hip_params &p) { return p.precision == SU2HIP_FP64 ? 8 : 4; }
 static int vol_of(const su2hip_params &p) { return p.L[0] * p.L[1] * p.L[2]; }
+// words of the links of a lattice with Vh sites per parity: 3 directions x 2
+// parities x the words of one stored link (4 in FP64, 3 in FP32: the packed
+// form of ldU / stU)
+static size_t link_words(size_t Vh, bool fp64) { return 6 * Vh * (fp64 ? 4 : 3); }
+static size_t link_words(const su2hip_params &p) {
+  return link_words((size_t)vol_of(p) / 2, p.precision == SU2HIP_FP64);
+}
 
 static size_t field_words(const su2hip_params &p) {
   size_t Vh = (size_t)vol_of(p) / 2;
-  size_t w = 24 * Vh;
+  size_t w = link_words(p);
   if (p.u1) w += 6 * Vh;
   if (p.nhiggs) w += 8 * Vh * p.nhiggs;
   if (p.triplet) w += 6 * Vh;
@@ -148,7 +168,7 @@ static size_t field_words(const su2hip_params &p) {
 
 static size_t flow_words(const su2hip_params &p) {
   size_t Vh = (size_t)vol_of(p) / 2;
-  size_t w = 48 * Vh + 18 * Vh;
+  size_t w = 2 * link_words(p) + 18 * Vh;
   if (p.triplet) w += 12 * Vh + 6 * Vh;
   return w;
 }
@@ -255,7 +275,7 @@ static void fill_desc(Slot &s) {
     off += align256(words * w);
     return q;
   };
-  L.U = take(24 * Vh);
+  L.U = take(link_words(p));
   L.A = p.u1 ? take(6 * Vh) : nullptr;
   L.F = p.nhiggs ? take(8 * Vh) : nullptr;
   L.F2 = p.nhiggs == 2 ? take(8 * Vh) : nullptr;
@@ -303,7 +323,7 @@ static void fill_desc(Slot &s) {
 
 static size_t slot_bytes(const su2hip_params &p) {
   size_t Vh = (size_t)vol_of(p) / 2, w = wsize(p);
-  size_t b = align256(24 * Vh * w);
+  size_t b = align256(link_words(p) * w);
   if (p.u1) b += align256(6 * Vh * w);
   if (p.nhiggs) b += align256(8 * Vh * w);
   if (p.nhiggs == 2) b += align256(8 * Vh * w);
@@ -984,6 +1004,7 @@ int su2hip_slot_active(su2hip_batch_t *b, int slot, int *active) {
 
 int su2hip_set_fields(su2hip_batch_t *b, int slot, const double *su2link, const double *u1link,
                       const double *doublet, const double *triplet, const double *singlet) {
+  TraceRange tr_("su2hip_set_fields");
   int rc = check_slot(b, slot);
   if (rc) return rc;
   if (b->slots[slot].fj.active)
@@ -1013,6 +1034,7 @@ int su2hip_set_fields(su2hip_batch_t *b, int slot, const double *su2link, const 
 
 int su2hip_get_fields(su2hip_batch_t *b, int slot, double *su2link, double *u1link,
                       double *doublet, double *triplet, double *singlet) {
+  TraceRange tr_("su2hip_get_fields");
   int rc = check_slot(b, slot);
   if (rc) return rc;
   rc = prepare(b);
@@ -1109,6 +1131,7 @@ int su2hip_zprofile(su2hip_batch_t *b, int slot, int dir, double *out, int max_r
 }
 
 int su2hip_update(su2hip_batch_t *b, int n_sweeps) {
+  TraceRange tr_("su2hip_update");
   if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
   if (n_sweeps < 0) return fail(SU2HIP_EINVAL, "n_sweeps < 0");
   int rc = prepare(b);
@@ -1336,6 +1359,7 @@ int su2hip_rng_set(su2hip_batch_t *b, int slot, uint64_t seed, uint64_t pass) {
 }
 
 int su2hip_measure_all(su2hip_batch_t *b, double *sums) {
+  TraceRange tr_("su2hip_measure");
   if (!b || !sums) return fail(SU2HIP_EINVAL, "NULL argument");
   int rc = prepare(b);
   if (rc) return rc;
@@ -1407,6 +1431,7 @@ int su2hip_measure_columns(const su2hip_params *p, const double *s, double weigh
 
 int su2hip_muca_set(su2hip_batch_t *b, int slot, const su2hip_muca *m, const double *pos,
                     const double *W) {
+  TraceRange tr_("su2hip_muca_set");
   int rc = check_slot(b, slot);
   if (rc) return rc;
   if (!m) return fail(SU2HIP_EINVAL, "NULL muca");
@@ -1562,6 +1587,7 @@ int su2hip_muca_get(su2hip_batch_t *b, int slot, su2hip_muca *m, double *pos, do
 
 int su2hip_correlators(su2hip_batch_t *b, int slot, int dir, double *out, int max_dist,
                        int *ndist) {
+  TraceRange tr_("su2hip_correlators");
   int rc = check_slot(b, slot);
   if (rc) return rc;
   const su2hip_params &p = b->slots[slot].p;
@@ -1654,7 +1680,8 @@ int su2hip_blocked_correlators(su2hip_batch_t *b, int slot, int levels, double *
   if (rc) return rc;
   LatDesc fine = b->slots[slot].hd;
   size_t w = wsize(p), Vh1 = (size_t)fine.Vh / 4;  // first level: a quarter of the sites
-  size_t words = align256(24 * Vh1 * w) + align256(6 * Vh1 * w);
+  size_t lw = link_words(Vh1, p.precision == SU2HIP_FP64);
+  size_t words = align256(lw * w) + align256(6 * Vh1 * w);
   char *mem = nullptr;
   LatDesc *ddesc = nullptr;
   if (hipMalloc(&mem, 2 * words) != hipSuccess) return fail(SU2HIP_ENOMEM, "blocked fields");
@@ -1666,7 +1693,7 @@ int su2hip_blocked_correlators(su2hip_batch_t *b, int slot, int levels, double *
   LatDesc base = fine;
   for (int k = 0; k < levels && e == hipSuccess; k++) {
     char *m = mem + (size_t)(k & 1) * words;
-    LatDesc B = blocked_desc(base, bmask, m, m + align256(24 * Vh1 * w));
+    LatDesc B = blocked_desc(base, bmask, m, m + align256(lw * w));
     e = launch_block(ctx_of(b), base, B, bmask);
     if (e == hipSuccess) e = hipMemcpyAsync(ddesc, &B, sizeof(LatDesc), hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -1857,7 +1884,7 @@ static int flow_enqueue(su2hip_batch *b, int slot, int scheme, double dt, double
     q += align256(words * w);
     return r;
   };
-  void *Ub[2] = {take(24 * Vh), take(24 * Vh)};
+  void *Ub[2] = {take(link_words(p)), take(link_words(p))};
   void *Z = take(18 * Vh);
   void *Gb[2] = {nullptr, nullptr}, *ZG = nullptr;
   if (p.triplet) {
@@ -1919,7 +1946,7 @@ static int flow_enqueue(su2hip_batch *b, int slot, int scheme, double dt, double
   j.active = true;
   hipError_t e = hipMemcpyAsync(j.ddesc, hd.data(), sizeof(LatDesc) * hd.size(), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(j.dblk, bm.data(), sizeof(BlockMap) * nb, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(Ub[0], s.hd.U, 24 * Vh * w, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(Ub[0], s.hd.U, link_words(p) * w, hipMemcpyDeviceToDevice, st);
   if (e == hipSuccess && p.triplet) e = hipMemcpyAsync(Gb[0], s.hd.G, 6 * Vh * w, hipMemcpyDeviceToDevice, st);
   int cur = 0;
   size_t r = 0;
@@ -1999,6 +2026,7 @@ static int flow_args(su2hip_batch *b, int slot, int scheme, double dt, int meas_
 
 int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max,
                 int meas_interval, double *rows, int max_rows, int *nrows) {
+  TraceRange tr_("su2hip_flow");
   int rc = flow_args(b, slot, scheme, dt, meas_interval);
   if (rc) return rc;
   rc = prepare(b);
@@ -2013,6 +2041,7 @@ int su2hip_flow(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max
 
 int su2hip_flow_start(su2hip_batch_t *b, int slot, int scheme, double dt, double t_max,
                       int meas_interval) {
+  TraceRange tr_("su2hip_flow_start");
   int rc = flow_args(b, slot, scheme, dt, meas_interval);
   if (rc) return rc;
   rc = prepare(b);
@@ -2047,6 +2076,7 @@ int su2hip_flow_poll(su2hip_batch_t *b, int slot, int *done) {
 }
 
 int su2hip_flow_finish(su2hip_batch_t *b, int slot, double *rows, int max_rows, int *nrows) {
+  TraceRange tr_("su2hip_flow_finish");
   int rc = check_slot(b, slot);
   if (rc) return rc;
   FlowJob &j = b->slots[slot].fj;
@@ -2167,6 +2197,7 @@ int su2hip_get_chunk(su2hip_batch_t *b, int kind, int *chunk) {
 }
 
 int su2hip_autotune(su2hip_batch_t *b, int repeats, double *table) {
+  TraceRange tr_("su2hip_autotune");
   if (!b || repeats < 1) return fail(SU2HIP_EINVAL, "NULL handle or repeats < 1");
   int rc = prepare(b);
   if (rc) return rc;

@@ -565,7 +565,7 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       R = qscale(R, frsqrt(qdot(R, R)));
     }
     Q<T> nu = qmul(old, R);
-    if (reunit) nu = normalise(nu);
+    (void)reunit;  // FP32 links are renormalised by their packed storage
     Q<T> du{nu.a - old.a, nu.b - old.b, nu.c - old.c, nu.d - old.d};
     T diff = (T)2 * qmul(du, V).a;
     if constexpr ((M & M_TR) != 0) {
@@ -592,7 +592,7 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       w = qconj(qscale(V, frsqrt(n2)));
     }
     Q<T> nu = qmul(qmul(w, qconj(old)), w);
-    if (reunit) nu = normalise(nu);
+    (void)reunit;  // FP32 links are renormalised by their packed storage
     if constexpr ((M & M_TR) != 0) {
       V3<T> a1 = ldG(G, Vh, par, s), a2 = G_at(L, G, shift(L, x, mu, +1));
       T diff = hop_triplet(old, a1, a2) - hop_triplet(nu, a1, a2);
@@ -1946,11 +1946,9 @@ __device__ __forceinline__ void *field_ptr(const LatDesc &L, int field) {
   }
 }
 
+// index of component k of a non-link field (links go through ldU / stU,
+// whose FP32 form is not one word per component)
 __device__ __forceinline__ size_t soa_index(int field, int Vh, int par, int s, int k) {
-  if (field == 0) {
-    int d = k >> 2, c = k & 3;
-    return ((size_t)(d * 2 + par) * Vh + s) * 4 + c;
-  }
   if (field == 2 || field == 5) return ((size_t)par * Vh + s) * 4 + k;
   return (size_t)(k * 2 + par) * Vh + s;
 }
@@ -1963,6 +1961,19 @@ __global__ void k_to_device(LatDesc L, int field, const double *__restrict__ h) 
   int n = field_dofs(field);
   for (int par = 0; par < 2; par++) {
     size_t i = (size_t)par * L.Vh + s;
+    if (field == 0) {
+      for (int d = 0; d < 3; d++) {
+        const double *q = h + i * 12 + d * 4;
+        Q<T> u{(T)q[0], (T)q[1], (T)q[2], (T)q[3]};
+        // the packed FP32 form holds unit links only
+        if constexpr (sizeof(T) == 4) {
+          double n = 1.0 / ::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+          u = Q<T>{(T)(q[0] * n), (T)(q[1] * n), (T)(q[2] * n), (T)(q[3] * n)};
+        }
+        stU((T *)dst, L.Vh, d, par, s, u);
+      }
+      continue;
+    }
     for (int k = 0; k < n; k++) ((T *)dst)[soa_index(field, L.Vh, par, s, k)] = (T)h[i * n + k];
   }
 }
@@ -1975,6 +1986,17 @@ __global__ void k_to_host(LatDesc L, int field, double *__restrict__ h) {
   int n = field_dofs(field);
   for (int par = 0; par < 2; par++) {
     size_t i = (size_t)par * L.Vh + s;
+    if (field == 0) {
+      for (int d = 0; d < 3; d++) {
+        Q<T> q = ldU((const T *)src, L.Vh, d, par, s);
+        double *o = h + i * 12 + d * 4;
+        o[0] = (double)q.a;
+        o[1] = (double)q.b;
+        o[2] = (double)q.c;
+        o[3] = (double)q.d;
+      }
+      continue;
+    }
     for (int k = 0; k < n; k++) h[i * n + k] = (double)((const T *)src)[soa_index(field, L.Vh, par, s, k)];
   }
 }
@@ -2021,10 +2043,11 @@ __device__ __forceinline__ V3<T> triplet_force(const LatDesc &L, const T *U, con
   return V3<T>{-(res.x + k * g.x), -(res.y + k * g.y), -(res.z + k * g.z)};
 }
 
-// exp(i z.sigma) v.  FP32 renormalises the product: without it the link
-// norm random-walks over the 1200 stages of an RK3 flow to t = 4 and the
-// flowed plaquette drifts by 1e-2 .. 1e-1 of 1 - P; FP64 keeps the
-// reference's arithmetic (src/gradflow.c:361-401).
+// exp(i z.sigma) v.  The FP32 flow's links are renormalised by their
+// packed storage at every stage: unrenormalised, the link norm would
+// random-walk over the 1200 stages of an RK3 flow to t = 4 and the flowed
+// 1 - P drift by 1e-2 .. 1e-1.  FP64 keeps the reference's arithmetic
+// (src/gradflow.c:361-401).
 template <typename T>
 __device__ __forceinline__ Q<T> expmul(const V3<T> &z, const Q<T> &v) {
   T mod = z.x * z.x + z.y * z.y + z.z * z.z;
@@ -2032,7 +2055,6 @@ __device__ __forceinline__ Q<T> expmul(const V3<T> &z, const Q<T> &v) {
   mod = msqrt(mod);
   T sn = msin(mod) / mod;
   Q<T> u{mcos(mod), sn * z.x, sn * z.y, sn * z.z};
-  if constexpr (sizeof(T) == 4) return normalise(qmul(u, v));
   return qmul(u, v);
 }
 

@@ -14,6 +14,9 @@ pytestmark = pytest.mark.gpu
 
 
 def test_fp32_roundtrip_rounds_to_float():
+    """Scalar fields and angles come back as their float rounding.  Links
+    are stored in the packed 12-byte form: they come back as the unit
+    quaternion U/|U| to a few units in the last place."""
     p = make_params(L=(8, 8, 8), nhiggs=1, u1=1, precision=FP32)
     b = Batch(p)
     rng = np.random.default_rng(1)
@@ -23,8 +26,48 @@ def test_fp32_roundtrip_rounds_to_float():
     f["u1link"][:] = rng.normal(size=f["u1link"].shape)
     b.set_fields(0, f)
     g = b.get_fields(0)
-    for k in ("su2link", "doublet", "u1link"):
+    for k in ("doublet", "u1link"):
         assert np.array_equal(g[k], f[k].astype(np.float32).astype(np.float64))
+    unit = f["su2link"] / np.linalg.norm(f["su2link"], axis=-1, keepdims=True)
+    assert np.abs(g["su2link"] - unit).max() < 4e-7
+    assert np.abs(np.linalg.norm(g["su2link"], axis=-1) - 1).max() < 4e-7
+
+
+def test_fp32_packed_links_every_position_and_sign():
+    """The packed FP32 link keeps the three components other than the
+    largest one, whose position and sign ride in the lowest mantissa bits:
+    unit quaternions whose largest component sits in each of the four
+    places with either sign, ties among equal magnitudes, and the axes,
+    come back to a few units in the last place with the sign of every
+    component kept."""
+    p = make_params(L=(4, 4, 4), nhiggs=1, precision=FP32)
+    V = abi.volume(p)
+    rng = np.random.default_rng(7)
+    q = rng.normal(size=(V, 3, 4))
+    fixed = np.array([[1, 0, 0, 0], [-1, 0, 0, 0], [0, 1, 0, 0], [0, 0, -1, 0], [0, 0, 0, 1],
+                      [0, 0, 0, -1], [.5, .5, .5, .5], [-.5, .5, -.5, .5], [.5, -.5, .5, -.5],
+                      [0, -.5, -.5, -.5 * np.sqrt(2)], [np.sqrt(.5), -np.sqrt(.5), 0, 0],
+                      [1e-30, 1, -1e-30, 0]], dtype=float)
+    q.reshape(-1, 4)[: len(fixed)] = fixed
+    # the largest component in each position with each sign, many times over
+    big = rng.integers(0, 4, size=(V, 3))
+    sign = rng.choice([-1.0, 1.0], size=(V, 3))
+    rest = q.reshape(-1, 4)[len(fixed):]
+    b_, s_ = big.reshape(-1)[len(fixed):], sign.reshape(-1)[len(fixed):]
+    rest[np.arange(len(rest)), b_] = s_ * (np.abs(rest).max(1) + 0.1)
+    q /= np.linalg.norm(q, axis=-1, keepdims=True)
+    f = abi.field_arrays(p)
+    f["su2link"][:] = q
+    b = Batch(p)
+    b.set_fields(0, f)
+    g = b.get_fields(0)["su2link"]
+    assert np.abs(g - q).max() < 4e-7
+    nz = np.abs(q) > 1e-6
+    assert (np.sign(g[nz]) == np.sign(q[nz])).all()
+    # a second round trip changes nothing: the packed form is a fixed point
+    f["su2link"][:] = g
+    b.set_fields(0, f)
+    assert np.abs(b.get_fields(0)["su2link"] - g).max() < 2.5e-7
 
 
 def test_fp32_links_stay_unitary_and_angles_wrapped():

@@ -514,10 +514,14 @@ def test_wall_start_and_zprofile(tag, precision):
     b = Batch(p)
     b.wall_start(0)
     fo, fg = o.get_fields(), b.get_fields(0)
-    tol = 1e-13 if precision == FP64 else 1e-7  # sqrt(1 - u^2) with a fused multiply-add
+    # FP32: sqrt(1 - u^2) with a fused multiply-add; links also through the
+    # packed 12-byte form (renormalised, lowest mantissa bits carry the
+    # position and sign of the rebuilt component: a few units in the last place)
+    tol = 1e-13 if precision == FP64 else 1e-7
     for k in FIELDS:
         if fg.get(k) is not None and k != "u1link":
-            assert np.abs(fo[k] - fg[k]).max() <= tol, k
+            t = 4e-7 if (k == "su2link" and precision == abi.FP32) else tol
+            assert np.abs(fo[k] - fg[k]).max() <= t, k
     if precision == abi.FP32:
         o.set_fields({k: v for k, v in fg.items() if v is not None})
     for d in (-1, 0, 2):

@@ -117,3 +117,72 @@ def test_realtime_stream_reports_latency():
     assert all(abs(r.t_arrival - a) < 1e-12 for r, a in zip(res, t))
     assert st["makespan"] >= t[-1]
     assert st["flows"] == 2
+
+
+def replay_alone(c, q):
+    """The chain of request q alone in its own handle, from the same cold
+    start and Philox key."""
+    from su2higgs_amd.lattice import Batch
+    p = make_params(L=(q.L,) * 3, precision=c.tmpl.precision, reunitarize=c.tmpl.reunitarize,
+                    link_overrelax=c.tmpl.link_overrelax, scalar_sweeps=c.tmpl.scalar_sweeps,
+                    betasu2=q.betasu2, msq_phi=q.msq_phi, lambda_phi=q.lambda_phi, seed=q.seed)
+    b = Batch(p)
+    b.cold_start(0, phi0=c.phi0, sigma0=c.sigma0, singlet0=c.singlet0)
+    b.use_graph(True)
+    b.update(int(q.sweeps))
+    return b
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32])
+def test_scan_serving_at_its_own_sizes(precision):
+    """The serving workload at its own sizes: the seeded Poisson stream at
+    20 requests/s of L in {24..64}, betaG in [6,12], msq in [-0.3,-0.1],
+    lambda in [0.005,0.03] and 100..1000 sweeps, whose 10th request is 64^3
+    with the RK3 flow dt = 0.01 to t = 4.0 (400 steps, 1200 stages).  Every
+    request completes.  The flowed request's result equals its chain run
+    alone in its own handle and flowed there, bit for bit (measure sums
+    after the sweeps and the flowed sums at t = 4)."""
+    reqs = serve.stream(2026, 20.0, 10, flow_every=10)
+    Ls = {int(q.L) for q in reqs}
+    assert reqs[9].flow == 1 and reqs[9].L == 64 and min(Ls) < 64
+    c = serve.config(realtime=1, tmpl_precision=precision,
+                     tmpl_reunitarize=1 if precision == abi.FP32 else 0)
+    assert (c.flow_scheme, c.flow_dt, c.flow_t_max) == (abi.FLOW_RK3, 0.01, 4.0)
+    res, st = serve.serve(c, reqs)
+    assert st["completed"] == 10 and st["failed"] == 0 and st["flows"] == 1
+    assert 0 < st["p50"] <= st["p99"]
+    r, q = res[9], reqs[9]
+    assert abs(r.flow_t - 4.0) < 1e-12
+    b = replay_alone(c, q)
+    rows = b.flow(0, c.flow_scheme, c.flow_dt, c.flow_t_max, 100)
+    assert np.array_equal(np.array(r.sums), b.measure(0))
+    assert abs(rows[-1][0] - 4.0) < 1e-12
+    assert np.array_equal(np.array(r.flow_sums), rows[-1][1:])
+    # the flow smooths: the Wilson action falls from each whole flow time to
+    # the next (the loop clips its last step at t_max, src/gradflow.c:94-96,
+    # so t = 4 may be measured twice, the second time after a step of ~1e-13)
+    t = rows[:, 0]
+    first = np.concatenate([[True], np.diff(t) > 1e-6])
+    w = rows[first, 1 + abi.SUM["wilson"]]
+    assert len(w) == 5 and (np.diff(w) < 0).all()
+    b.close()
+
+
+def test_flow_at_64_cubed_matches_the_oracle():
+    """The FP64 RK3 flow of a 64^3 configuration against the oracle's on the
+    same fields: every row of the first 5 steps (measured after each step)
+    to 1e-12 of the action's scale (the oracle's 400 steps would take
+    minutes; the kernel is the same at every step)."""
+    q = serve.stream(2026, 20.0, 10, flow_every=10)[9]
+    c = serve.config(tmpl_precision=abi.FP64, tmpl_reunitarize=0)
+    b = replay_alone(c, q)
+    rows = b.flow(0, abi.FLOW_RK3, 0.01, 0.05, 1)
+    p = make_params(L=(64,) * 3, betasu2=q.betasu2, msq_phi=q.msq_phi, lambda_phi=q.lambda_phi,
+                    seed=q.seed)
+    o = orc.Oracle(p)
+    o.set_fields(b.get_fields(0))
+    ro = o.flow(abi.FLOW_RK3, 0.01, 0.05, 1)
+    assert ro.shape == rows.shape == (6, 1 + abi.NSUMS)
+    scale = np.abs(ro[:, 1:]).max()
+    assert np.abs(ro - rows).max() <= 1e-12 * scale
+    b.close()
