@@ -1,0 +1,26 @@
+"""Write the FP64 reference values FP64_REF_48 of tests/test_gpu_fp32.py
+(needs the GPU, about 5 minutes):
+
+    python tests/fp64_reference_48.py
+
+128 independent FP64 chains (Philox keys 64001 + 17 k) of the 48^3
+SU(2)xU(1)+Higgs theory at the single-lattice workload's couplings, the
+lean sweep mix of the test, cold start, 8000 sweeps discarded, then 32000
+sweeps measured every 10.  Prints each observable's mean over chains of
+the chain's time average and its error from the chain-to-chain spread.
+The FP64 path is the one pinned site by site to the oracle."""
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from test_gpu_fp32 import chain_mean, chain_series  # noqa: E402
+from su2higgs_amd.abi import FP64  # noqa: E402
+
+if __name__ == "__main__":
+    x = chain_series(FP64, 128, 8000, 32000, 10, seed0=64001)
+    out = {name: [float(v) for v in chain_mean(series)] for name, series in x.items()}
+    print(json.dumps(out))

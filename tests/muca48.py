@@ -2,8 +2,10 @@
 and its comparison with the reference's own chains recorded in
 tests/golden/muca48_reference.npz (order parameter Sigmasq, READONLY weight,
 one global check per parity half-sweep, 500 + 2000 sweeps)."""
+import ctypes as C
 import os
 import subprocess
+import sys
 import tempfile
 import time
 
@@ -75,3 +77,87 @@ def compare(ours, ref):
             "hist_tv_distance": float(0.5 * np.abs(h_ref - h_us).sum()),
             "hist_edges": [float(edges[0]), float(edges[-1]), len(edges) - 1],
             "ref_chains": int(ro.shape[0]), "sweeps": int(len(ours["orderparam"]))}
+
+
+def run_batch(fp32=True, chains=16, seed0=480000):
+    """The fixture's run as `chains` independent chains (Philox keys seed0 +
+    k) in one handle, every kernel launch covering all of them: the config
+    parsed by su2hip_config_read, the weight by su2hip_weight_load
+    (READONLY), cold start, n_thermalize sweeps, then `iterations` sweeps
+    with the order parameter measured before each, in the driver's
+    measure-then-update order (src/main.c:302-353).  Returns the order
+    parameter as [chains, iterations] and the wall time."""
+    sys.path.insert(0, ROOT)
+    from su2higgs_amd import abi
+    from su2higgs_amd.lattice import Batch
+    ref = reference()
+    lib = abi.load()
+    with tempfile.TemporaryDirectory() as d:
+        cpath, wpath = os.path.join(d, "config"), os.path.join(d, "weight")
+        with open(cpath, "wb") as f:
+            f.write(ref["config"].tobytes())
+        with open(wpath, "wb") as f:
+            f.write(ref["weight"].tobytes())
+        cfg = abi.RunConfig()
+        abi.check(lib.su2hip_config_read(cpath.encode(), b"triplet", C.byref(cfg)))
+        m = abi.Muca()
+        C.memmove(C.byref(m), C.byref(cfg.muca), C.sizeof(abi.Muca))
+        n = m.bins + 2
+        pos, W = np.zeros(n), np.zeros(n)
+        abi.check(lib.su2hip_weight_load(wpath.encode(), C.byref(m), abi.ptr(pos), abi.ptr(W),
+                                         m.bins))
+    p = cfg.p
+    p.precision = abi.FP32 if fp32 else abi.FP64
+    p.reunitarize = 1 if fp32 else 0
+    ps = []
+    for k in range(chains):
+        q = abi.Params()
+        C.memmove(C.byref(q), C.byref(p), C.sizeof(abi.Params))
+        q.seed = seed0 + k
+        ps.append(q)
+    t0 = time.perf_counter()
+    b = Batch(ps)
+    for k in range(chains):
+        b.cold_start(k, phi0=cfg.phi0, sigma0=cfg.sigma0, singlet0=cfg.singlet0)
+        b.muca_set(k, m, pos, W)
+    b.use_graph(True)
+    b.update(int(cfg.n_thermalize))
+    V = float(abi.volume(p))
+    op = np.zeros((chains, int(cfg.iterations)))
+    for it in range(int(cfg.iterations)):
+        op[:, it] = b.measure_all()[:chains, abi.SUM["sigma2"]] / V
+        b.update(1)
+    b.sync()
+    wall = time.perf_counter() - t0
+    b.close()
+    return op, wall
+
+
+def tv(a, b, edges):
+    """Total-variation distance of the histograms of two samples."""
+    ha = np.histogram(np.ravel(a), edges)[0] / np.size(a)
+    hb = np.histogram(np.ravel(b), edges)[0] / np.size(b)
+    return float(0.5 * np.abs(ha - hb).sum())
+
+
+def compare_chains(ours, ref):
+    """Our chains (pooled) against the reference's: the order parameter's
+    mean with its error from our chain-to-chain spread, and the reference's
+    from its blocked chains; z of the difference; the histogram distance
+    between the two pools and, as its noise scale, the distance between each
+    reference chain and the pool of the other reference chains."""
+    ro = ref["orderparam"]
+    means = np.array([blocked(c)[0] for c in ro])
+    errs = np.array([blocked(c)[1] for c in ro])
+    m_ref = float(means.mean())
+    e_ref = float(np.sqrt((errs ** 2).sum()) / len(errs))
+    cm = ours.mean(1)
+    m_us, e_us = float(cm.mean()), float(cm.std(ddof=1) / np.sqrt(len(cm)))
+    lo, hi = min(ro.min(), ours.min()), max(ro.max(), ours.max())
+    edges = np.linspace(lo, hi, 41)
+    loo = [tv(ro[k], np.delete(ro, k, axis=0), edges) for k in range(len(ro))]
+    return {"chains": int(ours.shape[0]), "sweeps": int(ours.shape[1]), "mean": m_us, "err": e_us,
+            "ref_mean": m_ref, "ref_err": e_ref, "z": (m_us - m_ref) / float(np.hypot(e_us, e_ref)),
+            "hist_tv_distance": tv(ours, ro, edges), "ref_leave_one_out_tv": loo,
+            "spread_ratio": float(ours.std(axis=1).mean() / ro.std(axis=1).mean()),
+            "ref_chains": int(ro.shape[0])}

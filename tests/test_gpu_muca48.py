@@ -23,3 +23,22 @@ def test_multicanonical_48_matches_reference_statistics(fp32):
     print(c)
     assert abs(c["z"]) < 4, c
     assert 0.5 < c["spread_ratio"] < 2.0, c
+
+
+@pytest.mark.parametrize("fp32", [True, False], ids=["fp32", "fp64"])
+def test_multicanonical_48_many_chains_match_reference_histogram(fp32):
+    """16 chains of the same run in one handle (every launch covers all of
+    them), pooled, against the reference's four chains pooled: the order
+    parameter's mean within 3.5 combined standard errors (ours from the
+    chain-to-chain spread, the reference's blocked), the same per-chain
+    spread, and a histogram distance no larger than the largest distance
+    between one reference chain and the other three (the reference's own
+    chain-to-chain noise, which a biased distribution would exceed)."""
+    if not os.path.exists(muca48.FIXTURE):
+        pytest.skip("fixture absent")
+    ours, _ = muca48.run_batch(fp32=fp32, chains=16, seed0=480000 if fp32 else 490000)
+    c = muca48.compare_chains(ours, muca48.reference())
+    print(c)
+    assert abs(c["z"]) < 3.5, c
+    assert 0.8 < c["spread_ratio"] < 1.25, c
+    assert c["hist_tv_distance"] < max(c["ref_leave_one_out_tv"]), c
