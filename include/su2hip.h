@@ -299,6 +299,18 @@ int su2hip_rng_set(su2hip_batch_t *b, int slot, uint64_t seed, uint64_t pass);
 int su2hip_measure(su2hip_batch_t *b, int slot, double *sums);
 /* Sums of every slot at once: sums[max_slots][SU2HIP_NSUMS] (empty slots 0). */
 int su2hip_measure_all(su2hip_batch_t *b, double *sums);
+/* The sums of su2hip_measure for one lattice, allocating nothing: no
+ * reference counterpart; for callers that cannot allocate device memory
+ * (HIP graph capture, tight serving loops).  sums_device is device memory
+ * of SU2HIP_NSUMS doubles; the kernels are queued on the handle's stream
+ * after the work already there, and the call returns without waiting (the
+ * values are bitwise those of su2hip_measure).  The block partial sums go
+ * to a scratch array the library holds on every GPU it runs on: lattices of
+ * at most 2 * 256 * SU2HIP_NOALLOC_MAX_BLOCKS sites (SU2HIP_ERANGE beyond).
+ * The handle's device must be current.  Calls on one GPU share that scratch:
+ * calls whose kernels overlap (other streams, other threads) race. */
+#define SU2HIP_NOALLOC_MAX_BLOCKS 8192
+int su2hip_measure_noalloc(su2hip_batch_t *b, int slot, double *sums_device);
 /* Column values as written to `measure` (src/measure.c:252-306) from sums,
  * for the field content of p; weight/muca_param fill the first two columns.
  * Returns the number of columns in *ncols. */

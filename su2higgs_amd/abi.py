@@ -284,6 +284,7 @@ def load():
         "su2hip_rng_get": (C.c_int, [h, C.c_int, P(C.c_uint64), P(C.c_uint64)]),
         "su2hip_rng_set": (C.c_int, [h, C.c_int, C.c_uint64, C.c_uint64]),
         "su2hip_measure": (C.c_int, [h, C.c_int, dp]),
+        "su2hip_measure_noalloc": (C.c_int, [h, C.c_int, dp]),
         "su2hip_measure_all": (C.c_int, [h, dp]),
         "su2hip_measure_columns": (C.c_int, [P(Params), dp, C.c_double, C.c_double, dp, P(C.c_int)]),
         "su2hip_muca_set": (C.c_int, [h, C.c_int, P(Muca), dp, dp]),

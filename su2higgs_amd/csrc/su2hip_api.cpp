@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -854,6 +855,35 @@ int su2hip_internal_fail(int code, const char *msg) {
   return code;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of su2hip_measure_noalloc: the block partial sums of one lattice,
+// in a file-scope device array that every GPU holds from when the library
+// is loaded until the process ends, so the entry point allocates nothing.
+// In host code `noalloc_scratch` is only a stand-in for the symbol: its
+// device address, which differs from GPU to GPU, comes from
+// hipGetSymbolAddress once per device (noalloc_scratch_init, called by
+// su2hip_create with the handle's device current) and only that address is
+// passed to kernels.
+static __device__ __align__(256) double noalloc_scratch[SU2HIP_NOALLOC_MAX_BLOCKS * SU2HIP_NSUMS];
+
+static constexpr int kMaxDevices = 64;
+static double *g_noalloc_ptr[kMaxDevices];
+static std::mutex g_noalloc_mu;
+
+static int noalloc_scratch_init() {
+  int dev = 0;
+  HIPTRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SU2HIP_EDEVICE, "device %d beyond the table", dev);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[dev]) return SU2HIP_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(SU2HIP_EDEVICE, "scratch address on device %d: %s", dev, hipGetErrorString(e));
+  g_noalloc_ptr[dev] = (double *)ptr;
+  return SU2HIP_OK;
+}
+
 int su2hip_footprint(const su2hip_params *p, int with_flow, uint64_t *bytes) {
   int rc = validate(p);
   if (rc) return rc;
@@ -888,6 +918,8 @@ int su2hip_create(int n, const su2hip_params *params, int max_slots, uint64_t hb
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(SU2HIP_EDEVICE, "no HIP device");
+  int rs = noalloc_scratch_init();
+  if (rs) return rs;
   su2hip_batch *b = new su2hip_batch();
   b->max_slots = max_slots;
   b->hbm_budget = hbm_budget;
@@ -1380,6 +1412,26 @@ int su2hip_measure(su2hip_batch_t *b, int slot, double *sums) {
   rc = su2hip_measure_all(b, all.data());
   if (rc) return rc;
   memcpy(sums, all.data() + (size_t)slot * SU2HIP_NSUMS, sizeof(double) * SU2HIP_NSUMS);
+  return SU2HIP_OK;
+}
+
+int su2hip_measure_noalloc(su2hip_batch_t *b, int slot, double *sums_device) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  if (slot < 0 || slot >= b->max_slots) return fail(SU2HIP_ERANGE, "slot %d out of range", slot);
+  if (!b->slots[slot].used) return fail(SU2HIP_ERANGE, "slot %d is empty", slot);
+  if (!sums_device) return fail(SU2HIP_EINVAL, "NULL sums");
+  const LatDesc &L = b->slots[slot].hd;
+  int nblk = (L.Vh + BLK - 1) / BLK;
+  if (nblk > SU2HIP_NOALLOC_MAX_BLOCKS)
+    return fail(SU2HIP_ERANGE, "lattice of %d sites exceeds the scratch of su2hip_measure_noalloc "
+                "(at most %d sites)", L.vol, 2 * BLK * SU2HIP_NOALLOC_MAX_BLOCKS);
+  int dev = 0;
+  HIPTRY(hipGetDevice(&dev));
+  double *scratch = (dev >= 0 && dev < kMaxDevices) ? g_noalloc_ptr[dev] : nullptr;
+  if (!scratch) return fail(SU2HIP_ESTATE, "no handle was created on device %d", dev);
+  // every block writes all of its partials: nothing to clear
+  hipError_t e = launch_measure_one(b->stream, b->fp64, b->model, L, scratch, sums_device);
+  if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "measure_noalloc: %s", hipGetErrorString(e));
   return SU2HIP_OK;
 }
 

@@ -159,6 +159,10 @@ hipError_t launch_order(const LaunchCtx &c, uint8_t *order, const uint16_t *info
 // sweep epilogue: pass counter += passes, budget -= 1
 hipError_t launch_end_sweep(const LaunchCtx &c, int passes);
 hipError_t launch_measure(const LaunchCtx &c);
+// one lattice's measure sums without the block table, partials of
+// ceil(Vh / BLK) blocks in `partial` (su2hip_measure_noalloc)
+hipError_t launch_measure_one(hipStream_t stream, int fp64, int model, const LatDesc &L,
+                              double *partial, double *sums);
 // both parities' order-parameter contributions of lattice `only` (< 0: all)
 hipError_t launch_orderparam_init(const LaunchCtx &c, int only = -1);
 hipError_t launch_cold(const LaunchCtx &c, int slot, double phi0, double sigma0,

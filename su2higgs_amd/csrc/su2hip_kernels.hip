@@ -2275,6 +2275,57 @@ hipError_t launch_measure(const LaunchCtx &c) {
   return hipGetLastError();
 }
 
+// One lattice measured without the handle's block table: block k sums
+// sites k*BLK .. k*BLK+BLK-1 of both parities into partial[k], the same
+// blocks and order as the lattice's rows of the table, so the sums are
+// bitwise those of k_measure + k_measure_final.
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_measure_one(LatDesc L, double *__restrict__ partial) {
+  __shared__ double lds[BLK / 64];
+  int s = (int)blockIdx.x * BLK + (int)threadIdx.x;
+  double acc[SU2HIP_NSUMS];
+#pragma unroll
+  for (int j = 0; j < SU2HIP_NSUMS; j++) acc[j] = 0.0;
+  if (s < L.Vh) {
+    measure_site<T, M>(L, 0, s, acc);
+    measure_site<T, M>(L, 1, s, acc);
+  }
+  double *out = partial + (size_t)blockIdx.x * SU2HIP_NSUMS;
+#pragma unroll
+  for (int j = 0; j < SU2HIP_NSUMS; j++) {
+    double t = block_sum(acc[j], lds);
+    if (threadIdx.x == 0) out[j] = t;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_measure_one_final(const double *__restrict__ partial,
+                                                           int nblk, double *__restrict__ sums) {
+  __shared__ double lds[BLK / 64];
+  for (int j = 0; j < SU2HIP_NSUMS; j++) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += BLK) v += partial[(size_t)i * SU2HIP_NSUMS + j];
+    double t = block_sum(v, lds);
+    if (threadIdx.x == 0) sums[j] = t;
+  }
+}
+
+hipError_t launch_measure_one(hipStream_t stream, int fp64, int model, const LatDesc &L,
+                              double *partial, double *sums) {
+  int nb = (L.Vh + BLK - 1) / BLK;
+  hipError_t e = with_prec(fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model(model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_measure_one<T, M>), dim3(nb), dim3(BLK), 0, stream, L, partial);
+      return hipGetLastError();
+    });
+  });
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_measure_one_final, dim3(1), dim3(BLK), 0, stream, (const double *)partial, nb,
+                     sums);
+  return hipGetLastError();
+}
+
 hipError_t launch_corr(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, size_t P,
                        double *buf, double *planes) {
   int nb = (Vh + BLK - 1) / BLK;
