@@ -15,6 +15,7 @@ build timed on this host.
     python bench.py [--gpus 1] [--steps 500] [--warmup 20]
 """
 import argparse
+import ctypes as C
 import json
 import os
 import shutil
@@ -36,6 +37,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def slot_params(b, slot=0):
+    """The parameters a handle resolved for a slot (link storage included)."""
+    from su2higgs_amd import abi
+    p = abi.Params()
+    abi.check(b.lib.su2hip_slot_params(b.h, slot, C.byref(p)))
+    return p
+
+
 def workload_params(precision, L=96):
     from su2higgs_amd.abi import FP32, make_params
     return make_params(L=(L, L, L), nhiggs=1, u1=1, precision=precision, betasu2=9.0,
@@ -54,12 +63,14 @@ def passes_and_updates(p):
 
 
 def algo_words(kind, p):
-    """Compulsory HBM words per pass / V (each datum read or written once).
-    A link is 4 words in FP64 and 3 in FP32 (the packed form: three
-    components, the fourth rebuilt from the unit norm)."""
+    """Compulsory HBM words per pass / V (each datum read or written once),
+    for the handle's resolved parameters p (slot_params): a link is 4 words,
+    3 in the packed FP32 form (three components, the fourth rebuilt from
+    the unit norm)."""
     from su2higgs_amd import abi
     nh, u1, tr, sg = p.nhiggs, p.u1, p.triplet, p.singlet
-    lw = 1.0 if p.precision == abi.FP64 else 0.75  # link words / 4
+    packed = p.precision == abi.FP32 and p.link_format == abi.LINKS_PACKED
+    lw = 0.75 if packed else 1.0  # link words / 4
     if kind in (abi.K_LINK_HEATBATH, abi.K_LINK_METROPOLIS, abi.K_LINK_OVERRELAX):
         # U_mu(1-p) 2, U_nu 8, the write 2; phi, alpha_mu(p), Sigma
         w = (2 + 8 + 2) * lw + 4 * nh + 0.5 * u1 + 3 * tr
@@ -176,7 +187,9 @@ def run_precision(precision, steps, warmup, timing_steps=100, graph=True):
     b.update(steps)
     b.sync()
     wall = time.perf_counter() - t0
+    lp = slot_params(b)
     res = {"precision": precision, "V": V, "passes_per_sweep": npass, "updates_per_sweep_over_V": upd,
+           "link_format": {abi.LINKS_FULL: "full", abi.LINKS_PACKED: "packed"}[lp.link_format],
            "autotune_ms_64_128_256": tune, "chunk": picks,
            "wall_s": wall, "ms_per_sweep": 1e3 * wall / steps,
            "site_updates_per_s": upd * V * steps / wall}
@@ -201,9 +214,29 @@ def run_precision(precision, steps, warmup, timing_steps=100, graph=True):
         k = abi.KIND_NAMES.index(name)
         if k < abi.K_MEASURE:
             res["kernel_avg_ms"][name], res["kernel_span_ms"][name] = b.time_pass(k, 200)
-    res["p"] = p
+    res["p"] = lp
     b.close()
     return res
+
+
+def sweep_ms_with_links(precision, links, steps=200, L=96):
+    """ms per graph-replayed sweep of the workload with the FP32 links in the
+    given storage form (autotuned chunks): the other form beside `value`."""
+    from su2higgs_amd.lattice import Batch
+    p = workload_params(precision, L=L)
+    p.link_format = links
+    b = Batch(p)
+    b.cold_start(0, phi0=1.0)
+    b.autotune(repeats=50)
+    b.use_graph(True)
+    b.update(20)
+    b.sync()
+    t0 = time.perf_counter()
+    b.update(steps)
+    b.sync()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    b.close()
+    return ms
 
 
 def roofline(res):
@@ -259,6 +292,7 @@ def roofline_beyond_cache(L=192, precisions=None):
         b.update(2)
         V = abi.volume(p)
         w = 4 if prec == abi.FP32 else 8
+        p = slot_params(b)
         kinds = {}
         for k in (abi.K_LINK_OVERRELAX, abi.K_LINK_HEATBATH, abi.K_U1_METROPOLIS,
                   abi.K_DOUBLET_OVERRELAX, abi.K_DOUBLET_METROPOLIS):
@@ -356,10 +390,11 @@ def _scan_rate(ps, steps, graph, tune=True, kinds=()):
     Vtot = sum(abi.volume(p) for p in ps)
     _, upd = passes_and_updates(ps[0])
     w = 4 if ps[0].precision == abi.FP32 else 8
+    rp = slot_params(b)
     kr = {}
     for k in kinds:
         ms, _ = b.time_pass(k, 50)
-        byts = algo_words(k, ps[0]) * Vtot * w
+        byts = algo_words(k, rp) * Vtot * w
         ach = byts / (ms * 1e-3) / 1e9
         kr[abi.KIND_NAMES[k]] = {"avg_launch_us": ms * 1e3, "algorithmic_MB": byts / 1e6,
                                  "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
@@ -642,6 +677,11 @@ def main():
     log("[bench] FP64 plain 96^3, %d timed sweeps" % s64)
     r64 = run_precision(abi.FP64, s64, max(1, args.warmup // 2), graph=not args.no_graph)
     log("[bench] fp64 %.3f ms/sweep, %.3e site-updates/s" % (r64["ms_per_sweep"], r64["site_updates_per_s"]))
+    from su2higgs_amd.abi import LINKS_FULL, LINKS_PACKED
+    other = LINKS_PACKED if r32["link_format"] == "full" else LINKS_FULL
+    other_ms = sweep_ms_with_links(abi.FP32, other, steps=min(args.steps, 200))
+    log("[bench] fp32 with %s links: %.3f ms/sweep" % ("packed" if other == LINKS_PACKED else "full",
+                                                       other_ms))
     roof = roofline(r32)
     roof64 = roofline(r64)
     log("[bench] every pass kind beyond the Infinity Cache (192^3)")
@@ -701,6 +741,10 @@ def main():
                    "L": 96, "passes_per_sweep": r32["passes_per_sweep"],
                    "variable_updates_per_sweep": "%g V" % r32["updates_per_sweep_over_V"],
                    "precision": "FP32 fields/staples, FP64 reductions"},
+        "link_format_fp32": r32["link_format"],
+        "fp32_other_link_format": {"link_format": "packed" if other == LINKS_PACKED else "full",
+                                   "ms_per_step": other_ms,
+                                   "value": r32["updates_per_sweep_over_V"] * r32["V"] / (other_ms * 1e-3)},
         "fp64_value": r64["site_updates_per_s"],
         "fp64_ms_per_step": r64["ms_per_sweep"],
         "fp32_over_fp64": r32["site_updates_per_s"] / r64["site_updates_per_s"],

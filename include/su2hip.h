@@ -50,6 +50,23 @@ extern "C" {
 #define SU2HIP_FP64 64
 #define SU2HIP_FP32 32
 
+/* FP32 link storage (su2hip_params.link_format; FP64 always stores four
+ * components).  FULL: four floats per link (16 bytes).  PACKED: the three
+ * components other than the largest one (12 bytes); the largest is rebuilt
+ * from the unit norm when the link is loaded, which costs arithmetic and
+ * saves a quarter of the link bytes.  Every FP32 link is renormalised when
+ * it is stored, in both forms.  AUTO, the default: a handle created with
+ * its lattices takes PACKED when their FP32 fields in the full form exceed
+ * SU2HIP_LINKS_PACKED_ABOVE bytes (beyond what the 256 MB Infinity Cache
+ * serves well: there the bytes dominate, below it the arithmetic), FULL
+ * otherwise; a handle created empty (lattices added later) takes PACKED.
+ * A handle uses one format for all its lattices: the first lattice's
+ * setting decides, or the measured one when it is AUTO. */
+#define SU2HIP_LINKS_AUTO 0
+#define SU2HIP_LINKS_FULL 1
+#define SU2HIP_LINKS_PACKED 2
+#define SU2HIP_LINKS_PACKED_ABOVE (128ull << 20)
+
 /* Gradient-flow integrators. */
 #define SU2HIP_FLOW_EULER 1 /* the reference's scheme, src/gradflow.c:14-18 */
 #define SU2HIP_FLOW_RK3 3   /* Luscher's 3rd-order Runge-Kutta, arXiv:1006.4518 */
@@ -88,8 +105,9 @@ typedef struct su2hip_params {
   int32_t update_su2doublet; /* doublet passes per scalar round (update_doublet) */
   int32_t update_su2triplet; /* triplet passes per scalar round (update_triplet) */
   int32_t update_singlet;    /* singlet passes per scalar round (update_singlet) */
-  int32_t reunitarize;       /* FP32 only: renormalise every link a kernel writes */
-  int32_t reserved;
+  int32_t reunitarize;       /* FP32: links are stored unit (SU2HIP_LINKS_*); kept for
+                                the ABI, no further effect */
+  int32_t link_format;       /* FP32 link storage: SU2HIP_LINKS_AUTO / _FULL / _PACKED */
   double betasu2;
   double betau1, r_u1;
   double msq_phi, lambda_phi;

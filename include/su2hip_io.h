@@ -21,7 +21,8 @@ extern "C" {
 
 /* Everything a run reads from `config` (src/parameters.c:157-402), plus the
  * keys the reference does not know (model, seed, start, link_overrelax,
- * precision, flow_scheme), which it ignores when it reads the same file. */
+ * precision, link_format, flow_scheme), which it ignores when it reads the
+ * same file. */
 typedef struct su2hip_run_config {
   su2hip_params p;
   int32_t gradflow;          /* field content includes the gradient-flow keys */

@@ -71,6 +71,11 @@ typedef struct su2hip_serve_config {
   int32_t flow_scheme;   /* SU2HIP_FLOW_RK3 or SU2HIP_FLOW_EULER */
   double flow_dt, flow_t_max;
   double phi0, sigma0, singlet0; /* cold start of new chains (src/init.c:92-112) */
+  int32_t backfill;      /* while the queue's head waits for room, admit up to this
+                            many later new-chain requests that fit now (0: FIFO) */
+  int32_t pad;
+  double max_head_wait;  /* seconds: once the head has waited this long, nothing
+                            more goes ahead of it (room drains to the head) */
 } su2hip_serve_config;
 
 /* Totals of one su2hip_serve() call. */
@@ -88,9 +93,10 @@ typedef struct su2hip_serve_stats {
   int64_t flows, evictions;
 } su2hip_serve_stats;
 
-/* Defaults: FP32 SU(2)+doublet, heatbath + 4 link overrelaxation rounds,
+/* Defaults: FP32 SU(2)+doublet with packed links, heatbath + 4 link overrelaxation rounds,
  * 4 doublet overrelaxation rounds + the final Metropolis round, 64 slots,
- * 8 GiB cap, tick 16, graphs on, realtime, RK3 flow dt 0.01 to t 4.0. */
+ * 8 GiB cap, tick 16, graphs on, realtime, RK3 flow dt 0.01 to t 4.0,
+ * backfill 32 with max_head_wait 1 s. */
 void su2hip_serve_defaults(su2hip_serve_config *cfg);
 
 /* Variable updates per site in one compound sweep of p's schedule. */

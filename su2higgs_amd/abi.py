@@ -19,6 +19,7 @@ PHISQ, SIGMASQ, PHI2MINUSSIGMA2, PHI2SQ = 1, 2, 3, 4
 READONLY, FAST, SLOW = 0, 1, 2
 FP64, FP32 = 64, 32
 OK, EINVAL, ENOMEM, EDEVICE, ERANGE, EUNSUPPORTED, ESTATE, EIO = 0, -1, -2, -3, -4, -5, -6, -7
+LINKS_AUTO, LINKS_FULL, LINKS_PACKED = 0, 1, 2
 FLOW_EULER, FLOW_RK3 = 1, 3
 
 (K_LINK_HEATBATH, K_LINK_METROPOLIS, K_LINK_OVERRELAX, K_U1_METROPOLIS,
@@ -50,7 +51,7 @@ class Params(C.Structure):
         ("update_links", C.c_int32), ("link_overrelax", C.c_int32),
         ("scalar_sweeps", C.c_int32), ("update_su2doublet", C.c_int32),
         ("update_su2triplet", C.c_int32), ("update_singlet", C.c_int32),
-        ("reunitarize", C.c_int32), ("reserved", C.c_int32),
+        ("reunitarize", C.c_int32), ("link_format", C.c_int32),
         ("betasu2", C.c_double), ("betau1", C.c_double), ("r_u1", C.c_double),
         ("msq_phi", C.c_double), ("lambda_phi", C.c_double),
         ("msq_triplet", C.c_double), ("a2", C.c_double), ("b4", C.c_double),
@@ -147,7 +148,8 @@ class ServeConfig(C.Structure):
         ("tick_sweeps", C.c_int32), ("use_graph", C.c_int32), ("realtime", C.c_int32),
         ("keep_chains", C.c_int32), ("flow_scheme", C.c_int32), ("flow_dt", C.c_double),
         ("flow_t_max", C.c_double), ("phi0", C.c_double), ("sigma0", C.c_double),
-        ("singlet0", C.c_double),
+        ("singlet0", C.c_double), ("backfill", C.c_int32), ("pad", C.c_int32),
+        ("max_head_wait", C.c_double),
     ]
 
 

@@ -149,12 +149,16 @@ struct TraceRange {
 
 static size_t wsize(const su2hip_params &p) { return p.precision == SU2HIP_FP64 ? 8 : 4; }
 static int vol_of(const su2hip_params &p) { return p.L[0] * p.L[1] * p.L[2]; }
+// FP32 links of this lattice in the packed 12-byte form (link_format
+// resolved by the handle; FP64 links always have four components)
+static bool packed_links(const su2hip_params &p) {
+  return p.precision == SU2HIP_FP32 && p.link_format == SU2HIP_LINKS_PACKED;
+}
 // words of the links of a lattice with Vh sites per parity: 3 directions x 2
-// parities x the words of one stored link (4 in FP64, 3 in FP32: the packed
-// form of ldU / stU)
-static size_t link_words(size_t Vh, bool fp64) { return 6 * Vh * (fp64 ? 4 : 3); }
+// parities x the words of one stored link (3 packed, else 4)
+static size_t link_words(size_t Vh, bool packed) { return 6 * Vh * (packed ? 3 : 4); }
 static size_t link_words(const su2hip_params &p) {
-  return link_words((size_t)vol_of(p) / 2, p.precision == SU2HIP_FP64);
+  return link_words((size_t)vol_of(p) / 2, packed_links(p));
 }
 
 static size_t field_words(const su2hip_params &p) {
@@ -191,6 +195,8 @@ static int validate(const su2hip_params *p) {
   if (V > (1ll << 28)) return fail(SU2HIP_ERANGE, "volume %lld too large", V);
   if (p->nhiggs < 0 || p->nhiggs > 2)
     return fail(SU2HIP_EUNSUPPORTED, "nhiggs = %d (0, 1 or 2 supported)", p->nhiggs);
+  if (p->link_format < SU2HIP_LINKS_AUTO || p->link_format > SU2HIP_LINKS_PACKED)
+    return fail(SU2HIP_EINVAL, "link_format %d", p->link_format);
   if (!supported_model(model_bits(*p)))
     return fail(SU2HIP_EUNSUPPORTED,
                 "field content nhiggs=%d u1=%d triplet=%d singlet=%d is not supported",
@@ -752,7 +758,27 @@ static int check_slot(su2hip_batch *b, int slot) {
   return SU2HIP_OK;
 }
 
-static int alloc_slot(su2hip_batch *b, int i, const su2hip_params &p) {
+// The link storage of a handle (SU2HIP_LINKS_FULL or _PACKED) from its
+// first lattice's link_format; AUTO measures the FP32 fields of the n
+// lattices in the full form against SU2HIP_LINKS_PACKED_ABOVE (n = 0: the
+// lattices come later, as in serving: packed).
+static int resolve_links(int n, const su2hip_params *params) {
+  const su2hip_params &p0 = params[0];
+  if (p0.precision == SU2HIP_FP64) return SU2HIP_LINKS_FULL;
+  if (p0.link_format != SU2HIP_LINKS_AUTO) return p0.link_format;
+  if (n == 0) return SU2HIP_LINKS_PACKED;
+  uint64_t bytes = 0;
+  for (int i = 0; i < n; i++) {
+    su2hip_params q = params[i];
+    q.link_format = SU2HIP_LINKS_FULL;
+    bytes += field_words(q) * wsize(q);
+  }
+  return bytes > SU2HIP_LINKS_PACKED_ABOVE ? SU2HIP_LINKS_PACKED : SU2HIP_LINKS_FULL;
+}
+
+static int alloc_slot(su2hip_batch *b, int i, const su2hip_params &p_in) {
+  su2hip_params p = p_in;
+  p.link_format = b->tmpl.link_format;  // the handle's resolved storage
   size_t bytes = slot_bytes(p);
   if (b->hbm_budget && b->used + bytes > b->hbm_budget)
     return fail(SU2HIP_ENOMEM, "HBM budget: %llu + %zu bytes exceeds %llu",
@@ -888,6 +914,9 @@ int su2hip_footprint(const su2hip_params *p, int with_flow, uint64_t *bytes) {
   int rc = validate(p);
   if (rc) return rc;
   if (!bytes) return fail(SU2HIP_EINVAL, "bytes is NULL");
+  su2hip_params q = *p;
+  q.link_format = resolve_links(1, &q);  // AUTO: as a handle of this one lattice would
+  p = &q;
   size_t b = slot_bytes(*p);
   size_t Vh = (size_t)vol_of(*p) / 2;
   if (p->nhiggs || p->triplet) b += align256((p->nhiggs ? 8 : 6) * Vh * wsize(*p));
@@ -924,7 +953,8 @@ int su2hip_create(int n, const su2hip_params *params, int max_slots, uint64_t hb
   b->max_slots = max_slots;
   b->hbm_budget = hbm_budget;
   b->tmpl = params[0];
-  b->model = model_bits(params[0]);
+  b->tmpl.link_format = resolve_links(n, params);
+  b->model = model_bits(params[0]) | (packed_links(b->tmpl) ? M_PK : 0);
   b->fp64 = params[0].precision == SU2HIP_FP64;
   b->slots.resize(max_slots);
   int rc = SU2HIP_OK;
@@ -1732,7 +1762,7 @@ int su2hip_blocked_correlators(su2hip_batch_t *b, int slot, int levels, double *
   if (rc) return rc;
   LatDesc fine = b->slots[slot].hd;
   size_t w = wsize(p), Vh1 = (size_t)fine.Vh / 4;  // first level: a quarter of the sites
-  size_t lw = link_words(Vh1, p.precision == SU2HIP_FP64);
+  size_t lw = link_words(Vh1, packed_links(p));
   size_t words = align256(lw * w) + align256(6 * Vh1 * w);
   char *mem = nullptr;
   LatDesc *ddesc = nullptr;

@@ -305,31 +305,39 @@ __device__ __forceinline__ F3 link_pack(const Q<float> &q) {
   return v;
 }
 
-// words of one stored link
-template <typename T>
-constexpr int link_words() {
-  return sizeof(T) == 8 ? 4 : 3;
+// Link loads and stores.  M carries the model bits of the kernel; with
+// M_PK an FP32 link is in the packed 12-byte form, otherwise (and always in
+// FP64) it is the four components.  There is no default: every caller says
+// which layout it reads.  Every FP32 store renormalises (the packed form by
+// its construction); FP64 stores the reference's arithmetic as it is.
+template <int M, typename T>
+constexpr bool packed_links() {
+  return sizeof(T) == 4 && (M & M_PK) != 0;
 }
 
-template <typename T>
+template <int M, typename T>
 __device__ __forceinline__ Q<T> ldU(const T *__restrict__ U, int Vh, int d, int par, int s) {
-  if constexpr (sizeof(T) == 8) {
-    return ld4(U + ((size_t)(d * 2 + par) * Vh + s) * 4);
-  } else {
+  if constexpr (packed_links<M, T>()) {
     gptr<const float> p = gmem(U + ((size_t)(d * 2 + par) * Vh + s) * 3);
     return link_unpack(F3{p[0], p[1], p[2]});
+  } else {
+    return ld4(U + ((size_t)(d * 2 + par) * Vh + s) * 4);
   }
 }
-template <typename T>
+template <int M, typename T>
 __device__ __forceinline__ void stU(T *__restrict__ U, int Vh, int d, int par, int s, const Q<T> &q) {
-  if constexpr (sizeof(T) == 8) {
-    st4(U + ((size_t)(d * 2 + par) * Vh + s) * 4, q);
-  } else {
+  if constexpr (packed_links<M, T>()) {
     F3 v = link_pack(q);
     gptr<float> p = gmem(U + ((size_t)(d * 2 + par) * Vh + s) * 3);
     p[0] = v.x;
     p[1] = v.y;
     p[2] = v.z;
+  } else if constexpr (sizeof(T) == 4) {
+    // the full FP32 form renormalises every link it stores
+    float n = __builtin_amdgcn_rsqf(q.a * q.a + q.b * q.b + q.c * q.c + q.d * q.d);
+    st4(U + ((size_t)(d * 2 + par) * Vh + s) * 4, Q<float>{q.a * n, q.b * n, q.c * n, q.d * n});
+  } else {
+    st4(U + ((size_t)(d * 2 + par) * Vh + s) * 4, q);
   }
 }
 template <typename T>
@@ -364,9 +372,9 @@ __device__ __forceinline__ T ldS(const T *__restrict__ S, int Vh, int par, int s
 }
 
 // load by site
-template <typename T>
+template <int M, typename T>
 __device__ __forceinline__ Q<T> U_at(const LatDesc &L, const T *U, const Site &c, int d) {
-  return ldU(U, L.Vh, d, c.par, c.s);
+  return ldU<M>(U, L.Vh, d, c.par, c.s);
 }
 template <typename T>
 __device__ __forceinline__ Q<T> F_at(const LatDesc &L, const T *F, const Site &c) {
@@ -409,6 +417,27 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < BLK / 64; i++) r += lds[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+// N sums over the block at once, each in block_sum's fixed order (the
+// wavefront butterfly, then the waves in order); sum j is returned in
+// thread j < N.  One barrier pair for all of them; lds holds BLK/64 * N.
+template <int N>
+__device__ __forceinline__ double block_sums(const double *v, double *lds) {
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    double t = wave_sum(v[j]);
+    if ((threadIdx.x & 63) == 0) lds[w * N + j] = t;
+  }
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x < N) {
+#pragma unroll
+    for (int i = 0; i < BLK / 64; i++) r += lds[i * N + threadIdx.x];
   }
   __syncthreads();
   return r;

@@ -44,6 +44,8 @@ inline uint16_t step_info(bool scalar, int round, int pos) {
 
 // model bits; M_H2 (two doublets) always comes with M_NH
 enum : int { M_NH = 1, M_U1 = 2, M_TR = 4, M_SG = 8, M_H2 = 16 };
+// not field content: FP32 links stored in the packed 12-byte form (ldU)
+constexpr int M_PK = 64;
 
 inline int model_bits(const su2hip_params &p) {
   return (p.nhiggs ? M_NH : 0) | (p.nhiggs == 2 ? M_H2 : 0) | (p.u1 ? M_U1 : 0) |

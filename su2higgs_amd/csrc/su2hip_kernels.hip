@@ -70,7 +70,7 @@ constexpr bool kind_valid() {
 // su2link_staple (src/staples.c:134-193) with su2staple_wilson (:84-115):
 // -beta/2 sum_nu [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+
 // U_mu(x-nu)^+ U_nu(x-nu)] - 1/2 Phi(x+mu) exp(-i alpha sigma_3) Phi(x)^+
-template <typename T>
+template <typename T, int M>
 __device__ __forceinline__ Q<T> wilson_staple(const LatDesc &L, const T *U, const Site &x, int mu) {
   Q<T> W{(T)0, (T)0, (T)0, (T)0};
   Site xp = shift(L, x, mu, +1);
@@ -78,9 +78,9 @@ __device__ __forceinline__ Q<T> wilson_staple(const LatDesc &L, const T *U, cons
   for (int nu = 0; nu < 3; nu++) {
     if (nu == mu) continue;
     Site xn = shift(L, x, nu, +1), xm = shift(L, x, nu, -1), xpm = shift(L, xp, nu, -1);
-    Q<T> up = qmul(qmul(U_at(L, U, xp, nu), qconj(U_at(L, U, xn, mu))), qconj(U_at(L, U, x, nu)));
+    Q<T> up = qmul(qmul(U_at<M>(L, U, xp, nu), qconj(U_at<M>(L, U, xn, mu))), qconj(U_at<M>(L, U, x, nu)));
     W = qadd(W, up);
-    Q<T> lo = qmul(qmul(qconj(U_at(L, U, xpm, nu)), qconj(U_at(L, U, xm, mu))), U_at(L, U, xm, nu));
+    Q<T> lo = qmul(qmul(qconj(U_at<M>(L, U, xpm, nu)), qconj(U_at<M>(L, U, xm, mu))), U_at<M>(L, U, xm, nu));
     W = qadd(W, lo);
   }
   return W;
@@ -89,7 +89,7 @@ __device__ __forceinline__ Q<T> wilson_staple(const LatDesc &L, const T *U, cons
 template <typename T, int M>
 __device__ __forceinline__ Q<T> link_staple(const LatDesc &L, const T *U, const T *A, const T *F,
                                             const Site &x, int mu) {
-  Q<T> V = qscale(wilson_staple(L, U, x, mu), (T)(-0.5 * L.beta));
+  Q<T> V = qscale(wilson_staple<T, M>(L, U, x, mu), (T)(-0.5 * L.beta));
   if constexpr ((M & M_NH) != 0) {
     Site xp = shift(L, x, mu, +1);
     Q<T> b = F_at(L, F, xp);
@@ -114,10 +114,10 @@ template <typename T, int M>
 __device__ __forceinline__ void doublet_hops(const LatDesc &L, const T *U, const T *A, const T *F,
                                              const Site &x, int d, Q<T> &s) {
   Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);
-  Q<T> t = qmul(U_at(L, U, x, d), F_at(L, F, xp));
+  Q<T> t = qmul(U_at<M>(L, U, x, d), F_at(L, F, xp));
   if constexpr ((M & M_U1) != 0) t = qmul(t, u1twist<T>(A_at(L, A, x, d)));
   s = qadd(s, qscale(t, (T)-1));
-  Q<T> t2 = qmul(qconj(U_at(L, U, xm, d)), F_at(L, F, xm));
+  Q<T> t2 = qmul(qconj(U_at<M>(L, U, xm, d)), F_at(L, F, xm));
   if constexpr ((M & M_U1) != 0) t2 = qmul(t2, qconj(u1twist<T>(A_at(L, A, xm, d))));
   s = qadd(s, qscale(t2, (T)-1));
 }
@@ -136,10 +136,10 @@ __device__ __forceinline__ Q<T> doublet_staple(const LatDesc &L, const T *U, con
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);
-    Q<T> t = qmul(U_at(L, U, x, d), F_at(L, F, xp));
+    Q<T> t = qmul(U_at<M>(L, U, x, d), F_at(L, F, xp));
     if constexpr ((M & M_U1) != 0) t = qmul(t, u1twist<T>(A_at(L, A, x, d)));
     s = qadd(s, qscale(t, (T)-1));
-    Q<T> t2 = qmul(qconj(U_at(L, U, xm, d)), F_at(L, F, xm));
+    Q<T> t2 = qmul(qconj(U_at<M>(L, U, xm, d)), F_at(L, F, xm));
     if constexpr ((M & M_U1) != 0) t2 = qmul(t2, qconj(u1twist<T>(A_at(L, A, xm, d))));
     s = qadd(s, qscale(t2, (T)-1));
   }
@@ -147,17 +147,17 @@ __device__ __forceinline__ Q<T> doublet_staple(const LatDesc &L, const T *U, con
 }
 
 // hopping staple of the triplet (src/overrelax.c:143-180)
-template <typename T>
+template <typename T, int M>
 __device__ __forceinline__ V3<T> triplet_staple(const LatDesc &L, const T *U, const T *G, const Site &x) {
   V3<T> s{(T)0, (T)0, (T)0};
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);
-    V3<T> r = rotate(U_at(L, U, x, d), G_at(L, G, xp));
+    V3<T> r = rotate(U_at<M>(L, U, x, d), G_at(L, G, xp));
     s.x -= r.x;
     s.y -= r.y;
     s.z -= r.z;
-    r = rotate(qconj(U_at(L, U, xm, d)), G_at(L, G, xm));
+    r = rotate(qconj(U_at<M>(L, U, xm, d)), G_at(L, G, xm));
     s.x -= r.x;
     s.y -= r.y;
     s.z -= r.z;
@@ -464,7 +464,7 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
     T oldact = 0;
     V3<T> a1{}, a2{};
     if constexpr ((M & M_TR) != 0) {
-      old = ldU(U, Vh, mu, par, s);
+      old = ldU<M>(U, Vh, mu, par, s);
       a1 = ldG(G, Vh, par, s);
       a2 = G_at(L, G, shift(L, x, mu, +1));
       oldact = hop_triplet(old, a1, a2);
@@ -546,12 +546,12 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       T diff = oldact - hop_triplet(nu, a1, a2);
       if (!(uni<T>(r) < sexp(diff))) return 0;
     }
-    stU(U, Vh, mu, par, s, nu);
+    stU<M>(U, Vh, mu, par, s, nu);
     return 1;
   } else if constexpr (KIND == SU2HIP_K_LINK_METROPOLIS) {
     // metro_su2link (src/metropolis.c:15-41), random_su2link (src/init.c:43-58);
     // Delta S = 2 Re[(U'-U) V]_0 + Delta(triplet hop), from the staple
-    Q<T> old = ldU(U, Vh, mu, par, s);
+    Q<T> old = ldU<M>(U, Vh, mu, par, s);
     Q<T> V = link_staple<T, M>(L, U, A, F, x, mu);
     Q<T> R;
     R.a = (T)10 * (uni<T>(r) - (T)0.5);
@@ -573,13 +573,13 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       diff += hop_triplet(nu, a1, a2) - hop_triplet(old, a1, a2);
     }
     if (!metro_accept(diff, r)) return 0;
-    stU(U, Vh, mu, par, s, nu);
+    stU<M>(U, Vh, mu, par, s, nu);
     return 1;
   } else if constexpr (KIND == SU2HIP_K_LINK_OVERRELAX) {
     // SU(2) link overrelaxation, U' = W^+ U^+ W^+ with W = staple/|staple|
     // (no reference counterpart); with a triplet a Metropolis test on its
     // hopping term as in src/heatbath.c:107-120
-    Q<T> old = ldU(U, Vh, mu, par, s);
+    Q<T> old = ldU<M>(U, Vh, mu, par, s);
     Q<T> V = link_staple<T, M>(L, U, A, F, x, mu);
     Q<T> w;
     if constexpr (sizeof(T) == 8) {
@@ -598,7 +598,7 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       T diff = hop_triplet(old, a1, a2) - hop_triplet(nu, a1, a2);
       if (!(uni<T>(r) < sexp(diff))) return 0;
     }
-    stU(U, Vh, mu, par, s, nu);
+    stU<M>(U, Vh, mu, par, s, nu);
     return 1;
   } else if constexpr (KIND == SU2HIP_K_U1_METROPOLIS) {
     // metro_u1link (src/metropolis.c:48-70) with localact_u1link
@@ -631,10 +631,10 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       }
       dS = (T)2 * (T)L.betau1 * ::sin(rr * h) * sum;
       if constexpr ((M & M_NH) != 0) {
-        Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F, xp));
+        Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU<M>(U, Vh, mu, par, s)), F_at(L, F, xp));
         if constexpr ((M & M_H2) != 0) {
           const T *F2 = (const T *)L.F2;
-          q = qadd(q, qmul(qmul(qconj(ldF(F2, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F2, xp)));
+          q = qadd(q, qmul(qmul(qconj(ldF(F2, Vh, par, s)), ldU<M>(U, Vh, mu, par, s)), F_at(L, F2, xp)));
         }
         T sm, cm;
         ::sincos(m, &sm, &cm);
@@ -657,10 +657,10 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
       }
       dS = (T)2 * (T)L.betau1 * fsin(rr * h) * sum;
       if constexpr ((M & M_NH) != 0) {
-        Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F, xp));
+        Q<T> q = qmul(qmul(qconj(ldF(F, Vh, par, s)), ldU<M>(U, Vh, mu, par, s)), F_at(L, F, xp));
         if constexpr ((M & M_H2) != 0) {
           const T *F2 = (const T *)L.F2;
-          q = qadd(q, qmul(qmul(qconj(ldF(F2, Vh, par, s)), ldU(U, Vh, mu, par, s)), F_at(L, F2, xp)));
+          q = qadd(q, qmul(qmul(qconj(ldF(F2, Vh, par, s)), ldU<M>(U, Vh, mu, par, s)), F_at(L, F2, xp)));
         }
         // -(q0 (cos na - cos al) + q3 (sin na - sin al))
         dS -= (T)2 * fsin(h) * (q.d * fcos(m) - q.a * fsin(m));
@@ -762,7 +762,7 @@ __device__ __forceinline__ int site_update(const LatDesc &L, const Site &x, int 
   } else if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX ||
                        KIND == SU2HIP_K_TRIPLET_METROPOLIS) {
     V3<T> g = ldG(G, Vh, par, s);
-    V3<T> st = triplet_staple<T>(L, U, G, x);
+    V3<T> st = triplet_staple<T, M>(L, U, G, x);
     double C = 0.25 * L.b4;
     V3<T> ng;
     if constexpr (KIND == SU2HIP_K_TRIPLET_OVERRELAX && sizeof(T) == 4) {
@@ -1050,6 +1050,36 @@ __device__ double muca_weight(const MucaDev &m, double val) {
   return m.b[bin] + m.slope[bin] * val;
 }
 
+// The same lookup with the bin edges staged in LDS (pos = m.pos copied):
+// the same comparisons in the same order, without a dependent global load
+// per step of the search.
+__device__ int muca_whichbin_staged(const MucaDev &m, const double *pos, double val) {
+  if (val < m.min) return 0;
+  if (val >= m.max) return m.bins + 1;
+  int lo = 1, hi = m.bins;
+  while (lo <= hi) {
+    int bin = lo + (hi - lo) / 2;
+    double cur = pos[bin], nxt = pos[bin + 1];
+    if ((val < nxt && val > cur) || (::fabs(val - cur) < 1e-12)) return bin;
+    if (val > cur)
+      lo = bin + 1;
+    else
+      hi = bin - 1;
+  }
+  return m.bins;
+}
+__device__ double muca_weight_staged(const MucaDev &m, const double *pos, double val) {
+  int n = m.bins;
+  if (val >= m.max) return m.b[n + 1];
+  if (val < m.min) return m.b[0];
+  int bin = muca_whichbin_staged(m, pos, val);
+  if (bin >= n + 1) return m.b[n + 1];
+  return m.b[bin] + m.slope[bin] * val;
+}
+
+// bin edges that fit the accept kernel's LDS stage
+constexpr int MUCA_STAGE = 2048;
+
 // linearize_weight (src/multicanonical.c:337-346)
 __device__ void muca_linearize(MucaDev &m) {
   int n = m.bins;
@@ -1182,22 +1212,27 @@ __global__ __launch_bounds__(BLK) void k_muca_accept(const LatDesc *__restrict__
                                                      int hid, int par_in, int pass_off, int step,
                                                      const uint8_t *__restrict__ order, int npass) {
   __shared__ double lds[BLK / 64];
+  __shared__ double pos[MUCA_STAGE];
   const LatDesc &L = lats[blockIdx.x];
   MucaDev &m = mucas[blockIdx.x];
   if (!L.active || L.budget == 0 || !muca_field_on(m, kind, hid)) return;
   const int par = order ? (order[(size_t)blockIdx.x * npass + pass_off] & 1) : par_in;
   int lo, hi;
   if (!muca_step_range(L.Vh, m.cps, step, lo, hi)) return;
+  const bool staged = m.bins + 2 <= MUCA_STAGE;
+  if (staged)
+    for (int i = threadIdx.x; i < m.bins + 2; i += BLK) pos[i] = m.pos[i];
   // fixed-order sum of this lattice's block partials
   double v = 0.0;
   for (int i = threadIdx.x; i < L.nblk; i += BLK) v += partial[L.blk_base + i];
-  double tot = block_sum(v, lds);
+  double tot = block_sum(v, lds);  // its barriers also publish the staged edges
   if (threadIdx.x == 0) {
     double newp = tot / L.vol;
     double old = m.pv[0] + m.pv[1];
     m.pv[par] = newp;
     double nw = newp + m.pv[1 - par];
-    double diff = muca_weight(m, nw) - muca_weight(m, old);
+    double diff = staged ? muca_weight_staged(m, pos, nw) - muca_weight_staged(m, pos, old)
+                         : muca_weight(m, nw) - muca_weight(m, old);
     Rng r;
     rng_init(r, L.key0, L.key1, L.pass + (uint64_t)pass_off, MUCA_SITE);
     r.k = (uint32_t)step;
@@ -1392,10 +1427,10 @@ __device__ __forceinline__ Cm halfproj(const V3<T> &g) {
 }
 
 // project_u1 (src/magfield.c:115-184)
-template <typename T>
+template <typename T, int M>
 __device__ Cm project_u1(const LatDesc &L, const T *U, const T *G, const Site &x, int d) {
   Cm pl = halfproj(G_at(L, G, x)), pr = halfproj(G_at(L, G, shift(L, x, d, +1)));
-  Q<T> q = U_at(L, U, x, d);
+  Q<T> q = U_at<M>(L, U, x, d);
   Cm u;
   u.re[0] = q.a;
   u.im[0] = q.d;
@@ -1409,12 +1444,12 @@ __device__ Cm project_u1(const LatDesc &L, const T *U, const T *G, const Site &x
 }
 
 // alpha_proj (src/magfield.c:189-216)
-template <typename T>
+template <typename T, int M>
 __device__ double alpha_proj(const LatDesc &L, const T *U, const T *G, const Site &x, int d1, int d2) {
-  Cm u1 = project_u1(L, U, G, x, d1);
-  Cm u2 = project_u1(L, U, G, shift(L, x, d1, +1), d2);
-  Cm u3 = project_u1(L, U, G, shift(L, x, d2, +1), d1);
-  Cm u4 = project_u1(L, U, G, x, d2);
+  Cm u1 = project_u1<T, M>(L, U, G, x, d1);
+  Cm u2 = project_u1<T, M>(L, U, G, shift(L, x, d1, +1), d2);
+  Cm u3 = project_u1<T, M>(L, U, G, shift(L, x, d2, +1), d1);
+  Cm u4 = project_u1<T, M>(L, U, G, x, d2);
   u1 = cm_mul(u1, u2, false);
   u1 = cm_mul(u1, u3, true);
   u1 = cm_mul(u1, u4, true);
@@ -1422,14 +1457,14 @@ __device__ double alpha_proj(const LatDesc &L, const T *U, const T *G, const Sit
 }
 
 // magfield + magcharge_cube (src/magfield.c:221-280)
-template <typename T>
+template <typename T, int M>
 __device__ double magfield(const LatDesc &L, const T *U, const T *G, const Site &x, int dir) {
   double res = 0.0;
   for (int d1 = 0; d1 < 3; d1++) {
     if (d1 == dir) continue;
     for (int d2 = d1 + 1; d2 < 3; d2++) {
       if (d2 == dir) continue;
-      double al = alpha_proj(L, U, G, x, d1, d2);
+      double al = alpha_proj<T, M>(L, U, G, x, d1, d2);
       if (d1 > dir)
         res += al;
       else if (d2 > dir)
@@ -1452,9 +1487,9 @@ __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
   for (int d1 = 0; d1 < 3; d1++)
 #pragma unroll
     for (int d2 = 0; d2 < d1; d2++) {
-      Q<double> t = qmul(qmul(qmul(qd(U_at(L, U, x, d2)), qd(U_at(L, U, shift(L, x, d2, +1), d1))),
-                              qconj(qd(U_at(L, U, shift(L, x, d1, +1), d2)))),
-                         qconj(qd(U_at(L, U, x, d1))));
+      Q<double> t = qmul(qmul(qmul(qd(U_at<M>(L, U, x, d2)), qd(U_at<M>(L, U, shift(L, x, d2, +1), d1))),
+                              qconj(qd(U_at<M>(L, U, shift(L, x, d1, +1), d2)))),
+                         qconj(qd(U_at<M>(L, U, x, d1))));
       wl += (1.0 - 0.5 * (2.0 * t.a));
     }
   wl *= L.beta;
@@ -1483,7 +1518,7 @@ __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
     double cov = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      Q<double> t = qmul(qmul(qconj(f), qd(U_at(L, U, x, d))), qd(F_at(L, F, shift(L, x, d, +1))));
+      Q<double> t = qmul(qmul(qconj(f), qd(U_at<M>(L, U, x, d))), qd(F_at(L, F, shift(L, x, d, +1))));
       double hop;
       if constexpr ((M & M_U1) != 0) {
         double al = A_at(L, A, x, d);
@@ -1504,7 +1539,7 @@ __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
       double mod2 = 0.5 * qdot(f2, f2), cov2 = 0.0;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        Q<double> t = qmul(qmul(qconj(f2), qd(U_at(L, U, x, d))), qd(F_at(L, F2, shift(L, x, d, +1))));
+        Q<double> t = qmul(qmul(qconj(f2), qd(U_at<M>(L, U, x, d))), qd(F_at(L, F2, shift(L, x, d, +1))));
         double hop;
         if constexpr ((M & M_U1) != 0) {
           double al = A_at(L, A, x, d);
@@ -1532,7 +1567,7 @@ __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       V3<T> nb = G_at(L, G, shift(L, x, d, +1));
-      V3<double> r = rotate(qd(U_at(L, U, x, d)), V3<double>{(double)nb.x, (double)nb.y, (double)nb.z});
+      V3<double> r = rotate(qd(U_at<M>(L, U, x, d)), V3<double>{(double)nb.x, (double)nb.y, (double)nb.z});
       hop += -(g.x * r.x + g.y * r.y + g.z * r.z);
     }
     acc[SU2HIP_SUM_SIGMA2] += tmod;
@@ -1547,7 +1582,7 @@ __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
     double ch = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++)
-      ch += magfield(L, U, G, shift(L, x, d, +1), d) - magfield(L, U, G, x, d);
+      ch += magfield<T, M>(L, U, G, shift(L, x, d, +1), d) - magfield<T, M>(L, U, G, x, d);
     acc[SU2HIP_SUM_MAGCHARGE] += ch;
     acc[SU2HIP_SUM_MAGCHARGE_ABS] += ::fabs(ch);
   }
@@ -1581,7 +1616,7 @@ template <typename T, int M>
 __global__ __launch_bounds__(BLK) void k_measure(const LatDesc *__restrict__ lats,
                                                  const BlockMap *__restrict__ blk,
                                                  double *__restrict__ partial) {
-  __shared__ double lds[BLK / 64];
+  __shared__ double lds[BLK / 64 * SU2HIP_NSUMS];
   const BlockMap bm = blk[blockIdx.x];
   const LatDesc &L = lats[bm.lat];
   if (!L.active) return;
@@ -1593,31 +1628,32 @@ __global__ __launch_bounds__(BLK) void k_measure(const LatDesc *__restrict__ lat
     measure_site<T, M>(L, 0, s, acc);
     measure_site<T, M>(L, 1, s, acc);
   }
-  double *out = partial + (size_t)(L.blk_base + bm.s0 / BLK) * SU2HIP_NSUMS;
-#pragma unroll
-  for (int j = 0; j < SU2HIP_NSUMS; j++) {
-    double t = block_sum(acc[j], lds);
-    if (threadIdx.x == 0) out[j] = t;
-  }
+  double t = block_sums<SU2HIP_NSUMS>(acc, lds);
+  if (threadIdx.x < SU2HIP_NSUMS)
+    partial[(size_t)(L.blk_base + bm.s0 / BLK) * SU2HIP_NSUMS + threadIdx.x] = t;
 }
 
 __global__ __launch_bounds__(BLK) void k_measure_final(const LatDesc *__restrict__ lats,
                                                        const double *__restrict__ partial,
                                                        double *__restrict__ sums) {
-  __shared__ double lds[BLK / 64];
+  __shared__ double lds[BLK / 64 * SU2HIP_NSUMS];
   const LatDesc &L = lats[blockIdx.x];
   double *out = sums + (size_t)blockIdx.x * SU2HIP_NSUMS;
   if (!L.active) {
     if (threadIdx.x < SU2HIP_NSUMS) out[threadIdx.x] = 0.0;
     return;
   }
-  for (int j = 0; j < SU2HIP_NSUMS; j++) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < L.nblk; i += BLK)
-      v += partial[(size_t)(L.blk_base + i) * SU2HIP_NSUMS + j];
-    double t = block_sum(v, lds);
-    if (threadIdx.x == 0) out[j] = t;
+  // each thread its rows in order, all sums of a row at once
+  double v[SU2HIP_NSUMS];
+#pragma unroll
+  for (int j = 0; j < SU2HIP_NSUMS; j++) v[j] = 0.0;
+  for (int i = threadIdx.x; i < L.nblk; i += BLK) {
+    const double *row = partial + (size_t)(L.blk_base + i) * SU2HIP_NSUMS;
+#pragma unroll
+    for (int j = 0; j < SU2HIP_NSUMS; j++) v[j] += row[j];
   }
+  double t = block_sums<SU2HIP_NSUMS>(v, lds);
+  if (threadIdx.x < SU2HIP_NSUMS) out[threadIdx.x] = t;
 }
 
 // ------------------------------------------------------------------------
@@ -1633,9 +1669,9 @@ __device__ __forceinline__ Q<double> qmul4(const Q<double> &a, const Q<double> &
 
 // clover_su2 (src/su2u1.c:207-275): the four plaquettes of the (d1, d2)
 // plane with a corner at x
-template <typename T>
+template <typename T, int M>
 __device__ Q<double> clover(const LatDesc &L, const T *U, const Site &x, int d1, int d2) {
-  auto u = [&](const Site &c, int d) { return qd(U_at(L, U, c, d)); };
+  auto u = [&](const Site &c, int d) { return qd(U_at<M>(L, U, c, d)); };
   Site x1 = shift(L, x, d1, +1), x2 = shift(L, x, d2, +1);
   Site m1 = shift(L, x, d1, -1), m2 = shift(L, x, d2, -1);
   Site x2m1 = shift(L, x2, d1, -1), m1m2 = shift(L, m1, d2, -1), m2p1 = shift(L, m2, d1, +1);
@@ -1673,9 +1709,9 @@ __global__ __launch_bounds__(BLK) void k_corr_sites(const LatDesc *__restrict__ 
       V3<T> gt = G_at(L, G, x);
       double a0 = gt.x, a1 = gt.y, a2 = gt.z;
       v[1] = 0.5 * (a0 * a0 + a1 * a1 + a2 * a2);
-      Q<double> cl = clover(L, U, x, d1, d2);
+      Q<double> cl = clover<T, M>(L, U, x, d1, d2);
       double tr = a0 * (cl.b / 4.0) + a1 * (cl.c / 4.0) + a2 * (cl.d / 4.0);
-      double al = 2.0 * alpha_proj(L, U, G, x, d1, d2);
+      double al = 2.0 * alpha_proj<T, M>(L, U, G, x, d1, d2);
 #pragma unroll
       for (int m = 1; m <= 2; m++) {
         // photon_correl's momentum: transverse momentum m along x only when
@@ -1762,7 +1798,7 @@ __global__ void k_wall(LatDesc L, int zdir) {
     }
     for (int d = 0; d < 3; d++) {
       double u = 1.0 - (lo ? 0.03 : 0.4) * rng_u53(r);
-      stU((T *)L.U, L.Vh, d, par, s, Q<T>{(T)u, (T)::sqrt(1.0 - u * u), (T)0, (T)0});
+      stU<M>((T *)L.U, L.Vh, d, par, s, Q<T>{(T)u, (T)::sqrt(1.0 - u * u), (T)0, (T)0});
     }
   }
 }
@@ -1782,9 +1818,9 @@ __device__ __forceinline__ Site site_at(const LatDesc &L, int x, int y, int z) {
 }
 
 // su2staple_wilson_onedir with dagger = 1 (src/staples.c:197-234)
-template <typename T>
+template <typename T, int M>
 __device__ Q<double> staple_onedir_dag(const LatDesc &L, const T *U, const Site &x, int mu, int nu) {
-  auto u = [&](const Site &c, int d) { return qd(U_at(L, U, c, d)); };
+  auto u = [&](const Site &c, int d) { return qd(U_at<M>(L, U, c, d)); };
   Site xp = shift(L, x, mu, +1), xn = shift(L, x, nu, +1), xm = shift(L, x, nu, -1);
   Site xpm = shift(L, xp, nu, -1);
   Q<double> up = qmul(u(xp, nu), qmul(qconj(u(xn, mu)), qconj(u(x, nu))));
@@ -1796,7 +1832,7 @@ __device__ Q<double> staple_onedir_dag(const LatDesc &L, const T *U, const Site 
 // for one level: each blocked site takes the smeared links (blocked
 // directions) or the plain links, and the smeared triplet, of the fine site
 // at twice its coordinates in the blocked directions
-template <typename T>
+template <typename T, int M>
 __global__ void k_block(LatDesc F, LatDesc B, int bmask) {
   int sb = blockIdx.x * blockDim.x + threadIdx.x;
   if (sb >= B.Vh) return;
@@ -1807,15 +1843,15 @@ __global__ void k_block(LatDesc F, LatDesc B, int bmask) {
                      (bmask & 4) ? 2 * cb.z : cb.z);
     int paths = 1 + 2 * (__popc(bmask) - 1);
     for (int d = 0; d < 3; d++) {
-      Q<double> r = qd(U_at(F, U, x, d));
+      Q<double> r = qd(U_at<M>(F, U, x, d));
       if (bmask & (1 << d)) {
         // smear_link (src/su2u1.c:752-797)
         Site xd = shift(F, x, d, +1);
-        Q<double> v2 = qd(U_at(F, U, xd, d));
+        Q<double> v2 = qd(U_at<M>(F, U, xd, d));
         for (int j = 0; j < 3; j++) {
           if (j == d || !(bmask & (1 << j))) continue;
-          r = qadd(r, staple_onedir_dag(F, U, x, d, j));
-          v2 = qadd(v2, staple_onedir_dag(F, U, xd, d, j));
+          r = qadd(r, staple_onedir_dag<T, M>(F, U, x, d, j));
+          v2 = qadd(v2, staple_onedir_dag<T, M>(F, U, xd, d, j));
         }
         r = Q<double>{r.a / paths, r.b / paths, r.c / paths, r.d / paths};
         v2 = Q<double>{v2.a / paths, v2.b / paths, v2.c / paths, v2.d / paths};
@@ -1823,7 +1859,7 @@ __global__ void k_block(LatDesc F, LatDesc B, int bmask) {
         double det = ::sqrt(qdot(r, r));
         r = Q<double>{r.a / det, r.b / det, r.c / det, r.d / det};
       }
-      stU((T *)B.U, B.Vh, d, par, sb, Q<T>{(T)r.a, (T)r.b, (T)r.c, (T)r.d});
+      stU<M>((T *)B.U, B.Vh, d, par, sb, Q<T>{(T)r.a, (T)r.b, (T)r.c, (T)r.d});
     }
     // smear_triplet (src/su2u1.c:810-857)
     V3<double> g0 = [&] { V3<T> t = G_at(F, G, x); return V3<double>{t.x, t.y, t.z}; }();
@@ -1833,8 +1869,8 @@ __global__ void k_block(LatDesc F, LatDesc B, int bmask) {
       if (!(bmask & (1 << d))) continue;
       Site xn = shift(F, x, d, +1), xm = shift(F, x, d, -1);
       V3<T> a = G_at(F, G, xn), b = G_at(F, G, xm);
-      V3<double> r1 = rotate(qd(U_at(F, U, x, d)), V3<double>{a.x, a.y, a.z});
-      V3<double> r2 = rotate(qconj(qd(U_at(F, U, xm, d))), V3<double>{b.x, b.y, b.z});
+      V3<double> r1 = rotate(qd(U_at<M>(F, U, x, d)), V3<double>{a.x, a.y, a.z});
+      V3<double> r2 = rotate(qconj(qd(U_at<M>(F, U, xm, d))), V3<double>{b.x, b.y, b.z});
       cov.x += r1.x;
       cov.y += r1.y;
       cov.z += r1.z;
@@ -1869,7 +1905,7 @@ __global__ void k_cold(LatDesc L, double phi0, double sigma0, double singlet0) {
   if (s >= L.Vh) return;
   for (int par = 0; par < 2; par++) {
     for (int d = 0; d < 3; d++) {
-      stU((T *)L.U, L.Vh, d, par, s, Q<T>{(T)1, (T)0, (T)0, (T)0});
+      stU<M>((T *)L.U, L.Vh, d, par, s, Q<T>{(T)1, (T)0, (T)0, (T)0});
       if constexpr ((M & M_U1) != 0) ((T *)L.A)[(size_t)(d * 2 + par) * L.Vh + s] = 0;
     }
     if constexpr ((M & M_NH) != 0) stF((T *)L.F, L.Vh, par, s, Q<T>{(T)phi0, (T)0, (T)0, (T)0});
@@ -1900,7 +1936,7 @@ __global__ void k_hot(LatDesc L) {
       gauss2(r, z[0], z[1]);
       gauss2(r, z[2], z[3]);
       double n = ::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3]);
-      stU((T *)L.U, L.Vh, d, par, s, Q<T>{(T)(z[0] / n), (T)(z[1] / n), (T)(z[2] / n), (T)(z[3] / n)});
+      stU<M>((T *)L.U, L.Vh, d, par, s, Q<T>{(T)(z[0] / n), (T)(z[1] / n), (T)(z[2] / n), (T)(z[3] / n)});
     }
     if constexpr ((M & M_U1) != 0)
       for (int d = 0; d < 3; d++)
@@ -1953,7 +1989,7 @@ __device__ __forceinline__ size_t soa_index(int field, int Vh, int par, int s, i
   return (size_t)(k * 2 + par) * Vh + s;
 }
 
-template <typename T>
+template <typename T, int M>
 __global__ void k_to_device(LatDesc L, int field, const double *__restrict__ h) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.Vh) return;
@@ -1964,13 +2000,16 @@ __global__ void k_to_device(LatDesc L, int field, const double *__restrict__ h) 
     if (field == 0) {
       for (int d = 0; d < 3; d++) {
         const double *q = h + i * 12 + d * 4;
-        Q<T> u{(T)q[0], (T)q[1], (T)q[2], (T)q[3]};
-        // the packed FP32 form holds unit links only
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (packed_links<M, T>()) {
+          // the packed form holds unit links only: normalise in FP64 first
           double n = 1.0 / ::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-          u = Q<T>{(T)(q[0] * n), (T)(q[1] * n), (T)(q[2] * n), (T)(q[3] * n)};
+          stU<M>((T *)dst, L.Vh, d, par, s,
+                 Q<T>{(T)(q[0] * n), (T)(q[1] * n), (T)(q[2] * n), (T)(q[3] * n)});
+        } else {
+          // four components: the values as given, rounded to T
+          st4((T *)dst + ((size_t)(d * 2 + par) * L.Vh + s) * 4,
+              Q<T>{(T)q[0], (T)q[1], (T)q[2], (T)q[3]});
         }
-        stU((T *)dst, L.Vh, d, par, s, u);
       }
       continue;
     }
@@ -1978,7 +2017,7 @@ __global__ void k_to_device(LatDesc L, int field, const double *__restrict__ h) 
   }
 }
 
-template <typename T>
+template <typename T, int M>
 __global__ void k_to_host(LatDesc L, int field, double *__restrict__ h) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.Vh) return;
@@ -1988,7 +2027,7 @@ __global__ void k_to_host(LatDesc L, int field, double *__restrict__ h) {
     size_t i = (size_t)par * L.Vh + s;
     if (field == 0) {
       for (int d = 0; d < 3; d++) {
-        Q<T> q = ldU((const T *)src, L.Vh, d, par, s);
+        Q<T> q = ldU<M>((const T *)src, L.Vh, d, par, s);
         double *o = h + i * 12 + d * 4;
         o[0] = (double)q.a;
         o[1] = (double)q.b;
@@ -2016,8 +2055,8 @@ __global__ void k_copy(const double2 *__restrict__ a, double2 *__restrict__ b, s
 template <typename T, int M>
 __device__ __forceinline__ V3<T> link_force(const LatDesc &L, const T *U, const T *G, const Site &x,
                                            int d) {
-  Q<T> st = qscale(wilson_staple(L, U, x, d), (T)(-L.beta / 8.0));
-  Q<T> u = U_at(L, U, x, d);
+  Q<T> st = qscale(wilson_staple<T, M>(L, U, x, d), (T)(-L.beta / 8.0));
+  Q<T> u = U_at<M>(L, U, x, d);
   if constexpr ((M & M_TR) != 0) {
     V3<T> g2 = G_at(L, G, shift(L, x, d, +1)), g1 = G_at(L, G, x);
     Q<T> t = qmul(qmul(Q<T>{(T)0, g2.x, g2.y, g2.z}, qconj(u)), Q<T>{(T)0, g1.x, g1.y, g1.z});
@@ -2032,7 +2071,7 @@ __device__ __forceinline__ V3<T> link_force(const LatDesc &L, const T *U, const 
 template <typename T, int M>
 __device__ __forceinline__ V3<T> triplet_force(const LatDesc &L, const T *U, const T *F, const T *G,
                                               const Site &x, double msq_t) {
-  V3<T> res = triplet_staple<T>(L, U, G, x);
+  V3<T> res = triplet_staple<T, M>(L, U, G, x);
   V3<T> g = G_at(L, G, x);
   T tsq = (T)0.5 * (g.x * g.x + g.y * g.y + g.z * g.z);
   T k = (T)(2.0 * 3 + msq_t) + (T)2 * (T)L.b4 * tsq;
@@ -2058,23 +2097,42 @@ __device__ __forceinline__ Q<T> expmul(const V3<T> &z, const Q<T> &v) {
   return qmul(u, v);
 }
 
-// stage 0: Euler (flow_gauge, src/gradflow.c:361-401); stages 1..3: RK3
+// Threads per site of a flow stage: FP32 one per link (6 per site), FP64
+// one per site doing its six links in turn -- measured at 64^3, 20.4 vs
+// 21.9 us per FP32 stage and 28.5 vs 35.6 us per FP64 stage; the per-site
+// form shares index arithmetic and cached neighbours across the six links.
+template <typename T>
+constexpr int flow_groups() {
+  return sizeof(T) == 4 ? 6 : 1;
+}
+
+// stage 0: Euler (flow_gauge, src/gradflow.c:361-401); stages 1..3: RK3.
+// Block b * G + g of the grid covers the sites of block-table entry b; with
+// G = 6 group g is parity g / 3, direction g % 3, with G = 1 the thread
+// does both parities and all directions.  The links of a stage are
+// independent (each reads Uin and its own Z, writes Uout and its own Z), so
+// the split does not change a bit; the direction-0 links also step the
+// triplet.
 template <typename T, int M>
 __global__ __launch_bounds__(BLK) void k_flow(const LatDesc *__restrict__ lats,
                                               const BlockMap *__restrict__ blk, const T *__restrict__ Uin,
                                               T *__restrict__ Uout, const T *__restrict__ Gin,
                                               T *__restrict__ Gout, T *__restrict__ Z,
                                               T *__restrict__ ZG, double dt, double msq_t, int stage) {
-  const BlockMap bm = blk[blockIdx.x];
+  constexpr int G = flow_groups<T>();
+  const int g = (int)(blockIdx.x % (unsigned)G);
+  const BlockMap bm = blk[blockIdx.x / (unsigned)G];
   const LatDesc &L = lats[bm.lat];
   int s = bm.s0 + (int)threadIdx.x;
   if (s >= L.Vh) return;
   const double cs[4] = {1.0, 0.25, 8.0 / 9.0, 0.75};
   const double ds[4] = {0.0, 0.0, 17.0 / 9.0, 1.0};
   T c = (T)(cs[stage] * dt), dd = (T)ds[stage];
-  for (int par = 0; par < 2; par++) {
+  const int p0 = G == 6 ? g / 3 : 0, p1 = G == 6 ? p0 + 1 : 2;
+  const int d0 = G == 6 ? g % 3 : 0, d1 = G == 6 ? d0 + 1 : 3;
+  for (int par = p0; par < p1; par++) {
     Site x = site_of(L, par, s);
-    for (int d = 0; d < 3; d++) {
+    for (int d = d0; d < d1; d++) {
       V3<T> f = link_force<T, M>(L, Uin, Gin, x, d);
       V3<T> z{c * f.x, c * f.y, c * f.z};
       if (stage > 0) {
@@ -2088,24 +2146,26 @@ __global__ __launch_bounds__(BLK) void k_flow(const LatDesc *__restrict__ lats,
         Z[o + st] = z.y;
         Z[o + 2 * st] = z.z;
       }
-      stU(Uout, L.Vh, d, par, s, expmul(z, ldU(Uin, L.Vh, d, par, s)));
+      stU<M>(Uout, L.Vh, d, par, s, expmul(z, ldU<M>(Uin, L.Vh, d, par, s)));
     }
     if constexpr ((M & M_TR) != 0) {
-      V3<T> f = triplet_force<T, M>(L, Uin, (const T *)L.F, Gin, x, msq_t);
-      V3<T> z{c * f.x, c * f.y, c * f.z};
-      if (stage > 0) {
-        size_t o = (size_t)par * L.Vh + s, st = (size_t)2 * L.Vh;
-        if (stage > 1) {
-          z.x -= dd * ZG[o];
-          z.y -= dd * ZG[o + st];
-          z.z -= dd * ZG[o + 2 * st];
+      if (d0 == 0) {
+        V3<T> f = triplet_force<T, M>(L, Uin, (const T *)L.F, Gin, x, msq_t);
+        V3<T> z{c * f.x, c * f.y, c * f.z};
+        if (stage > 0) {
+          size_t o = (size_t)par * L.Vh + s, st = (size_t)2 * L.Vh;
+          if (stage > 1) {
+            z.x -= dd * ZG[o];
+            z.y -= dd * ZG[o + st];
+            z.z -= dd * ZG[o + 2 * st];
+          }
+          ZG[o] = z.x;
+          ZG[o + st] = z.y;
+          ZG[o + 2 * st] = z.z;
         }
-        ZG[o] = z.x;
-        ZG[o + st] = z.y;
-        ZG[o + 2 * st] = z.z;
+        V3<T> gg = ldG(Gin, L.Vh, par, s);
+        stG(Gout, L.Vh, par, s, V3<T>{gg.x + z.x, gg.y + z.y, gg.z + z.z});
       }
-      V3<T> g = ldG(Gin, L.Vh, par, s);
-      stG(Gout, L.Vh, par, s, V3<T>{g.x + z.x, g.y + z.y, g.z + z.z});
     }
   }
 }
@@ -2151,6 +2211,20 @@ static hipError_t with_model(int model, Fn &&fn) {
   return hipErrorInvalidValue;
 }
 
+// The model bits of a launch in precision T: FP32 handles may add M_PK
+// (packed links), FP64 never does (no FP64 kernel is built with it).
+template <typename T, typename Fn>
+static hipError_t with_model_of(int model, Fn &&fn) {
+  if constexpr (sizeof(T) == 4) {
+    if (model & M_PK)
+      return with_model(model & ~M_PK, [&](auto m) -> hipError_t {
+        return fn(std::integral_constant<int, decltype(m)::value | M_PK>{});
+      });
+  }
+  if (model & M_PK) return hipErrorInvalidValue;
+  return with_model(model, fn);
+}
+
 template <typename Fn>
 static hipError_t with_prec(int fp64, Fn &&fn) {
   if (fp64) return fn(double(0));
@@ -2193,7 +2267,7 @@ hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir, int pa
   if (c.nblk == 0) return hipSuccess;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       switch (kind) {
         case SU2HIP_K_LINK_HEATBATH: return pass_one<T, M, SU2HIP_K_LINK_HEATBATH>(c, parity, dir, pass_off, muca_step, reunit);
@@ -2230,7 +2304,7 @@ hipError_t launch_orderparam_init(const LaunchCtx &c, int only) {
   if (c.nblk == 0) return hipSuccess;
   hipError_t e = with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_orderparam<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc,
                          c.muca, c.blk, c.nblk, c.partial, only);
@@ -2261,7 +2335,7 @@ hipError_t launch_measure(const LaunchCtx &c) {
   if (c.nblk > 0) {
     hipError_t e = with_prec(c.fp64, [&](auto t) {
       using T = decltype(t);
-      return with_model(c.model, [&](auto m) -> hipError_t {
+      return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
         constexpr int M = decltype(m)::value;
         hipLaunchKernelGGL((k_measure<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.blk,
                            c.partial);
@@ -2281,7 +2355,7 @@ hipError_t launch_measure(const LaunchCtx &c) {
 // bitwise those of k_measure + k_measure_final.
 template <typename T, int M>
 __global__ __launch_bounds__(BLK) void k_measure_one(LatDesc L, double *__restrict__ partial) {
-  __shared__ double lds[BLK / 64];
+  __shared__ double lds[BLK / 64 * SU2HIP_NSUMS];
   int s = (int)blockIdx.x * BLK + (int)threadIdx.x;
   double acc[SU2HIP_NSUMS];
 #pragma unroll
@@ -2290,23 +2364,23 @@ __global__ __launch_bounds__(BLK) void k_measure_one(LatDesc L, double *__restri
     measure_site<T, M>(L, 0, s, acc);
     measure_site<T, M>(L, 1, s, acc);
   }
-  double *out = partial + (size_t)blockIdx.x * SU2HIP_NSUMS;
-#pragma unroll
-  for (int j = 0; j < SU2HIP_NSUMS; j++) {
-    double t = block_sum(acc[j], lds);
-    if (threadIdx.x == 0) out[j] = t;
-  }
+  double t = block_sums<SU2HIP_NSUMS>(acc, lds);
+  if (threadIdx.x < SU2HIP_NSUMS) partial[(size_t)blockIdx.x * SU2HIP_NSUMS + threadIdx.x] = t;
 }
 
 __global__ __launch_bounds__(BLK) void k_measure_one_final(const double *__restrict__ partial,
                                                            int nblk, double *__restrict__ sums) {
-  __shared__ double lds[BLK / 64];
-  for (int j = 0; j < SU2HIP_NSUMS; j++) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += BLK) v += partial[(size_t)i * SU2HIP_NSUMS + j];
-    double t = block_sum(v, lds);
-    if (threadIdx.x == 0) sums[j] = t;
+  __shared__ double lds[BLK / 64 * SU2HIP_NSUMS];
+  double v[SU2HIP_NSUMS];
+#pragma unroll
+  for (int j = 0; j < SU2HIP_NSUMS; j++) v[j] = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += BLK) {
+    const double *row = partial + (size_t)i * SU2HIP_NSUMS;
+#pragma unroll
+    for (int j = 0; j < SU2HIP_NSUMS; j++) v[j] += row[j];
   }
+  double t = block_sums<SU2HIP_NSUMS>(v, lds);
+  if (threadIdx.x < SU2HIP_NSUMS) sums[threadIdx.x] = t;
 }
 
 hipError_t launch_measure_one(hipStream_t stream, int fp64, int model, const LatDesc &L,
@@ -2314,7 +2388,7 @@ hipError_t launch_measure_one(hipStream_t stream, int fp64, int model, const Lat
   int nb = (L.Vh + BLK - 1) / BLK;
   hipError_t e = with_prec(fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_measure_one<T, M>), dim3(nb), dim3(BLK), 0, stream, L, partial);
       return hipGetLastError();
@@ -2331,7 +2405,7 @@ hipError_t launch_corr(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, si
   int nb = (Vh + BLK - 1) / BLK;
   hipError_t e = with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_corr_sites<T, M>), dim3(nb), dim3(BLK), 0, c.stream, c.desc, slot, dir,
                          buf);
@@ -2348,8 +2422,11 @@ hipError_t launch_block(const LaunchCtx &c, const LatDesc &fine, const LatDesc &
   int nb = (blocked.Vh + BLK - 1) / BLK;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    hipLaunchKernelGGL((k_block<T>), dim3(nb), dim3(BLK), 0, c.stream, fine, blocked, bmask);
-    return hipGetLastError();
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_block<T, M>), dim3(nb), dim3(BLK), 0, c.stream, fine, blocked, bmask);
+      return hipGetLastError();
+    });
   });
 }
 
@@ -2358,7 +2435,7 @@ hipError_t launch_zprof(const LaunchCtx &c, int slot, int dir, int Vh, int Ld, s
   int nb = (Vh + BLK - 1) / BLK;
   hipError_t e = with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_zprof_sites<T, M>), dim3(nb), dim3(BLK), 0, c.stream, c.desc, slot, dir,
                          buf);
@@ -2385,7 +2462,7 @@ hipError_t launch_cold(const LaunchCtx &c, int slot, double phi0, double sigma0,
   int nb = (L.Vh + BLK - 1) / BLK;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_cold<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L, phi0, sigma0, singlet0);
       return hipGetLastError();
@@ -2400,7 +2477,7 @@ hipError_t launch_hot(const LaunchCtx &c, int slot) {
   int nb = (L.Vh + BLK - 1) / BLK;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_hot<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L);
       return hipGetLastError();
@@ -2415,7 +2492,7 @@ hipError_t launch_wall(const LaunchCtx &c, int slot, int zdir) {
   int nb = (L.Vh + BLK - 1) / BLK;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_wall<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L, zdir);
       return hipGetLastError();
@@ -2430,8 +2507,11 @@ hipError_t launch_to_device_layout(const LaunchCtx &c, int slot, int field, cons
   int nb = (L.Vh + BLK - 1) / BLK;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    hipLaunchKernelGGL(k_to_device<T>, dim3(nb), dim3(BLK), 0, c.stream, L, field, h);
-    return hipGetLastError();
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_to_device<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L, field, h);
+      return hipGetLastError();
+    });
   });
 }
 
@@ -2442,8 +2522,11 @@ hipError_t launch_to_host_layout(const LaunchCtx &c, int slot, int field, double
   int nb = (L.Vh + BLK - 1) / BLK;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    hipLaunchKernelGGL(k_to_host<T>, dim3(nb), dim3(BLK), 0, c.stream, L, field, h);
-    return hipGetLastError();
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      hipLaunchKernelGGL((k_to_host<T, M>), dim3(nb), dim3(BLK), 0, c.stream, L, field, h);
+      return hipGetLastError();
+    });
   });
 }
 
@@ -2461,9 +2544,10 @@ hipError_t launch_flow_stage(const LaunchCtx &c, int slot, const void *Uin, void
   if (c.nblk == 0) return hipSuccess;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
-      hipLaunchKernelGGL((k_flow<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.blk,
+      hipLaunchKernelGGL((k_flow<T, M>), dim3(flow_groups<T>() * c.nblk), dim3(BLK), 0, c.stream,
+                         c.desc, c.blk,
                          (const T *)Uin, (T *)Uout, (const T *)Gin, (T *)Gout, (T *)Z, (T *)ZG, dt,
                          msq_t, stage);
       return hipGetLastError();
@@ -2476,7 +2560,7 @@ hipError_t launch_flow_force(const LaunchCtx &c, int slot, double *force) {
   if (c.nblk == 0) return hipSuccess;
   return with_prec(c.fp64, [&](auto t) {
     using T = decltype(t);
-    return with_model(c.model, [&](auto m) -> hipError_t {
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_flow_force<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.blk,
                          force);

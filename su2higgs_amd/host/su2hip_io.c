@@ -356,6 +356,17 @@ int su2hip_config_read(const char *path, const char *model, su2hip_run_config *o
     }
   }
   p->reunitarize = p->precision == SU2HIP_FP32;
+  p->link_format = SU2HIP_LINKS_AUTO;
+  if (cfg_opt(&f, "link_format", v)) {
+    if (!strcasecmp(v, "full"))
+      p->link_format = SU2HIP_LINKS_FULL;
+    else if (!strcasecmp(v, "packed"))
+      p->link_format = SU2HIP_LINKS_PACKED;
+    else if (strcasecmp(v, "auto")) {
+      cfg_free(&f);
+      return ioerr(SU2HIP_EINVAL, "link_format '%s' (auto, full or packed)", v);
+    }
+  }
   if (cfg_opt(&f, "link_overrelax", v)) p->link_overrelax = (int)strtol(v, NULL, 10);
   if (cfg_opt(&f, "seed", v)) p->seed = strtoull(v, NULL, 10);
   if (cfg_opt(&f, "start", v)) c.hot_start = !strcasecmp(v, "hot");

@@ -4,7 +4,9 @@
  *   1. moves requests whose arrival time has passed into a FIFO queue,
  *   2. admits queued requests head first while their device footprint fits
  *      under the HBM cap and a slot is free (idle kept chains are evicted
- *      oldest first to make room),
+ *      oldest first to make room); while the head waits for room, up to
+ *      `backfill` later requests for new chains that fit now go ahead of it,
+ *      until the head has waited `max_head_wait` seconds,
  *   3. sweeps every running chain by min(tick_sweeps, smallest remaining
  *      budget) compound sweeps -- lattices whose budget is spent sit out --
  *      and waits for the stream,
@@ -55,6 +57,7 @@ void su2hip_serve_defaults(su2hip_serve_config *c) {
   p->update_su2triplet = 1;
   p->update_singlet = 1;
   p->reunitarize = 1;
+  p->link_format = SU2HIP_LINKS_PACKED; /* many resident chains: bytes dominate */
   p->betasu2 = 8.0;
   p->betau1 = 8.0;
   p->r_u1 = 1.0;
@@ -71,6 +74,8 @@ void su2hip_serve_defaults(su2hip_serve_config *c) {
   c->flow_dt = 0.01;
   c->flow_t_max = 4.0;
   c->phi0 = 1.0;
+  c->backfill = 32;
+  c->max_head_wait = 1.0;
   c->sigma0 = 0.0;
   c->singlet0 = 0.0;
 }
@@ -443,8 +448,25 @@ int su2hip_serve(const su2hip_serve_config *cfg, const su2hip_request *req, int 
       }
       if (a == 0) break;
       qh++;
+      while (qh < qt && queue[qh] < 0) qh++; /* admitted earlier by backfill */
     }
     if (rc) break;
+    if (cfg->backfill > 0 && qh < qt &&
+        elapsed(&S) - res[queue[qh]].t_arrival < cfg->max_head_wait) {
+      /* the head waits for room: later new chains that fit now go ahead
+       * (a continuation keeps the order of its chain) */
+      for (int i = qh + 1, looked = 0; i < qt && looked < cfg->backfill; i++) {
+        if (queue[i] < 0 || req[queue[i]].chain >= 0) continue;
+        looked++;
+        int a = admit(&S, queue[i]);
+        if (a < 0) {
+          rc = a;
+          break;
+        }
+        if (a == 1) queue[i] = -1;
+      }
+      if (rc) break;
+    }
     if (S.running == 0 && S.flowing) {
       /* only flows in flight: wait for the next one, or for an arrival */
       if (next >= n && qh == qt) {

@@ -3,6 +3,8 @@
 There is no per-site oracle for FP32 (different draw mapping and rounding);
 these tests pin it by invariants and by agreement with the FP64 path within
 statistical errors."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -13,11 +15,12 @@ from su2higgs_amd.lattice import Batch
 pytestmark = pytest.mark.gpu
 
 
-def test_fp32_roundtrip_rounds_to_float():
-    """Scalar fields and angles come back as their float rounding.  Links
-    are stored in the packed 12-byte form: they come back as the unit
+@pytest.mark.parametrize("links", [abi.LINKS_FULL, abi.LINKS_PACKED], ids=["full", "packed"])
+def test_fp32_roundtrip_rounds_to_float(links):
+    """Scalar fields and angles come back as their float rounding, and so
+    do links in the full form.  Packed links come back as the unit
     quaternion U/|U| to a few units in the last place."""
-    p = make_params(L=(8, 8, 8), nhiggs=1, u1=1, precision=FP32)
+    p = make_params(L=(8, 8, 8), nhiggs=1, u1=1, precision=FP32, link_format=links)
     b = Batch(p)
     rng = np.random.default_rng(1)
     f = abi.field_arrays(p)
@@ -26,11 +29,64 @@ def test_fp32_roundtrip_rounds_to_float():
     f["u1link"][:] = rng.normal(size=f["u1link"].shape)
     b.set_fields(0, f)
     g = b.get_fields(0)
-    for k in ("doublet", "u1link"):
-        assert np.array_equal(g[k], f[k].astype(np.float32).astype(np.float64))
-    unit = f["su2link"] / np.linalg.norm(f["su2link"], axis=-1, keepdims=True)
-    assert np.abs(g["su2link"] - unit).max() < 4e-7
-    assert np.abs(np.linalg.norm(g["su2link"], axis=-1) - 1).max() < 4e-7
+    for k in ("doublet", "u1link") + (("su2link",) if links == abi.LINKS_FULL else ()):
+        assert np.array_equal(g[k], f[k].astype(np.float32).astype(np.float64)), k
+    if links == abi.LINKS_PACKED:
+        unit = f["su2link"] / np.linalg.norm(f["su2link"], axis=-1, keepdims=True)
+        assert np.abs(g["su2link"] - unit).max() < 4e-7
+        assert np.abs(np.linalg.norm(g["su2link"], axis=-1) - 1).max() < 4e-7
+
+
+def slot_links(b, slot=0):
+    p = abi.Params()
+    abi.check(b.lib.su2hip_slot_params(b.h, slot, C.byref(p)))
+    return p.link_format
+
+
+def test_link_format_resolution():
+    """AUTO takes the full form below SU2HIP_LINKS_PACKED_ABOVE bytes of FP32
+    fields (in the full form) and the packed form above it, and packed for a
+    handle created empty; FP64 is always full; an explicit choice is kept."""
+    small = make_params(L=(32, 32, 32), nhiggs=1, u1=1, precision=FP32)   # 2.5 MB
+    big = make_params(L=(130, 130, 130), nhiggs=1, precision=FP32)         # 141 MB
+    assert slot_links(Batch(small)) == abi.LINKS_FULL
+    assert slot_links(Batch(big)) == abi.LINKS_PACKED
+    assert slot_links(Batch([small] * 60)) == abi.LINKS_PACKED            # 151 MB together
+    assert slot_links(Batch(make_params(L=(128,) * 3, precision=FP64))) == abi.LINKS_FULL
+    forced = make_params(L=(8, 8, 8), precision=FP32, link_format=abi.LINKS_PACKED)
+    assert slot_links(Batch(forced)) == abi.LINKS_PACKED
+    lib = abi.load()
+    h = C.c_void_p()
+    abi.check(lib.su2hip_create(0, C.byref(small), 2, 0, C.byref(h)))
+    slot = C.c_int()
+    abi.check(lib.su2hip_add(h, C.byref(small), C.byref(slot)))
+    p = abi.Params()
+    abi.check(lib.su2hip_slot_params(h, slot.value, C.byref(p)))
+    assert p.link_format == abi.LINKS_PACKED
+    lib.su2hip_destroy(h)
+
+
+@pytest.mark.parametrize("kind", [abi.K_LINK_OVERRELAX, abi.K_DOUBLET_OVERRELAX],
+                         ids=["link_or", "doublet_or"])
+def test_packed_and_full_links_make_the_same_move(kind):
+    """The same deterministic overrelaxation pass on the same fields in the
+    two FP32 link forms agrees to FP32 rounding (the packed form rebuilds a
+    component and moves the others by at most one unit in the last place)."""
+    import orc
+    p64 = make_params(L=(8, 8, 12), nhiggs=1, u1=1, betau1=9.0, seed=5)
+    o = orc.Oracle(p64)
+    o.hot_start()
+    f = o.get_fields()
+    out = {}
+    for links in (abi.LINKS_FULL, abi.LINKS_PACKED):
+        b = Batch(make_params(L=(8, 8, 12), nhiggs=1, u1=1, betau1=9.0, seed=5, precision=FP32,
+                              link_format=links))
+        b.set_fields(0, f)
+        b.pass_(kind, 1, 2)
+        out[links] = b.get_fields(0)
+    for k in ("su2link", "doublet"):
+        d = np.abs(out[abi.LINKS_FULL][k] - out[abi.LINKS_PACKED][k])
+        assert d.max() < 3e-6, (k, d.max())
 
 
 def test_fp32_packed_links_every_position_and_sign():
@@ -40,7 +96,7 @@ def test_fp32_packed_links_every_position_and_sign():
     places with either sign, ties among equal magnitudes, and the axes,
     come back to a few units in the last place with the sign of every
     component kept."""
-    p = make_params(L=(4, 4, 4), nhiggs=1, precision=FP32)
+    p = make_params(L=(4, 4, 4), nhiggs=1, precision=FP32, link_format=abi.LINKS_PACKED)
     V = abi.volume(p)
     rng = np.random.default_rng(7)
     q = rng.normal(size=(V, 3, 4))
@@ -70,9 +126,10 @@ def test_fp32_packed_links_every_position_and_sign():
     assert np.abs(b.get_fields(0)["su2link"] - g).max() < 2.5e-7
 
 
-def test_fp32_links_stay_unitary_and_angles_wrapped():
+@pytest.mark.parametrize("links", [abi.LINKS_FULL, abi.LINKS_PACKED], ids=["full", "packed"])
+def test_fp32_links_stay_unitary_and_angles_wrapped(links):
     p = make_params(L=(16, 16, 16), nhiggs=1, u1=1, precision=FP32, link_overrelax=4,
-                    betasu2=9.0, betau1=27.0, msq_phi=-0.2, lambda_phi=0.015)
+                    betasu2=9.0, betau1=27.0, msq_phi=-0.2, lambda_phi=0.015, link_format=links)
     b = Batch(p)
     b.hot_start(0)
     b.use_graph(True)
@@ -181,6 +238,27 @@ def test_fp32_heatbath_distribution():
     cdf /= cdf[-1]
     stat = stats.kstest(a0, lambda x: np.interp(x, xs, cdf))
     assert stat.pvalue > 1e-4, stat
+
+
+@pytest.mark.parametrize("links", [abi.LINKS_FULL, abi.LINKS_PACKED], ids=["full", "packed"])
+def test_fp32_heatbath_direction_is_isotropic(links):
+    """The FP32 heatbath draws the direction of a = (a0, a) on S^2 directly
+    (cos theta uniform in [-1, 1], phi uniform) instead of by the
+    reference's cube rejection: on a cold lattice (every staple the same)
+    the directions of the new links pass KS tests for uniform cos theta and
+    uniform phi, and their mean vector is zero within its error."""
+    from scipy import stats
+    p = make_params(L=(32, 32, 32), nhiggs=0, precision=FP32, betasu2=1.5, link_format=links)
+    b = Batch(p)
+    b.cold_start(0)
+    b.pass_(abi.K_LINK_HEATBATH, 0, 0)
+    u = b.get_fields(0)["su2link"][: abi.volume(p) // 2, 0, 1:]
+    n = u / np.linalg.norm(u, axis=1, keepdims=True)
+    assert stats.kstest(n[:, 2], "uniform", args=(-1, 2)).pvalue > 1e-4
+    phi = np.arctan2(n[:, 1], n[:, 0])
+    assert stats.kstest(phi, "uniform", args=(-np.pi, 2 * np.pi)).pvalue > 1e-4
+    m = n.mean(0)
+    assert (np.abs(m) < 4.5 * np.sqrt(1 / 3 / len(n))).all(), m
 
 
 MOVES = [

@@ -139,10 +139,15 @@ def scan_params(precision):
                         scalar_sweeps=4, seed=1000 + k) for k, L in enumerate(Ls)]
 
 
-def test_batched_scan_full_shape_same_alone_and_in_mix():
+@pytest.mark.parametrize("links", [abi.LINKS_PACKED, abi.LINKS_FULL], ids=["packed", "full"])
+def test_batched_scan_full_shape_same_alone_and_in_mix(links):
     """FP32, the whole 96-lattice batch in one launch per pass: a lattice's
-    chain is bitwise the one it follows alone."""
+    chain is bitwise the one it follows alone in the same link storage (the
+    batch alone would take the packed form, a small lattice alone the full
+    one: both are forced here, each in turn)."""
     ps = scan_params(FP32)
+    for p in ps:
+        p.link_format = links
     b = Batch(ps)
     for i in range(len(ps)):
         b.hot_start(i)

@@ -89,6 +89,36 @@ def test_hbm_cap_queues_but_does_not_change_results():
     assert res[-1].status == abi.ENOMEM and st["failed"] == 1 and st["completed"] == len(rows)
 
 
+@pytest.mark.parametrize("backfill", [0, 32], ids=["fifo", "backfill"])
+def test_backfill_admits_small_requests_past_a_blocked_head(backfill):
+    """Under an HBM cap that holds A with C but not A with B: with FIFO
+    admission C waits behind B, which waits for A to finish; with backfill
+    C starts beside A at once.  Every result is the same either way."""
+    import ctypes as C
+    lib = abi.load()
+
+    def fp(L):
+        v = C.c_uint64()
+        abi.check(lib.su2hip_footprint(C.byref(make_params(L=(L,) * 3)), 0, C.byref(v)))
+        return v.value
+
+    rows = [dict(L=16, sweeps=60, betasu2=8, msq_phi=-0.2, lambda_phi=0.02, seed=1),
+            dict(L=20, sweeps=10, betasu2=8, msq_phi=-0.2, lambda_phi=0.02, seed=2),
+            dict(L=8, sweeps=10, betasu2=8, msq_phi=-0.2, lambda_phi=0.02, seed=3)]
+    cap = fp(16) + fp(20) - 1
+    assert fp(16) + fp(8) <= cap
+    res, st = serve.serve(cfg(hbm_budget=cap, backfill=backfill, max_head_wait=60.0,
+                              tick_sweeps=2), serve.requests(rows))
+    assert st["completed"] == 3 and st["failed"] == 0
+    a, b, c = res
+    if backfill:
+        assert c.t_admit < b.t_admit and c.t_admit <= a.t_done
+    else:
+        assert c.t_admit >= b.t_admit >= a.t_done
+    ref, _ = serve.serve(cfg(), serve.requests(rows))  # no cap: all at once
+    assert all(same(x, y) for x, y in zip(res, ref))
+
+
 def test_chain_continuation_equals_one_long_request():
     one = [dict(L=12, sweeps=10, betasu2=9, msq_phi=-0.2, lambda_phi=0.02, seed=11)]
     long_, _ = serve.serve(cfg(), serve.requests(one))
@@ -125,7 +155,8 @@ def replay_alone(c, q):
     from su2higgs_amd.lattice import Batch
     p = make_params(L=(q.L,) * 3, precision=c.tmpl.precision, reunitarize=c.tmpl.reunitarize,
                     link_overrelax=c.tmpl.link_overrelax, scalar_sweeps=c.tmpl.scalar_sweeps,
-                    betasu2=q.betasu2, msq_phi=q.msq_phi, lambda_phi=q.lambda_phi, seed=q.seed)
+                    betasu2=q.betasu2, msq_phi=q.msq_phi, lambda_phi=q.lambda_phi, seed=q.seed,
+                    link_format=c.tmpl.link_format)  # the server's link storage
     b = Batch(p)
     b.cold_start(0, phi0=c.phi0, sigma0=c.sigma0, singlet0=c.singlet0)
     b.use_graph(True)

@@ -198,6 +198,23 @@ def test_config_missing_key_is_an_error(lib, tmp_path):
     assert code == -7 and b"lambda" in lib.su2hip_last_error()
 
 
+@pytest.mark.parametrize("value,want", [("", 0), ("link_format auto\n", 0),
+                                        ("link_format full\n", 1), ("link_format PACKED\n", 2),
+                                        ("link_format quarter\n", None)])
+def test_config_link_format(lib, tmp_path, value, want):
+    """The FP32 link storage key of this build (the reference ignores it):
+    auto (the default), full or packed, case-insensitive; anything else is
+    an error."""
+    path = str(tmp_path / "config")
+    with open(path, "w") as f:
+        f.write(CONFIG.format(L1=4, muca=0, op="phisq") + "precision fp32\n" + value)
+    code, rc = read_config(lib, path, "higgs")
+    if want is None:
+        assert code == -1 and b"link_format" in lib.su2hip_last_error()
+    else:
+        assert code == 0 and rc.p.link_format == want
+
+
 def test_config_odd_extent_is_an_error(lib, tmp_path):
     path = str(tmp_path / "config")
     with open(path, "w") as f:
