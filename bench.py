@@ -120,6 +120,10 @@ def observables(chains=OBS_CHAINS, L=OBS_L, therm=OBS_THERM, sweeps=OBS_SWEEPS, 
     from su2higgs_amd.lattice import Batch
     out = {"L": L, "chains": chains, "thermalization_sweeps": therm, "sampled_sweeps": sweeps,
            "every": every}
+    # 128 independent FP64 chains of the 48^3 theory at these couplings
+    # (written by tests/fp64_reference_48.py)
+    with open(os.path.join(ROOT, "tests", "fp64_reference_48.json")) as f:
+        ref = json.load(f)
     series = {}
     for prec in (abi.FP32, abi.FP64):
         ps = [workload_params(prec, L=L) for _ in range(chains)]
@@ -160,6 +164,13 @@ def observables(chains=OBS_CHAINS, L=OBS_L, therm=OBS_THERM, sweeps=OBS_SWEEPS, 
         se = float(np.hypot(e["fp32"][1], e["fp64"][1]))
         e["fp32_minus_fp64"] = [d, se]
         e["sigma"] = d / se
+        if L == 48 and name in ref:
+            # each precision against the 128-chain FP64 value of the same theory
+            r, re_ = ref[name]
+            e["fp64_reference_128"] = [r, re_]
+            for tag in ("fp32", "fp64"):
+                m, err = e[tag]
+                e[tag + "_vs_reference_sigma"] = (m - r) / float(np.hypot(err, re_))
         out[name] = e
     return out
 

@@ -1456,26 +1456,6 @@ __device__ double alpha_proj(const LatDesc &L, const T *U, const T *G, const Sit
   return ::atan2(u1.im[0] + u1.im[3], u1.re[0] + u1.re[3]) * ::sqrt(L.beta);
 }
 
-// magfield + magcharge_cube (src/magfield.c:221-280)
-template <typename T, int M>
-__device__ double magfield(const LatDesc &L, const T *U, const T *G, const Site &x, int dir) {
-  double res = 0.0;
-  for (int d1 = 0; d1 < 3; d1++) {
-    if (d1 == dir) continue;
-    for (int d2 = d1 + 1; d2 < 3; d2++) {
-      if (d2 == dir) continue;
-      double al = alpha_proj<T, M>(L, U, G, x, d1, d2);
-      if (d1 > dir)
-        res += al;
-      else if (d2 > dir)
-        res -= al;
-      else
-        res += al;
-    }
-  }
-  return res;
-}
-
 template <typename T, int M>
 __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
   const T *U = (const T *)L.U, *A = (const T *)L.A, *F = (const T *)L.F, *G = (const T *)L.G,
@@ -1579,12 +1559,7 @@ __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
       action += L.a2 * mod * tmod;
       acc[SU2HIP_SUM_PHI2SIGMA2] += mod * tmod;
     }
-    double ch = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-      ch += magfield<T, M>(L, U, G, shift(L, x, d, +1), d) - magfield<T, M>(L, U, G, x, d);
-    acc[SU2HIP_SUM_MAGCHARGE] += ch;
-    acc[SU2HIP_SUM_MAGCHARGE_ABS] += ::fabs(ch);
+    // the monopole charge of the cube at x is summed by k_magcharge
   }
   if constexpr ((M & M_SG) != 0) {
     double Sx = S_at(L, S, x);
@@ -1612,6 +1587,77 @@ __device__ void measure_site(const LatDesc &L, int par, int s, double *acc) {
   acc[SU2HIP_SUM_ACTION] += action;
 }
 
+// One edge of a projected plaquette, given the projectors at its two ends:
+// project_u1 (src/magfield.c:115-184) with halfproj evaluated by the caller.
+template <typename T, int M>
+__device__ __forceinline__ Cm edge_u1(const LatDesc &L, const T *U, const Cm &pl, const Site &x,
+                                      int d, const Cm &pr) {
+  Q<T> q = U_at<M>(L, U, x, d);
+  Cm u;
+  u.re[0] = q.a;
+  u.im[0] = q.d;
+  u.re[1] = q.c;
+  u.im[1] = q.b;
+  u.re[2] = -q.c;
+  u.im[2] = q.b;
+  u.re[3] = q.a;
+  u.im[3] = -q.d;
+  return cm_mul(cm_mul(pl, u, false), pr, false);
+}
+
+// alpha_proj (src/magfield.c:189-216) from its four projected edges
+__device__ __forceinline__ double face_angle(const LatDesc &L, Cm u1, const Cm &u2, const Cm &u3,
+                                             const Cm &u4) {
+  u1 = cm_mul(u1, u2, false);
+  u1 = cm_mul(u1, u3, true);
+  u1 = cm_mul(u1, u4, true);
+  return ::atan2(u1.im[0] + u1.im[3], u1.re[0] + u1.re[3]) * ::sqrt(L.beta);
+}
+
+// Monopole charge of the elementary cube at x, magcharge_cube
+// (src/magfield.c:262-279): sum over dir of magfield(x + dir) - magfield(x),
+// where magfield(y, 0) = +alpha(y; 1,2), magfield(y, 1) = -alpha(y; 0,2) and
+// magfield(y, 2) = +alpha(y; 0,1) (src/magfield.c:221-256).  The cube's six
+// faces share its eight corners; the projector of each corner is computed
+// once instead of once per edge end, which leaves every value unchanged.
+// Corner cabc sits at x + a e0 + b e1 + c e2.
+template <typename T, int M>
+__device__ __forceinline__ double magcharge_site(const LatDesc &L, int par, int s) {
+  const T *U = (const T *)L.U, *G = (const T *)L.G;
+  const Site c000 = site_of(L, par, s);
+  const Site c100 = shift(L, c000, 0, +1), c010 = shift(L, c000, 1, +1), c001 = shift(L, c000, 2, +1);
+  const Site c110 = shift(L, c100, 1, +1), c101 = shift(L, c100, 2, +1), c011 = shift(L, c010, 2, +1);
+  const Site c111 = shift(L, c110, 2, +1);
+  const Cm p000 = halfproj(G_at(L, G, c000)), p100 = halfproj(G_at(L, G, c100));
+  const Cm p010 = halfproj(G_at(L, G, c010)), p001 = halfproj(G_at(L, G, c001));
+  const Cm p110 = halfproj(G_at(L, G, c110)), p101 = halfproj(G_at(L, G, c101));
+  const Cm p011 = halfproj(G_at(L, G, c011)), p111 = halfproj(G_at(L, G, c111));
+  auto e = [&](const Cm &pl, const Site &y, int d, const Cm &pr) {
+    return edge_u1<T, M>(L, U, pl, y, d, pr);
+  };
+  // alpha(y; d1,d2) = face(P_y U_d1 P_y+d1, P_y+d1 U_d2 P_y+d1+d2, P_y+d2 U_d1 P_y+d1+d2, P_y U_d2 P_y+d2)
+  const double b0 = 0.0 + face_angle(L, e(p000, c000, 1, p010), e(p010, c010, 2, p011),
+                                     e(p001, c001, 1, p011), e(p000, c000, 2, p001));
+  const double b0n = 0.0 + face_angle(L, e(p100, c100, 1, p110), e(p110, c110, 2, p111),
+                                      e(p101, c101, 1, p111), e(p100, c100, 2, p101));
+  const double b1 = 0.0 - face_angle(L, e(p000, c000, 0, p100), e(p100, c100, 2, p101),
+                                     e(p001, c001, 0, p101), e(p000, c000, 2, p001));
+  const double b1n = 0.0 - face_angle(L, e(p010, c010, 0, p110), e(p110, c110, 2, p111),
+                                      e(p011, c011, 0, p111), e(p010, c010, 2, p011));
+  const double b2 = 0.0 + face_angle(L, e(p000, c000, 0, p100), e(p100, c100, 1, p110),
+                                     e(p010, c010, 0, p110), e(p000, c000, 1, p010));
+  const double b2n = 0.0 + face_angle(L, e(p001, c001, 0, p101), e(p101, c101, 1, p111),
+                                      e(p011, c011, 0, p111), e(p001, c001, 1, p011));
+  double ch = 0.0;
+  ch += b0n - b0;
+  ch += b1n - b1;
+  ch += b2n - b2;
+  return ch;
+}
+
+// Each thread measures site s of parity 0, then of parity 1, so every sum
+// of a thread is (0 + site0) + site1 in that order.  The loop is kept
+// rolled: interleaving the two sites doubles the live registers.
 template <typename T, int M>
 __global__ __launch_bounds__(BLK) void k_measure(const LatDesc *__restrict__ lats,
                                                  const BlockMap *__restrict__ blk,
@@ -1625,12 +1671,40 @@ __global__ __launch_bounds__(BLK) void k_measure(const LatDesc *__restrict__ lat
 #pragma unroll
   for (int j = 0; j < SU2HIP_NSUMS; j++) acc[j] = 0.0;
   if (s < L.Vh) {
-    measure_site<T, M>(L, 0, s, acc);
-    measure_site<T, M>(L, 1, s, acc);
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) measure_site<T, M>(L, par, s, acc);
   }
   double t = block_sums<SU2HIP_NSUMS>(acc, lds);
   if (threadIdx.x < SU2HIP_NSUMS)
     partial[(size_t)(L.blk_base + bm.s0 / BLK) * SU2HIP_NSUMS + threadIdx.x] = t;
+}
+
+// The two monopole sums of the adjoint model, written into the rows
+// k_measure has just filled (which hold 0 there): same threads, same sites,
+// same order, so each is bitwise what one fused kernel would give.  Kept
+// apart because the 24 projected links of a cube, in FP64, do not fit in
+// registers beside the other sums.
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_magcharge(const LatDesc *__restrict__ lats,
+                                                   const BlockMap *__restrict__ blk,
+                                                   double *__restrict__ partial) {
+  __shared__ double lds[BLK / 64 * 2];
+  const BlockMap bm = blk[blockIdx.x];
+  const LatDesc &L = lats[bm.lat];
+  if (!L.active) return;
+  int s = bm.s0 + (int)threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (s < L.Vh) {
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+      double ch = magcharge_site<T, M>(L, par, s);
+      acc[0] += ch;
+      acc[1] += ::fabs(ch);
+    }
+  }
+  double t = block_sums<2>(acc, lds);
+  if (threadIdx.x < 2)
+    partial[(size_t)(L.blk_base + bm.s0 / BLK) * SU2HIP_NSUMS + SU2HIP_SUM_MAGCHARGE + threadIdx.x] = t;
 }
 
 __global__ __launch_bounds__(BLK) void k_measure_final(const LatDesc *__restrict__ lats,
@@ -2098,30 +2172,51 @@ __device__ __forceinline__ Q<T> expmul(const V3<T> &z, const Q<T> &v) {
 }
 
 // Threads per site of a flow stage: FP32 one per link (6 per site), FP64
-// one per site doing its six links in turn -- measured at 64^3, 20.4 vs
-// 21.9 us per FP32 stage and 28.5 vs 35.6 us per FP64 stage; the per-site
-// form shares index arithmetic and cached neighbours across the six links.
+// one per site doing its six links in turn -- measured at 64^3, 13.3 us
+// per FP32 stage per link (with the six workgroups of a chunk on one XCD,
+// flow_grid) against 21.9 us per site, and 28.5 us per FP64 stage per site
+// against 35.6 us per link; the per-site form shares index arithmetic and
+// cached neighbours across the six links.
 template <typename T>
 constexpr int flow_groups() {
   return sizeof(T) == 4 ? 6 : 1;
 }
 
+// Work-groups go to the device's NXCD chiplets round-robin (block b to
+// XCD b % NXCD), each chiplet with its own L2.  A grid of flow_grid(n)
+// blocks covers block-table entries k < n with G blocks each, all G on
+// XCD k % NXCD so that they share the entry's neighbourhood in one L2
+// instead of fetching it into G of them.
+constexpr unsigned NXCD = 8;
+
+template <typename T>
+constexpr unsigned flow_grid(unsigned n) {
+  return flow_groups<T>() == 1 ? n : flow_groups<T>() * NXCD * ((n + NXCD - 1) / NXCD);
+}
+
 // stage 0: Euler (flow_gauge, src/gradflow.c:361-401); stages 1..3: RK3.
-// Block b * G + g of the grid covers the sites of block-table entry b; with
-// G = 6 group g is parity g / 3, direction g % 3, with G = 1 the thread
-// does both parities and all directions.  The links of a stage are
-// independent (each reads Uin and its own Z, writes Uout and its own Z), so
-// the split does not change a bit; the direction-0 links also step the
-// triplet.
+// With G = 6 groups per block-table entry, group g is parity g / 3 and
+// direction g % 3; with G = 1 the thread does both parities and all
+// directions.  The links of a stage are independent (each reads Uin and its
+// own Z, writes Uout and its own Z), so the split does not change a bit;
+// the direction-0 links also step the triplet.
 template <typename T, int M>
 __global__ __launch_bounds__(BLK) void k_flow(const LatDesc *__restrict__ lats,
-                                              const BlockMap *__restrict__ blk, const T *__restrict__ Uin,
+                                              const BlockMap *__restrict__ blk, int nblk,
+                                              const T *__restrict__ Uin,
                                               T *__restrict__ Uout, const T *__restrict__ Gin,
                                               T *__restrict__ Gout, T *__restrict__ Z,
                                               T *__restrict__ ZG, double dt, double msq_t, int stage) {
   constexpr int G = flow_groups<T>();
-  const int g = (int)(blockIdx.x % (unsigned)G);
-  const BlockMap bm = blk[blockIdx.x / (unsigned)G];
+  unsigned k = blockIdx.x;
+  int g = 0;
+  if constexpr (G > 1) {
+    const unsigned xcd = blockIdx.x % NXCD, i = blockIdx.x / NXCD;
+    k = (i / G) * NXCD + xcd;
+    g = (int)(i % G);
+    if (k >= (unsigned)nblk) return;
+  }
+  const BlockMap bm = blk[k];
   const LatDesc &L = lats[bm.lat];
   int s = bm.s0 + (int)threadIdx.x;
   if (s >= L.Vh) return;
@@ -2339,6 +2434,9 @@ hipError_t launch_measure(const LaunchCtx &c) {
         constexpr int M = decltype(m)::value;
         hipLaunchKernelGGL((k_measure<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.blk,
                            c.partial);
+        if constexpr ((M & M_TR) != 0)
+          hipLaunchKernelGGL((k_magcharge<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc,
+                             c.blk, c.partial);
         return hipGetLastError();
       });
     });
@@ -2361,11 +2459,29 @@ __global__ __launch_bounds__(BLK) void k_measure_one(LatDesc L, double *__restri
 #pragma unroll
   for (int j = 0; j < SU2HIP_NSUMS; j++) acc[j] = 0.0;
   if (s < L.Vh) {
-    measure_site<T, M>(L, 0, s, acc);
-    measure_site<T, M>(L, 1, s, acc);
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) measure_site<T, M>(L, par, s, acc);
   }
   double t = block_sums<SU2HIP_NSUMS>(acc, lds);
   if (threadIdx.x < SU2HIP_NSUMS) partial[(size_t)blockIdx.x * SU2HIP_NSUMS + threadIdx.x] = t;
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_magcharge_one(LatDesc L, double *__restrict__ partial) {
+  __shared__ double lds[BLK / 64 * 2];
+  int s = (int)blockIdx.x * BLK + (int)threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (s < L.Vh) {
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+      double ch = magcharge_site<T, M>(L, par, s);
+      acc[0] += ch;
+      acc[1] += ::fabs(ch);
+    }
+  }
+  double t = block_sums<2>(acc, lds);
+  if (threadIdx.x < 2)
+    partial[(size_t)blockIdx.x * SU2HIP_NSUMS + SU2HIP_SUM_MAGCHARGE + threadIdx.x] = t;
 }
 
 __global__ __launch_bounds__(BLK) void k_measure_one_final(const double *__restrict__ partial,
@@ -2391,6 +2507,8 @@ hipError_t launch_measure_one(hipStream_t stream, int fp64, int model, const Lat
     return with_model_of<T>(model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
       hipLaunchKernelGGL((k_measure_one<T, M>), dim3(nb), dim3(BLK), 0, stream, L, partial);
+      if constexpr ((M & M_TR) != 0)
+        hipLaunchKernelGGL((k_magcharge_one<T, M>), dim3(nb), dim3(BLK), 0, stream, L, partial);
       return hipGetLastError();
     });
   });
@@ -2546,9 +2664,8 @@ hipError_t launch_flow_stage(const LaunchCtx &c, int slot, const void *Uin, void
     using T = decltype(t);
     return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
-      hipLaunchKernelGGL((k_flow<T, M>), dim3(flow_groups<T>() * c.nblk), dim3(BLK), 0, c.stream,
-                         c.desc, c.blk,
-                         (const T *)Uin, (T *)Uout, (const T *)Gin, (T *)Gout, (T *)Z, (T *)ZG, dt,
+      hipLaunchKernelGGL((k_flow<T, M>), dim3(flow_grid<T>(c.nblk)), dim3(BLK), 0, c.stream,
+                         c.desc, c.blk, c.nblk, (const T *)Uin, (T *)Uout, (const T *)Gin, (T *)Gout, (T *)Z, (T *)ZG, dt,
                          msq_t, stage);
       return hipGetLastError();
     });

@@ -1,5 +1,5 @@
-"""Write the FP64 reference values FP64_REF_48 of tests/test_gpu_fp32.py
-(needs the GPU, about 5 minutes):
+"""Write the FP64 reference values tests/fp64_reference_48.json, read by
+tests/test_gpu_fp32.py and bench.py (needs the GPU, about 5 minutes):
 
     python tests/fp64_reference_48.py
 
@@ -7,7 +7,8 @@
 SU(2)xU(1)+Higgs theory at the single-lattice workload's couplings, the
 lean sweep mix of the test, cold start, 8000 sweeps discarded, then 32000
 sweeps measured every 10.  Prints each observable's mean over chains of
-the chain's time average and its error from the chain-to-chain spread.
+the chain's time average and its error from the chain-to-chain spread
+(the stored file keeps 9 and 2 significant digits).
 The FP64 path is the one pinned site by site to the oracle."""
 import json
 import os
@@ -24,3 +25,6 @@ if __name__ == "__main__":
     x = chain_series(FP64, 128, 8000, 32000, 10, seed0=64001)
     out = {name: [float(v) for v in chain_mean(series)] for name, series in x.items()}
     print(json.dumps(out))
+    with open(os.path.join(HERE, "fp64_reference_48.json"), "w") as f:
+        json.dump({k: [float("%.9f" % m), float("%.2g" % e)] for k, (m, e) in out.items()}, f)
+        f.write("\n")

@@ -4,6 +4,8 @@ There is no per-site oracle for FP32 (different draw mapping and rounding);
 these tests pin it by invariants and by agreement with the FP64 path within
 statistical errors."""
 import ctypes as C
+import json
+import os
 
 import numpy as np
 import pytest
@@ -146,8 +148,8 @@ def test_fp32_links_stay_unitary_and_angles_wrapped(links):
 # start, 40000 sweeps each measured every 10, the first 8000 discarded; each
 # value is the mean over chains of the chain's time average, its error the
 # chain-to-chain spread / sqrt(128).  Made with tests/fp64_reference_48.py.
-FP64_REF_48 = {"plaquette": (0.885931540, 3.3e-07), "phi2": (0.522692999, 4.2e-06),
-               "u1_plaquette": (0.987590678, 6.3e-08)}
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "fp64_reference_48.json")) as _f:
+    FP64_REF_48 = {k: tuple(v) for k, v in json.load(_f).items()}
 
 
 def lean48(precision, seed):
