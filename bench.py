@@ -581,7 +581,8 @@ def profile_set(sweeps=10):
     global check per parity half-sweep: k_muca_accept, k_muca_restore); the
     RK3 flow of a 64^3 SU(2)+Higgs lattice (dt 0.01 to t 0.5, a measurement
     every 10 steps); the 96^3 sweep's pass kinds at 192^3; the 96-lattice
-    scan.  Each in FP32-mixed and FP64.  Returns the wall time of each part."""
+    scan (autotuned and warmed up first, as the batched scan is timed).
+    Each in FP32-mixed and FP64.  Returns the wall time of each part."""
     from su2higgs_amd import abi
     from su2higgs_amd.abi import FP32, FP64, make_params
     from su2higgs_amd.lattice import Batch
@@ -637,7 +638,10 @@ def profile_set(sweeps=10):
         b = Batch(ps)
         for i in range(len(ps)):
             b.hot_start(i)
-        b.update(3)
+        # as the batched scan is timed: autotuned chunks and warmed-up
+        # fields (every launch here kernel by kernel)
+        b.autotune(repeats=20)
+        b.update(23)
         b.sync()
         b.close()
         out["scan96_" + tag] = time.perf_counter() - t0

@@ -2171,35 +2171,30 @@ __device__ __forceinline__ Q<T> expmul(const V3<T> &z, const Q<T> &v) {
   return qmul(u, v);
 }
 
-// Threads per site of a flow stage: FP32 one per link (6 per site), FP64
-// one per site doing its six links in turn -- measured at 64^3, 13.3 us
-// per FP32 stage per link (with the six workgroups of a chunk on one XCD,
-// flow_grid) against 21.9 us per site, and 28.5 us per FP64 stage per site
-// against 35.6 us per link; the per-site form shares index arithmetic and
-// cached neighbours across the six links.
-template <typename T>
-constexpr int flow_groups() {
-  return sizeof(T) == 4 ? 6 : 1;
-}
+// A flow stage runs one thread per link: six workgroups per block-table
+// entry, one per (parity, direction).  Measured at 64^3 per RK3 stage:
+// FP32 13.3-14.2 us per link against 21.9 us with one thread per site doing
+// its six links, FP64 26.8 against 30.0 us.
+constexpr int FLOW_GROUPS = 6;
 
 // Work-groups go to the device's NXCD chiplets round-robin (block b to
 // XCD b % NXCD), each chiplet with its own L2.  A grid of flow_grid(n)
-// blocks covers block-table entries k < n with G blocks each, all G on
-// XCD k % NXCD so that they share the entry's neighbourhood in one L2
-// instead of fetching it into G of them.
+// blocks covers block-table entries k < n with FLOW_GROUPS blocks each, all
+// on XCD k % NXCD (the XCD the block table gives entry k), so that they
+// share the entry's neighbourhood in one L2 instead of fetching it into six
+// of them (2.7x the algorithmic bytes when consecutive blocks took the six
+// groups of one entry).
 constexpr unsigned NXCD = 8;
 
-template <typename T>
 constexpr unsigned flow_grid(unsigned n) {
-  return flow_groups<T>() == 1 ? n : flow_groups<T>() * NXCD * ((n + NXCD - 1) / NXCD);
+  return FLOW_GROUPS * NXCD * ((n + NXCD - 1) / NXCD);
 }
 
 // stage 0: Euler (flow_gauge, src/gradflow.c:361-401); stages 1..3: RK3.
-// With G = 6 groups per block-table entry, group g is parity g / 3 and
-// direction g % 3; with G = 1 the thread does both parities and all
-// directions.  The links of a stage are independent (each reads Uin and its
-// own Z, writes Uout and its own Z), so the split does not change a bit;
-// the direction-0 links also step the triplet.
+// Group g of a block-table entry is parity g / 3, direction g % 3.  The
+// links of a stage are independent (each reads Uin and its own Z, writes
+// Uout and its own Z), so the split does not change a bit; the
+// direction-0 links also step the triplet.
 template <typename T, int M>
 __global__ __launch_bounds__(BLK) void k_flow(const LatDesc *__restrict__ lats,
                                               const BlockMap *__restrict__ blk, int nblk,
@@ -2207,15 +2202,10 @@ __global__ __launch_bounds__(BLK) void k_flow(const LatDesc *__restrict__ lats,
                                               T *__restrict__ Uout, const T *__restrict__ Gin,
                                               T *__restrict__ Gout, T *__restrict__ Z,
                                               T *__restrict__ ZG, double dt, double msq_t, int stage) {
-  constexpr int G = flow_groups<T>();
-  unsigned k = blockIdx.x;
-  int g = 0;
-  if constexpr (G > 1) {
-    const unsigned xcd = blockIdx.x % NXCD, i = blockIdx.x / NXCD;
-    k = (i / G) * NXCD + xcd;
-    g = (int)(i % G);
-    if (k >= (unsigned)nblk) return;
-  }
+  const unsigned xcd = blockIdx.x % NXCD, i = blockIdx.x / NXCD;
+  const unsigned k = (i / FLOW_GROUPS) * NXCD + xcd;
+  const int g = (int)(i % FLOW_GROUPS);
+  if (k >= (unsigned)nblk) return;
   const BlockMap bm = blk[k];
   const LatDesc &L = lats[bm.lat];
   int s = bm.s0 + (int)threadIdx.x;
@@ -2223,8 +2213,8 @@ __global__ __launch_bounds__(BLK) void k_flow(const LatDesc *__restrict__ lats,
   const double cs[4] = {1.0, 0.25, 8.0 / 9.0, 0.75};
   const double ds[4] = {0.0, 0.0, 17.0 / 9.0, 1.0};
   T c = (T)(cs[stage] * dt), dd = (T)ds[stage];
-  const int p0 = G == 6 ? g / 3 : 0, p1 = G == 6 ? p0 + 1 : 2;
-  const int d0 = G == 6 ? g % 3 : 0, d1 = G == 6 ? d0 + 1 : 3;
+  const int p0 = g / 3, p1 = p0 + 1;
+  const int d0 = g % 3, d1 = d0 + 1;
   for (int par = p0; par < p1; par++) {
     Site x = site_of(L, par, s);
     for (int d = d0; d < d1; d++) {
@@ -2664,7 +2654,7 @@ hipError_t launch_flow_stage(const LaunchCtx &c, int slot, const void *Uin, void
     using T = decltype(t);
     return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
       constexpr int M = decltype(m)::value;
-      hipLaunchKernelGGL((k_flow<T, M>), dim3(flow_grid<T>(c.nblk)), dim3(BLK), 0, c.stream,
+      hipLaunchKernelGGL((k_flow<T, M>), dim3(flow_grid(c.nblk)), dim3(BLK), 0, c.stream,
                          c.desc, c.blk, c.nblk, (const T *)Uin, (T *)Uout, (const T *)Gin, (T *)Gout, (T *)Z, (T *)ZG, dt,
                          msq_t, stage);
       return hipGetLastError();

@@ -221,6 +221,56 @@ def test_fp32_observables_match_fp64_over_many_chains():
         assert agrees(m, e, ref, ref_err), (name, "FP64 check of the stored value", m, e)
 
 
+def adjoint_series(precision, chains, therm, sweeps, every, seed0, L=24):
+    """[samples, chains] series of the plaquette, the triplet moments
+    <Sigma^2/2> and <(Sigma^2/2)^2> per site and the hopping term per link
+    of independent SU(2)+adjoint chains at the
+    multicanonical workload's couplings (betaG = 10, msq = -0.12, b4 = 0.01),
+    swept in one batch from a cold start with sigma = 0.5."""
+    ps = [make_params(L=(L, L, L), nhiggs=0, triplet=1, precision=precision, betasu2=10.0,
+                      msq_triplet=-0.12, b4=0.01, a2=0.0, link_overrelax=4, scalar_sweeps=4,
+                      reunitarize=1 if precision == FP32 else 0, seed=seed0 + 13 * k)
+          for k in range(chains)]
+    b = Batch(ps)
+    for k in range(chains):
+        b.cold_start(k, sigma0=0.5)
+    b.use_graph(True)
+    b.update(therm)
+    V = float(abi.volume(ps[0]))
+    rows = []
+    for _ in range(sweeps // every):
+        b.update(every)
+        rows.append(b.measure_all()[:chains])
+    b.close()
+    s = np.array(rows)
+    return {"plaquette": 1 - s[:, :, abi.SUM["wilson"]] / (V * 10.0) / 3,
+            "sigma2": s[:, :, abi.SUM["sigma2"]] / V,
+            "sigma4": s[:, :, abi.SUM["sigma4"]] / V,
+            "hopping": s[:, :, abi.SUM["hop_sigma"]] / (3 * V)}
+
+
+def test_fp32_adjoint_observables_match_fp64_over_many_chains():
+    """SU(2)+adjoint, 24^3: 64 FP32-mixed chains against 64 FP64 chains
+    (independent keys), each observable's mean over chains within 3.5
+    combined standard errors after a plateau check in both precisions.
+    The errors are small enough that a shift of 1e-4 in the plaquette or of
+    0.2% in a triplet moment would fail the same check."""
+    x32 = adjoint_series(FP32, 64, 1000, 4000, 10, seed0=240001)
+    x64 = adjoint_series(FP64, 64, 1000, 4000, 10, seed0=640001)
+    for name in ("plaquette", "sigma2", "sigma4", "hopping"):
+        for x in (x32[name], x64[name]):
+            h = len(x) // 2
+            m1, e1 = chain_mean(x[:h])
+            m2, e2 = chain_mean(x[h:])
+            assert agrees(m1, e1, m2, e2), (name, "no plateau", m1, e1, m2, e2)
+        m32, e32 = chain_mean(x32[name])
+        m64, e64 = chain_mean(x64[name])
+        assert agrees(m32, e32, m64, e64), (name, m32, e32, m64, e64)
+        shift = 1e-4 if name == "plaquette" else 2e-3 * abs(m64)
+        for sh in (shift, -shift):
+            assert not agrees(m32 + sh, e32, m64, e64), (name, "too weak to see", sh, e32, e64)
+
+
 def test_fp32_heatbath_distribution():
     """a0 = (U W)_0 / |W| of freshly heatbathed links follows
     sqrt(1-a0^2) exp(2 k a0) (Kennedy-Pendleton); checked on a cold lattice
