@@ -88,15 +88,22 @@ def algo_words(kind, p):
     return None
 
 
-def load_traffic(kernel, workload):
-    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/*.json)."""
+def load_traffic(kernel, workload, precision, packed):
+    """HBM bytes per launch of one pass kind, measured by rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, calibrated) and summarised in
+    profiles/hbm_traffic.json: the row of that workload, precision and link
+    form with the most launches (the chunk the profiled run kept)."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
-            d = json.load(f)
-        e = d.get(workload, {}).get(kernel)
-        return None if e is None else float(e["bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
+            rows = json.load(f)["kernels"]
+        cand = [r for r in rows if r["workload"] == workload and r["kernel"] == kernel
+                and r["precision"] == precision and r["hbm_MB"] is not None
+                and bool((r["model_bits"] or 0) & 64) == packed]
+        if not cand:
+            return None
+        return float(max(cand, key=lambda r: r["launches"])["hbm_MB"]) * 1e6
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
@@ -261,11 +268,12 @@ def roofline(res):
     words = algo_words(kid, p)
     bytes_per_launch = words * res["V"] * w
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    workload = "96^3 SU(2)xU(1)+Higgs fp%d" % (64 if p.precision == abi.FP64 else 32)
+    prec = "fp64" if p.precision == abi.FP64 else "fp32"
+    packed = res["link_format"] == "packed"
     span_ms = res["kernel_span_ms"][kind]
     return {"bound": "hbm", "kernel": kind, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(kind, workload),
+            "traffic": load_traffic(kind, "96cubed", prec, packed),
             "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
             "avg_span_ms": span_ms,
             "frac_with_launch_gaps": bytes_per_launch / (span_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
