@@ -198,6 +198,7 @@ def run_precision(precision, steps, warmup, timing_steps=100, graph=True):
     # workgroup chunk per pass kind picked by the autotuner (state restored)
     tune = b.autotune(repeats=50)
     picks = {name: b.chunk(abi.KIND_NAMES.index(name)) for name in tune}
+    waves = {name: b.waves(abi.KIND_NAMES.index(name)) for name in tune}
     b.use_graph(graph)
     b.update(warmup)
     b.sync()
@@ -208,7 +209,7 @@ def run_precision(precision, steps, warmup, timing_steps=100, graph=True):
     lp = slot_params(b)
     res = {"precision": precision, "V": V, "passes_per_sweep": npass, "updates_per_sweep_over_V": upd,
            "link_format": {abi.LINKS_FULL: "full", abi.LINKS_PACKED: "packed"}[lp.link_format],
-           "autotune_ms_64_128_256": tune, "chunk": picks,
+           "autotune_ms_64_128_256": tune, "chunk": picks, "waves": waves,
            "wall_s": wall, "ms_per_sweep": 1e3 * wall / steps,
            "site_updates_per_s": upd * V * steps / wall}
     b.use_graph(False)
@@ -786,6 +787,8 @@ def main():
         "kernel_avg_ms_fp64": r64["kernel_avg_ms"],
         "chunk_fp32": r32["chunk"],
         "chunk_fp64": r64["chunk"],
+        "waves_fp32": r32["waves"],
+        "waves_fp64": r64["waves"],
         "autotune_fp32": r32["autotune_ms_64_128_256"],
         "kernel_span_ms_fp32": r32["kernel_span_ms"],
         "kernel_span_ms_fp64": r64["kernel_span_ms"],

@@ -451,13 +451,23 @@ int su2hip_time_pass(su2hip_batch_t *b, int kind, int repeats, double *ms_kernel
 int su2hip_set_chunk(su2hip_batch_t *b, int kind, int chunk);
 /* The chunk a pass kind launches with now. */
 int su2hip_get_chunk(su2hip_batch_t *b, int kind, int *chunk);
-/* Time every pass kind of the batch's sweep with each chunk (kernel
- * durations of `repeats` back-to-back launches, as su2hip_time_pass) and
- * keep the fastest; 256 stays unless a smaller chunk is 2% faster.  Fields,
- * RNG positions, budgets and counters are restored afterwards (the handle
- * needs one more copy of its lattices' memory meanwhile).  table, if not
- * NULL, receives the milliseconds as table[kind * 3 + {0,1,2}] for chunks
- * 64, 128, 256 (entries of kinds not in the sweep are left alone). */
+/* Occupancy target of a pass kind's kernel: 0 = the compiler's own register
+ * allocation, 5 = registers capped so that 5 waves fit per SIMD.  The link
+ * overrelaxation is built both ways (default 5); every other kind only
+ * with 0, and asking it for 5 fails with SU2HIP_EINVAL.  Which one is
+ * faster depends on the lattice; results do not depend on it. */
+int su2hip_set_waves(su2hip_batch_t *b, int kind, int waves);
+/* The occupancy target a pass kind launches with now. */
+int su2hip_get_waves(su2hip_batch_t *b, int kind, int *waves);
+/* Time every pass kind of the batch's sweep with each chunk, and with each
+ * occupancy target it is built with (kernel durations of `repeats`
+ * back-to-back launches, as su2hip_time_pass), and keep the fastest; 256
+ * stays unless a smaller chunk is 2% faster, and the default target unless
+ * the other is 1% faster.  Fields, RNG positions, budgets and counters are
+ * restored afterwards (the handle needs one more copy of its lattices'
+ * memory meanwhile).  table, if not NULL, receives the milliseconds as
+ * table[kind * 3 + {0,1,2}] for chunks 64, 128, 256, the faster target's
+ * for each chunk (entries of kinds not in the sweep are left alone). */
 int su2hip_autotune(su2hip_batch_t *b, int repeats, double *table);
 
 /* Copy `bytes` of device memory with the library's streaming copy kernel and

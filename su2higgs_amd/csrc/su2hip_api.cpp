@@ -127,6 +127,7 @@ struct su2hip_batch {
   AltMap alt[2];                  // chunks 64 and 128; 256 is the main table
   hipStream_t flow_stream = nullptr;  // asynchronous flows (su2hip_flow_start)
   int tune[SU2HIP_NKINDS] = {};   // chunk per pass kind, 0 = chosen by size
+  int waves[SU2HIP_NKINDS] = {};  // occupancy target per pass kind (su2hip_set_waves)
   bool random_order = false;      // the reference's random_sweeps
   uint8_t *d_order = nullptr;     // [max_slots][passes] parity | dir << 1
   uint16_t *d_info = nullptr;     // [passes] step_info of each step
@@ -260,6 +261,7 @@ static int chunk_for(su2hip_batch *b, int kind) {
 
 static LaunchCtx ctx_for(su2hip_batch *b, int kind) {
   LaunchCtx c = ctx_of(b);
+  if (kind >= 0 && kind < SU2HIP_NKINDS) c.waves = b->waves[kind];
   int ch = chunk_for(b, kind);
   for (int k = 0; k < 2; k++)
     if (ch == b->alt[k].chunk) {
@@ -677,6 +679,7 @@ static hipError_t enqueue_step(su2hip_batch *b, const Step &st, bool in_sweep = 
       return launch_pass(ct, st.kind, st.parity, st.dir, st.pass_off, -1, reunit);
     });
   }
+  c.waves = b->waves[st.kind];  // multicanonical steps keep the 256-site table
   for (int k = 0; k < b->muca_steps; k++) {
     hipError_t e = timed(b, st.kind, [&] {
       return launch_pass(c, st.kind, st.parity, st.dir, st.pass_off, k, reunit);
@@ -950,6 +953,7 @@ int su2hip_create(int n, const su2hip_params *params, int max_slots, uint64_t hb
   int rs = noalloc_scratch_init();
   if (rs) return rs;
   su2hip_batch *b = new su2hip_batch();
+  b->waves[SU2HIP_K_LINK_OVERRELAX] = 5;
   b->max_slots = max_slots;
   b->hbm_budget = hbm_budget;
   b->tmpl = params[0];
@@ -2278,6 +2282,28 @@ int su2hip_get_chunk(su2hip_batch_t *b, int kind, int *chunk) {
   return SU2HIP_OK;
 }
 
+// the occupancy targets a pass kind's kernel is built with
+static bool waves_valid(int kind, int waves) {
+  return waves == 0 || (waves == 5 && kind == SU2HIP_K_LINK_OVERRELAX);
+}
+
+int su2hip_set_waves(su2hip_batch_t *b, int kind, int waves) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  if (kind < 0 || kind >= NCOUNT) return fail(SU2HIP_EINVAL, "kind %d is not a pass kind", kind);
+  if (!waves_valid(kind, waves))
+    return fail(SU2HIP_EINVAL, "waves %d for kind %d (0, or 5 for link overrelaxation)", waves, kind);
+  b->waves[kind] = waves;
+  b->graph_valid = false;
+  return SU2HIP_OK;
+}
+
+int su2hip_get_waves(su2hip_batch_t *b, int kind, int *waves) {
+  if (!b || !waves) return fail(SU2HIP_EINVAL, "NULL argument");
+  if (kind < 0 || kind >= NCOUNT) return fail(SU2HIP_EINVAL, "kind %d is not a pass kind", kind);
+  *waves = b->waves[kind];
+  return SU2HIP_OK;
+}
+
 int su2hip_autotune(su2hip_batch_t *b, int repeats, double *table) {
   TraceRange tr_("su2hip_autotune");
   if (!b || repeats < 1) return fail(SU2HIP_EINVAL, "NULL handle or repeats < 1");
@@ -2316,24 +2342,41 @@ int su2hip_autotune(su2hip_batch_t *b, int repeats, double *table) {
       return fail(SU2HIP_EDEVICE, "autotune snapshot: %s", hipGetErrorString(e));
     }
   }
-  int saved_tune[SU2HIP_NKINDS];
+  int saved_tune[SU2HIP_NKINDS], saved_waves[SU2HIP_NKINDS];
   memcpy(saved_tune, b->tune, sizeof(saved_tune));
+  memcpy(saved_waves, b->waves, sizeof(saved_waves));
   for (int k = 0; k < NCOUNT && !rc; k++) {
     if (!in_sweep[k]) continue;
+    // candidates: every chunk, with each occupancy target the kind has; a
+    // candidate's time is weighed by 1.02 unless its chunk is 256 and by
+    // 1.01 unless its target is the kind's default, so a smaller chunk has
+    // to win by 2% and the other target by 1% to be kept
+    const int dflt = k == SU2HIP_K_LINK_OVERRELAX ? 5 : 0;
     double best = 0.0;
-    int pick = 256;
+    int pick = 256, pick_w = dflt;
+    bool first = true;
     for (int c = 0; c < NCHUNK && !rc; c++) {
-      b->tune[k] = CHUNKS[c];
-      double ms = 0.0;
-      rc = time_pass_impl(b, k, repeats, &ms, nullptr);
-      if (table) table[k * NCHUNK + c] = ms;
-      // a smaller workgroup has to win by 2% to replace 256
-      if (!rc && (c == 0 || ms < best * (CHUNKS[c] == 256 ? 1.02 : 1.0))) {
-        best = ms;
-        pick = CHUNKS[c];
+      double row = 0.0;
+      for (int w : {5, 0}) {
+        if (!waves_valid(k, w) || rc) continue;
+        b->tune[k] = CHUNKS[c];
+        b->waves[k] = w;
+        double ms = 0.0;
+        rc = time_pass_impl(b, k, repeats, &ms, nullptr);
+        if (rc) break;
+        row = (row == 0.0 || ms < row) ? ms : row;
+        double score = ms * (CHUNKS[c] == 256 ? 1.0 : 1.02) * (w == dflt ? 1.0 : 1.01);
+        if (first || score < best) {
+          best = score;
+          pick = CHUNKS[c];
+          pick_w = w;
+          first = false;
+        }
       }
+      if (table && !rc) table[k * NCHUNK + c] = row;
     }
     b->tune[k] = rc ? saved_tune[k] : pick;
+    b->waves[k] = rc ? saved_waves[k] : pick_w;
   }
   // restore the state
   int rc2 = fold_counts(b);

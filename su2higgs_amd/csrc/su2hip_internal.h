@@ -138,6 +138,10 @@ struct LaunchCtx {
   // threads per workgroup of pass kernels = sites per block-table chunk
   // (64, 128 or 256); blk/nblk/cnt then belong to the table of that chunk
   int block = BLK;
+  // occupancy target of the pass kernel: 0 = the compiler's own register
+  // allocation, 5 = registers capped for 5 waves per SIMD (link
+  // overrelaxation only; see su2hip_set_waves)
+  int waves = 0;
   // random pass order: per lattice and sweep step, parity | direction << 1
   // ([max_slots][npass]); null = the launch's parity and direction
   const uint8_t *order = nullptr;

@@ -1013,9 +1013,11 @@ __global__ __launch_bounds__(BLK) void k_pass(K_PASS_ARGS) {
                         partial, order, npass, lds);
 }
 
-// link overrelaxation: the same pass with an occupancy target of 5 waves per
-// SIMD, which schedules the FP64 pass 4% faster at 96^3 (the other kinds
-// either do not change or spill under it)
+// The same pass with an occupancy target of 5 waves per SIMD, built for the
+// link overrelaxation only: whether it wins depends on the lattice (FP64
+// 96^3 22.5 against 22.9 us, 192^3 232 against 222 us, in one GPU
+// session), so the autotuner picks per handle; the other kinds either do
+// not change or spill under it.
 template <typename T, int M, int KIND>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5))) void k_pass_w5(
     K_PASS_ARGS) {
@@ -2324,16 +2326,20 @@ static hipError_t pass_one(const LaunchCtx &c, int parity, int dir, int pass_off
     // partials are indexed by 256-site chunk
     if (step >= 0 && c.block != BLK) return hipErrorInvalidValue;
     if constexpr (KIND == SU2HIP_K_LINK_OVERRELAX) {
-      if (c.ev_start || c.ev_stop) {
-        hipExtLaunchKernelGGL((k_pass_w5<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream,
-                              c.ev_start, c.ev_stop, 0, c.desc, c.muca, c.blk, parity, dir,
-                              pass_off, step, reunit, c.cnt, c.nblk, c.partial, c.order, c.npass);
-      } else {
-        hipLaunchKernelGGL((k_pass_w5<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream,
-                           c.desc, c.muca, c.blk, parity, dir, pass_off, step, reunit, c.cnt,
-                           c.nblk, c.partial, c.order, c.npass);
+      if (c.waves == 5) {
+        if (c.ev_start || c.ev_stop) {
+          hipExtLaunchKernelGGL((k_pass_w5<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream,
+                                c.ev_start, c.ev_stop, 0, c.desc, c.muca, c.blk, parity, dir,
+                                pass_off, step, reunit, c.cnt, c.nblk, c.partial, c.order, c.npass);
+        } else {
+          hipLaunchKernelGGL((k_pass_w5<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream,
+                             c.desc, c.muca, c.blk, parity, dir, pass_off, step, reunit, c.cnt,
+                             c.nblk, c.partial, c.order, c.npass);
+        }
+        return hipGetLastError();
       }
-    } else if (c.ev_start || c.ev_stop) {
+    }
+    if (c.ev_start || c.ev_stop) {
       hipExtLaunchKernelGGL((k_pass<T, M, KIND>), dim3(c.nblk), dim3(c.block), 0, c.stream,
                             c.ev_start, c.ev_stop, 0, c.desc, c.muca, c.blk, parity, dir, pass_off,
                             step, reunit, c.cnt, c.nblk, c.partial, c.order, c.npass);

@@ -195,9 +195,18 @@ class Batch:
         check(self.lib.su2hip_get_chunk(self.h, kind, C.byref(c)))
         return c.value
 
+    def set_waves(self, kind, waves):
+        check(self.lib.su2hip_set_waves(self.h, kind, waves))
+
+    def waves(self, kind):
+        w = C.c_int()
+        check(self.lib.su2hip_get_waves(self.h, kind, C.byref(w)))
+        return w.value
+
     def autotune(self, repeats=50):
-        """Pick the fastest workgroup chunk per pass kind; returns
-        {kind name: (ms at 64, 128, 256)} for the kinds of the sweep."""
+        """Pick the fastest workgroup chunk (and, where a kind has two, the
+        faster occupancy target) per pass kind; returns {kind name: (ms at
+        64, 128, 256)} for the kinds of the sweep."""
         t = np.full(abi.NKINDS * 3, np.nan)
         check(self.lib.su2hip_autotune(self.h, repeats, ptr(t)))
         return {abi.KIND_NAMES[k]: tuple(t[3 * k:3 * k + 3]) for k in range(abi.NKINDS)

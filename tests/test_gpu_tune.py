@@ -1,5 +1,6 @@
-"""Launch shapes: results do not depend on the workgroup chunk of the pass
-kernels, and the autotuner leaves the lattices exactly as it found them."""
+"""Launch shapes: results do not depend on the workgroup chunk or the
+occupancy target of the pass kernels, and the autotuner leaves the lattices
+exactly as it found them."""
 import numpy as np
 import pytest
 
@@ -65,6 +66,7 @@ def test_autotune_restores_the_state_and_picks_a_chunk():
         assert all(t > 0 for t in ms), (name, ms)
         k = abi.KIND_NAMES.index(name)
         assert b.chunk(k) in (64, 128, 256)
+        assert b.waves(k) in ((0, 5) if k == abi.K_LINK_OVERRELAX else (0,))
     assert same(state(b, n), before)
     assert list(b.budgets()) == list(budgets)
     # and the tuned handle continues the chains exactly as an untuned one
@@ -72,3 +74,35 @@ def test_autotune_restores_the_state_and_picks_a_chunk():
     c, _ = batch(FP32)
     c.update(4)
     assert same(state(b, n), state(c, n))
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+@pytest.mark.parametrize("graph", [False, True])
+def test_results_do_not_depend_on_the_occupancy_target(precision, graph):
+    """The link overrelaxation built for 5 waves per SIMD and built with the
+    compiler's own registers make the same moves, bit for bit, launched one
+    by one or from a captured graph."""
+    out = {}
+    for waves in (5, 0):
+        b, n = batch(precision)
+        b.use_graph(graph)
+        b.set_waves(abi.K_LINK_OVERRELAX, waves)
+        assert b.waves(abi.K_LINK_OVERRELAX) == waves
+        b.update(3)
+        out[waves] = state(b, n)
+        b.close()
+    assert same(out[5], out[0])
+
+
+def test_occupancy_targets_each_kind_is_built_with():
+    b, _ = batch(FP32)
+    assert b.waves(abi.K_LINK_OVERRELAX) == 5  # the default
+    assert b.waves(abi.K_LINK_HEATBATH) == 0
+    for kind, waves in ((abi.K_LINK_HEATBATH, 5), (abi.K_LINK_OVERRELAX, 3),
+                        (abi.K_DOUBLET_OVERRELAX, 5), (abi.K_MEASURE + 5, 0)):
+        with pytest.raises(abi.Su2hipError):
+            b.set_waves(kind, waves)
+    b.set_waves(abi.K_LINK_HEATBATH, 0)
+    b.set_waves(abi.K_LINK_OVERRELAX, 0)
+    assert b.waves(abi.K_LINK_OVERRELAX) == 0
+    b.close()
