@@ -2017,8 +2017,10 @@ static int flow_enqueue(su2hip_batch *b, int slot, int scheme, double dt, double
         memset(&L, 0, sizeof(L));
       }
     }
-  std::vector<BlockMap> bm(nb);
-  for (int k = 0; k < nb; k++) bm[k] = BlockMap{slot, k * BLK};
+  std::vector<BlockMap> seq(nb);
+  for (int k = 0; k < nb; k++) seq[k] = BlockMap{slot, k * BLK};
+  // XCD j gets a contiguous run of chunks, as in the sweep's table
+  std::vector<BlockMap> bm = deal_to_xcds(seq);
   size_t nrows = j.t.size();
   if (hipMalloc(&j.ddesc, sizeof(LatDesc) * hd.size()) != hipSuccess ||
       hipMalloc(&j.dblk, sizeof(BlockMap) * nb) != hipSuccess ||

@@ -122,6 +122,26 @@ def test_link_heatbath_wavefront_rounds(L, beta):
     assert o.rng_get()[1] == b.rng_get(0)[1]
 
 
+@pytest.mark.parametrize("L", [(8, 8, 8), (16, 8, 12)])
+def test_link_heatbath_loop_limit(L):
+    """Pure gauge at beta = 1e-6: the staples nearly vanish (a = 2|V| ~
+    1e-5), so every Kennedy-Pendleton try has d ~ 1e5 > 2 and is rejected,
+    and the reference's loop stops after 200 tries with a0 = 1e-9
+    (src/heatbath.c:64-68).  Sites and accepts are the oracle's, and so is
+    the stream position the loop leaves (the direction's draws start
+    there)."""
+    p, o, b = pair("g", L=L, betasu2=1e-6)
+    for d in range(3):
+        for par in (0, 1):
+            o.pass_(K_LINK_HEATBATH, par, d)
+            b.pass_(K_LINK_HEATBATH, par, d)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-12
+    co, cg = o.counters(), b.counters(0)
+    for k in co:
+        assert co[k] == cg[k], k
+    assert o.rng_get()[1] == b.rng_get(0)[1]
+
+
 @pytest.mark.parametrize("tag", list(MODELS))
 def test_compound_sweeps(tag):
     p, o, b = pair(tag, link_overrelax=2)
