@@ -240,7 +240,10 @@ def run_precision(precision, steps, warmup, timing_steps=100, graph=True):
 
 def sweep_ms_with_links(precision, links, steps=200, L=96):
     """ms per graph-replayed sweep of the workload with the FP32 links in the
-    given storage form (autotuned chunks): the other form beside `value`."""
+    given storage form (autotuned chunks), and its link overrelaxation's
+    average launch and fraction of 8 TB/s on that form's algorithmic bytes:
+    the other form beside `value`."""
+    from su2higgs_amd import abi
     from su2higgs_amd.lattice import Batch
     p = workload_params(precision, L=L)
     p.link_format = links
@@ -254,8 +257,13 @@ def sweep_ms_with_links(precision, links, steps=200, L=96):
     b.update(steps)
     b.sync()
     ms = 1e3 * (time.perf_counter() - t0) / steps
+    b.use_graph(False)
+    lo_ms, _ = b.time_pass(abi.K_LINK_OVERRELAX, 200)
+    byts = algo_words(abi.K_LINK_OVERRELAX, slot_params(b)) * abi.volume(p) * 4
     b.close()
-    return ms
+    return {"ms_per_step": ms, "link_overrelax_us": lo_ms * 1e3,
+            "link_overrelax_algorithmic_MB": byts / 1e6,
+            "link_overrelax_frac": byts / (lo_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def roofline(res):
@@ -702,10 +710,10 @@ def main():
     r64 = run_precision(abi.FP64, s64, max(1, args.warmup // 2), graph=not args.no_graph)
     log("[bench] fp64 %.3f ms/sweep, %.3e site-updates/s" % (r64["ms_per_sweep"], r64["site_updates_per_s"]))
     from su2higgs_amd.abi import LINKS_FULL, LINKS_PACKED
-    other = LINKS_PACKED if r32["link_format"] == "full" else LINKS_FULL
-    other_ms = sweep_ms_with_links(abi.FP32, other, steps=min(args.steps, 200))
-    log("[bench] fp32 with %s links: %.3f ms/sweep" % ("packed" if other == LINKS_PACKED else "full",
-                                                       other_ms))
+    other_fmt = LINKS_PACKED if r32["link_format"] == "full" else LINKS_FULL
+    other = sweep_ms_with_links(abi.FP32, other_fmt, steps=min(args.steps, 200))
+    log("[bench] fp32 with %s links: %.3f ms/sweep" % (
+        "packed" if other_fmt == LINKS_PACKED else "full", other["ms_per_step"]))
     roof = roofline(r32)
     roof64 = roofline(r64)
     log("[bench] every pass kind beyond the Infinity Cache (192^3)")
@@ -766,9 +774,10 @@ def main():
                    "variable_updates_per_sweep": "%g V" % r32["updates_per_sweep_over_V"],
                    "precision": "FP32 fields/staples, FP64 reductions"},
         "link_format_fp32": r32["link_format"],
-        "fp32_other_link_format": {"link_format": "packed" if other == LINKS_PACKED else "full",
-                                   "ms_per_step": other_ms,
-                                   "value": r32["updates_per_sweep_over_V"] * r32["V"] / (other_ms * 1e-3)},
+        "fp32_other_link_format": dict(
+            {"link_format": "packed" if other_fmt == LINKS_PACKED else "full",
+             "value": r32["updates_per_sweep_over_V"] * r32["V"] / (other["ms_per_step"] * 1e-3)},
+            **other),
         "fp64_value": r64["site_updates_per_s"],
         "fp64_ms_per_step": r64["ms_per_sweep"],
         "fp32_over_fp64": r32["site_updates_per_s"] / r64["site_updates_per_s"],

@@ -51,21 +51,22 @@ extern "C" {
 #define SU2HIP_FP32 32
 
 /* FP32 link storage (su2hip_params.link_format; FP64 always stores four
- * components).  FULL: four floats per link (16 bytes).  PACKED: the three
- * components other than the largest one (12 bytes); the largest is rebuilt
- * from the unit norm when the link is loaded, which costs arithmetic and
- * saves a quarter of the link bytes.  Every FP32 link is renormalised when
- * it is stored, in both forms.  AUTO, the default: a handle created with
- * its lattices takes PACKED when their FP32 fields in the full form exceed
- * SU2HIP_LINKS_PACKED_ABOVE bytes (beyond what the 256 MB Infinity Cache
- * serves well: there the bytes dominate, below it the arithmetic), FULL
- * otherwise; a handle created empty (lattices added later) takes PACKED.
- * A handle uses one format for all its lattices: the first lattice's
- * setting decides, or the measured one when it is AUTO. */
+ * components).  FULL: four floats per link (16 bytes).  PACKED: three
+ * floats (12 bytes), the stereographic projection of the unit quaternion
+ * from the pole opposite the sign of its real part, that sign in the lowest
+ * mantissa bit; a load rebuilds the link with one reciprocal, which costs a
+ * little arithmetic and saves a quarter of the link bytes.  Every FP32 link
+ * is renormalised when it is stored, in both forms.  AUTO, the default: a
+ * handle created with its lattices takes PACKED when their FP32 fields in
+ * the full form exceed SU2HIP_LINKS_PACKED_ABOVE bytes (above it the saved
+ * bytes outweigh the rebuild, below it the rebuild does), FULL otherwise; a
+ * handle created empty (lattices added later) takes PACKED.  A handle uses
+ * one format for all its lattices: the first lattice's setting decides, or
+ * the measured one when it is AUTO. */
 #define SU2HIP_LINKS_AUTO 0
 #define SU2HIP_LINKS_FULL 1
 #define SU2HIP_LINKS_PACKED 2
-#define SU2HIP_LINKS_PACKED_ABOVE (128ull << 20)
+#define SU2HIP_LINKS_PACKED_ABOVE (32ull << 20)
 
 /* Gradient-flow integrators. */
 #define SU2HIP_FLOW_EULER 1 /* the reference's scheme, src/gradflow.c:14-18 */

@@ -256,52 +256,43 @@ __device__ __forceinline__ void st4<double>(double *p, const Q<double> &q) {
   *gmem(reinterpret_cast<vd2 *>(p + 2)) = vd2{q.c, q.d};
 }
 
-// FP32 links are stored in 12 bytes instead of 16: the three components
-// other than the one of largest magnitude.  That one is at least 1/2 in
-// magnitude for a unit quaternion, so it is rebuilt well conditioned as
-// +-sqrt(1 - (sum of the other squares)).  Its position (2 bits) and sign
-// (1 bit) ride in the lowest mantissa bit of the three stored floats, which
-// moves each by at most one unit in the last place.  A loaded link is
-// therefore unit to FP32 rounding whatever was stored (the store needs no
-// renormalisation of its own), and the rebuilt component takes one Newton
-// step so that the approximate hardware square root leaves no bias in the
-// norm.  FP64 links keep all four components (the reference's arithmetic).
+// FP32 links are stored in 12 bytes instead of 16, by stereographic
+// projection from the pole opposite the real part's sign: a unit
+// quaternion (a, b, c, d) is kept as w = (b, c, d) / (1 + |a|), |w| <= 1,
+// with the sign of a in the lowest mantissa bit of w.x (which moves w.x by
+// at most one unit in the last place).  It comes back as
+// a = +-(1 - s) / (1 + s), (b, c, d) = 2 w / (1 + s) with s = |w|^2 in
+// [0, 1]: one reciprocal of a number in [1, 2] (the hardware one and a
+// Newton step), no square root and no case by case placement, and unit in
+// exact arithmetic for every w, so a loaded link is unit to FP32 rounding
+// whatever was stored (the store needs no renormalisation of its own).
+// FP64 links keep all four components (the reference's arithmetic).
 struct F3 {
   float x, y, z;
 };
 
 __device__ __forceinline__ Q<float> link_unpack(const F3 &v) {
-  uint32_t bx = __float_as_uint(v.x) & 1u, by = __float_as_uint(v.y) & 1u;
-  uint32_t neg = __float_as_uint(v.z) & 1u;
-  uint32_t i = bx | (by << 1);
-  float r2 = v.x * v.x + v.y * v.y + v.z * v.z;
-  float y = fmaxf(1.0f - r2, 0.25f);  // >= 1/4 for the largest component
-  float m0 = __builtin_amdgcn_sqrtf(y);
-  float m = __builtin_fmaf(__builtin_fmaf(-m0, m0, y), 0.5f * __builtin_amdgcn_rcpf(m0), m0);
-  m = __uint_as_float(__float_as_uint(m) ^ (neg << 31));
+  const uint32_t neg = __float_as_uint(v.x) & 1u;
+  const float s = v.x * v.x + v.y * v.y + v.z * v.z;
+  const float den = 1.0f + s;
+  float t = __builtin_amdgcn_rcpf(den);
+  t = __builtin_fmaf(__builtin_fmaf(-den, t, 1.0f), t, t);
+  const float t2 = t + t;
   Q<float> q;
-  q.a = i == 0 ? m : v.x;
-  q.b = i == 0 ? v.x : (i == 1 ? m : v.y);
-  q.c = i < 2 ? v.y : (i == 2 ? m : v.z);
-  q.d = i < 3 ? v.z : m;
+  q.a = __uint_as_float(__float_as_uint((1.0f - s) * t) ^ (neg << 31));
+  q.b = v.x * t2;
+  q.c = v.y * t2;
+  q.d = v.z * t2;
   return q;
 }
 
 // q must be unit up to rounding (every link a kernel writes is)
 __device__ __forceinline__ F3 link_pack(const Q<float> &q) {
-  float ma = fabsf(q.a), mb = fabsf(q.b), mc = fabsf(q.c), md = fabsf(q.d);
-  uint32_t i = 0;
-  float big = q.a, mx = ma;
-  if (mb > mx) { i = 1; mx = mb; big = q.b; }
-  if (mc > mx) { i = 2; mx = mc; big = q.c; }
-  if (md > mx) { i = 3; big = q.d; }
-  float x = i == 0 ? q.b : q.a;
-  float y = i < 2 ? q.c : q.b;
-  float z = i < 3 ? q.d : q.c;
-  F3 v;
-  v.x = __uint_as_float((__float_as_uint(x) & ~1u) | (i & 1u));
-  v.y = __uint_as_float((__float_as_uint(y) & ~1u) | (i >> 1));
-  v.z = __uint_as_float((__float_as_uint(z) & ~1u) | (big < 0.0f ? 1u : 0u));
+  const float den = 1.0f + fabsf(q.a);
+  float t = __builtin_amdgcn_rcpf(den);
+  t = __builtin_fmaf(__builtin_fmaf(-den, t, 1.0f), t, t);
+  F3 v{q.b * t, q.c * t, q.d * t};
+  v.x = __uint_as_float((__float_as_uint(v.x) & ~1u) | (q.a < 0.0f ? 1u : 0u));
   return v;
 }
 

@@ -50,10 +50,13 @@ def test_link_format_resolution():
     fields (in the full form) and the packed form above it, and packed for a
     handle created empty; FP64 is always full; an explicit choice is kept."""
     small = make_params(L=(32, 32, 32), nhiggs=1, u1=1, precision=FP32)   # 2.5 MB
-    big = make_params(L=(130, 130, 130), nhiggs=1, precision=FP32)         # 141 MB
+    mid = make_params(L=(64, 64, 64), nhiggs=1, precision=FP32)            # 16.8 MB
+    big = make_params(L=(96, 96, 96), nhiggs=1, u1=1, precision=FP32)      # 67 MB
     assert slot_links(Batch(small)) == abi.LINKS_FULL
+    assert slot_links(Batch(mid)) == abi.LINKS_FULL
     assert slot_links(Batch(big)) == abi.LINKS_PACKED
-    assert slot_links(Batch([small] * 60)) == abi.LINKS_PACKED            # 151 MB together
+    assert slot_links(Batch([small] * 12)) == abi.LINKS_FULL              # 29.9 MB together
+    assert slot_links(Batch([small] * 14)) == abi.LINKS_PACKED            # 34.9 MB together
     assert slot_links(Batch(make_params(L=(128,) * 3, precision=FP64))) == abi.LINKS_FULL
     forced = make_params(L=(8, 8, 8), precision=FP32, link_format=abi.LINKS_PACKED)
     assert slot_links(Batch(forced)) == abi.LINKS_PACKED
@@ -72,8 +75,8 @@ def test_link_format_resolution():
                          ids=["link_or", "doublet_or"])
 def test_packed_and_full_links_make_the_same_move(kind):
     """The same deterministic overrelaxation pass on the same fields in the
-    two FP32 link forms agrees to FP32 rounding (the packed form rebuilds a
-    component and moves the others by at most one unit in the last place)."""
+    two FP32 link forms agrees to FP32 rounding (the packed form rebuilds
+    every link from its projection, to a few units in the last place)."""
     import orc
     p64 = make_params(L=(8, 8, 12), nhiggs=1, u1=1, betau1=9.0, seed=5)
     o = orc.Oracle(p64)
@@ -92,12 +95,12 @@ def test_packed_and_full_links_make_the_same_move(kind):
 
 
 def test_fp32_packed_links_every_position_and_sign():
-    """The packed FP32 link keeps the three components other than the
-    largest one, whose position and sign ride in the lowest mantissa bits:
-    unit quaternions whose largest component sits in each of the four
-    places with either sign, ties among equal magnitudes, and the axes,
-    come back to a few units in the last place with the sign of every
-    component kept."""
+    """The packed FP32 link is the stereographic projection of the unit
+    quaternion from the pole opposite its real part's sign, that sign in the
+    lowest mantissa bit: unit quaternions with either sign of the real part
+    and each component dominant in turn, ties among equal magnitudes, the
+    poles (a = +-1), the equator (a = 0) and the axes come back to a few
+    units in the last place with the sign of every component kept."""
     p = make_params(L=(4, 4, 4), nhiggs=1, precision=FP32, link_format=abi.LINKS_PACKED)
     V = abi.volume(p)
     rng = np.random.default_rng(7)
@@ -107,7 +110,7 @@ def test_fp32_packed_links_every_position_and_sign():
                       [0, -.5, -.5, -.5 * np.sqrt(2)], [np.sqrt(.5), -np.sqrt(.5), 0, 0],
                       [1e-30, 1, -1e-30, 0]], dtype=float)
     q.reshape(-1, 4)[: len(fixed)] = fixed
-    # the largest component in each position with each sign, many times over
+    # each component dominant in turn, with each sign, many times over
     big = rng.integers(0, 4, size=(V, 3))
     sign = rng.choice([-1.0, 1.0], size=(V, 3))
     rest = q.reshape(-1, 4)[len(fixed):]
@@ -122,7 +125,7 @@ def test_fp32_packed_links_every_position_and_sign():
     assert np.abs(g - q).max() < 4e-7
     nz = np.abs(q) > 1e-6
     assert (np.sign(g[nz]) == np.sign(q[nz])).all()
-    # a second round trip changes nothing: the packed form is a fixed point
+    # a second round trip moves nothing beyond rounding
     f["su2link"][:] = g
     b.set_fields(0, f)
     assert np.abs(b.get_fields(0)["su2link"] - g).max() < 2.5e-7
