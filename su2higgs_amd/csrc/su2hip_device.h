@@ -286,9 +286,25 @@ __device__ __forceinline__ Q<float> link_unpack(const F3 &v) {
   return q;
 }
 
-// q must be unit up to rounding (every link a kernel writes is)
+// |q|^2 - 1 of a link within rounding of unit, as an FMA chain started at
+// a^2 - 1: the small result is never rounded on the float grid near 1,
+// whose spacing changes there (a renormalisation through 1/sqrt(|q|^2),
+// even a correctly rounded one, keeps a bias of about 1e-9 in the norm;
+// the hardware estimate alone left links about 3e-8 short, which a long
+// gradient flow turned into a plaquette 1.4e-7 low).
+__device__ __forceinline__ float unit_excess(const Q<float> &q) {
+  float dx = __builtin_fmaf(q.a, q.a, -1.0f);
+  dx = __builtin_fmaf(q.b, q.b, dx);
+  dx = __builtin_fmaf(q.c, q.c, dx);
+  return __builtin_fmaf(q.d, q.d, dx);
+}
+
+// The projection of q / |q|: w = (b, c, d) / (|q| + |a|), with
+// |q| = 1 + dx/2 (an error of order dx^2 ~ 1e-14), so a link that a kernel
+// computed a little off unit is stored as its normalised direction rather
+// than tilted by the excess norm.
 __device__ __forceinline__ F3 link_pack(const Q<float> &q) {
-  const float den = 1.0f + fabsf(q.a);
+  const float den = __builtin_fmaf(0.5f, unit_excess(q), 1.0f + fabsf(q.a));
   float t = __builtin_amdgcn_rcpf(den);
   t = __builtin_fmaf(__builtin_fmaf(-den, t, 1.0f), t, t);
   F3 v{q.b * t, q.c * t, q.d * t};
@@ -324,9 +340,14 @@ __device__ __forceinline__ void stU(T *__restrict__ U, int Vh, int d, int par, i
     p[1] = v.y;
     p[2] = v.z;
   } else if constexpr (sizeof(T) == 4) {
-    // the full FP32 form renormalises every link it stores
-    float n = __builtin_amdgcn_rsqf(q.a * q.a + q.b * q.b + q.c * q.c + q.d * q.d);
-    st4(U + ((size_t)(d * 2 + par) * Vh + s) * 4, Q<float>{q.a * n, q.b * n, q.c * n, q.d * n});
+    // the full FP32 form renormalises every link it stores: q (1 + e) with
+    // e = 1/sqrt(1 + dx) - 1 to second order in the excess dx, one rounding
+    // per component
+    const float dx = unit_excess(q);
+    const float e = dx * __builtin_fmaf(0.375f, dx, -0.5f);
+    st4(U + ((size_t)(d * 2 + par) * Vh + s) * 4,
+        Q<float>{__builtin_fmaf(q.a, e, q.a), __builtin_fmaf(q.b, e, q.b), __builtin_fmaf(q.c, e, q.c),
+                 __builtin_fmaf(q.d, e, q.d)});
   } else {
     st4(U + ((size_t)(d * 2 + par) * Vh + s) * 4, q);
   }

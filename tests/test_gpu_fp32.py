@@ -147,10 +147,12 @@ def test_fp32_links_stay_unitary_and_angles_wrapped(links):
 
 # FP64 values of the 48^3 SU(2)xU(1)+Higgs theory at betaG = 9, betaY = 27,
 # r = 1, msq = -0.20, lambda = 0.015 (the single-lattice workload's
-# couplings): 128 independent FP64 chains of the LEAN sweep mix below, cold
-# start, 40000 sweeps each measured every 10, the first 8000 discarded; each
-# value is the mean over chains of the chain's time average, its error the
-# chain-to-chain spread / sqrt(128).  Made with tests/fp64_reference_48.py.
+# couplings): 512 independent FP64 chains of the LEAN sweep mix below, cold
+# start, 16000 sweeps each measured every 10, the first 4000 discarded (the
+# protocol of the FP32 chains here, so both precisions are compared at the
+# same Monte Carlo time from the same start); each value is the mean over
+# chains of the chain's time average, its error the chain-to-chain spread /
+# sqrt(512).  Made with tests/fp64_reference_48.py.
 with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "fp64_reference_48.json")) as _f:
     FP64_REF_48 = {k: tuple(v) for k, v in json.load(_f).items()}
 
