@@ -71,9 +71,14 @@ def algo_words(kind, p):
     nh, u1, tr, sg = p.nhiggs, p.u1, p.triplet, p.singlet
     packed = p.precision == abi.FP32 and p.link_format == abi.LINKS_PACKED
     lw = 0.75 if packed else 1.0  # link words / 4
+    if kind == "hop":
+        # the hopping-term cache of one doublet (k_hop): phi 4, alpha 3, the
+        # write 12
+        return 4 + 3 + 12 if nh == 1 else None
     if kind in (abi.K_LINK_HEATBATH, abi.K_LINK_METROPOLIS, abi.K_LINK_OVERRELAX):
-        # U_mu(1-p) 2, U_nu 8, the write 2; phi, alpha_mu(p), Sigma
-        w = (2 + 8 + 2) * lw + 4 * nh + 0.5 * u1 + 3 * tr
+        # U_mu(1-p) 2, U_nu 8, the write 2; with one doublet its cached
+        # hopping term H_mu(p) 2, else phi 4 and alpha_mu(p) 0.5; Sigma
+        w = (2 + 8 + 2) * lw + (2 if nh == 1 else 4 * nh + 0.5 * u1) + 3 * tr
         if kind != abi.K_LINK_HEATBATH:
             w += 2 * lw  # the old link U_mu(p)
         return w

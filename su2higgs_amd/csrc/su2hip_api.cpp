@@ -290,6 +290,7 @@ static void fill_desc(Slot &s) {
   L.F2 = p.nhiggs == 2 ? take(8 * Vh) : nullptr;
   L.G = p.triplet ? take(6 * Vh) : nullptr;
   L.S = p.singlet ? take(2 * Vh) : nullptr;
+  L.H = p.nhiggs == 1 ? take(24 * Vh) : nullptr;
   L.B = s.bu_mem;
   L.L0 = p.L[0];
   L.L1 = p.L[1];
@@ -338,6 +339,7 @@ static size_t slot_bytes(const su2hip_params &p) {
   if (p.nhiggs == 2) b += align256(8 * Vh * w);
   if (p.triplet) b += align256(6 * Vh * w);
   if (p.singlet) b += align256(2 * Vh * w);
+  if (p.nhiggs == 1) b += align256(24 * Vh * w);  // hopping-term cache
   return b;
 }
 
@@ -692,6 +694,12 @@ static hipError_t enqueue_step(su2hip_batch *b, const Step &st, bool in_sweep = 
   return hipSuccess;
 }
 
+// the SU(2) link kinds, whose staples read the hopping-term cache
+static bool su2_link_kind(int kind) {
+  return kind == SU2HIP_K_LINK_HEATBATH || kind == SU2HIP_K_LINK_METROPOLIS ||
+         kind == SU2HIP_K_LINK_OVERRELAX;
+}
+
 static hipError_t enqueue_sweep(su2hip_batch *b) {
   if (b->random_order) {
     const su2hip_params &p = b->tmpl;
@@ -701,11 +709,21 @@ static hipError_t enqueue_sweep(su2hip_batch *b) {
     });
     if (e != hipSuccess) return e;
   }
+  // the hopping-term cache is filled before each round of link passes: the
+  // doublet and the U(1) links change only in the passes between rounds
+  bool hop_fresh = false;
+  LaunchCtx c = ctx_of(b);
   for (const Step &st : b->sched) {
-    hipError_t e = enqueue_step(b, st);
+    hipError_t e = hipSuccess;
+    if (!su2_link_kind(st.kind))
+      hop_fresh = false;
+    else if (!hop_fresh) {
+      e = timed(b, SU2HIP_K_AUX, [&] { return launch_hop(c); });
+      hop_fresh = true;
+    }
+    if (e == hipSuccess) e = enqueue_step(b, st);
     if (e != hipSuccess) return e;
   }
-  LaunchCtx c = ctx_of(b);
   return timed(b, SU2HIP_K_AUX, [&] { return launch_end_sweep(c, b->passes); });
 }
 
@@ -1269,7 +1287,8 @@ int su2hip_pass(su2hip_batch_t *b, int kind, int parity, int dir) {
   bool muca = false;
   for (auto &s : b->slots)
     if (s.used && field && muca_on_field(s.hm, field)) muca = true;
-  hipError_t e = enqueue_step(b, Step{kind, parity, dir, 0, muca, 0}, false);
+  hipError_t e = su2_link_kind(kind) ? launch_hop(ctx_of(b)) : hipSuccess;
+  if (e == hipSuccess) e = enqueue_step(b, Step{kind, parity, dir, 0, muca, 0}, false);
   if (e == hipErrorInvalidValue)
     return fail(SU2HIP_EINVAL, "kind %d is not part of this model", kind);
   if (e != hipSuccess) return fail(SU2HIP_EDEVICE, "pass launch: %s", hipGetErrorString(e));
@@ -2232,6 +2251,8 @@ static int time_pass_impl(su2hip_batch *b, int kind, int repeats, double *ms_ker
   LaunchCtx c = ctx_for(b, kind);
   int reunit = b->tmpl.reunitarize && !b->fp64;
   hipEvent_t s0 = ev[2 * (size_t)repeats], s1 = ev[2 * (size_t)repeats + 1];
+  // link passes read the hopping-term cache, filled once outside the timing
+  if (e == hipSuccess && su2_link_kind(kind)) e = launch_hop(ctx_of(b));
   if (e == hipSuccess) e = hipEventRecord(s0, b->stream);
   for (int i = 0; i < repeats && e == hipSuccess; i++) {
     c.ev_start = ev[2 * (size_t)i];

@@ -81,6 +81,9 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 struct LatDesc {
   void *U, *A, *F, *G, *S, *B;  // field arrays (element type = precision)
   void *F2;                     // second doublet (nhiggs = 2), else null
+  // one doublet: the doublet's hopping term of every link staple, H_mu(x)
+  // (k_hop), 4 values per link in the link layout; else null
+  void *H;
   int L0, L1, L2, h0;           // extents, h0 = L0/2
   int Vh, vol;                  // sites per parity, volume
   FastDiv div_h0, div_L1;
@@ -153,6 +156,10 @@ struct LaunchCtx {
 // `muca_step` of a multicanonical parity sweep.
 hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir,
                        int pass_off, int muca_step, int reunit);
+// the doublet's hopping term of every link staple (one doublet; a no-op
+// otherwise), read by the link passes that follow until the doublet or the
+// U(1) links change
+hipError_t launch_hop(const LaunchCtx &c);
 // multicanonical global accept/reject after a scalar pass step (hid: the
 // doublet of a doublet kind)
 hipError_t launch_muca_accept(const LaunchCtx &c, int kind, int hid, int parity,

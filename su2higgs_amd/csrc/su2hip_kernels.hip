@@ -86,26 +86,33 @@ __device__ __forceinline__ Q<T> wilson_staple(const LatDesc &L, const T *U, cons
   return W;
 }
 
+// With one doublet the hopping term of the staple is read from the cache
+// k_hop fills (the same arithmetic, done once per round of link passes).
+template <int M>
+constexpr bool hop_cached() {
+  return (M & M_NH) != 0 && (M & M_H2) == 0;
+}
+
+// -1/2 Phi(x+mu) exp(-i alpha sigma_3) Phi(x)^+ of link (x, mu)
+template <typename T, int M>
+__device__ __forceinline__ Q<T> hop_term(const LatDesc &L, const T *A, const T *F, const Site &x,
+                                         int mu) {
+  Q<T> b = F_at(L, F, shift(L, x, mu, +1));
+  if constexpr ((M & M_U1) != 0) b = qmul(b, u1twist<T>(A_at(L, A, x, mu)));
+  Q<T> t = qmul(b, qconj(F_at(L, F, x)));
+  return qscale(t, (T)-0.5);
+}
+
 template <typename T, int M>
 __device__ __forceinline__ Q<T> link_staple(const LatDesc &L, const T *U, const T *A, const T *F,
                                             const Site &x, int mu) {
   Q<T> V = qscale(wilson_staple<T, M>(L, U, x, mu), (T)(-0.5 * L.beta));
-  if constexpr ((M & M_NH) != 0) {
-    Site xp = shift(L, x, mu, +1);
-    Q<T> b = F_at(L, F, xp);
-    if constexpr ((M & M_U1) != 0) b = qmul(b, u1twist<T>(A_at(L, A, x, mu)));
-    Q<T> t = qmul(b, qconj(F_at(L, F, x)));
-    V = qadd(V, qscale(t, (T)-0.5));
-  }
-  if constexpr ((M & M_H2) != 0) {
-    // the second doublet's hopping term (the doublet loop of
-    // src/staples.c:150-190)
-    const T *F2 = (const T *)L.F2;
-    Q<T> b = F_at(L, F2, shift(L, x, mu, +1));
-    if constexpr ((M & M_U1) != 0) b = qmul(b, u1twist<T>(A_at(L, A, x, mu)));
-    Q<T> t = qmul(b, qconj(F_at(L, F2, x)));
-    V = qadd(V, qscale(t, (T)-0.5));
-  }
+  if constexpr (hop_cached<M>())
+    return qadd(V, ld4((const T *)L.H + ((size_t)(mu * 2 + x.par) * L.Vh + x.s) * 4));
+  if constexpr ((M & M_NH) != 0) V = qadd(V, hop_term<T, M>(L, A, F, x, mu));
+  // the second doublet's hopping term (the doublet loop of
+  // src/staples.c:150-190)
+  if constexpr ((M & M_H2) != 0) V = qadd(V, hop_term<T, M>(L, A, (const T *)L.F2, x, mu));
   return V;
 }
 
@@ -1024,6 +1031,31 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5))) void k
   __shared__ double lds[BLK / 64];
   pass_body<T, M, KIND>(lats, mucas, blk, par_in, mu_in, pass_off, step, reunit, cnt, nblk,
                         partial, order, npass, lds);
+}
+
+// The hopping-term cache of one doublet: for every link, the doublet part
+// of su2link_staple (src/staples.c:150-190) as the link passes would form
+// it.  Neither the link passes nor the link staple change the doublet or
+// the U(1) links, so a round of link passes reads 4 values per link here
+// instead of two doublets and a U(1) link, and FP64 evaluates each U(1)
+// sincos once per round instead of once per pass.
+template <typename T, int M>
+__global__ __launch_bounds__(BLK) void k_hop(const LatDesc *__restrict__ lats,
+                                             const BlockMap *__restrict__ blk) {
+  const BlockMap bm = blk[blockIdx.x];
+  const LatDesc &L = lats[bm.lat];
+  if (!L.active || L.budget == 0) return;
+  const int s = bm.s0 + (int)threadIdx.x;
+  if (s >= L.Vh) return;
+  const T *A = (const T *)L.A, *F = (const T *)L.F;
+  T *H = (T *)L.H;
+#pragma unroll 1
+  for (int par = 0; par < 2; par++) {
+    const Site x = site_of(L, par, s);
+#pragma unroll
+    for (int mu = 0; mu < 3; mu++)
+      st4(H + ((size_t)(mu * 2 + par) * L.Vh + s) * 4, hop_term<T, M>(L, A, F, x, mu));
+  }
 }
 
 // get_weight (src/multicanonical.c:350-379) with whichbin (:671-704)
@@ -2373,6 +2405,21 @@ hipError_t launch_pass(const LaunchCtx &c, int kind, int parity, int dir, int pa
         case SU2HIP_K_SINGLET_METROPOLIS: return pass_one<T, M, SU2HIP_K_SINGLET_METROPOLIS>(c, parity, dir, pass_off, muca_step, reunit);
       }
       return hipErrorInvalidValue;
+    });
+  });
+}
+
+hipError_t launch_hop(const LaunchCtx &c) {
+  if (c.nblk == 0) return hipSuccess;
+  return with_prec(c.fp64, [&](auto t) {
+    using T = decltype(t);
+    return with_model_of<T>(c.model, [&](auto m) -> hipError_t {
+      constexpr int M = decltype(m)::value;
+      if constexpr (hop_cached<M>()) {
+        hipLaunchKernelGGL((k_hop<T, M>), dim3(c.nblk), dim3(BLK), 0, c.stream, c.desc, c.blk);
+        return hipGetLastError();
+      }
+      return hipSuccess;
     });
   });
 }

@@ -156,6 +156,41 @@ def test_compound_sweeps(tag):
     assert o.rng_get()[1] == b.rng_get(0)[1]
 
 
+@pytest.mark.parametrize("tag", ["h", "hu", "ht", "hsu"])
+@pytest.mark.parametrize("random_order", [0, 1])
+def test_two_link_rounds_per_sweep(tag, random_order):
+    """update_links = 2: the U(1) passes of the first round change the
+    hopping term that the second round's link staples read (the link passes
+    take it from a cache filled before each round), and the doublet passes
+    of one sweep change it for the next."""
+    p, o, b = pair(tag, link_overrelax=1, update_links=2)
+    if random_order:
+        o.lib.orc_set_random_order(o.h, 1)
+        b.set_random_order(True)
+    o.update(3)
+    b.update(3)
+    assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-10
+    co, cg = o.counters(), b.counters(0)
+    for k in co:
+        assert co[k] == cg[k], k
+
+
+@pytest.mark.parametrize("tag", ["h", "hu"])
+def test_link_pass_after_scalar_and_u1_passes(tag):
+    """Single passes interleaved: each link pass sees the doublet and U(1)
+    links that the passes before it left."""
+    p, o, b = pair(tag)
+    seq = [(K_LINK_OVERRELAX, 0, 1), (K_DOUBLET_OVERRELAX, 1, 0), (K_LINK_HEATBATH, 1, 1),
+           (K_DOUBLET_METROPOLIS, 0, 0), (K_LINK_OVERRELAX, 0, 2)]
+    if p.u1:
+        seq += [(K_U1_METROPOLIS, 1, 2), (K_LINK_OVERRELAX, 1, 2), (K_U1_METROPOLIS, 0, 0),
+                (K_LINK_HEATBATH, 0, 0)]
+    for kind, par, d in seq:
+        o.pass_(kind, par, d)
+        b.pass_(kind, par, d)
+        assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-11, (kind, par, d)
+
+
 def test_graph_replay_matches_eager():
     p = make_params(L=(8, 8, 8), nhiggs=1, link_overrelax=2)
     b1, b2 = Batch(p), Batch(p)
