@@ -62,11 +62,12 @@ def passes_and_updates(p):
     return links + scal, (links + scal) / 2
 
 
-def algo_words(kind, p):
+def algo_words(kind, p, hop=True):
     """Compulsory HBM words per pass / V (each datum read or written once),
     for the handle's resolved parameters p (slot_params): a link is 4 words,
     3 in the packed FP32 form (three components, the fourth rebuilt from
-    the unit norm)."""
+    the unit norm); hop: the handle keeps the hopping-term cache
+    (Batch.hop_cache())."""
     from su2higgs_amd import abi
     nh, u1, tr, sg = p.nhiggs, p.u1, p.triplet, p.singlet
     packed = p.precision == abi.FP32 and p.link_format == abi.LINKS_PACKED
@@ -74,11 +75,11 @@ def algo_words(kind, p):
     if kind == "hop":
         # the hopping-term cache of one doublet (k_hop): phi 4, alpha 3, the
         # write 12
-        return 4 + 3 + 12 if nh == 1 else None
+        return 4 + 3 + 12 if nh == 1 and hop else None
     if kind in (abi.K_LINK_HEATBATH, abi.K_LINK_METROPOLIS, abi.K_LINK_OVERRELAX):
         # U_mu(1-p) 2, U_nu 8, the write 2; with one doublet its cached
         # hopping term H_mu(p) 2, else phi 4 and alpha_mu(p) 0.5; Sigma
-        w = (2 + 8 + 2) * lw + (2 if nh == 1 else 4 * nh + 0.5 * u1) + 3 * tr
+        w = (2 + 8 + 2) * lw + (2 if nh == 1 and hop else 4 * nh + 0.5 * u1) + 3 * tr
         if kind != abi.K_LINK_HEATBATH:
             w += 2 * lw  # the old link U_mu(p)
         return w
@@ -217,6 +218,7 @@ def run_precision(precision, steps, warmup, timing_steps=100, graph=True):
     lp = slot_params(b)
     res = {"precision": precision, "V": V, "passes_per_sweep": npass, "updates_per_sweep_over_V": upd,
            "link_format": {abi.LINKS_FULL: "full", abi.LINKS_PACKED: "packed"}[lp.link_format],
+           "hop_cache": b.hop_cache(),
            "autotune_ms_64_128_256": tune, "chunk": picks, "waves": waves,
            "wall_s": wall, "ms_per_sweep": 1e3 * wall / steps,
            "site_updates_per_s": upd * V * steps / wall}
@@ -246,11 +248,11 @@ def run_precision(precision, steps, warmup, timing_steps=100, graph=True):
     return res
 
 
-def sweep_ms_with_links(precision, links, steps=200, L=96):
-    """ms per graph-replayed sweep of the workload with the FP32 links in the
-    given storage form (autotuned chunks), and its link overrelaxation's
-    average launch and fraction of 8 TB/s on that form's algorithmic bytes:
-    the other form beside `value`."""
+def sweep_ms_with_links(precision, links, steps=200, L=96, graph=True):
+    """ms per sweep of the workload with the FP32 links in the given
+    storage form (autotuned chunks; graph-replayed unless graph=False), and
+    its link overrelaxation's average launch and fraction of 8 TB/s on that
+    form's algorithmic bytes: the other form beside `value`."""
     from su2higgs_amd import abi
     from su2higgs_amd.lattice import Batch
     p = workload_params(precision, L=L)
@@ -258,7 +260,7 @@ def sweep_ms_with_links(precision, links, steps=200, L=96):
     b = Batch(p)
     b.cold_start(0, phi0=1.0)
     b.autotune(repeats=50)
-    b.use_graph(True)
+    b.use_graph(graph)
     b.update(20)
     b.sync()
     t0 = time.perf_counter()
@@ -267,7 +269,7 @@ def sweep_ms_with_links(precision, links, steps=200, L=96):
     ms = 1e3 * (time.perf_counter() - t0) / steps
     b.use_graph(False)
     lo_ms, _ = b.time_pass(abi.K_LINK_OVERRELAX, 200)
-    byts = algo_words(abi.K_LINK_OVERRELAX, slot_params(b)) * abi.volume(p) * 4
+    byts = algo_words(abi.K_LINK_OVERRELAX, slot_params(b), b.hop_cache()) * abi.volume(p) * 4
     b.close()
     return {"ms_per_step": ms, "link_overrelax_us": lo_ms * 1e3,
             "link_overrelax_algorithmic_MB": byts / 1e6,
@@ -282,7 +284,7 @@ def roofline(res):
     kind = max((k for k in kms if k not in ("measure", "aux", "muca")), key=lambda k: kms[k])
     kid = abi.KIND_NAMES.index(kind)
     avg_ms = res["kernel_avg_ms"][kind]
-    words = algo_words(kid, p)
+    words = algo_words(kid, p, res["hop_cache"])
     bytes_per_launch = words * res["V"] * w
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     prec = "fp64" if p.precision == abi.FP64 else "fp32"
@@ -305,7 +307,7 @@ def kernel_rooflines(res):
     w = 8 if p.precision == abi.FP64 else 4
     out = {}
     for kind, avg_ms in res["kernel_avg_ms"].items():
-        byts = algo_words(abi.KIND_NAMES.index(kind), p) * res["V"] * w
+        byts = algo_words(abi.KIND_NAMES.index(kind), p, res["hop_cache"]) * res["V"] * w
         ach = byts / (avg_ms * 1e-3) / 1e9
         out[kind] = {"avg_launch_us": avg_ms * 1e3, "algorithmic_MB": byts / 1e6,
                      "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
@@ -333,7 +335,7 @@ def roofline_beyond_cache(L=192, precisions=None):
         for k in (abi.K_LINK_OVERRELAX, abi.K_LINK_HEATBATH, abi.K_U1_METROPOLIS,
                   abi.K_DOUBLET_OVERRELAX, abi.K_DOUBLET_METROPOLIS):
             ms, span = b.time_pass(k, 60)
-            byts = algo_words(k, p) * V * w
+            byts = algo_words(k, p, b.hop_cache()) * V * w
             ach = byts / (ms * 1e-3) / 1e9
             kinds[abi.KIND_NAMES[k]] = {"avg_launch_us": ms * 1e3, "algorithmic_MB": byts / 1e6,
                                         "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
@@ -430,7 +432,7 @@ def _scan_rate(ps, steps, graph, tune=True, kinds=()):
     kr = {}
     for k in kinds:
         ms, _ = b.time_pass(k, 50)
-        byts = algo_words(k, rp) * Vtot * w
+        byts = algo_words(k, rp, b.hop_cache()) * Vtot * w
         ach = byts / (ms * 1e-3) / 1e9
         kr[abi.KIND_NAMES[k]] = {"avg_launch_us": ms * 1e3, "algorithmic_MB": byts / 1e6,
                                  "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
@@ -719,7 +721,7 @@ def main():
     log("[bench] fp64 %.3f ms/sweep, %.3e site-updates/s" % (r64["ms_per_sweep"], r64["site_updates_per_s"]))
     from su2higgs_amd.abi import LINKS_FULL, LINKS_PACKED
     other_fmt = LINKS_PACKED if r32["link_format"] == "full" else LINKS_FULL
-    other = sweep_ms_with_links(abi.FP32, other_fmt, steps=min(args.steps, 200))
+    other = sweep_ms_with_links(abi.FP32, other_fmt, steps=min(args.steps, 200), graph=not args.no_graph)
     log("[bench] fp32 with %s links: %.3f ms/sweep" % (
         "packed" if other_fmt == LINKS_PACKED else "full", other["ms_per_step"]))
     roof = roofline(r32)
@@ -782,6 +784,7 @@ def main():
                    "variable_updates_per_sweep": "%g V" % r32["updates_per_sweep_over_V"],
                    "precision": "FP32 fields/staples, FP64 reductions"},
         "link_format_fp32": r32["link_format"],
+        "hop_cache_fp32": r32["hop_cache"], "hop_cache_fp64": r64["hop_cache"],
         "fp32_other_link_format": dict(
             {"link_format": "packed" if other_fmt == LINKS_PACKED else "full",
              "value": r32["updates_per_sweep_over_V"] * r32["V"] / (other["ms_per_step"] * 1e-3)},

@@ -460,6 +460,25 @@ int su2hip_get_chunk(su2hip_batch_t *b, int kind, int *chunk);
 int su2hip_set_waves(su2hip_batch_t *b, int kind, int waves);
 /* The occupancy target a pass kind launches with now. */
 int su2hip_get_waves(su2hip_batch_t *b, int kind, int *waves);
+/* Hopping-term cache (no reference counterpart; the doublet part of the
+ * link staple, src/staples.c:150-190).  With one doublet the handle can keep,
+ * per lattice, -1/2 Phi(x+mu) exp(-i alpha sigma_3) Phi(x)^+ for every link
+ * (4 values per link, 12 per site, inside the HBM budget), filled before
+ * each round of link passes: the link passes then read it instead of two
+ * doublets and a U(1) link.  The arithmetic is the same either way; FP64
+ * results agree to rounding, FP32 ones to FP32 rounding.  Default: kept,
+ * except when the handle is created with lattices whose fields fit in
+ * SU2HIP_INFINITY_CACHE_BYTES and whose fields and caches together do not
+ * (there the other passes lose more to the footprint than the link passes
+ * gain).  A lattice added to a handle that keeps it gets one if it fits the
+ * budget, and otherwise runs without.  Enabling it for a model without
+ * exactly one doublet fails with SU2HIP_EINVAL; enabling it beyond the
+ * budget fails with SU2HIP_ENOMEM and leaves it off for every lattice. */
+#define SU2HIP_INFINITY_CACHE_BYTES (256ull << 20)
+int su2hip_set_hop_cache(su2hip_batch_t *b, int enable);
+/* 1 if the handle keeps the hopping-term cache for every one of its
+ * lattices, else 0. */
+int su2hip_get_hop_cache(su2hip_batch_t *b, int *enabled);
 /* Time every pass kind of the batch's sweep with each chunk, and with each
  * occupancy target it is built with (kernel durations of `repeats`
  * back-to-back launches, as su2hip_time_pass), and keep the fastest; 256

@@ -311,6 +311,8 @@ def load():
         "su2hip_get_chunk": (C.c_int, [h, C.c_int, P(C.c_int)]),
         "su2hip_set_waves": (C.c_int, [h, C.c_int, C.c_int]),
         "su2hip_get_waves": (C.c_int, [h, C.c_int, P(C.c_int)]),
+        "su2hip_set_hop_cache": (C.c_int, [h, C.c_int]),
+        "su2hip_get_hop_cache": (C.c_int, [h, P(C.c_int)]),
         "su2hip_autotune": (C.c_int, [h, C.c_int, dp]),
         "su2hip_time_pass": (C.c_int, [h, C.c_int, C.c_int, dp, dp]),
         # host I/O layer (include/su2hip_io.h)

@@ -65,6 +65,8 @@ struct Slot {
   MucaDev hm{};
   void *flow_mem = nullptr;
   size_t flow_bytes = 0;
+  void *hop_mem = nullptr;  // hopping-term cache (LatDesc::H), when the handle keeps one
+  size_t hop_bytes = 0;
   FlowJob fj;
 };
 
@@ -129,6 +131,7 @@ struct su2hip_batch {
   int tune[SU2HIP_NKINDS] = {};   // chunk per pass kind, 0 = chosen by size
   int waves[SU2HIP_NKINDS] = {};  // occupancy target per pass kind (su2hip_set_waves)
   bool random_order = false;      // the reference's random_sweeps
+  bool hop_on = false;            // hopping-term cache kept (su2hip_set_hop_cache)
   uint8_t *d_order = nullptr;     // [max_slots][passes] parity | dir << 1
   uint16_t *d_info = nullptr;     // [passes] step_info of each step
   size_t order_cap = 0, info_cap = 0;
@@ -290,7 +293,7 @@ static void fill_desc(Slot &s) {
   L.F2 = p.nhiggs == 2 ? take(8 * Vh) : nullptr;
   L.G = p.triplet ? take(6 * Vh) : nullptr;
   L.S = p.singlet ? take(2 * Vh) : nullptr;
-  L.H = p.nhiggs == 1 ? take(24 * Vh) : nullptr;
+  L.H = s.hop_mem;
   L.B = s.bu_mem;
   L.L0 = p.L[0];
   L.L1 = p.L[1];
@@ -339,8 +342,13 @@ static size_t slot_bytes(const su2hip_params &p) {
   if (p.nhiggs == 2) b += align256(8 * Vh * w);
   if (p.triplet) b += align256(6 * Vh * w);
   if (p.singlet) b += align256(2 * Vh * w);
-  if (p.nhiggs == 1) b += align256(24 * Vh * w);  // hopping-term cache
   return b;
+}
+
+// the hopping-term cache of a lattice with one doublet: 4 values per link
+static size_t hop_cache_bytes(const su2hip_params &p) {
+  size_t Vh = (size_t)vol_of(p) / 2;
+  return p.nhiggs == 1 ? align256(24 * Vh * wsize(p)) : 0;
 }
 
 // pull the device-owned descriptor fields (pass counter, budget)
@@ -717,7 +725,7 @@ static hipError_t enqueue_sweep(su2hip_batch *b) {
     hipError_t e = hipSuccess;
     if (!su2_link_kind(st.kind))
       hop_fresh = false;
-    else if (!hop_fresh) {
+    else if (!hop_fresh && b->hop_on) {
       e = timed(b, SU2HIP_K_AUX, [&] { return launch_hop(c); });
       hop_fresh = true;
     }
@@ -797,6 +805,51 @@ static int resolve_links(int n, const su2hip_params *params) {
   return bytes > SU2HIP_LINKS_PACKED_ABOVE ? SU2HIP_LINKS_PACKED : SU2HIP_LINKS_FULL;
 }
 
+// the slot's hopping-term cache, inside the handle's HBM budget
+static int hop_alloc(su2hip_batch *b, int i) {
+  Slot &s = b->slots[i];
+  size_t bytes = hop_cache_bytes(s.p);
+  if (s.hop_mem || bytes == 0) return SU2HIP_OK;
+  if (b->hbm_budget && b->used + bytes > b->hbm_budget)
+    return fail(SU2HIP_ENOMEM, "HBM budget: %llu + %zu bytes of hopping-term cache exceeds %llu",
+                (unsigned long long)b->used, bytes, (unsigned long long)b->hbm_budget);
+  if (hipMalloc(&s.hop_mem, bytes) != hipSuccess) {
+    s.hop_mem = nullptr;
+    return fail(SU2HIP_ENOMEM, "device allocation of %zu bytes failed", bytes);
+  }
+  s.hop_bytes = bytes;
+  b->used += bytes;
+  s.hd.H = s.hop_mem;
+  return SU2HIP_OK;
+}
+
+static void hop_release(su2hip_batch *b, int i) {
+  Slot &s = b->slots[i];
+  if (s.hop_mem) (void)hipFree(s.hop_mem);
+  b->used -= s.hop_bytes;
+  s.hop_mem = nullptr;
+  s.hop_bytes = 0;
+  s.hd.H = nullptr;
+}
+
+// AUTO choice of the hopping-term cache for a handle created with n
+// lattices: kept with one doublet, except where the fields fit the
+// Infinity Cache and the fields and the cache together do not (the link
+// passes save less than the other passes then lose to the extra footprint);
+// a handle created empty keeps it
+static bool resolve_hop(int n, const su2hip_params *params, int link_format) {
+  if (params[0].nhiggs != 1) return false;
+  if (n == 0) return true;
+  uint64_t f = 0, h = 0;
+  for (int i = 0; i < n; i++) {
+    su2hip_params q = params[i];
+    q.link_format = link_format;
+    f += slot_bytes(q);
+    h += hop_cache_bytes(q);
+  }
+  return !(f <= SU2HIP_INFINITY_CACHE_BYTES && f + h > SU2HIP_INFINITY_CACHE_BYTES);
+}
+
 static int alloc_slot(su2hip_batch *b, int i, const su2hip_params &p_in) {
   su2hip_params p = p_in;
   p.link_format = b->tmpl.link_format;  // the handle's resolved storage
@@ -819,6 +872,9 @@ static int alloc_slot(su2hip_batch *b, int i, const su2hip_params &p_in) {
   s.hd.pass = 0;
   s.hm = MucaDev();
   b->map_dirty = true;
+  // a lattice whose cache does not fit the budget runs without one (its
+  // link passes form the hopping term themselves)
+  if (b->hop_on) (void)hop_alloc(b, i);
   return SU2HIP_OK;
 }
 
@@ -842,7 +898,8 @@ static void free_slot(su2hip_batch *b, int i) {
   if (s.bu_mem) (void)hipFree(s.bu_mem);
   if (s.muca_mem) (void)hipFree(s.muca_mem);
   if (s.flow_mem) (void)hipFree(s.flow_mem);
-  b->used -= s.bytes + s.bu_bytes + s.flow_bytes;
+  if (s.hop_mem) (void)hipFree(s.hop_mem);
+  b->used -= s.bytes + s.bu_bytes + s.flow_bytes + s.hop_bytes;
   s = Slot();
   b->map_dirty = true;
 }
@@ -939,6 +996,7 @@ int su2hip_footprint(const su2hip_params *p, int with_flow, uint64_t *bytes) {
   q.link_format = resolve_links(1, &q);  // AUTO: as a handle of this one lattice would
   p = &q;
   size_t b = slot_bytes(*p);
+  if (resolve_hop(1, p, p->link_format)) b += hop_cache_bytes(*p);
   size_t Vh = (size_t)vol_of(*p) / 2;
   if (p->nhiggs || p->triplet) b += align256((p->nhiggs ? 8 : 6) * Vh * wsize(*p));
   if (with_flow) b += align256(flow_words(*p) * wsize(*p)) + 4096;  // as su2hip_flow allocates
@@ -976,6 +1034,7 @@ int su2hip_create(int n, const su2hip_params *params, int max_slots, uint64_t hb
   b->hbm_budget = hbm_budget;
   b->tmpl = params[0];
   b->tmpl.link_format = resolve_links(n, params);
+  b->hop_on = resolve_hop(n, params, b->tmpl.link_format);
   b->model = model_bits(params[0]) | (packed_links(b->tmpl) ? M_PK : 0);
   b->fp64 = params[0].precision == SU2HIP_FP64;
   b->slots.resize(max_slots);
@@ -1287,7 +1346,7 @@ int su2hip_pass(su2hip_batch_t *b, int kind, int parity, int dir) {
   bool muca = false;
   for (auto &s : b->slots)
     if (s.used && field && muca_on_field(s.hm, field)) muca = true;
-  hipError_t e = su2_link_kind(kind) ? launch_hop(ctx_of(b)) : hipSuccess;
+  hipError_t e = b->hop_on && su2_link_kind(kind) ? launch_hop(ctx_of(b)) : hipSuccess;
   if (e == hipSuccess) e = enqueue_step(b, Step{kind, parity, dir, 0, muca, 0}, false);
   if (e == hipErrorInvalidValue)
     return fail(SU2HIP_EINVAL, "kind %d is not part of this model", kind);
@@ -2252,7 +2311,7 @@ static int time_pass_impl(su2hip_batch *b, int kind, int repeats, double *ms_ker
   int reunit = b->tmpl.reunitarize && !b->fp64;
   hipEvent_t s0 = ev[2 * (size_t)repeats], s1 = ev[2 * (size_t)repeats + 1];
   // link passes read the hopping-term cache, filled once outside the timing
-  if (e == hipSuccess && su2_link_kind(kind)) e = launch_hop(ctx_of(b));
+  if (e == hipSuccess && b->hop_on && su2_link_kind(kind)) e = launch_hop(ctx_of(b));
   if (e == hipSuccess) e = hipEventRecord(s0, b->stream);
   for (int i = 0; i < repeats && e == hipSuccess; i++) {
     c.ev_start = ev[2 * (size_t)i];
@@ -2324,6 +2383,41 @@ int su2hip_get_waves(su2hip_batch_t *b, int kind, int *waves) {
   if (!b || !waves) return fail(SU2HIP_EINVAL, "NULL argument");
   if (kind < 0 || kind >= NCOUNT) return fail(SU2HIP_EINVAL, "kind %d is not a pass kind", kind);
   *waves = b->waves[kind];
+  return SU2HIP_OK;
+}
+
+int su2hip_set_hop_cache(su2hip_batch_t *b, int enable) {
+  if (!b) return fail(SU2HIP_EINVAL, "NULL handle");
+  bool on = enable != 0;
+  if (on && b->tmpl.nhiggs != 1)
+    return fail(SU2HIP_EINVAL, "the hopping-term cache needs a model with one doublet");
+  HIPTRY(hipStreamSynchronize(b->stream));
+  int rc = pull_desc(b);  // keep the device-owned pass counters and budgets
+  if (rc) return rc;
+  b->hop_on = on;
+  for (int i = 0; i < b->max_slots && !rc; i++)
+    if (b->slots[i].used) rc = on ? hop_alloc(b, i) : (hop_release(b, i), SU2HIP_OK);
+  if (rc) {  // off everywhere, as documented
+    std::string keep = g_err;
+    for (int i = 0; i < b->max_slots; i++)
+      if (b->slots[i].used) hop_release(b, i);
+    b->hop_on = false;
+    b->graph_valid = false;
+    (void)push_desc(b);
+    g_err = keep;
+    return rc;
+  }
+  b->graph_valid = false;
+  return push_desc(b);
+}
+
+// kept: on for the handle and present for each of its lattices
+int su2hip_get_hop_cache(su2hip_batch_t *b, int *enabled) {
+  if (!b || !enabled) return fail(SU2HIP_EINVAL, "NULL argument");
+  bool on = b->hop_on;
+  for (int i = 0; i < b->max_slots; i++)
+    if (b->slots[i].used && !b->slots[i].hop_mem) on = false;
+  *enabled = on ? 1 : 0;
   return SU2HIP_OK;
 }
 

@@ -81,8 +81,9 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 struct LatDesc {
   void *U, *A, *F, *G, *S, *B;  // field arrays (element type = precision)
   void *F2;                     // second doublet (nhiggs = 2), else null
-  // one doublet: the doublet's hopping term of every link staple, H_mu(x)
-  // (k_hop), 4 values per link in the link layout; else null
+  // one doublet, when the handle keeps the cache: the doublet's hopping
+  // term of every link staple, H_mu(x) (k_hop), 4 values per link in the
+  // link layout; else null
   void *H;
   int L0, L1, L2, h0;           // extents, h0 = L0/2
   int Vh, vol;                  // sites per parity, volume

@@ -87,7 +87,8 @@ __device__ __forceinline__ Q<T> wilson_staple(const LatDesc &L, const T *U, cons
 }
 
 // With one doublet the hopping term of the staple is read from the cache
-// k_hop fills (the same arithmetic, done once per round of link passes).
+// k_hop fills (the same arithmetic, done once per round of link passes),
+// when the handle keeps one (LatDesc::H not null).
 template <int M>
 constexpr bool hop_cached() {
   return (M & M_NH) != 0 && (M & M_H2) == 0;
@@ -108,7 +109,7 @@ __device__ __forceinline__ Q<T> link_staple(const LatDesc &L, const T *U, const 
                                             const Site &x, int mu) {
   Q<T> V = qscale(wilson_staple<T, M>(L, U, x, mu), (T)(-0.5 * L.beta));
   if constexpr (hop_cached<M>())
-    return qadd(V, ld4((const T *)L.H + ((size_t)(mu * 2 + x.par) * L.Vh + x.s) * 4));
+    if (L.H) return qadd(V, ld4((const T *)L.H + ((size_t)(mu * 2 + x.par) * L.Vh + x.s) * 4));
   if constexpr ((M & M_NH) != 0) V = qadd(V, hop_term<T, M>(L, A, F, x, mu));
   // the second doublet's hopping term (the doublet loop of
   // src/staples.c:150-190)
@@ -1044,7 +1045,7 @@ __global__ __launch_bounds__(BLK) void k_hop(const LatDesc *__restrict__ lats,
                                              const BlockMap *__restrict__ blk) {
   const BlockMap bm = blk[blockIdx.x];
   const LatDesc &L = lats[bm.lat];
-  if (!L.active || L.budget == 0) return;
+  if (!L.active || L.budget == 0 || !L.H) return;
   const int s = bm.s0 + (int)threadIdx.x;
   if (s >= L.Vh) return;
   const T *A = (const T *)L.A, *F = (const T *)L.F;

@@ -203,6 +203,14 @@ class Batch:
         check(self.lib.su2hip_get_waves(self.h, kind, C.byref(w)))
         return w.value
 
+    def set_hop_cache(self, on=True):
+        check(self.lib.su2hip_set_hop_cache(self.h, 1 if on else 0))
+
+    def hop_cache(self):
+        v = C.c_int()
+        check(self.lib.su2hip_get_hop_cache(self.h, C.byref(v)))
+        return bool(v.value)
+
     def autotune(self, repeats=50):
         """Pick the fastest workgroup chunk (and, where a kind has two, the
         faster occupancy target) per pass kind; returns {kind name: (ms at

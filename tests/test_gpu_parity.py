@@ -158,12 +158,17 @@ def test_compound_sweeps(tag):
 
 @pytest.mark.parametrize("tag", ["h", "hu", "ht", "hsu"])
 @pytest.mark.parametrize("random_order", [0, 1])
-def test_two_link_rounds_per_sweep(tag, random_order):
+@pytest.mark.parametrize("cache", [True, False], ids=["cache", "inline"])
+def test_two_link_rounds_per_sweep(tag, random_order, cache):
     """update_links = 2: the U(1) passes of the first round change the
     hopping term that the second round's link staples read (the link passes
     take it from a cache filled before each round), and the doublet passes
-    of one sweep change it for the next."""
+    of one sweep change it for the next.  Without the cache the passes form
+    the term themselves."""
     p, o, b = pair(tag, link_overrelax=1, update_links=2)
+    assert b.hop_cache()
+    b.set_hop_cache(cache)
+    assert b.hop_cache() == cache
     if random_order:
         o.lib.orc_set_random_order(o.h, 1)
         b.set_random_order(True)
@@ -173,6 +178,64 @@ def test_two_link_rounds_per_sweep(tag, random_order):
     co, cg = o.counters(), b.counters(0)
     for k in co:
         assert co[k] == cg[k], k
+
+
+def test_hop_cache_switched_between_sweeps():
+    """The cache dropped and kept again between sweeps (graph replay on): the
+    chain is the oracle's throughout, pass counters and budgets included."""
+    p, o, b = pair("hu", link_overrelax=1)
+    b.use_graph(True)
+    b.set_budget(0, 7)
+    for on in (True, False, True, False):
+        b.set_hop_cache(on)
+        o.update(1)
+        b.update(1)
+        assert maxdiff(o.get_fields(), b.get_fields(0)) < 1e-10, on
+    assert o.rng_get()[1] == b.rng_get(0)[1]
+    assert b.budgets()[0] == 3
+
+
+def test_hop_cache_needs_one_doublet_and_fits_the_budget():
+    import ctypes as C
+    for tag in ("g", "t", "d"):
+        b = Batch(make_params(L=(4, 4, 4), **MODELS[tag]))
+        assert not b.hop_cache()
+        with pytest.raises(abi.Su2hipError, match="one doublet"):
+            b.set_hop_cache(True)
+        b.close()
+    p = make_params(L=(8, 8, 8), **MODELS["hu"])
+    lib = abi.load()
+    fp = C.c_uint64()
+    assert lib.su2hip_footprint(C.byref(p), 0, C.byref(fp)) == 0
+    # the footprint counts the fields, the cache (12 words per site) and a
+    # doublet backup for a weight (4 words per site): a budget one byte short
+    # of fields and cache
+    V, w = abi.volume(p), 8
+    b = Batch(p, hbm_budget=fp.value - 4 * V * w - 1)
+    assert not b.hop_cache()
+    with pytest.raises(abi.Su2hipError, match="budget"):
+        b.set_hop_cache(True)
+    assert not b.hop_cache()
+    b.update(1)
+    b.close()
+
+
+def test_hop_cache_default_follows_the_infinity_cache():
+    """Kept for the 96^3 workload and for lattices beyond the Infinity Cache;
+    dropped for the FP32 96-lattice scan, whose fields fit it (about 200 MB)
+    and whose fields and caches (about 390 MB) do not."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    for prec, want in ((abi.FP32, False), (FP64, True)):
+        ps = bench.scan_params(prec)
+        b = Batch(ps)
+        assert b.hop_cache() == want, prec
+        b.close()
+    b = Batch(bench.workload_params(abi.FP32))
+    assert b.hop_cache()
+    b.close()
 
 
 @pytest.mark.parametrize("tag", ["h", "hu"])
