@@ -143,6 +143,19 @@ __device__ __forceinline__ Q<T> u1twist(T alpha) {
   }
   return Q<T>{c, (T)0, (T)0, -s};
 }
+// x times a quaternion (c, 0, 0, d): the U(1) twist (d = -sin alpha) or its
+// conjugate (d = sin alpha), the product rule of qmul with the two zero
+// components dropped (the same value, half the multiplies)
+template <typename T>
+__device__ __forceinline__ Q<T> qmul_u1(const Q<T> &x, const Q<T> &u) {
+  const T c = u.a, d = u.d;
+  Q<T> r;
+  r.a = x.a * c - x.d * d;
+  r.b = x.b * c - x.c * d;
+  r.c = x.c * c + x.b * d;
+  r.d = x.a * d + x.d * c;
+  return r;
+}
 
 template <typename T>
 struct V3 {
@@ -258,14 +271,16 @@ __device__ __forceinline__ void st4<double>(double *p, const Q<double> &q) {
 
 // FP32 links are stored in 12 bytes instead of 16, by stereographic
 // projection from the pole opposite the real part's sign: a unit
-// quaternion (a, b, c, d) is kept as w = (b, c, d) / (1 + |a|), |w| <= 1,
-// with the sign of a in the lowest mantissa bit of w.x (which moves w.x by
-// at most one unit in the last place).  It comes back as
-// a = +-(1 - s) / (1 + s), (b, c, d) = 2 w / (1 + s) with s = |w|^2 in
-// [0, 1]: one reciprocal of a number in [1, 2] (the hardware one and a
-// Newton step), no square root and no case by case placement, and unit in
-// exact arithmetic for every w, so a loaded link is unit to FP32 rounding
-// whatever was stored (the store needs no renormalisation of its own).
+// quaternion (a, b, c, d) is kept as v = 2 w, w = (b, c, d) / (1 + |a|),
+// |w| <= 1, with the sign of a in the lowest mantissa bit of v.x (which
+// moves it by at most one unit in the last place).  It comes back as
+// a = +-(1 - s) / (1 + s), (b, c, d) = 2 w / (1 + s) = v / (1 + s) with
+// s = |w|^2 = |v|^2 / 4 in [0, 1]: one reciprocal of a number in [1, 2]
+// (the hardware one and a Newton step), no square root and no case by case
+// placement, and unit in exact arithmetic for every v, so a loaded link is
+// unit to FP32 rounding whatever was stored (the store needs no
+// renormalisation of its own).  Keeping 2 w rather than w saves an
+// instruction per load and rounds the same way (the factor is a power of 2).
 // FP64 links keep all four components (the reference's arithmetic).
 struct F3 {
   float x, y, z;
@@ -273,16 +288,15 @@ struct F3 {
 
 __device__ __forceinline__ Q<float> link_unpack(const F3 &v) {
   const uint32_t neg = __float_as_uint(v.x) & 1u;
-  const float s = v.x * v.x + v.y * v.y + v.z * v.z;
-  const float den = 1.0f + s;
+  const float s4 = __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));  // 4 s
+  const float den = __builtin_fmaf(s4, 0.25f, 1.0f);
   float t = __builtin_amdgcn_rcpf(den);
   t = __builtin_fmaf(__builtin_fmaf(-den, t, 1.0f), t, t);
-  const float t2 = t + t;
   Q<float> q;
-  q.a = __uint_as_float(__float_as_uint((1.0f - s) * t) ^ (neg << 31));
-  q.b = v.x * t2;
-  q.c = v.y * t2;
-  q.d = v.z * t2;
+  q.a = __uint_as_float(__float_as_uint(__builtin_fmaf(s4, -0.25f, 1.0f) * t) ^ (neg << 31));
+  q.b = v.x * t;
+  q.c = v.y * t;
+  q.d = v.z * t;
   return q;
 }
 
@@ -299,7 +313,7 @@ __device__ __forceinline__ float unit_excess(const Q<float> &q) {
   return __builtin_fmaf(q.d, q.d, dx);
 }
 
-// The projection of q / |q|: w = (b, c, d) / (|q| + |a|), with
+// The projection of q / |q|: v = 2 (b, c, d) / (|q| + |a|), with
 // |q| = 1 + dx/2 (an error of order dx^2 ~ 1e-14), so a link that a kernel
 // computed a little off unit is stored as its normalised direction rather
 // than tilted by the excess norm.
@@ -307,6 +321,7 @@ __device__ __forceinline__ F3 link_pack(const Q<float> &q) {
   const float den = __builtin_fmaf(0.5f, unit_excess(q), 1.0f + fabsf(q.a));
   float t = __builtin_amdgcn_rcpf(den);
   t = __builtin_fmaf(__builtin_fmaf(-den, t, 1.0f), t, t);
+  t += t;
   F3 v{q.b * t, q.c * t, q.d * t};
   v.x = __uint_as_float((__float_as_uint(v.x) & ~1u) | (q.a < 0.0f ? 1u : 0u));
   return v;

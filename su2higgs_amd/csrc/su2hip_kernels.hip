@@ -99,7 +99,7 @@ template <typename T, int M>
 __device__ __forceinline__ Q<T> hop_term(const LatDesc &L, const T *A, const T *F, const Site &x,
                                          int mu) {
   Q<T> b = F_at(L, F, shift(L, x, mu, +1));
-  if constexpr ((M & M_U1) != 0) b = qmul(b, u1twist<T>(A_at(L, A, x, mu)));
+  if constexpr ((M & M_U1) != 0) b = qmul_u1(b, u1twist<T>(A_at(L, A, x, mu)));
   Q<T> t = qmul(b, qconj(F_at(L, F, x)));
   return qscale(t, (T)-0.5);
 }
@@ -123,10 +123,10 @@ __device__ __forceinline__ void doublet_hops(const LatDesc &L, const T *U, const
                                              const Site &x, int d, Q<T> &s) {
   Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);
   Q<T> t = qmul(U_at<M>(L, U, x, d), F_at(L, F, xp));
-  if constexpr ((M & M_U1) != 0) t = qmul(t, u1twist<T>(A_at(L, A, x, d)));
+  if constexpr ((M & M_U1) != 0) t = qmul_u1(t, u1twist<T>(A_at(L, A, x, d)));
   s = qadd(s, qscale(t, (T)-1));
   Q<T> t2 = qmul(qconj(U_at<M>(L, U, xm, d)), F_at(L, F, xm));
-  if constexpr ((M & M_U1) != 0) t2 = qmul(t2, qconj(u1twist<T>(A_at(L, A, xm, d))));
+  if constexpr ((M & M_U1) != 0) t2 = qmul_u1(t2, qconj(u1twist<T>(A_at(L, A, xm, d))));
   s = qadd(s, qscale(t2, (T)-1));
 }
 
@@ -145,10 +145,10 @@ __device__ __forceinline__ Q<T> doublet_staple(const LatDesc &L, const T *U, con
   for (int d = 0; d < 3; d++) {
     Site xp = shift(L, x, d, +1), xm = shift(L, x, d, -1);
     Q<T> t = qmul(U_at<M>(L, U, x, d), F_at(L, F, xp));
-    if constexpr ((M & M_U1) != 0) t = qmul(t, u1twist<T>(A_at(L, A, x, d)));
+    if constexpr ((M & M_U1) != 0) t = qmul_u1(t, u1twist<T>(A_at(L, A, x, d)));
     s = qadd(s, qscale(t, (T)-1));
     Q<T> t2 = qmul(qconj(U_at<M>(L, U, xm, d)), F_at(L, F, xm));
-    if constexpr ((M & M_U1) != 0) t2 = qmul(t2, qconj(u1twist<T>(A_at(L, A, xm, d))));
+    if constexpr ((M & M_U1) != 0) t2 = qmul_u1(t2, qconj(u1twist<T>(A_at(L, A, xm, d))));
     s = qadd(s, qscale(t2, (T)-1));
   }
   return s;
